@@ -1,0 +1,181 @@
+/* palabos_amd — MI355X-native D3Q19 collide-and-stream engine, C ABI.
+ *
+ * Every entry point returns 0 on success and a negative code on failure; the
+ * message of the last failure on the calling thread is available from
+ * plbamd_last_error(). No C++ exception crosses this boundary. All pointers
+ * are host pointers unless stated otherwise; sizes are in bytes or cells as
+ * named. Calls on one lattice must be serialised by the caller (Palabos is
+ * single-threaded per rank, multiBlock/multiBlockLattice3D.hh:318-360).
+ *
+ * Populations cross the boundary in Palabos's static cell layout: C order
+ * [nx][ny][nz][19] of shifted populations f-bar = f - t_i, D3Q19 velocity order
+ * of latticeBoltzmann/nearestNeighborLattices3D.hh:166-178
+ * (BlockLatticeDataTransfer3D::send, atomicBlock/blockLattice3D.hh:650-690).
+ */
+#ifndef PALABOS_AMD_H
+#define PALABOS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes. */
+#define PLBAMD_OK 0
+#define PLBAMD_EINVAL (-1)   /* bad argument                                  */
+#define PLBAMD_ENOMEM (-2)   /* device allocation failed                      */
+#define PLBAMD_EHIP (-3)     /* HIP runtime error                             */
+#define PLBAMD_ESTATE (-4)   /* call not valid in the lattice's current state */
+
+/* Storage precision of the populations on the device. F16/BF16 store f-bar in
+ * 16 bits and compute in FP32; F64 and F32 compute in their own precision. */
+#define PLBAMD_F64 0
+#define PLBAMD_F32 1
+#define PLBAMD_F16 2
+#define PLBAMD_BF16 3
+
+/* Bulk (background) dynamics of a lattice. */
+#define PLBAMD_BGK 0   /* BGKdynamics, basicDynamics/isoThermalDynamics.hh:318-330             */
+#define PLBAMD_RBGK 1  /* RegularizedBGKdynamics, basicDynamics/isoThermalDynamics.hh:1587-1600 */
+
+/* Per-cell dynamics tags accepted by plbamd_set_tags. */
+#define PLBAMD_TAG_BULK 0        /* background dynamics                                     */
+#define PLBAMD_TAG_BOUNCE_BACK 1 /* BounceBack (full-way, node), core/dynamics.hh:783-790     */
+#define PLBAMD_TAG_NO_DYNAMICS 2 /* NoDynamics, core/dynamics.hh:968-971                      */
+#define PLBAMD_TAG_MOMENTUM_EXCHANGE 3 /* MomentumExchangeBounceBack, bounceBackModels.hh:98-135 */
+#define PLBAMD_TAG_WALL_LINK 4   /* solid for link-wise half-way bounce-back: neighbouring
+                                    fluid cells reflect their outgoing populations in the same
+                                    step, as Palabos does at a non-periodic domain edge
+                                    (blockLattice3D.hh:403-427); the cell itself is not updated */
+
+/* Boundary-plane families and kinds (plbamd_add_plane). */
+#define PLBAMD_ZOUHE 0        /* ZouHeVelocity/PressureDynamics, boundaryCondition/zouHeDynamics.hh:44-107 */
+#define PLBAMD_REGULARIZED 1  /* RegularizedVelocity/DensityBoundaryDynamics,
+                                 boundaryCondition/regularizedBoundaryDynamics.hh:94-106,166-178 */
+#define PLBAMD_VELOCITY 0
+#define PLBAMD_PRESSURE 1
+
+/* Statistics layout returned by plbamd_get_statistics (FP64). */
+#define PLBAMD_STAT_AVG_RHOBAR 0 /* average pre-collision rhoBar of cells taking statistics   */
+#define PLBAMD_STAT_AVG_USQR 1   /* average pre-collision |u|^2                               */
+#define PLBAMD_STAT_MAX_USQR 2   /* maximum pre-collision |u|^2                               */
+#define PLBAMD_STAT_FORCE_X 3    /* momentum-exchange force on the obstacle                   */
+#define PLBAMD_STAT_FORCE_Y 4
+#define PLBAMD_STAT_FORCE_Z 5
+#define PLBAMD_STAT_COUNT 6      /* number of cells that took statistics                      */
+#define PLBAMD_STAT_POROUS_X 7   /* momentum removed by the Brinkman term (x, y, z)           */
+#define PLBAMD_STAT_POROUS_Y 8
+#define PLBAMD_STAT_POROUS_Z 9
+#define PLBAMD_NUM_STATS 10
+
+typedef struct {
+    int32_t x0, x1, y0, y1, z0, z1; /* inclusive, like plb::Box3D (core/geometry3D.h) */
+} plbamd_box;
+
+typedef struct {
+    int32_t nx, ny, nz;   /* lattice extent in cells                                   */
+    int32_t storage;      /* PLBAMD_F64 | F32 | F16 | BF16                             */
+    int32_t dynamics;     /* PLBAMD_BGK | PLBAMD_RBGK                                  */
+    int32_t periodic[3];  /* 1 = periodic along x/y/z, 0 = same-step edge reflection   */
+    double omega;         /* relaxation frequency                                      */
+    double force[3];      /* uniform Guo body force (ForcedD3Q19, externalForceDynamics.hh:215-232);
+                             all zero = plain dynamics                                */
+} plbamd_lattice_desc;
+
+/* Selects and initialises the HIP device for the calling process. */
+int plbamd_init(int device);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* plbamd_last_error(void);
+
+/* Creates a lattice (one Palabos block, envelope included) and returns its handle.
+ * Replaces CoProcessor3D<T>::addDomain (coProcessors/coProcessor3D.h:77) and the
+ * BlockLattice3D(nx,ny,nz,Dynamics*) constructor (atomicBlock/blockLattice3D.h:111).
+ * Populations start at rest (f-bar = 0). */
+int plbamd_lattice_create(const plbamd_lattice_desc* desc, int* handle);
+
+/* Releases a lattice. Replaces BlockLattice3D's destructor (blockLattice3D.hh:120-130). */
+int plbamd_lattice_destroy(int handle);
+
+/* Sets per-cell dynamics tags, C order [nx][ny][nz], PLBAMD_TAG_*.
+ * Replaces defineDynamics(lattice, domain, Dynamics*) for the BounceBack,
+ * NoDynamics and MomentumExchangeBounceBack prototypes
+ * (dataProcessors/dataInitializerWrapper3D.hh:232-235); the momentum-exchange
+ * fluid directions are derived as initializeMomentumExchange does
+ * (boundaryCondition/bounceBackModels3D.hh:42-75). */
+int plbamd_set_tags(int handle, const uint8_t* tags);
+
+/* Adds a boundary plane (face of the box must be one cell thick along `dir`).
+ * Replaces OnLatticeBoundaryCondition3D::add{Velocity,Pressure}Boundary{0,1,2}{N,P}
+ * (boundaryCondition/boundaryCondition3D.h:40-200) of the factories
+ * createDynamicsBasedZouHeBoundaryCondition3D (zouHeBoundary3D.hh:345) and
+ * createDynamicsBasedLocalBoundaryCondition3D (boundaryCondition3D.hh:1170):
+ * the completion runs inside the cell's collision. Returns the plane id. */
+int plbamd_add_plane(int handle, int family, int kind, int dir, int ori,
+                     const plbamd_box* box, int* plane);
+
+/* Sets the imposed values of a plane, x,y,z C order over its box: velocity planes
+ * take 3 doubles (u) per cell, pressure planes 1 double (rho) per cell.
+ * Replaces setBoundaryVelocity / setBoundaryDensity
+ * (dataProcessors/dataInitializerFunctional3D.hh:830-845, :878-888). */
+int plbamd_set_plane_values(int handle, int plane, const double* values, size_t count);
+
+/* Sets a Brinkman porosity field alpha in [0,1] per cell ([nx][ny][nz], FP64);
+ * NULL removes it. The equilibrium velocity of a porous cell is (1-alpha) u.
+ * No Palabos counterpart. */
+int plbamd_set_porosity(int handle, const double* alpha);
+
+/* Copies populations of `box` between host and device. `elem` is the host
+ * element type (PLBAMD_F64 or PLBAMD_F32 values, or PLBAMD_F16/BF16 raw
+ * 16-bit patterns when it equals the storage type); `bytes` must equal
+ * box cells * 19 * sizeof(elem). Conversion to the storage type rounds to
+ * nearest even. Replaces CoProcessor3D<T>::send / receive
+ * (coProcessors/coProcessor3D.h:93,103) and Cell::operator[] access. */
+int plbamd_upload(int handle, const plbamd_box* box, int elem, const void* data, size_t bytes);
+int plbamd_download(int handle, const plbamd_box* box, int elem, void* data, size_t bytes);
+
+/* Sets the populations of `box` to equilibrium for per-cell rho and u
+ * (FP64, box cells and box cells x 3). Replaces initializeAtEquilibrium
+ * (dataProcessors/dataInitializerFunctional3D.hh:925-944). */
+int plbamd_initialize_at_equilibrium(int handle, const plbamd_box* box,
+                                     const double* rho, const double* u);
+
+/* Advances the lattice by nsteps collide-and-stream steps (blocking).
+ * Replaces CoProcessor3D<T>::collideAndStream (coProcessors/coProcessor3D.h:115)
+ * and BlockLattice3D::collideAndStream() (atomicBlock/blockLattice3D.hh:253-330). */
+int plbamd_collide_and_stream(int handle, int nsteps);
+
+/* As plbamd_collide_and_stream, and returns the elapsed device time of the
+ * steps in milliseconds, measured with HIP events on the lattice's stream. */
+int plbamd_collide_and_stream_timed(int handle, int nsteps, double* ms);
+
+/* Turns the per-step statistics on or off (on by default). Replaces
+ * MultiBlock3D::toggleInternalStatistics (multiBlock/multiBlock3D.cpp:455). */
+int plbamd_toggle_statistics(int handle, int on);
+
+/* Statistics of the last step, PLBAMD_NUM_STATS doubles (PLBAMD_STAT_*).
+ * Replaces getStoredAverageDensity / getStoredAverageEnergy and the
+ * momentum-exchange sums of BlockStatistics (core/blockStatistics.cpp:77-120). */
+int plbamd_get_statistics(int handle, double* out);
+
+/* Per-cell density and velocity of `box` (FP64; box cells and box cells x 3),
+ * rho = rhoBar + 1, u = j / rho of the current populations
+ * (momentTemplates3D.h:441-449). */
+int plbamd_compute_macroscopic(int handle, const plbamd_box* box, double* rho, double* u);
+
+/* FP64 sums over `box` of the current populations: out[0] = sum rho,
+ * out[1] = sum |u|^2 / 2, out[2..4] = sum j. computeAverageDensity /
+ * computeAverageEnergy divide these by the cell count
+ * (dataProcessors/dataAnalysisWrapper3D.h:628-645). */
+int plbamd_reduce_moments(int handle, const plbamd_box* box, double* out);
+
+/* Number of completed steps. */
+int plbamd_get_time(int handle, int64_t* t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PALABOS_AMD_H */

@@ -1,0 +1,766 @@
+// ORACLE — test infrastructure only.
+//
+// A CPU restatement of Palabos's D3Q19 collide-and-stream path, written to
+// follow the reference's arithmetic and its operation order term by term, so
+// that in FP64 (and FP32) it reproduces compiled Palabos bit for bit when both
+// are compiled without floating-point contraction. Only tests/, the smoke check
+// and the benchmark's CPU-baseline leg may load it, and only as the checker.
+// The product library never links or calls it.
+//
+// Streaming is done with two arrays (pull), not with Palabos's in-place swap;
+// the cell update order does not change any value (each cell's collision reads
+// only its own populations), so the results are identical.
+//
+// Parity status of each function (see the tests named in brackets):
+//   d3q19 constants, moments, BGK, RBGK, streaming + edge reflection,
+//   periodicity, BounceBack, NoDynamics, MomentumExchangeBounceBack,
+//   Zou-He velocity/pressure, regularized velocity/pressure
+//        -> pinned bit-exact against compiled Palabos golden vectors
+//           [tests/test_oracle_golden.py]
+//   FP16 / BF16 storage rounding
+//        -> pinned against numpy's float16 conversion and against exhaustive
+//           bit-pattern checks [tests/test_oracle_invariants.py]
+//   BGK + Guo body force
+//        -> pinned bit-exact against compiled Palabos (ForcedD3Q19 +
+//           GuoExternalForceBGKdynamics) golden vectors
+//   RBGK + Guo, Brinkman porosity term (no Palabos counterpart)
+//        -> parity unpinned against a reference; checked by invariants only
+//           (alpha = 0 and F = 0 reduce to the pinned paths, the Guo source
+//           adds exactly F of momentum per step, alpha = 1 cells relax to rest)
+//           [tests/test_oracle_invariants.py]
+
+#include <cstdint>
+#include <cstring>
+#include <cmath>
+#include <vector>
+#include <algorithm>
+
+namespace {
+
+// D3Q19 velocity set in Palabos order: rest, nine "negative-first" velocities,
+// then their opposites (i <-> i+9).
+// latticeBoltzmann/nearestNeighborLattices3D.hh:166-178
+const int C[19][3] = {
+    { 0, 0, 0},
+    {-1, 0, 0}, { 0,-1, 0}, { 0, 0,-1},
+    {-1,-1, 0}, {-1, 1, 0}, {-1, 0,-1},
+    {-1, 0, 1}, { 0,-1,-1}, { 0,-1, 1},
+    { 1, 0, 0}, { 0, 1, 0}, { 0, 0, 1},
+    { 1, 1, 0}, { 1,-1, 0}, { 1, 0, 1},
+    { 1, 0,-1}, { 0, 1, 1}, { 0, 1,-1}
+};
+
+inline int opposite(int i) { return i == 0 ? 0 : (i <= 9 ? i + 9 : i - 9); }  // indexTemplates.h:42-46
+
+template<typename T> struct W {
+    // nearestNeighborLattices3D.hh:186-199
+    static T t(int i) {
+        if (i == 0) return (T)1 / (T)3;
+        int n2 = C[i][0]*C[i][0] + C[i][1]*C[i][1] + C[i][2]*C[i][2];
+        return n2 == 1 ? (T)1 / (T)18 : (T)1 / (T)36;
+    }
+    static T cs2() { return (T)1 / (T)3; }
+};
+
+// ---------------------------------------------------------------------------
+// Storage conversions (round to nearest even).
+
+inline uint16_t f32_to_f16(float f) {
+    uint32_t x; std::memcpy(&x, &f, 4);
+    uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) {  // inf or nan
+        return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u));
+    }
+    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);  // rounds to >= 65520 -> inf
+    int e = (int)(ax >> 23) - 127;
+    uint32_t mant = ax & 0x7fffffu;
+    if (e >= -14) {  // normal half
+        uint32_t m = mant | 0x800000u;           // 24-bit significand
+        uint32_t shift = 13;
+        uint32_t q = m >> shift, r = m & ((1u << shift) - 1), half = 1u << (shift - 1);
+        if (r > half || (r == half && (q & 1u))) q += 1;
+        // q in [2^10, 2^11]; carry to 2^11 bumps the exponent
+        uint32_t he = (uint32_t)(e + 15);
+        if (q == 0x800u) { q = 0x400u; he += 1; }
+        if (he >= 31) return (uint16_t)(sign | 0x7c00u);
+        return (uint16_t)(sign | (he << 10) | (q & 0x3ffu));
+    }
+    // subnormal half: value = q * 2^-24
+    if (e < -25) return (uint16_t)sign;  // below half of the smallest subnormal -> 0 (ties handled below)
+    uint32_t m = mant | 0x800000u;           // value = m * 2^(e-23)
+    uint32_t shift = (uint32_t)(-e - 1);     // q = m * 2^(e-23+24) = m >> (-1-e)
+    uint32_t q = m >> shift;
+    uint32_t r = m & ((1u << shift) - 1), half = 1u << (shift - 1);
+    if (r > half || (r == half && (q & 1u))) q += 1;
+    return (uint16_t)(sign | q);             // q == 0x400 becomes the smallest normal, as it should
+}
+
+inline float f16_to_f32(uint16_t h) {
+    uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    uint32_t x;
+    if (e == 0) {
+        if (m == 0) x = sign;
+        else {
+            float v = (float)m * 5.9604644775390625e-08f;  // m * 2^-24, exact
+            std::memcpy(&x, &v, 4);
+            x |= sign;
+        }
+    } else if (e == 31) {
+        x = sign | 0x7f800000u | (m << 13);
+    } else {
+        x = sign | ((e + 112u) << 23) | (m << 13);
+    }
+    float f; std::memcpy(&f, &x, 4);
+    return f;
+}
+
+inline uint16_t f32_to_bf16(float f) {
+    uint32_t x; std::memcpy(&x, &f, 4);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);  // quiet nan
+    uint32_t lsb = (x >> 16) & 1u;
+    x += 0x7fffu + lsb;
+    return (uint16_t)(x >> 16);
+}
+
+inline float bf16_to_f32(uint16_t h) {
+    uint32_t x = (uint32_t)h << 16;
+    float f; std::memcpy(&f, &x, 4);
+    return f;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Layouts shared with the Python side (tests/oracle.py builds these with ctypes).
+typedef struct {
+    int32_t family;     // 0 Zou-He, 1 regularized
+    int32_t kind;       // 0 velocity, 1 pressure
+    int32_t dir, ori;   // plane normal axis, orientation (-1 or +1)
+    int32_t box[6];     // x0 x1 y0 y1 z0 z1, inclusive
+    const double* values;  // velocity: 3 per box cell; pressure: 1 (rho) per box cell; x,y,z C order
+} OraclePlane;
+
+typedef struct {
+    int32_t nx, ny, nz;
+    int32_t storage;     // 0 f64, 1 f32, 2 f16 storage / f32 compute, 3 bf16 storage / f32 compute
+    int32_t dynamics;    // 0 BGK, 1 regularized BGK
+    int32_t periodic[3];
+    double omega;
+    const uint8_t* tags; // nullable; per cell: 0 bulk, 1 BounceBack, 2 NoDynamics, 3 MomentumExchangeBounceBack
+    int32_t nplanes;
+    const OraclePlane* planes;
+    double force[3];     // Guo body force per unit volume (all zero: plain dynamics)
+    const double* alpha; // nullable; Brinkman porosity per cell in [0,1]
+    int32_t bc_wrapped;  // unused by the step; documents the snapshot convention of the caller
+} OracleDesc;
+
+}  // extern "C"
+
+namespace {
+
+enum { TAG_BULK = 0, TAG_BB = 1, TAG_NODYN = 2, TAG_MEBB = 3, TAG_WALL = 4, TAG_PLANE = 5 };
+
+// Moments ------------------------------------------------------------------
+
+// momentTemplates3D.h:351-373 (partial_rho) and :411-423 (get_rhoBar_j)
+template<typename T>
+inline void get_rhoBar_j(const T* f, T& rhoBar, T j[3]) {
+    T sXM1 = f[1] + f[4] + f[5] + f[6] + f[7];
+    T sXP1 = f[10] + f[13] + f[14] + f[15] + f[16];
+    T sYM1 = f[2] + f[4] + f[8] + f[9] + f[14];
+    T sYP1 = f[5] + f[11] + f[13] + f[17] + f[18];
+    T sZM1 = f[3] + f[6] + f[8] + f[16] + f[18];
+    T sZP1 = f[7] + f[9] + f[12] + f[15] + f[17];
+    T sX0 = f[0] + f[2] + f[3] + f[8] + f[9] + f[11] + f[12] + f[17] + f[18];
+    rhoBar = sXM1 + sX0 + sXP1;
+    j[0] = (sXP1 - sXM1);
+    j[1] = (sYP1 - sYM1);
+    j[2] = (sZP1 - sZM1);
+}
+
+template<typename T> inline T invRhoOf(T rhoBar) { return (T)1 / (rhoBar + (T)1); }  // roundOffPolicy.h:86-88
+
+// momentTemplates3D.h:517-539 (compute_rhoBar_j_PiNeq, D3Q19); PiNeq order xx xy xz yy yz zz
+template<typename T>
+inline void get_rhoBar_j_PiNeq(const T* f, T& rhoBar, T j[3], T pi[6]) {
+    T sXM1 = f[1] + f[4] + f[5] + f[6] + f[7];
+    T sXP1 = f[10] + f[13] + f[14] + f[15] + f[16];
+    T sYM1 = f[2] + f[4] + f[8] + f[9] + f[14];
+    T sYP1 = f[5] + f[11] + f[13] + f[17] + f[18];
+    T sZM1 = f[3] + f[6] + f[8] + f[16] + f[18];
+    T sZP1 = f[7] + f[9] + f[12] + f[15] + f[17];
+    T sX0 = f[0] + f[2] + f[3] + f[8] + f[9] + f[11] + f[12] + f[17] + f[18];
+    rhoBar = sXM1 + sX0 + sXP1;
+    j[0] = (sXP1 - sXM1);
+    j[1] = (sYP1 - sYM1);
+    j[2] = (sZP1 - sZM1);
+    T invRho = invRhoOf(rhoBar);
+    T cs2 = W<T>::cs2();
+    pi[0] = sXP1 + sXM1 - cs2*rhoBar - invRho*j[0]*j[0];
+    pi[3] = sYP1 + sYM1 - cs2*rhoBar - invRho*j[1]*j[1];
+    pi[5] = sZP1 + sZM1 - cs2*rhoBar - invRho*j[2]*j[2];
+    pi[1] = f[4] - f[5] + f[13] - f[14] - invRho*j[0]*j[1];
+    pi[2] = f[6] - f[7] + f[15] - f[16] - invRho*j[0]*j[2];
+    pi[4] = f[8] - f[9] + f[17] - f[18] - invRho*j[1]*j[2];
+}
+
+// dynamicsTemplates3D.h:1994-1997 (bgk_ma2_equilibrium, D3Q19)
+template<typename T>
+inline T equilibrium(int i, T rhoBar, T invRho, const T j[3], T jSqr) {
+    T c_j = C[i][0]*j[0] + C[i][1]*j[1] + C[i][2]*j[2];
+    return W<T>::t(i) * (rhoBar + (T)3.*c_j + invRho*((T)4.5*c_j*c_j - (T)1.5*jSqr));
+}
+
+// momentTemplates3D.h:375-382 (get_rhoBar, D3Q19: plain sum in index order)
+template<typename T>
+inline T get_rhoBar(const T* f) {
+    return f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8]
+         + f[9] + f[10] + f[11] + f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+}
+
+// externalForceTemplates3D.h:73-150 (addGuoForce, ForcedD3Q19, amplitude 1)
+template<typename T>
+inline void addGuoForce(T* f, const T u[3], const T force[3], T omega) {
+    T mu = (T)1*((T)1-omega/(T)2);
+    const T oneOver6 = (T)1/(T)6;
+    const T oneOver12 = (T)1/(T)12;
+    f[0] += -mu*(force[0]*u[0]+force[1]*u[1]+force[2]*u[2]);
+    f[1] += oneOver6*mu*(force[0]*(-(T)1+2*u[0])-force[1]*u[1]-force[2]*u[2]);
+    f[2] += -oneOver6*mu*(force[0]*u[0]+force[1]*((T)1-2*u[1])+force[2]*u[2]);
+    f[3] += -oneOver6*mu*(force[0]*u[0] + force[1]*u[1] + force[2]*((T)1-2*u[2]));
+    f[4] += oneOver12*mu*(force[0]*(-(T)1+2*u[0]+3*u[1]) + force[1]*(-(T)1+2*u[1]+3*u[0]) - force[2]*u[2]);
+    f[5] += oneOver12*mu*(force[0]*(-(T)1+2*u[0]-3*u[1]) + force[1]*((T)1+2*u[1]-3*u[0]) - force[2]*u[2]);
+    f[6] += oneOver12*mu*(force[0]*(-(T)1+2*u[0]+3*u[2]) - force[1]*u[1] + force[2]*(-(T)1+2*u[2]+3*u[0]));
+    f[7] += oneOver12*mu*(force[0]*(-(T)1+2*u[0]-3*u[2]) - force[1]*u[1] + force[2]*((T)1+2*u[2]-3*u[0]));
+    f[8] += -oneOver12*mu*(force[0]*u[0] + force[1]*((T)1-2*u[1]-3*u[2]) + force[2]*((T)1-2*u[2]-3*u[1]));
+    f[9] += -oneOver12*mu*(force[0]*u[0] + force[1]*((T)1-2*u[1]+3*u[2]) + force[2]*(-(T)1-2*u[2]+3*u[1]));
+    f[10] += oneOver6*mu*(force[0]*((T)1+2*u[0]) -force[1]*u[1] -force[2]*u[2]);
+    f[11] += -oneOver6*mu*(force[0]*u[0] +force[1]*(-(T)1-2*u[1]) +force[2]*u[2]);
+    f[12] += -oneOver6*mu*(force[0]*u[0] +force[1]*u[1] +force[2]*(-(T)1-2*u[2]));
+    f[13] += oneOver12*mu*(force[0]*((T)1+2*u[0]+3*u[1]) +force[1]*((T)1+2*u[1]+3*u[0]) -force[2]*u[2]);
+    f[14] += oneOver12*mu*(force[0]*((T)1+2*u[0]-3*u[1]) +force[1]*(-(T)1+2*u[1]-3*u[0]) -force[2]*u[2]);
+    f[15] += oneOver12*mu*(force[0]*((T)1+2*u[0]+3*u[2]) -force[1]*u[1] +force[2]*((T)1+2*u[2]+3*u[0]));
+    f[16] += oneOver12*mu*(force[0]*((T)1+2*u[0]-3*u[2]) -force[1]*u[1] +force[2]*(-(T)1+2*u[2]-3*u[0]));
+    f[17] += -oneOver12*mu*(force[0]*u[0] +force[1]*(-(T)1-2*u[1]-3*u[2]) +force[2]*(-(T)1-2*u[2]-3*u[1]));
+    f[18] += -oneOver12*mu*(force[0]*u[0] +force[1]*(-(T)1-2*u[1]+3*u[2]) +force[2]*((T)1-2*u[2]+3*u[1]));
+}
+
+// Collisions ---------------------------------------------------------------
+
+// dynamicsTemplates3D.h:2289-2368 (bgk_ma2_collision_base, D3Q19)
+template<typename T>
+inline T bgk_collision(T* f, T rhoBar, const T j[3], T omega, T invRho) {
+    T one_m_omega = (T)1 - omega;
+    T t0_omega = W<T>::t(0) * omega;
+    T t1_omega = W<T>::t(1) * omega;
+    T t4_omega = W<T>::t(4) * omega;
+    T jSqr = j[0]*j[0] + j[1]*j[1] + j[2]*j[2];
+    T kx = (T)3 * j[0];
+    T ky = (T)3 * j[1];
+    T kz = (T)3 * j[2];
+    T kxSqr_ = invRho / (T)2 * kx*kx;
+    T kySqr_ = invRho / (T)2 * ky*ky;
+    T kzSqr_ = invRho / (T)2 * kz*kz;
+    T kxky_ = invRho * kx*ky;
+    T kxkz_ = invRho * kx*kz;
+    T kykz_ = invRho * ky*kz;
+    T C1 = rhoBar + invRho*(T)3*jSqr;
+    T C2, C3;
+    C3 = -kxSqr_ - kySqr_ - kzSqr_;
+    f[0] *= one_m_omega; f[0] += t0_omega * (C1+C3);
+    C2 = -kx; C3 = -kySqr_ - kzSqr_;
+    f[1] *= one_m_omega; f[1] += t1_omega * (C1+C2+C3);
+    f[10] *= one_m_omega; f[10] += t1_omega * (C1-C2+C3);
+    C2 = -ky; C3 = -kxSqr_ - kzSqr_;
+    f[2] *= one_m_omega; f[2] += t1_omega * (C1+C2+C3);
+    f[11] *= one_m_omega; f[11] += t1_omega * (C1-C2+C3);
+    C2 = -kz; C3 = -kxSqr_ - kySqr_;
+    f[3] *= one_m_omega; f[3] += t1_omega * (C1+C2+C3);
+    f[12] *= one_m_omega; f[12] += t1_omega * (C1-C2+C3);
+    C2 = -kx - ky; C3 = kxky_ - kzSqr_;
+    f[4] *= one_m_omega; f[4] += t4_omega * (C1+C2+C3);
+    f[13] *= one_m_omega; f[13] += t4_omega * (C1-C2+C3);
+    C2 = -kx + ky; C3 = -kxky_ - kzSqr_;
+    f[5] *= one_m_omega; f[5] += t4_omega * (C1+C2+C3);
+    f[14] *= one_m_omega; f[14] += t4_omega * (C1-C2+C3);
+    C2 = -kx - kz; C3 = kxkz_ - kySqr_;
+    f[6] *= one_m_omega; f[6] += t4_omega * (C1+C2+C3);
+    f[15] *= one_m_omega; f[15] += t4_omega * (C1-C2+C3);
+    C2 = -kx + kz; C3 = -kxkz_ - kySqr_;
+    f[7] *= one_m_omega; f[7] += t4_omega * (C1+C2+C3);
+    f[16] *= one_m_omega; f[16] += t4_omega * (C1-C2+C3);
+    C2 = -ky - kz; C3 = kykz_ - kxSqr_;
+    f[8] *= one_m_omega; f[8] += t4_omega * (C1+C2+C3);
+    f[17] *= one_m_omega; f[17] += t4_omega * (C1-C2+C3);
+    C2 = -ky + kz; C3 = -kykz_ - kxSqr_;
+    f[9] *= one_m_omega; f[9] += t4_omega * (C1+C2+C3);
+    f[18] *= one_m_omega; f[18] += t4_omega * (C1-C2+C3);
+    return invRho*invRho*jSqr;
+}
+
+// dynamicsTemplates3D.h:1902-1973 (neqPiD3Q19::fromPiToFneq0..9); pi order xx xy xz yy yz zz
+template<typename T>
+inline void neqPi(const T pi[6], T p[10]) {
+    const T xx = pi[0], xy = pi[1], xz = pi[2], yy = pi[3], yz = pi[4], zz = pi[5];
+    p[0] = (T)3./(T)2 * (-(T)1./(T)3*xx - (T)1./(T)3*yy - (T)1./(T)3*zz);
+    p[1] = (T)1./(T)4 * ((T)2./(T)3*xx - (T)1./(T)3*yy - (T)1./(T)3*zz);
+    p[2] = (T)1./(T)4 * (-(T)1./(T)3*xx + (T)2./(T)3*yy - (T)1./(T)3*zz);
+    p[3] = (T)1./(T)4 * (-(T)1./(T)3*xx - (T)1./(T)3*yy + (T)2./(T)3*zz);
+    p[4] = (T)1./(T)8 * ((T)2./(T)3*xx + (T)2./(T)3*yy - (T)1./(T)3*zz + (T)2*xy);
+    p[5] = (T)1./(T)8 * ((T)2./(T)3*xx + (T)2./(T)3*yy - (T)1./(T)3*zz - (T)2*xy);
+    p[6] = (T)1./(T)8 * ((T)2./(T)3*xx - (T)1./(T)3*yy + (T)2./(T)3*zz + (T)2*xz);
+    p[7] = (T)1./(T)8 * ((T)2./(T)3*xx - (T)1./(T)3*yy + (T)2./(T)3*zz - (T)2*xz);
+    p[8] = (T)1./(T)8 * (-(T)1./(T)3*xx + (T)2./(T)3*yy + (T)2./(T)3*zz + (T)2*yz);
+    p[9] = (T)1./(T)8 * (-(T)1./(T)3*xx + (T)2./(T)3*yy + (T)2./(T)3*zz - (T)2*yz);
+}
+
+// dynamicsTemplates3D.h:2500-2554 (rlb_collision, D3Q19)
+template<typename T>
+inline T rlb_collision(T* f, T rhoBar, T invRho, const T j[3], const T pi[6], T omega) {
+    const T jSqr = j[0]*j[0] + j[1]*j[1] + j[2]*j[2];
+    T p[10];
+    neqPi(pi, p);
+    for (int i = 0; i < 10; ++i)
+        f[i] = equilibrium(i, rhoBar, invRho, j, jSqr) + ((T)1-omega) * p[i];
+    for (int i = 10; i < 19; ++i)
+        f[i] = equilibrium(i, rhoBar, invRho, j, jSqr) + ((T)1-omega) * p[i-9];
+    return jSqr*invRho*invRho;
+}
+
+// offEquilibriumTemplates.h:91-113 (generic fromPiToFneq, used by regularize)
+template<typename T>
+inline T fromPiToFneq(int i, const T pi[6]) {
+    T fNeq = T();
+    int iPi = 0;
+    for (int a = 0; a < 3; ++a) {
+        fNeq += pi[iPi] * (C[i][a]*C[i][a] - W<T>::cs2());
+        ++iPi;
+        for (int b = a + 1; b < 3; ++b) {
+            fNeq += pi[iPi] * (T)2 * C[i][a]*C[i][b];
+            ++iPi;
+        }
+    }
+    fNeq *= W<T>::t(i) * (T)3 * (T)3 / (T)2;
+    return fNeq;
+}
+
+// ---------------------------------------------------------------------------
+// The lattice being stepped: compute type T, storage type S.
+
+struct Stats {
+    double sumRhoBar = 0, sumUSqr = 0, maxUSqr = 0;
+    long count = 0;
+    double force[3] = {0, 0, 0};
+    double porous[3] = {0, 0, 0};
+};
+
+template<typename T>
+struct Engine {
+    const OracleDesc& d;
+    long N;
+    std::vector<uint8_t> tag;          // TAG_*
+    std::vector<int> planeOf;          // per cell, -1 or plane index
+    std::vector<long> boxIndexOf;      // per cell, index in the plane box
+    std::vector<uint32_t> fluidDirs;   // per cell, bit i set: direction i is a fluid direction (MEBB)
+
+    explicit Engine(const OracleDesc& d_) : d(d_) {
+        N = (long)d.nx * d.ny * d.nz;
+        tag.assign(N, TAG_BULK);
+        planeOf.assign(N, -1);
+        boxIndexOf.assign(N, 0);
+        fluidDirs.assign(N, 0);
+        if (d.tags) for (long k = 0; k < N; ++k) tag[k] = d.tags[k];
+        for (int p = 0; p < d.nplanes; ++p) {
+            const int* b = d.planes[p].box;
+            long idx = 0;
+            for (int x = b[0]; x <= b[1]; ++x)
+                for (int y = b[2]; y <= b[3]; ++y)
+                    for (int z = b[4]; z <= b[5]; ++z, ++idx) {
+                        long k = lin(x, y, z);
+                        tag[k] = TAG_PLANE;
+                        planeOf[k] = p;
+                        boxIndexOf[k] = idx;
+                    }
+        }
+        // bounceBackModels3D.hh:48-65 (fluidDirections of a MomentumExchangeBounceBack cell).
+        // A neighbour outside a non-periodic domain edge holds the background dynamics
+        // in Palabos's envelope, so it counts as fluid.
+        for (int x = 0; x < d.nx; ++x) for (int y = 0; y < d.ny; ++y) for (int z = 0; z < d.nz; ++z) {
+            long k = lin(x, y, z);
+            if (tag[k] != TAG_MEBB) continue;
+            uint32_t m = 0;
+            for (int i = 1; i < 19; ++i) {
+                int nx_ = x + C[i][0], ny_ = y + C[i][1], nz_ = z + C[i][2];
+                bool inside = wrap(nx_, ny_, nz_);
+                if (!inside) { m |= 1u << i; continue; }
+                uint8_t pt = tag[lin(nx_, ny_, nz_)];
+                if (pt != TAG_MEBB && pt != TAG_BB && pt != TAG_NODYN && pt != TAG_WALL) m |= 1u << i;
+            }
+            fluidDirs[k] = m;
+        }
+    }
+
+    long lin(int x, int y, int z) const { return ((long)x * d.ny + y) * d.nz + z; }
+
+    // Applies periodic wrap; returns false if the point is outside a non-periodic edge.
+    bool wrap(int& x, int& y, int& z) const {
+        if (x < 0 || x >= d.nx) { if (!d.periodic[0]) return false; x = (x + d.nx) % d.nx; }
+        if (y < 0 || y >= d.ny) { if (!d.periodic[1]) return false; y = (y + d.ny) % d.ny; }
+        if (z < 0 || z >= d.nz) { if (!d.periodic[2]) return false; z = (z + d.nz) % d.nz; }
+        return true;
+    }
+
+    // Base dynamics collision, with statistics (isoThermalDynamics.hh:318-330 and :1587-1600).
+    void baseCollide(T* f, Stats& st, long k) const {
+        T omega = (T)d.omega;
+        T uSqr;
+        T rhoBar;
+        T j[3];
+        bool forced = d.force[0] != 0 || d.force[1] != 0 || d.force[2] != 0;
+        if (d.alpha || forced) {
+            uSqr = forcedCollide(f, k, rhoBar, st);
+        } else if (d.dynamics == 0) {
+            get_rhoBar_j(f, rhoBar, j);
+            uSqr = bgk_collision(f, rhoBar, j, omega, invRhoOf(rhoBar));
+        } else {
+            T pi[6];
+            get_rhoBar_j_PiNeq(f, rhoBar, j, pi);
+            T invRho = invRhoOf(rhoBar);
+            uSqr = rlb_collision(f, rhoBar, invRho, j, pi, omega);
+        }
+        // latticeStatistics.h:38-43
+        st.sumRhoBar += (double)rhoBar;
+        st.sumUSqr += (double)uSqr;
+        st.maxUSqr = std::max(st.maxUSqr, (double)uSqr);
+        st.count += 1;
+    }
+
+    // BGK or RBGK with a Guo body force and/or a Brinkman porosity term.
+    // BGK + Guo follows GuoExternalForceBGKdynamics::collide
+    // (externalForceDynamics.hh:215-232, velocity :59-74) and addGuoForce
+    // (externalForceTemplates3D.h:73-150) term by term. Palabos has no
+    // RBGK + Guo and no Brinkman term; they are defined here:
+    //   RBGK + Guo: rhoBar, j, PiNeq from compute_rhoBar_j_PiNeq, u = j/rho + F/2,
+    //               rlb_collision around j_eq = rho u, then addGuoForce.
+    //   Brinkman:   the equilibrium velocity is damped, u <- (1 - alpha) u,
+    //               before j_eq = rho u is formed (alpha in [0,1], 1 = solid).
+    T forcedCollide(T* f, long k, T& rhoBar, Stats& st) const {
+        T omega = (T)d.omega;
+        T j[3];
+        T pi[6];
+        T rhoBarP;
+        if (d.dynamics == 0) {
+            rhoBar = get_rhoBar(f);
+            get_rhoBar_j(f, rhoBarP, j);
+        } else {
+            get_rhoBar_j_PiNeq(f, rhoBarP, j, pi);
+            rhoBar = rhoBarP;
+        }
+        T invRhoP = invRhoOf(rhoBarP);
+        T F[3] = {(T)d.force[0], (T)d.force[1], (T)d.force[2]};
+        T u[3];
+        for (int c = 0; c < 3; ++c) u[c] = j[c]*invRhoP + F[c]/(T)2;
+        // Brinkman damping; the momentum it removes, omega*rho*(u - u_damped),
+        // is summed as the porous drag.
+        T damp = (T)1 - (d.alpha ? (T)d.alpha[k] : (T)0);
+        T rho = rhoBar + (T)1;
+        T jeq[3];
+        for (int c = 0; c < 3; ++c) {
+            T u0 = u[c];
+            u[c] = u0 * damp;
+            jeq[c] = rho * u[c];
+            st.porous[c] += (double)(omega * (rho * u0 - jeq[c]));
+        }
+        T uSqr;
+        if (d.dynamics == 0) uSqr = bgk_collision(f, rhoBar, jeq, omega, invRhoOf(rhoBar));
+        else uSqr = rlb_collision(f, rhoBar, invRhoOf(rhoBar), jeq, pi, omega);
+        if (F[0] != 0 || F[1] != 0 || F[2] != 0) addGuoForce(f, u, F, omega);
+        return uSqr;
+    }
+
+    // Plane boundary completion (dynamics-based or wrapped variant: same arithmetic).
+    void completePlane(T* f, long k) const {
+        const OraclePlane& P = d.planes[planeOf[k]];
+        const long bi = boxIndexOf[k];
+        const int dir = P.dir, ori = P.ori;
+        // Index subsets (indexTemplates.h:86-133): onWall c[dir]==0, normal c[dir]==ori,
+        // outgoing (missing) = opposite(normal).
+        int onWall[19], nOnWall = 0, normal[19], nNormal = 0;
+        for (int i = 0; i < 19; ++i) {
+            if (C[i][dir] == 0) onWall[nOnWall++] = i;
+            if (C[i][dir] == ori) normal[nNormal++] = i;
+        }
+        // computeRhoBarJ: boundaryDynamics.hh:643-700 (velocity) and :776-822 (pressure)
+        T rhoBar;
+        T j[3];
+        if (P.kind == 0) {
+            T u[3] = {(T)P.values[3*bi], (T)P.values[3*bi+1], (T)P.values[3*bi+2]};
+            T rhoOnWall = T();
+            for (int n = 0; n < nOnWall; ++n) rhoOnWall += f[onWall[n]];
+            T rhoNormal = T();
+            for (int n = 0; n < nNormal; ++n) rhoNormal += f[normal[n]];
+            T velNormal = (T)ori * u[dir];
+            rhoBar = ((T)2*rhoNormal + rhoOnWall - velNormal) / ((T)1 + velNormal);
+            T rho = rhoBar + (T)1;
+            for (int c = 0; c < 3; ++c) j[c] = rho * u[c];
+        } else {
+            rhoBar = (T)P.values[bi] - (T)1;  // StoreDensityDynamics::defineDensity, boundaryDynamics.hh:251-255
+            j[0] = j[1] = j[2] = T();
+            T rhoOnWall = T();
+            for (int n = 0; n < nOnWall; ++n) rhoOnWall += f[onWall[n]];
+            T rhoNormal = T();
+            for (int n = 0; n < nNormal; ++n) rhoNormal += f[normal[n]];
+            j[dir] = (T)ori * ((T)2*rhoNormal + rhoOnWall - rhoBar);
+        }
+        T jSqr = j[0]*j[0] + j[1]*j[1] + j[2]*j[2];
+        T invRho = invRhoOf(rhoBar);
+        if (P.family == 0) {
+            // zouHeDynamics.hh:44-107 (ZouHeClosure)
+            int missing[5], nMissing = 0;
+            for (int n = 0; n < nNormal; ++n) missing[nMissing++] = opposite(normal[n]);
+            int diag[5], nDiag = 0;
+            bool removed = false;
+            for (int n = 0; n < nMissing; ++n) {
+                int p = missing[n];
+                int nz_ = std::abs(C[p][0]) + std::abs(C[p][1]) + std::abs(C[p][2]);
+                if (nz_ == 1 && !removed) { removed = true; continue; }
+                diag[nDiag++] = p;
+            }
+            for (int n = 0; n < nMissing; ++n) {
+                int p = missing[n], o = opposite(p);
+                f[p] = f[o] - equilibrium(o, rhoBar, invRho, j, jSqr) + equilibrium(p, rhoBar, invRho, j, jSqr);
+            }
+            // compute_rho_uLb (momentTemplates3D.h:441-449)
+            T fRhoBar, fJ[3];
+            get_rhoBar_j(f, fRhoBar, fJ);
+            T fInvRho = invRhoOf(fRhoBar);
+            T falseRho = fRhoBar + (T)1;
+            T falseU[3] = {fJ[0]*fInvRho, fJ[1]*fInvRho, fJ[2]*fInvRho};
+            T diff[3];
+            for (int c = 0; c < 3; ++c) diff[c] = (j[c] - falseRho*falseU[c]) / (T)nDiag;
+            for (int n = 0; n < nDiag; ++n)
+                for (int iDim = 1; iDim < 3; ++iDim) {
+                    int a = (dir + iDim) % 3;
+                    f[diag[n]] += C[diag[n]][a] * diff[a];
+                }
+        } else {
+            // regularizedBoundaryDynamics.hh:94-106 / :166-178, boundaryTemplates.h:42-90,
+            // isoThermalDynamics.hh:55-69 (regularize)
+            T fNeq[19];
+            for (int n = 0; n < nOnWall; ++n) {
+                int i = onWall[n];
+                fNeq[i] = f[i] - equilibrium(i, rhoBar, invRho, j, jSqr);
+            }
+            for (int n = 0; n < nNormal; ++n) {
+                int i = normal[n];
+                fNeq[i] = (i == 0) ? T() : f[i] - equilibrium(i, rhoBar, invRho, j, jSqr);
+            }
+            T pi[6];
+            int iPi = 0;
+            for (int a = 0; a < 3; ++a)
+                for (int b = a; b < 3; ++b) {
+                    pi[iPi] = T();
+                    for (int n = 0; n < nOnWall; ++n) {
+                        int i = onWall[n];
+                        pi[iPi] += C[i][a]*C[i][b]*fNeq[i];
+                    }
+                    for (int n = 0; n < nNormal; ++n) {
+                        int i = normal[n];
+                        pi[iPi] += (T)2 * C[i][a]*C[i][b]*fNeq[i];
+                    }
+                    ++iPi;
+                }
+            f[0] = equilibrium(0, rhoBar, invRho, j, jSqr) + fromPiToFneq(0, pi);
+            for (int i = 1; i <= 9; ++i) {
+                f[i] = equilibrium(i, rhoBar, invRho, j, jSqr);
+                f[i+9] = equilibrium(i+9, rhoBar, invRho, j, jSqr);
+                T fn = fromPiToFneq(i, pi);
+                f[i] += fn;
+                f[i+9] += fn;
+            }
+        }
+    }
+
+    // True if population i of cell (x,y,z) streams from a source that reflects it:
+    // outside a non-periodic edge, or a link-wise wall cell.
+    bool sourceReflects(int x, int y, int z, int i, long& src) const {
+        int sx = x - C[i][0], sy = y - C[i][1], sz = z - C[i][2];
+        if (!wrap(sx, sy, sz)) return true;
+        src = lin(sx, sy, sz);
+        return tag[src] == TAG_WALL;
+    }
+
+    // One collision of cell k, in place on f (the cell's 19 populations).
+    void collideCell(T* f, long k, Stats& st) const {
+        switch (tag[k]) {
+        case TAG_BULK:
+            baseCollide(f, st, k);
+            break;
+        case TAG_BB:  // dynamics.hh:783-790
+            for (int i = 1; i <= 9; ++i) std::swap(f[i], f[i+9]);
+            break;
+        case TAG_NODYN:  // dynamics.hh:968-971
+        case TAG_WALL:   // solid of link-wise half-way bounce-back: not updated
+            break;
+        case TAG_MEBB: {  // bounceBackModels.hh:98-135
+            T mom[3] = {T(), T(), T()};
+            uint32_t m = fluidDirs[k];
+            for (int i = 1; i < 19; ++i) {
+                if (!(m & (1u << i))) continue;
+                int o = opposite(i);
+                for (int c = 0; c < 3; ++c) mom[c] += (T)2*C[i][c]*f[o];
+            }
+            for (int c = 0; c < 3; ++c) st.force[c] += (double)(-mom[c]);
+            for (int i = 1; i <= 9; ++i) std::swap(f[i], f[i+9]);
+            break;
+        }
+        case TAG_PLANE:  // core/dynamics.hh:470-475 + :709-711 (completion, then base collision)
+            completePlane(f, k);
+            baseCollide(f, st, k);
+            break;
+        }
+    }
+};
+
+template<typename S, typename T> struct Store;
+template<> struct Store<double, double> {
+    static double load(double s) { return s; }
+    static double store(double v) { return v; }
+};
+template<> struct Store<float, float> {
+    static float load(float s) { return s; }
+    static float store(float v) { return v; }
+};
+struct Half { uint16_t bits; };
+struct BHalf { uint16_t bits; };
+template<> struct Store<Half, float> {
+    static float load(Half s) { return f16_to_f32(s.bits); }
+    static Half store(float v) { Half h; h.bits = f32_to_f16(v); return h; }
+};
+template<> struct Store<BHalf, float> {
+    static float load(BHalf s) { return bf16_to_f32(s.bits); }
+    static BHalf store(float v) { BHalf h; h.bits = f32_to_bf16(v); return h; }
+};
+
+template<typename S, typename T>
+void runSteps(const OracleDesc& d, S* state, int nsteps, double* statsOut) {
+    Engine<T> eng(d);
+    const long N = eng.N;
+    std::vector<S> post((size_t)N * 19);
+    Stats st;
+    for (int step = 0; step < nsteps; ++step) {
+        st = Stats();
+        // Collision (every cell reads only its own populations).
+        for (long k = 0; k < N; ++k) {
+            T f[19];
+            for (int i = 0; i < 19; ++i) f[i] = Store<S, T>::load(state[k*19 + i]);
+            eng.collideCell(f, k, st);
+            for (int i = 0; i < 19; ++i) post[k*19 + i] = Store<S, T>::store(f[i]);
+        }
+        // Streaming with same-step reflection at non-periodic edges
+        // (blockLattice3D.hh:403-427, latticeTemplates3D.h:50-63, periodicity :554-624).
+        // Link-wise walls reflect the same way; the force on them is the
+        // momentum 2 c_i f*_i of every reflected population.
+        for (int x = 0; x < d.nx; ++x) for (int y = 0; y < d.ny; ++y) for (int z = 0; z < d.nz; ++z) {
+            long k = eng.lin(x, y, z);
+            if (eng.tag[k] == TAG_WALL) continue;
+            state[k*19] = post[k*19];
+            for (int i = 1; i < 19; ++i) {
+                long src = -1;
+                if (!eng.sourceReflects(x, y, z, i, src)) {
+                    state[k*19 + i] = post[src*19 + i];
+                } else {
+                    state[k*19 + i] = post[k*19 + opposite(i)];
+                    if (src >= 0) {  // reflected by a wall cell: momentum 2 c_o f*_o, o = opposite(i)
+                        int o = opposite(i);
+                        T fo = Store<S, T>::load(post[k*19 + o]);
+                        for (int c = 0; c < 3; ++c) st.force[c] += (double)((T)2 * C[o][c] * fo);
+                    }
+                }
+            }
+        }
+    }
+    if (statsOut) {
+        statsOut[0] = st.count ? st.sumRhoBar / (double)st.count : 0.0;
+        statsOut[1] = st.count ? st.sumUSqr / (double)st.count : 0.0;
+        statsOut[2] = st.maxUSqr;
+        statsOut[3] = st.force[0];
+        statsOut[4] = st.force[1];
+        statsOut[5] = st.force[2];
+        statsOut[6] = (double)st.count;
+        statsOut[7] = st.porous[0];
+        statsOut[8] = st.porous[1];
+        statsOut[9] = st.porous[2];
+    }
+}
+
+template<typename S, typename T>
+void completeAll(const OracleDesc& d, S* state) {
+    Engine<T> eng(d);
+    for (long k = 0; k < eng.N; ++k) {
+        if (eng.tag[k] != TAG_PLANE) continue;
+        T f[19];
+        for (int i = 0; i < 19; ++i) f[i] = Store<S, T>::load(state[k*19 + i]);
+        eng.completePlane(f, k);
+        for (int i = 0; i < 19; ++i) state[k*19 + i] = Store<S, T>::store(f[i]);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Advances `state` (natural layout, AoS [x][y][z][19] f-bar in the storage type)
+// by nsteps collide-and-stream steps. statsOut (nullable, 10 doubles) receives the
+// last step's average rhoBar, average uSqr, max uSqr (pre-collision moments of
+// bulk and plane cells), the force on the obstacle (momentum exchange of
+// MomentumExchangeBounceBack nodes plus link-wise walls; x,y,z), the number of
+// cells that took statistics and the momentum removed by the Brinkman term
+// (x,y,z). Populations of link-wise wall cells are left unchanged.
+// Returns 0, or -1 on a bad descriptor.
+int oracle_run(const OracleDesc* d, void* state, int nsteps, double* statsOut) {
+    if (!d || !state || d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || nsteps < 0) return -1;
+    switch (d->storage) {
+    case 0: runSteps<double, double>(*d, (double*)state, nsteps, statsOut); return 0;
+    case 1: runSteps<float, float>(*d, (float*)state, nsteps, statsOut); return 0;
+    case 2: runSteps<Half, float>(*d, (Half*)state, nsteps, statsOut); return 0;
+    case 3: runSteps<BHalf, float>(*d, (BHalf*)state, nsteps, statsOut); return 0;
+    }
+    return -1;
+}
+
+// Applies the plane-boundary completion to the plane cells of `state` in place
+// (what Palabos's wrapped boundary processor leaves in the lattice after a step,
+// wrappedLocalBoundaryProcessor3D.hh:41-62).
+int oracle_complete(const OracleDesc* d, void* state) {
+    if (!d || !state) return -1;
+    switch (d->storage) {
+    case 0: completeAll<double, double>(*d, (double*)state); return 0;
+    case 1: completeAll<float, float>(*d, (float*)state); return 0;
+    case 2: completeAll<Half, float>(*d, (Half*)state); return 0;
+    case 3: completeAll<BHalf, float>(*d, (BHalf*)state); return 0;
+    }
+    return -1;
+}
+
+// Bulk equilibrium populations f-bar for (rho, u) per cell, FP64
+// (dynamicsTemplates3D.h:1994-1997 with j = rho u, cell.hh:124-132).
+void oracle_equilibrium(long n, const double* rho, const double* u, double* fOut) {
+    for (long k = 0; k < n; ++k) {
+        double rhoBar = rho[k] - 1.0;
+        double invRho = invRhoOf(rhoBar);
+        double j[3] = {rho[k]*u[3*k], rho[k]*u[3*k+1], rho[k]*u[3*k+2]};
+        double jSqr = j[0]*j[0] + j[1]*j[1] + j[2]*j[2];
+        for (int i = 0; i < 19; ++i) fOut[k*19 + i] = equilibrium(i, rhoBar, invRho, j, jSqr);
+    }
+}
+
+void oracle_f32_to_f16(long n, const float* in, uint16_t* out) { for (long k = 0; k < n; ++k) out[k] = f32_to_f16(in[k]); }
+void oracle_f16_to_f32(long n, const uint16_t* in, float* out) { for (long k = 0; k < n; ++k) out[k] = f16_to_f32(in[k]); }
+void oracle_f32_to_bf16(long n, const float* in, uint16_t* out) { for (long k = 0; k < n; ++k) out[k] = f32_to_bf16(in[k]); }
+void oracle_bf16_to_f32(long n, const uint16_t* in, float* out) { for (long k = 0; k < n; ++k) out[k] = bf16_to_f32(in[k]); }
+
+}  // extern "C"

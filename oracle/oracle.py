@@ -1,0 +1,196 @@
+"""ctypes wrapper of the oracle restatement (oracle/liboracle.so).
+
+Test infrastructure only: tests/, the smoke check and the benchmark's
+CPU-baseline leg use it as the checker. The product never imports it.
+
+A lattice case is any object with the attributes of ``tests/cases.py:Case``:
+nx, ny, nz, storage ('f64' | 'f32' | 'f16' | 'bf16'), dynamics ('bgk' | 'rbgk'),
+omega, periodic (3 bools), tags (uint8 [nx,ny,nz] or None), planes (list of
+Plane), force (3 floats), alpha (float64 [nx,ny,nz] or None).
+
+Populations are numpy arrays in Palabos's static-data order, [nx][ny][nz][19]
+of shifted populations f-bar (``BlockLatticeDataTransfer3D::send``,
+atomicBlock/blockLattice3D.hh:650-690), in the storage dtype
+(float64, float32, or uint16 bit patterns for f16/bf16).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+STORAGE_CODES = {"f64": 0, "f32": 1, "f16": 2, "bf16": 3}
+STORAGE_DTYPES = {"f64": np.float64, "f32": np.float32, "f16": np.uint16, "bf16": np.uint16}
+DYNAMICS_CODES = {"bgk": 0, "rbgk": 1}
+FAMILY_CODES = {"zouhe": 0, "regularized": 1}
+KIND_CODES = {"velocity": 0, "pressure": 1}
+
+
+class _Plane(ctypes.Structure):
+    _fields_ = [("family", ctypes.c_int32), ("kind", ctypes.c_int32),
+                ("dir", ctypes.c_int32), ("ori", ctypes.c_int32),
+                ("box", ctypes.c_int32 * 6), ("values", ctypes.c_void_p)]
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("storage", ctypes.c_int32), ("dynamics", ctypes.c_int32),
+                ("periodic", ctypes.c_int32 * 3), ("omega", ctypes.c_double),
+                ("tags", ctypes.c_void_p), ("nplanes", ctypes.c_int32),
+                ("planes", ctypes.c_void_p), ("force", ctypes.c_double * 3),
+                ("alpha", ctypes.c_void_p), ("bc_wrapped", ctypes.c_int32)]
+
+
+def build():
+    """Compiles oracle/liboracle.so with the committed Makefile."""
+    root = os.path.dirname(_HERE)
+    subprocess.run(["make", "-s", "-f", "oracle/Makefile", "all"], cwd=root, check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.oracle_run.argtypes = [ctypes.POINTER(_Desc), ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.c_void_p]
+        _lib.oracle_run.restype = ctypes.c_int
+        _lib.oracle_complete.argtypes = [ctypes.POINTER(_Desc), ctypes.c_void_p]
+        _lib.oracle_complete.restype = ctypes.c_int
+        _lib.oracle_equilibrium.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p]
+        for name in ("oracle_f32_to_f16", "oracle_f16_to_f32", "oracle_f32_to_bf16",
+                     "oracle_bf16_to_f32"):
+            getattr(_lib, name).argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
+    return _lib
+
+
+class _Keep:
+    """Holds the numpy buffers a descriptor points into."""
+
+    def __init__(self):
+        self.refs = []
+
+    def ptr(self, arr):
+        self.refs.append(arr)
+        return arr.ctypes.data
+
+
+def _desc(case, keep):
+    d = _Desc()
+    d.nx, d.ny, d.nz = case.nx, case.ny, case.nz
+    d.storage = STORAGE_CODES[case.storage]
+    d.dynamics = DYNAMICS_CODES[case.dynamics]
+    for a in range(3):
+        d.periodic[a] = int(bool(case.periodic[a]))
+    d.omega = float(case.omega)
+    if case.tags is not None:
+        tags = np.ascontiguousarray(case.tags, dtype=np.uint8)
+        assert tags.shape == (case.nx, case.ny, case.nz)
+        d.tags = keep.ptr(tags)
+    planes = list(case.planes or [])
+    d.nplanes = len(planes)
+    if planes:
+        arr = (_Plane * len(planes))()
+        for i, p in enumerate(planes):
+            arr[i].family = FAMILY_CODES[p.family]
+            arr[i].kind = KIND_CODES[p.kind]
+            arr[i].dir = p.dir
+            arr[i].ori = p.ori
+            for k in range(6):
+                arr[i].box[k] = p.box[k]
+            vals = np.ascontiguousarray(p.values, dtype=np.float64)
+            arr[i].values = keep.ptr(vals)
+        keep.refs.append(arr)
+        d.planes = ctypes.addressof(arr)
+    for a in range(3):
+        d.force[a] = float(case.force[a]) if case.force is not None else 0.0
+    if case.alpha is not None:
+        alpha = np.ascontiguousarray(case.alpha, dtype=np.float64)
+        assert alpha.shape == (case.nx, case.ny, case.nz)
+        d.alpha = keep.ptr(alpha)
+    return d
+
+
+def run(case, f, nsteps):
+    """Returns (populations after nsteps, stats dict). `f` is not modified."""
+    dtype = STORAGE_DTYPES[case.storage]
+    state = np.array(f, dtype=dtype, order="C", copy=True)
+    assert state.shape == (case.nx, case.ny, case.nz, 19)
+    keep = _Keep()
+    d = _desc(case, keep)
+    stats = np.zeros(10, dtype=np.float64)
+    rc = lib().oracle_run(ctypes.byref(d), state.ctypes.data, int(nsteps), stats.ctypes.data)
+    if rc != 0:
+        raise ValueError("oracle_run rejected the case")
+    return state, {"avg_rhobar": stats[0], "avg_usqr": stats[1], "max_usqr": stats[2],
+                   "force": stats[3:6].copy(), "count": int(stats[6]),
+                   "porous": stats[7:10].copy()}
+
+
+def complete(case, f):
+    """Applies the plane-boundary completion (wrapped-processor snapshot convention)."""
+    dtype = STORAGE_DTYPES[case.storage]
+    state = np.array(f, dtype=dtype, order="C", copy=True)
+    keep = _Keep()
+    d = _desc(case, keep)
+    if lib().oracle_complete(ctypes.byref(d), state.ctypes.data) != 0:
+        raise ValueError("oracle_complete rejected the case")
+    return state
+
+
+def equilibrium(rho, u):
+    """FP64 bulk equilibrium f-bar for per-cell rho [...] and u [...,3]."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.empty(rho.shape + (19,), dtype=np.float64)
+    lib().oracle_equilibrium(rho.size, rho.ctypes.data, u.ctypes.data, out.ctypes.data)
+    return out
+
+
+def _conv(name, arr, out_dtype):
+    arr = np.ascontiguousarray(arr)
+    out = np.empty(arr.shape, dtype=out_dtype)
+    getattr(lib(), name)(arr.size, arr.ctypes.data, out.ctypes.data)
+    return out
+
+
+def f32_to_f16(a):
+    return _conv("oracle_f32_to_f16", np.asarray(a, np.float32), np.uint16)
+
+
+def f16_to_f32(a):
+    return _conv("oracle_f16_to_f32", np.asarray(a, np.uint16), np.float32)
+
+
+def f32_to_bf16(a):
+    return _conv("oracle_f32_to_bf16", np.asarray(a, np.float32), np.uint16)
+
+
+def bf16_to_f32(a):
+    return _conv("oracle_bf16_to_f32", np.asarray(a, np.uint16), np.float32)
+
+
+def to_storage(f64, storage):
+    """Converts FP64 f-bar to the storage representation (RNE)."""
+    if storage == "f64":
+        return np.ascontiguousarray(f64, dtype=np.float64)
+    if storage == "f32":
+        return np.ascontiguousarray(f64, dtype=np.float32)
+    f32 = np.ascontiguousarray(f64, dtype=np.float32)
+    return f32_to_f16(f32) if storage == "f16" else f32_to_bf16(f32)
+
+
+def from_storage(s, storage):
+    """Converts storage-typed f-bar to float64."""
+    if storage in ("f64", "f32"):
+        return np.asarray(s, dtype=np.float64)
+    f32 = f16_to_f32(s) if storage == "f16" else bf16_to_f32(s)
+    return f32.astype(np.float64)

@@ -1,0 +1,342 @@
+// Drives the compiled Palabos reference (built from the sources under the
+// reference tree by oracle/Makefile.ref) on a case described by a small text
+// file, and writes the post-step populations and statistics as raw binary.
+//
+// This program is test infrastructure: it produces the golden vectors that pin
+// the oracle restatement (oracle/d3q19_oracle.cpp). It is never linked into the
+// product library.
+//
+// Case file, one "key values..." per line:
+//   T double|float
+//   size nx ny nz
+//   dyn bgk|rbgk                      base bulk dynamics
+//   omega w
+//   periodic px py pz                 0/1 per axis
+//   bcstyle dynamics|wrapped          createDynamicsBased...() or wrapped factory
+//   bcfamily zouhe|regularized        createZouHe... or createLocal...
+//   init file                         double f-bar, C order [nx][ny][nz][19]
+//   tags file                         uint8 [nx][ny][nz]: 0 bulk 1 BounceBack 2 NoDynamics 3 MomentumExchangeBounceBack
+//   bc vel|press dir ori x0 x1 y0 y1 z0 z1 file
+//                                     vel: 3 doubles per box cell, press: 1 double rho per box cell,
+//                                     box cells in x,y,z C order
+//   block x0 x1 y0 y1 z0 z1           optional; any number -> SparseBlockStructure3D multi-block
+//   dump s1 s2 ...                    steps after which populations are written (0 = after initialize)
+//   force fx fy fz                    optional; ForcedD3Q19 + GuoExternalForceBGKdynamics (bgk, no planes)
+//   out prefix
+// Outputs: prefix_s<step>.bin (T, [nx][ny][nz][19]) and prefix_stats.txt lines
+//   step storedAvgRho storedAvgEnergy avgRho(computeAverageDensity) avgEnergy force_x force_y force_z
+//
+// Bench mode: ref_driver --bench T n steps  -> periodic BGK(1.8) n^3 lattice, prints MLUPS.
+
+// The umbrella palabos3D.hh does not compile with g++ 11 (multiGrid), so the
+// lattice-core subset is included instead.
+#include "core/globalDefs.h"
+#include "parallelism/headers3D.h"
+#include "latticeBoltzmann/headers3D.h"
+#include "core/headers3D.h"
+#include "basicDynamics/headers3D.h"
+#include "boundaryCondition/headers3D.h"
+#include "io/headers3D.h"
+#include "atomicBlock/headers3D.h"
+#include "multiBlock/headers3D.h"
+#include "algorithm/headers3D.h"
+#include "dataProcessors/headers3D.h"
+#include "coProcessors/headers3D.h"
+#include "parallelism/headers3D.hh"
+#include "latticeBoltzmann/headers3D.hh"
+#include "core/headers3D.hh"
+#include "basicDynamics/headers3D.hh"
+#include "boundaryCondition/headers3D.hh"
+#include "io/headers3D.hh"
+#include "atomicBlock/headers3D.hh"
+#include "multiBlock/headers3D.hh"
+#include "algorithm/headers3D.hh"
+#include "dataProcessors/headers3D.hh"
+#include "coProcessors/headers3D.hh"
+
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+#include <ctime>
+
+using namespace plb;
+
+namespace {
+
+struct BcSpec {
+    std::string kind;
+    int dir, ori;
+    Box3D box;
+    std::string file;
+};
+
+struct CaseSpec {
+    std::string T = "double";
+    plint nx = 0, ny = 0, nz = 0;
+    std::string dyn = "bgk";
+    double omega = 1.0;
+    int periodic[3] = {0, 0, 0};
+    std::string bcstyle = "dynamics";
+    std::string bcfamily = "zouhe";
+    std::string init, tags, out;
+    std::vector<BcSpec> bcs;
+    std::vector<Box3D> blocks;
+    std::vector<plint> dumps;
+    bool forced = false;
+    double force[3] = {0, 0, 0};
+};
+
+std::vector<char> readFile(std::string const& name) {
+    std::ifstream f(name.c_str(), std::ios::binary);
+    if (!f) { std::fprintf(stderr, "cannot open %s\n", name.c_str()); std::exit(2); }
+    return std::vector<char>((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+}
+
+CaseSpec parseCase(std::string const& name) {
+    CaseSpec c;
+    std::ifstream f(name.c_str());
+    if (!f) { std::fprintf(stderr, "cannot open %s\n", name.c_str()); std::exit(2); }
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream s(line);
+        std::string key;
+        if (!(s >> key) || key[0] == '#') continue;
+        if (key == "T") s >> c.T;
+        else if (key == "size") s >> c.nx >> c.ny >> c.nz;
+        else if (key == "dyn") s >> c.dyn;
+        else if (key == "omega") s >> c.omega;
+        else if (key == "periodic") s >> c.periodic[0] >> c.periodic[1] >> c.periodic[2];
+        else if (key == "bcstyle") s >> c.bcstyle;
+        else if (key == "bcfamily") s >> c.bcfamily;
+        else if (key == "init") s >> c.init;
+        else if (key == "tags") s >> c.tags;
+        else if (key == "out") s >> c.out;
+        else if (key == "force") { s >> c.force[0] >> c.force[1] >> c.force[2]; c.forced = true; }
+        else if (key == "bc") {
+            BcSpec b;
+            s >> b.kind >> b.dir >> b.ori >> b.box.x0 >> b.box.x1 >> b.box.y0 >> b.box.y1
+              >> b.box.z0 >> b.box.z1 >> b.file;
+            c.bcs.push_back(b);
+        } else if (key == "block") {
+            Box3D b;
+            s >> b.x0 >> b.x1 >> b.y0 >> b.y1 >> b.z0 >> b.z1;
+            c.blocks.push_back(b);
+        } else if (key == "dump") {
+            plint v;
+            while (s >> v) c.dumps.push_back(v);
+        } else {
+            std::fprintf(stderr, "unknown key %s\n", key.c_str());
+            std::exit(2);
+        }
+    }
+    return c;
+}
+
+template<typename T, template<typename U> class D>
+void addVelocityPlane(OnLatticeBoundaryCondition3D<T,D>& bc,
+                      MultiBlockLattice3D<T,D>& lat,
+                      int dir, int ori, Box3D box) {
+    if (dir == 0 && ori < 0) bc.addVelocityBoundary0N(box, lat);
+    else if (dir == 0) bc.addVelocityBoundary0P(box, lat);
+    else if (dir == 1 && ori < 0) bc.addVelocityBoundary1N(box, lat);
+    else if (dir == 1) bc.addVelocityBoundary1P(box, lat);
+    else if (dir == 2 && ori < 0) bc.addVelocityBoundary2N(box, lat);
+    else bc.addVelocityBoundary2P(box, lat);
+}
+
+template<typename T, template<typename U> class D>
+void addPressurePlane(OnLatticeBoundaryCondition3D<T,D>& bc,
+                      MultiBlockLattice3D<T,D>& lat,
+                      int dir, int ori, Box3D box) {
+    if (dir == 0 && ori < 0) bc.addPressureBoundary0N(box, lat);
+    else if (dir == 0) bc.addPressureBoundary0P(box, lat);
+    else if (dir == 1 && ori < 0) bc.addPressureBoundary1N(box, lat);
+    else if (dir == 1) bc.addPressureBoundary1P(box, lat);
+    else if (dir == 2 && ori < 0) bc.addPressureBoundary2N(box, lat);
+    else bc.addPressureBoundary2P(box, lat);
+}
+
+template<typename T, template<typename U> class D>
+int runCase(CaseSpec const& c) {
+    typedef D<T> L;
+    typedef MultiBlockLattice3D<T,D> Lattice;
+    const int Q = 19;
+
+    Dynamics<T,D>* bulk = 0;
+    if (c.forced) bulk = new GuoExternalForceBGKdynamics<T,D>((T)c.omega);
+    else if (c.dyn == "bgk") bulk = new BGKdynamics<T,D>((T)c.omega);
+    else if (c.dyn == "rbgk") bulk = new RegularizedBGKdynamics<T,D>((T)c.omega);
+    else { std::fprintf(stderr, "unknown dyn\n"); return 2; }
+
+    Lattice* lattice = 0;
+    if (c.blocks.empty()) {
+        lattice = new Lattice(c.nx, c.ny, c.nz, bulk);
+    } else {
+        SparseBlockStructure3D sparse(Box3D(0, c.nx-1, 0, c.ny-1, 0, c.nz-1));
+        for (pluint i = 0; i < c.blocks.size(); ++i) sparse.addBlock(c.blocks[i], (plint)i);
+        MultiBlockManagement3D management(sparse, new OneToOneThreadAttribution, 1);
+        lattice = new Lattice(management,
+                              defaultMultiBlockPolicy3D().getBlockCommunicator(),
+                              defaultMultiBlockPolicy3D().getCombinedStatistics(),
+                              defaultMultiBlockPolicy3D().getMultiCellAccess<T,D>(),
+                              bulk);
+    }
+    Lattice& lat = *lattice;
+    lat.periodicity().toggle(0, c.periodic[0] != 0);
+    lat.periodicity().toggle(1, c.periodic[1] != 0);
+    lat.periodicity().toggle(2, c.periodic[2] != 0);
+
+    // Guo body force, uniform (external scalars of ForcedD3Q19).
+    if (c.forced) {
+        Array<T,3> F((T)c.force[0], (T)c.force[1], (T)c.force[2]);
+        setExternalVector(lat, lat.getBoundingBox(), D<T>::ExternalField::forceBeginsAt, F);
+    }
+
+    // Solid cells.
+    Array<plint,3> forceIds;
+    bool haveMomentumExchange = false;
+    if (!c.tags.empty()) {
+        std::vector<char> tags = readFile(c.tags);
+        for (plint x = 0; x < c.nx; ++x) for (plint y = 0; y < c.ny; ++y) for (plint z = 0; z < c.nz; ++z)
+            if (tags[(x*c.ny + y)*c.nz + z] == 3) haveMomentumExchange = true;
+        if (haveMomentumExchange) {
+            forceIds[0] = lat.internalStatSubscription().subscribeSum();
+            forceIds[1] = lat.internalStatSubscription().subscribeSum();
+            forceIds[2] = lat.internalStatSubscription().subscribeSum();
+        }
+        for (plint x = 0; x < c.nx; ++x) for (plint y = 0; y < c.ny; ++y) for (plint z = 0; z < c.nz; ++z) {
+            char t = tags[(x*c.ny + y)*c.nz + z];
+            Box3D cell(x, x, y, y, z, z);
+            if (t == 1) defineDynamics(lat, cell, new BounceBack<T,D>((T)1));
+            else if (t == 2) defineDynamics(lat, cell, new NoDynamics<T,D>((T)1));
+            else if (t == 3) defineDynamics(lat, cell,
+                     new MomentumExchangeBounceBack<T,D>(forceIds));
+        }
+        if (haveMomentumExchange) initializeMomentumExchange(lat, lat.getBoundingBox());
+    }
+
+    // Boundary planes.
+    OnLatticeBoundaryCondition3D<T,D>* bc = 0;
+    if (!c.bcs.empty()) {
+        if (c.bcfamily == "zouhe")
+            bc = (c.bcstyle == "dynamics") ? createDynamicsBasedZouHeBoundaryCondition3D<T,D>()
+                                           : createZouHeBoundaryCondition3D<T,D>();
+        else
+            bc = (c.bcstyle == "dynamics") ? createDynamicsBasedLocalBoundaryCondition3D<T,D>()
+                                           : createLocalBoundaryCondition3D<T,D>();
+        for (pluint i = 0; i < c.bcs.size(); ++i) {
+            BcSpec const& b = c.bcs[i];
+            if (b.kind == "vel") addVelocityPlane<T,D>(*bc, lat, b.dir, b.ori, b.box);
+            else addPressurePlane<T,D>(*bc, lat, b.dir, b.ori, b.box);
+        }
+    }
+
+    // Initial populations.
+    {
+        std::vector<char> raw = readFile(c.init);
+        const double* f = reinterpret_cast<const double*>(&raw[0]);
+        for (plint x = 0; x < c.nx; ++x) for (plint y = 0; y < c.ny; ++y) for (plint z = 0; z < c.nz; ++z) {
+            Cell<T,D>& cell = lat.get(x, y, z);
+            for (int i = 0; i < Q; ++i) cell[i] = (T)f[((x*c.ny + y)*c.nz + z)*Q + i];
+        }
+    }
+    // Boundary values (after the composite dynamics exist).
+    for (pluint i = 0; i < c.bcs.size(); ++i) {
+        BcSpec const& b = c.bcs[i];
+        std::vector<char> raw = readFile(b.file);
+        const double* v = reinterpret_cast<const double*>(&raw[0]);
+        plint k = 0;
+        for (plint x = b.box.x0; x <= b.box.x1; ++x)
+            for (plint y = b.box.y0; y <= b.box.y1; ++y)
+                for (plint z = b.box.z0; z <= b.box.z1; ++z, ++k) {
+                    Cell<T,D>& cell = lat.get(x, y, z);
+                    if (b.kind == "vel") {
+                        Array<T,3> u((T)v[3*k], (T)v[3*k+1], (T)v[3*k+2]);
+                        cell.defineVelocity(u);
+                    } else {
+                        cell.defineDensity((T)v[k]);
+                    }
+                }
+    }
+    lat.initialize();
+
+    std::string statsName = c.out + "_stats.txt";
+    FILE* stats = std::fopen(statsName.c_str(), "w");
+    plint maxStep = 0;
+    for (pluint i = 0; i < c.dumps.size(); ++i) if (c.dumps[i] > maxStep) maxStep = c.dumps[i];
+    std::vector<T> buf((size_t)(c.nx*c.ny*c.nz*Q));
+    for (plint step = 0; step <= maxStep; ++step) {
+        if (step > 0) lat.collideAndStream();
+        bool dumpNow = false;
+        for (pluint i = 0; i < c.dumps.size(); ++i) if (c.dumps[i] == step) dumpNow = true;
+        if (!dumpNow) continue;
+        for (plint x = 0; x < c.nx; ++x) for (plint y = 0; y < c.ny; ++y) for (plint z = 0; z < c.nz; ++z) {
+            Cell<T,D>& cell = lat.get(x, y, z);
+            for (int i = 0; i < Q; ++i) buf[((x*c.ny + y)*c.nz + z)*Q + i] = cell[i];
+        }
+        char name[512];
+        std::snprintf(name, sizeof(name), "%s_s%ld.bin", c.out.c_str(), (long)step);
+        FILE* o = std::fopen(name, "wb");
+        std::fwrite(&buf[0], sizeof(T), buf.size(), o);
+        std::fclose(o);
+        double fx = 0, fy = 0, fz = 0;
+        if (haveMomentumExchange && step > 0) {
+            fx = lat.getInternalStatistics().getSum(forceIds[0]);
+            fy = lat.getInternalStatistics().getSum(forceIds[1]);
+            fz = lat.getInternalStatistics().getSum(forceIds[2]);
+        }
+        double sRho = step > 0 ? getStoredAverageDensity<T>(lat) : 0.0;
+        double sEn = step > 0 ? getStoredAverageEnergy<T>(lat) : 0.0;
+        std::fprintf(stats, "%ld %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", (long)step,
+                     sRho, sEn, (double)computeAverageDensity(lat), (double)computeAverageEnergy(lat),
+                     fx, fy, fz);
+    }
+    std::fclose(stats);
+    delete bc;
+    delete lattice;
+    return 0;
+}
+
+template<typename T>
+int runBench(plint n, plint steps) {
+    typedef MultiBlockLattice3D<T,descriptors::D3Q19Descriptor> Lattice;
+    Lattice lat(n, n, n, new BGKdynamics<T,descriptors::D3Q19Descriptor>((T)1.8));
+    lat.periodicity().toggleAll(true);
+    lat.toggleInternalStatistics(false);
+    Array<T,3> u((T)0.01, (T)0.02, (T)0.0);
+    initializeAtEquilibrium(lat, lat.getBoundingBox(), (T)1, u);
+    lat.initialize();
+    lat.collideAndStream();  // warm-up
+    timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (plint s = 0; s < steps; ++s) lat.collideAndStream();
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    double sec = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+    double mlups = (double)n*n*n*steps / sec / 1e6;
+    std::printf("{\"mlups\": %.6f, \"seconds\": %.6f, \"n\": %ld, \"steps\": %ld, \"T\": \"%s\"}\n",
+                mlups, sec, (long)n, (long)steps, sizeof(T) == 8 ? "double" : "float");
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+    plbInit(&argc, &argv);
+    if (argc >= 2 && std::string(argv[1]) == "--bench") {
+        if (argc < 5) { std::fprintf(stderr, "usage: --bench double|float n steps\n"); return 2; }
+        plint n = std::atol(argv[3]), steps = std::atol(argv[4]);
+        return std::string(argv[2]) == "float" ? runBench<float>(n, steps) : runBench<double>(n, steps);
+    }
+    if (argc < 2) { std::fprintf(stderr, "usage: ref_driver case.txt | --bench T n steps\n"); return 2; }
+    CaseSpec c = parseCase(argv[1]);
+    if (c.forced) {
+        if (!c.bcs.empty() || c.dyn != "bgk") { std::fprintf(stderr, "force: bgk without planes only\n"); return 2; }
+        return c.T == "float" ? runCase<float, descriptors::ForcedD3Q19Descriptor>(c)
+                              : runCase<double, descriptors::ForcedD3Q19Descriptor>(c);
+    }
+    return c.T == "float" ? runCase<float, descriptors::D3Q19Descriptor>(c)
+                          : runCase<double, descriptors::D3Q19Descriptor>(c);
+}

@@ -1,0 +1,304 @@
+"""Python host mirror of Palabos's lattice API for the D3Q19 hot path.
+
+Names and argument meaning follow the reference's Python binding (pypal, as
+used by jlabos/src/testing/test_cylinder.py:78-130: Block3D, BGK,
+initializeAtEquilibrium, collideAndStream, averageEnergy, averageDensity) and
+the C++ API (BlockLattice3D::collideAndStream, defineDynamics,
+setBoundaryVelocity, setBoundaryDensity, getStoredAverageDensity/Energy).
+Everything runs through the native library (include/palabos_amd.h); there is
+no CPU path.
+
+Boxes are inclusive (x0, x1, y0, y1, z0, z1) tuples, as plb::Box3D.
+Populations are numpy arrays [x][y][z][19] of shifted f-bar in Palabos's
+D3Q19 order.
+"""
+import ctypes
+
+import numpy as np
+
+from . import native
+from .native import Box, check, lib
+
+D3Q19 = "D3Q19"
+
+
+class BGK:
+    """BGKdynamics (basicDynamics/isoThermalDynamics.hh:318-330)."""
+    code = native.BGK
+
+    def __init__(self, omega):
+        self.omega = float(omega)
+
+
+class RegularizedBGK:
+    """RegularizedBGKdynamics (basicDynamics/isoThermalDynamics.hh:1587-1600)."""
+    code = native.RBGK
+
+    def __init__(self, omega):
+        self.omega = float(omega)
+
+
+class BounceBack:
+    """BounceBack node (core/dynamics.hh:783-790)."""
+    tag = native.TAG_BOUNCE_BACK
+
+
+class NoDynamics:
+    """NoDynamics (core/dynamics.hh:968-971)."""
+    tag = native.TAG_NO_DYNAMICS
+
+
+class MomentumExchangeBounceBack:
+    """MomentumExchangeBounceBack (boundaryCondition/bounceBackModels.hh:98-135)."""
+    tag = native.TAG_MOMENTUM_EXCHANGE
+
+
+class WallLink:
+    """Solid of link-wise half-way bounce-back (same-step reflection of the
+    neighbours' outgoing populations, as at a non-periodic domain edge)."""
+    tag = native.TAG_WALL_LINK
+
+
+class Bulk:
+    tag = native.TAG_BULK
+
+
+def _box(b, shape):
+    if b is None:
+        return Box(0, shape[0] - 1, 0, shape[1] - 1, 0, shape[2] - 1)
+    return Box.of(b)
+
+
+class Block3D:
+    """One lattice block (a Palabos BlockLattice3D / single-block MultiBlockLattice3D)."""
+
+    def __init__(self, nx, ny, nz, descriptor=D3Q19, dynamics=None, storage="f64",
+                 periodic=False, force=None, device=0):
+        if descriptor != D3Q19:
+            raise ValueError("only the D3Q19 descriptor is supported")
+        if dynamics is None:
+            raise ValueError("a bulk dynamics (BGK or RegularizedBGK) is required")
+        native.init(device)
+        if isinstance(periodic, bool):
+            periodic = (periodic,) * 3
+        d = native.LatticeDesc()
+        d.nx, d.ny, d.nz = int(nx), int(ny), int(nz)
+        d.storage = native.STORAGE_CODES[storage]
+        d.dynamics = dynamics.code
+        for a in range(3):
+            d.periodic[a] = int(bool(periodic[a]))
+        d.omega = dynamics.omega
+        if force is not None:
+            for a in range(3):
+                d.force[a] = float(force[a])
+        h = ctypes.c_int(0)
+        check(lib().plbamd_lattice_create(ctypes.byref(d), ctypes.byref(h)))
+        self.handle = h.value
+        self.shape = (int(nx), int(ny), int(nz))
+        self.storage = storage
+        self.dynamics = dynamics
+        self.periodic = tuple(bool(p) for p in periodic)
+        self._tags = None
+        self._planes = []
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            check(lib().plbamd_lattice_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nx(self):
+        return self.shape[0]
+
+    @property
+    def ny(self):
+        return self.shape[1]
+
+    @property
+    def nz(self):
+        return self.shape[2]
+
+    def getBoundingBox(self):
+        return (0, self.nx - 1, 0, self.ny - 1, 0, self.nz - 1)
+
+    # -- geometry / dynamics ------------------------------------------------
+    def setTags(self, tags):
+        tags = np.ascontiguousarray(tags, dtype=np.uint8)
+        if tags.shape != self.shape:
+            raise ValueError("tags must have the lattice shape")
+        self._tags = tags.copy()
+        check(lib().plbamd_set_tags(self.handle, tags.ctypes.data))
+
+    def defineDynamics(self, where, dynamics):
+        """defineDynamics(lattice, box|mask, dynamics) (dataInitializerWrapper3D.hh:232-240)."""
+        if self._tags is None:
+            self._tags = np.zeros(self.shape, dtype=np.uint8)
+        if isinstance(where, np.ndarray):
+            self._tags[where.astype(bool)] = dynamics.tag
+        else:
+            x0, x1, y0, y1, z0, z1 = where
+            self._tags[x0:x1 + 1, y0:y1 + 1, z0:z1 + 1] = dynamics.tag
+        check(lib().plbamd_set_tags(self.handle, self._tags.ctypes.data))
+
+    def _add_plane(self, family, kind, direction, orientation, box):
+        fam = {"zouhe": native.ZOUHE, "regularized": native.REGULARIZED}[family]
+        pid = ctypes.c_int(-1)
+        b = Box.of(box)
+        check(lib().plbamd_add_plane(self.handle, fam, kind, int(direction), int(orientation),
+                                     ctypes.byref(b), ctypes.byref(pid)))
+        self._planes.append((kind, b))
+        return pid.value
+
+    def addVelocityBoundary(self, direction, orientation, box, family="zouhe"):
+        """add{Velocity}Boundary{dir}{N|P} of the dynamics-based Zou-He or regularized
+        boundary condition (boundaryCondition3D.h:40-200). Returns a plane id."""
+        return self._add_plane(family, native.VELOCITY, direction, orientation, box)
+
+    def addPressureBoundary(self, direction, orientation, box, family="zouhe"):
+        return self._add_plane(family, native.PRESSURE, direction, orientation, box)
+
+    def setBoundaryVelocity(self, plane, u):
+        """setBoundaryVelocity: a 3-vector or an array [box cells, 3] in box C order."""
+        kind, b = self._planes[plane]
+        if kind != native.VELOCITY:
+            raise ValueError("not a velocity plane")
+        n = b.cells()
+        u = np.asarray(u, dtype=np.float64)
+        vals = np.broadcast_to(u, (n, 3)) if u.shape == (3,) else u.reshape(n, 3)
+        vals = np.ascontiguousarray(vals)
+        check(lib().plbamd_set_plane_values(self.handle, plane, vals.ctypes.data, vals.size))
+
+    def setBoundaryDensity(self, plane, rho):
+        kind, b = self._planes[plane]
+        if kind != native.PRESSURE:
+            raise ValueError("not a pressure plane")
+        n = b.cells()
+        vals = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, dtype=np.float64), (n,)).reshape(n))
+        check(lib().plbamd_set_plane_values(self.handle, plane, vals.ctypes.data, vals.size))
+
+    def setPorosity(self, alpha):
+        """Brinkman porosity field alpha in [0,1] (None removes it)."""
+        if alpha is None:
+            check(lib().plbamd_set_porosity(self.handle, None))
+            return
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError("alpha must have the lattice shape")
+        check(lib().plbamd_set_porosity(self.handle, a.ctypes.data))
+
+    # -- data ---------------------------------------------------------------
+    def initializeAtEquilibrium(self, rho, u, box=None):
+        b = _box(box, self.shape)
+        n = b.cells()
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, dtype=np.float64), b.shape()).reshape(n))
+        uu = np.asarray(u, dtype=np.float64)
+        uu = np.ascontiguousarray(np.broadcast_to(uu, b.shape() + (3,)).reshape(n, 3))
+        check(lib().plbamd_initialize_at_equilibrium(self.handle, ctypes.byref(b), r.ctypes.data,
+                                                      uu.ctypes.data))
+
+    def setPopulations(self, f, box=None):
+        """Uploads populations (float64/float32 values, or raw uint16 bits of the storage type)."""
+        b = _box(box, self.shape)
+        f = np.ascontiguousarray(f)
+        if f.shape != b.shape() + (19,):
+            raise ValueError(f"populations must have shape {b.shape() + (19,)}")
+        elem = {np.dtype(np.float64): native.F64, np.dtype(np.float32): native.F32}.get(f.dtype)
+        if elem is None:
+            if f.dtype != np.uint16 or self.storage not in ("f16", "bf16"):
+                raise ValueError("unsupported population dtype")
+            elem = native.STORAGE_CODES[self.storage]
+        check(lib().plbamd_upload(self.handle, ctypes.byref(b), elem, f.ctypes.data, f.nbytes))
+
+    def getPopulations(self, box=None, dtype=np.float64):
+        """Downloads populations as float64/float32 values, or as raw storage bits (dtype=np.uint16)."""
+        b = _box(box, self.shape)
+        dtype = np.dtype(dtype)
+        if dtype == np.float64:
+            elem = native.F64
+        elif dtype == np.float32:
+            elem = native.F32
+        elif dtype == np.uint16 and self.storage in ("f16", "bf16"):
+            elem = native.STORAGE_CODES[self.storage]
+        else:
+            raise ValueError("unsupported population dtype")
+        out = np.empty(b.shape() + (19,), dtype=dtype)
+        check(lib().plbamd_download(self.handle, ctypes.byref(b), elem, out.ctypes.data, out.nbytes))
+        return out
+
+    # -- stepping -----------------------------------------------------------
+    def collideAndStream(self, nsteps=1):
+        check(lib().plbamd_collide_and_stream(self.handle, int(nsteps)))
+
+    def collideAndStreamTimed(self, nsteps):
+        """Runs nsteps steps and returns the device time in ms (HIP events on the lattice stream)."""
+        ms = ctypes.c_double(0.0)
+        check(lib().plbamd_collide_and_stream_timed(self.handle, int(nsteps), ctypes.byref(ms)))
+        return ms.value
+
+    def toggleInternalStatistics(self, on):
+        check(lib().plbamd_toggle_statistics(self.handle, int(bool(on))))
+
+    @property
+    def time(self):
+        t = ctypes.c_int64(0)
+        check(lib().plbamd_get_time(self.handle, ctypes.byref(t)))
+        return t.value
+
+    # -- analysis -----------------------------------------------------------
+    def getStatistics(self):
+        s = np.zeros(native.NUM_STATS, dtype=np.float64)
+        check(lib().plbamd_get_statistics(self.handle, s.ctypes.data))
+        return {"avg_rhobar": s[0], "avg_usqr": s[1], "max_usqr": s[2], "force": s[3:6].copy(),
+                "count": int(s[6]), "porous": s[7:10].copy()}
+
+    def getStoredAverageDensity(self):
+        """multiBlockLattice3D.hh:531-535."""
+        return 1.0 + self.getStatistics()["avg_rhobar"]
+
+    def getStoredAverageEnergy(self):
+        """multiBlockLattice3D.hh:538-541."""
+        return 0.5 * self.getStatistics()["avg_usqr"]
+
+    def macroscopic(self, box=None):
+        b = _box(box, self.shape)
+        n = b.cells()
+        rho = np.empty(n, dtype=np.float64)
+        u = np.empty((n, 3), dtype=np.float64)
+        check(lib().plbamd_compute_macroscopic(self.handle, ctypes.byref(b), rho.ctypes.data,
+                                                u.ctypes.data))
+        return rho.reshape(b.shape()), u.reshape(b.shape() + (3,))
+
+    def density(self, box=None):
+        return self.macroscopic(box)[0]
+
+    def velocity(self, box=None):
+        return self.macroscopic(box)[1]
+
+    def reduceMoments(self, box=None):
+        b = _box(box, self.shape)
+        out = np.zeros(5, dtype=np.float64)
+        check(lib().plbamd_reduce_moments(self.handle, ctypes.byref(b), out.ctypes.data))
+        return {"sum_rho": out[0], "sum_energy": out[1], "sum_j": out[2:5].copy(), "cells": b.cells()}
+
+    def averageDensity(self, box=None):
+        """computeAverageDensity (dataAnalysisWrapper3D.h:628-631)."""
+        r = self.reduceMoments(box)
+        return r["sum_rho"] / r["cells"]
+
+    def averageEnergy(self, box=None):
+        """computeAverageEnergy (dataAnalysisWrapper3D.hh:1252-1257): mean of |u|^2 / 2."""
+        r = self.reduceMoments(box)
+        return r["sum_energy"] / r["cells"]

@@ -1,0 +1,129 @@
+"""ctypes binding of the C ABI declared in include/palabos_amd.h.
+
+The library is loaded from the package directory (palabos_amd/libpalabos_amd.so,
+built by palabos_amd/_build.py). There is no fallback: if the library is
+missing or fails to load, NativeLibraryError is raised.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpalabos_amd.so")
+
+OK, EINVAL, ENOMEM, EHIP, ESTATE = 0, -1, -2, -3, -4
+F64, F32, F16, BF16 = 0, 1, 2, 3
+BGK, RBGK = 0, 1
+TAG_BULK, TAG_BOUNCE_BACK, TAG_NO_DYNAMICS, TAG_MOMENTUM_EXCHANGE, TAG_WALL_LINK = 0, 1, 2, 3, 4
+ZOUHE, REGULARIZED = 0, 1
+VELOCITY, PRESSURE = 0, 1
+NUM_STATS = 10
+
+STORAGE_CODES = {"f64": F64, "f32": F32, "f16": F16, "bf16": BF16}
+STORAGE_NUMPY = {"f64": np.float64, "f32": np.float32, "f16": np.uint16, "bf16": np.uint16}
+
+# Every symbol include/palabos_amd.h declares.
+EXPORTS = (
+    "plbamd_init", "plbamd_last_error", "plbamd_lattice_create", "plbamd_lattice_destroy",
+    "plbamd_set_tags", "plbamd_add_plane", "plbamd_set_plane_values", "plbamd_set_porosity",
+    "plbamd_upload", "plbamd_download", "plbamd_initialize_at_equilibrium",
+    "plbamd_collide_and_stream", "plbamd_collide_and_stream_timed", "plbamd_toggle_statistics",
+    "plbamd_get_statistics", "plbamd_compute_macroscopic", "plbamd_reduce_moments",
+    "plbamd_get_time",
+)
+
+
+class NativeLibraryError(RuntimeError):
+    pass
+
+
+class PlbError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"palabos_amd error {code}: {message}")
+        self.code = code
+
+
+class Box(ctypes.Structure):
+    _fields_ = [("x0", ctypes.c_int32), ("x1", ctypes.c_int32), ("y0", ctypes.c_int32),
+                ("y1", ctypes.c_int32), ("z0", ctypes.c_int32), ("z1", ctypes.c_int32)]
+
+    @classmethod
+    def of(cls, b):
+        return cls(*[int(v) for v in b])
+
+    def cells(self):
+        return (self.x1 - self.x0 + 1) * (self.y1 - self.y0 + 1) * (self.z1 - self.z0 + 1)
+
+    def shape(self):
+        return (self.x1 - self.x0 + 1, self.y1 - self.y0 + 1, self.z1 - self.z0 + 1)
+
+
+class LatticeDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("storage", ctypes.c_int32), ("dynamics", ctypes.c_int32),
+                ("periodic", ctypes.c_int32 * 3), ("omega", ctypes.c_double),
+                ("force", ctypes.c_double * 3)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads the native library once; raises NativeLibraryError if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeLibraryError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    try:
+        L = ctypes.CDLL(LIB_PATH)
+    except OSError as e:
+        raise NativeLibraryError(f"cannot load {LIB_PATH}: {e}") from e
+    vp, i32, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
+    pbox = ctypes.POINTER(Box)
+    sig = {
+        "plbamd_init": [i32],
+        "plbamd_lattice_create": [ctypes.POINTER(LatticeDesc), ctypes.POINTER(ctypes.c_int)],
+        "plbamd_lattice_destroy": [i32],
+        "plbamd_set_tags": [i32, vp],
+        "plbamd_add_plane": [i32, i32, i32, i32, i32, pbox, ctypes.POINTER(ctypes.c_int)],
+        "plbamd_set_plane_values": [i32, i32, vp, sz],
+        "plbamd_set_porosity": [i32, vp],
+        "plbamd_upload": [i32, pbox, i32, vp, sz],
+        "plbamd_download": [i32, pbox, i32, vp, sz],
+        "plbamd_initialize_at_equilibrium": [i32, pbox, vp, vp],
+        "plbamd_collide_and_stream": [i32, i32],
+        "plbamd_collide_and_stream_timed": [i32, i32, ctypes.POINTER(ctypes.c_double)],
+        "plbamd_toggle_statistics": [i32, i32],
+        "plbamd_get_statistics": [i32, vp],
+        "plbamd_compute_macroscopic": [i32, pbox, vp, vp],
+        "plbamd_reduce_moments": [i32, pbox, vp],
+        "plbamd_get_time": [i32, ctypes.POINTER(ctypes.c_int64)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    L.plbamd_last_error.argtypes = []
+    L.plbamd_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != OK:
+        msg = lib().plbamd_last_error().decode(errors="replace")
+        raise PlbError(rc, msg)
+    return rc
+
+
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    if not _initialised:
+        check(lib().plbamd_init(int(device)))
+        _initialised = True

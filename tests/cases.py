@@ -1,0 +1,184 @@
+"""Lattice cases shared by the parity tests and the golden-vector generator.
+
+A Case describes one lattice (size, precision, dynamics, periodicity, per-cell
+tags, boundary planes, body force, porosity). The same Case drives the oracle
+(oracle/oracle.py), the HIP engine (palabos_amd.Block3D) and, for golden
+vectors, the compiled Palabos reference (oracle/ref_driver.cpp).
+"""
+import dataclasses
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+TAG_BULK, TAG_BB, TAG_NODYN, TAG_MEBB, TAG_WALL = 0, 1, 2, 3, 4
+
+
+@dataclasses.dataclass
+class Plane:
+    family: str            # 'zouhe' | 'regularized'
+    kind: str              # 'velocity' | 'pressure'
+    dir: int
+    ori: int
+    box: Tuple[int, int, int, int, int, int]
+    values: np.ndarray     # velocity: [cells, 3]; pressure: [cells] (rho)
+
+    def cells(self):
+        x0, x1, y0, y1, z0, z1 = self.box
+        return (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1)
+
+
+@dataclasses.dataclass
+class Case:
+    nx: int
+    ny: int
+    nz: int
+    storage: str = "f64"
+    dynamics: str = "bgk"
+    omega: float = 1.0
+    periodic: Tuple[bool, bool, bool] = (False, False, False)
+    tags: Optional[np.ndarray] = None
+    planes: List[Plane] = dataclasses.field(default_factory=list)
+    force: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    alpha: Optional[np.ndarray] = None
+
+    def with_storage(self, storage):
+        return dataclasses.replace(self, storage=storage)
+
+    @property
+    def shape(self):
+        return (self.nx, self.ny, self.nz)
+
+
+# ---------------------------------------------------------------------------
+# Deterministic pseudo-random fields (splitmix64 of index and seed), identical
+# on every platform and numpy version.
+
+def _splitmix(idx, seed):
+    with np.errstate(over="ignore"):
+        z = (idx.astype(np.uint64) + np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15)
+             + np.uint64(0x632BE59BD9B4E019))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(shape, seed):
+    """Uniform values in [-1, 1), from 53-bit integers."""
+    n = int(np.prod(shape))
+    z = _splitmix(np.arange(n, dtype=np.uint64), seed)
+    return ((z >> np.uint64(11)).astype(np.float64) * (2.0 ** -53) * 2.0 - 1.0).reshape(shape)
+
+
+def equilibrium(rho, u):
+    """FP64 shifted equilibrium f-bar (same formula as the oracle and Palabos)."""
+    from oracle import oracle
+    return oracle.equilibrium(rho, u)
+
+
+def random_state(case, seed, u_amp=0.05, rho_amp=0.02, noise=5e-4):
+    """A non-equilibrium FP64 state: equilibrium of random rho/u plus noise."""
+    shape = case.shape
+    rho = 1.0 + rho_amp * uniform(shape, seed)
+    u = u_amp * uniform(shape + (3,), seed + 1)
+    f = equilibrium(rho, u)
+    f = f + noise * uniform(shape + (19,), seed + 2)
+    return np.ascontiguousarray(f)
+
+
+def rest_state(case):
+    return np.zeros(case.shape + (19,), dtype=np.float64)
+
+
+# ---------------------------------------------------------------------------
+# Geometry helpers
+
+def sphere_tags(shape, center, radius, tag):
+    x, y, z = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    r2 = (x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2
+    tags = np.zeros(shape, dtype=np.uint8)
+    tags[r2 <= radius * radius] = tag
+    return tags
+
+
+def poiseuille_profile(ny, nz, u_max):
+    """u_x on the inlet interior (1..ny-2, 1..nz-2): 16 u_max y'(H-y') z'(H-z') / (Hy^2 Hz^2),
+    y', z' distances to the half-way walls at -0.5 and n-0.5."""
+    y = np.arange(1, ny - 1, dtype=np.float64) + 0.5
+    z = np.arange(1, nz - 1, dtype=np.float64) + 0.5
+    hy, hz = float(ny), float(nz)
+    py = y * (hy - y) / (hy * hy)
+    pz = z * (hz - z) / (hz * hz)
+    return 16.0 * u_max * py[:, None] * pz[None, :]
+
+
+def channel_case(nx, ny, nz, omega, u_max, storage="f64", family="zouhe", sphere=None,
+                 sphere_tag=TAG_MEBB, dynamics="bgk"):
+    """Velocity inlet (x=0) and pressure outlet (x=nx-1) on the face interiors;
+    the side walls are the domain edges (same-step reflection)."""
+    planes = []
+    prof = poiseuille_profile(ny, nz, u_max)
+    vel = np.zeros(((ny - 2) * (nz - 2), 3))
+    vel[:, 0] = prof.reshape(-1)
+    planes.append(Plane(family, "velocity", 0, -1, (0, 0, 1, ny - 2, 1, nz - 2), vel))
+    planes.append(Plane(family, "pressure", 0, 1, (nx - 1, nx - 1, 1, ny - 2, 1, nz - 2),
+                        np.ones((ny - 2) * (nz - 2))))
+    tags = None
+    if sphere is not None:
+        center, radius = sphere
+        tags = sphere_tags((nx, ny, nz), center, radius, sphere_tag)
+    return Case(nx, ny, nz, storage=storage, dynamics=dynamics, omega=omega, tags=tags,
+                planes=planes)
+
+
+def cavity_case(n, omega, u_lid, storage="f64", family="zouhe", ny=None, nz=None, dynamics="bgk"):
+    """Lid-driven cavity: Zou-He (or regularized) velocity lid on the top z face
+    interior Box3D(1, nx-2, 1, ny-2, nz-1, nz-1); every other wall is the domain edge."""
+    nx = n
+    ny = ny or n
+    nz = nz or n
+    box = (1, nx - 2, 1, ny - 2, nz - 1, nz - 1)
+    cells = (nx - 2) * (ny - 2)
+    vel = np.zeros((cells, 3))
+    vel[:, 0] = u_lid
+    return Case(nx, ny, nz, storage=storage, dynamics=dynamics, omega=omega,
+                planes=[Plane(family, "velocity", 2, 1, box, vel)])
+
+
+# ---------------------------------------------------------------------------
+# Running a case on the HIP engine
+
+def gpu_lattice(case):
+    import palabos_amd as plb
+    dyn = plb.BGK(case.omega) if case.dynamics == "bgk" else plb.RegularizedBGK(case.omega)
+    force = case.force if any(case.force) else None
+    lat = plb.Block3D(case.nx, case.ny, case.nz, plb.D3Q19, dyn, storage=case.storage,
+                      periodic=case.periodic, force=force)
+    if case.tags is not None:
+        lat.setTags(case.tags)
+    for p in case.planes:
+        if p.kind == "velocity":
+            pid = lat.addVelocityBoundary(p.dir, p.ori, p.box, family=p.family)
+            lat.setBoundaryVelocity(pid, p.values)
+        else:
+            pid = lat.addPressureBoundary(p.dir, p.ori, p.box, family=p.family)
+            lat.setBoundaryDensity(pid, p.values)
+    if case.alpha is not None:
+        lat.setPorosity(case.alpha)
+    return lat
+
+
+def gpu_run(case, f_storage, nsteps, stats=True):
+    """Uploads a storage-typed state, runs nsteps on the GPU, returns (state, stats)."""
+    lat = gpu_lattice(case)
+    try:
+        lat.setPopulations(f_storage)
+        lat.toggleInternalStatistics(stats)
+        lat.collideAndStream(nsteps)
+        dtype = np.uint16 if case.storage in ("f16", "bf16") else (
+            np.float64 if case.storage == "f64" else np.float32)
+        out = lat.getPopulations(dtype=dtype)
+        st = lat.getStatistics() if (stats and nsteps > 0) else None
+        return out, st
+    finally:
+        lat.close()

@@ -1,0 +1,63 @@
+"""Runs a Case through the compiled Palabos reference (oracle/_ref/ref_driver).
+
+Test infrastructure: used by tests/golden/make_golden.py and by the CPU tests
+that compare the oracle with live Palabos when oracle/_ref is built.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+
+def available():
+    return os.path.exists(DRIVER)
+
+
+def run(case, init_f64, dumps, bcstyle="dynamics", blocks=None, timeout=1800):
+    """Returns ({step: populations in Palabos's T}, {step: stats row})."""
+    T = {"f64": "double", "f32": "float"}[case.storage]
+    with tempfile.TemporaryDirectory() as tmp:
+        lines = [f"T {T}", f"size {case.nx} {case.ny} {case.nz}", f"dyn {case.dynamics}",
+                 f"omega {case.omega!r}",
+                 "periodic " + " ".join(str(int(bool(p))) for p in case.periodic),
+                 f"bcstyle {bcstyle}"]
+        fams = {p.family for p in case.planes}
+        assert len(fams) <= 1, "one boundary family per case"
+        if fams:
+            lines.append(f"bcfamily {fams.pop()}")
+        init_path = os.path.join(tmp, "init.bin")
+        np.ascontiguousarray(init_f64, dtype=np.float64).tofile(init_path)
+        lines.append(f"init {init_path}")
+        if case.tags is not None:
+            tp = os.path.join(tmp, "tags.bin")
+            np.ascontiguousarray(case.tags, dtype=np.uint8).tofile(tp)
+            lines.append(f"tags {tp}")
+        for k, p in enumerate(case.planes):
+            vp = os.path.join(tmp, f"bc{k}.bin")
+            np.ascontiguousarray(p.values, dtype=np.float64).tofile(vp)
+            kind = "vel" if p.kind == "velocity" else "press"
+            lines.append(f"bc {kind} {p.dir} {p.ori} " + " ".join(map(str, p.box)) + f" {vp}")
+        if any(case.force):
+            lines.append("force " + " ".join(repr(float(v)) for v in case.force))
+        for b in blocks or []:
+            lines.append("block " + " ".join(map(str, b)))
+        lines.append("dump " + " ".join(map(str, dumps)))
+        out = os.path.join(tmp, "out")
+        lines.append(f"out {out}")
+        cfile = os.path.join(tmp, "case.txt")
+        with open(cfile, "w") as fh:
+            fh.write("\n".join(lines) + "\n")
+        subprocess.run([DRIVER, cfile], check=True, timeout=timeout, capture_output=True)
+        dt = np.float64 if T == "double" else np.float32
+        pops = {s: np.fromfile(f"{out}_s{s}.bin", dtype=dt).reshape(case.shape + (19,)) for s in dumps}
+        stats = {}
+        for line in open(f"{out}_stats.txt"):
+            v = line.split()
+            stats[int(v[0])] = {"stored_avg_rho": float(v[1]), "stored_avg_energy": float(v[2]),
+                                "avg_rho": float(v[3]), "avg_energy": float(v[4]),
+                                "force": np.array([float(v[5]), float(v[6]), float(v[7])])}
+        return pops, stats
