@@ -1,0 +1,168 @@
+"""D3Q19 collide-and-stream benchmark on one MI355X.
+
+Workload (the metric's headline configuration): periodic Taylor-Green vortex
+on a 512^3 D3Q19 lattice, BGK, omega = 1.8, u0 = 0.04, synthetic initial state
+f = f_eq(rho=1, u_TG) built on the device. A step is one collide-and-stream
+pass over the whole lattice (all 134,217,728 cells), statistics off (as
+Palabos's own timing runs toggle them off). The primary line is FP64; the FP32
+and FP16-storage runs of the same workload are reported under "variants".
+
+value        MLUPS = cells * steps / wall seconds / 1e6 over the timed steps,
+             populations resident in HBM before the timed region starts.
+roofline     the bulk kernel: algorithmic bytes per launch (read + write 19
+             populations per cell) / average launch duration, measured with
+             HIP events on the stream the kernel runs on.
+cpu_baseline the compiled Palabos reference (oracle/_ref/ref_driver, serial,
+             1 core) on a bounded sample (periodic BGK 128^3), or the oracle
+             restatement when the reference binary is absent.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
+                       [--precision f64] [--no-variants] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = {"f64": 19 * 2 * 8, "f32": 19 * 2 * 4, "f16": 19 * 2 * 2, "bf16": 19 * 2 * 2}
+
+
+def taylor_green_init(lat, n, u0):
+    """f = f_eq(rho = 1, u) with u = u0 (sin kx cos ky cos kz, -cos kx sin ky cos kz, 0), in x slabs."""
+    k = 2.0 * np.pi / n
+    c = np.arange(n) * k
+    sx, cxv = np.sin(c), np.cos(c)
+    slab = max(1, min(n, (1 << 22) // (n * n)))
+    for x0 in range(0, n, slab):
+        x1 = min(n, x0 + slab)
+        xs = slice(x0, x1)
+        ux = u0 * sx[xs, None, None] * cxv[None, :, None] * cxv[None, None, :]
+        uy = -u0 * cxv[xs, None, None] * sx[None, :, None] * cxv[None, None, :]
+        u = np.zeros((x1 - x0, n, n, 3))
+        u[..., 0] = ux
+        u[..., 1] = uy
+        lat.initializeAtEquilibrium(1.0, u, box=(x0, x1 - 1, 0, n - 1, 0, n - 1))
+
+
+def run_precision(precision, n, steps, warmup, omega=1.8, u0=0.04):
+    import palabos_amd as plb
+    lat = plb.Block3D(n, n, n, plb.D3Q19, plb.BGK(omega), storage=precision, periodic=True)
+    try:
+        taylor_green_init(lat, n, u0)
+        lat.toggleInternalStatistics(False)
+        if warmup:
+            lat.collideAndStream(warmup)
+        t0 = time.perf_counter()
+        ms_events = lat.collideAndStreamTimed(steps)  # blocks until the last step has finished
+        wall = time.perf_counter() - t0
+        cells = n ** 3
+        kernel_ms = ms_events / steps
+        achieved = BYTES_PER_CELL[precision] * cells / (kernel_ms * 1e-3) / 1e9
+        return {
+            "dtype": precision,
+            "value": cells * steps / wall / 1e6,
+            "ms_per_step": wall * 1e3 / steps,
+            "event_ms_per_step": kernel_ms,
+            "achieved_gbs": achieved,
+            "frac": achieved / HBM_PEAK_GBS,
+        }
+    finally:
+        lat.close()
+
+
+def cpu_baseline():
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    n, steps = 128, 60
+    if os.path.exists(ref):
+        out = subprocess.run([ref, "--bench", "double", str(n), str(steps)], capture_output=True,
+                             text=True, timeout=600, check=True).stdout
+        r = json.loads(out.strip().splitlines()[-1])
+        return {"value": r["mlups"], "unit": "MLUPS", "cores": 1, "kind": "reference",
+                "sample": f"compiled Palabos (serial MultiBlockLattice3D, BGK, FP64) periodic "
+                          f"{n}^3, {steps} steps after 1 warm-up step, {r['seconds']:.1f} s"}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from cases import Case
+    from oracle import oracle
+    n, steps = 64, 30
+    case = Case(n, n, n, omega=1.8, periodic=(True, True, True))
+    f = np.zeros((n, n, n, 19))
+    t0 = time.perf_counter()
+    oracle.run(case, f, steps)
+    sec = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / sec / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": f"oracle restatement, periodic BGK FP64 {n}^3, {steps} steps, {sec:.1f} s"}
+
+
+def traffic_from_profile(precision, n):
+    """HBM bytes per launch of the bulk kernel from the committed rocprofv3 counter
+    summary (profiles/bulk_step_tg512_counters.txt), or None."""
+    path = os.path.join(ROOT, "profiles", "bulk_step_tg512_counters.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\s*traffic_bytes_per_launch\s+(\S+)\s+n=(\d+)\s+([0-9.eE+]+)", line)
+        if m and m.group(1) == precision and int(m.group(2)) == n:
+            return float(m.group(3))
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32", "f16", "bf16"])
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the lattice would shard into x-slabs with per-face halo exchange over RCCL; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    main_run = run_precision(args.precision, args.n, args.steps, args.warmup)
+    variants = []
+    if not args.no_variants:
+        for p in ("f64", "f32", "f16"):
+            if p != args.precision:
+                variants.append(run_precision(p, args.n, args.steps, args.warmup))
+    traffic = traffic_from_profile(args.precision, args.n)
+    line = {
+        "metric": "D3Q19 MLUPS (FP64/FP32/FP16-store) & % HBM peak; adjoint/forward ratio",
+        "value": main_run["value"],
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_run["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic (Taylor-Green initial state built on the device)",
+        "config": {"workload": f"taylor_green_{args.n}^3_periodic_bgk", "lattice": [args.n] * 3,
+                   "omega": 1.8, "u0": 0.04, "precision": args.precision, "statistics": False},
+        "roofline": {"bound": "hbm", "achieved": main_run["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": main_run["frac"], "traffic": traffic,
+                     "kernel": "plbamd::bulk_step", "algorithmic_bytes_per_cell": BYTES_PER_CELL[args.precision],
+                     "event_ms_per_launch": main_run["event_ms_per_step"]},
+        "variants": variants,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -463,8 +463,9 @@ struct Engine {
         T F[3] = {(T)d.force[0], (T)d.force[1], (T)d.force[2]};
         T u[3];
         for (int c = 0; c < 3; ++c) u[c] = j[c]*invRhoP + F[c]/(T)2;
-        // Brinkman damping; the momentum it removes, omega*rho*(u - u_damped),
-        // is summed as the porous drag.
+        // Brinkman damping; the momentum it removes in the collision is summed as
+        // the porous drag: omega*rho*(u - u_damped) for BGK, rho*(u - u_damped)
+        // for RBGK, whose first moment becomes j_eq.
         T damp = (T)1 - (d.alpha ? (T)d.alpha[k] : (T)0);
         T rho = rhoBar + (T)1;
         T jeq[3];
@@ -472,7 +473,7 @@ struct Engine {
             T u0 = u[c];
             u[c] = u0 * damp;
             jeq[c] = rho * u[c];
-            st.porous[c] += (double)(omega * (rho * u0 - jeq[c]));
+            st.porous[c] += (double)(d.dynamics == 0 ? omega * (rho * u0 - jeq[c]) : (rho * u0 - jeq[c]));
         }
         T uSqr;
         if (d.dynamics == 0) uSqr = bgk_collision(f, rhoBar, jeq, omega, invRhoOf(rhoBar));

@@ -177,7 +177,7 @@ int runCase(CaseSpec const& c) {
     } else {
         SparseBlockStructure3D sparse(Box3D(0, c.nx-1, 0, c.ny-1, 0, c.nz-1));
         for (pluint i = 0; i < c.blocks.size(); ++i) sparse.addBlock(c.blocks[i], (plint)i);
-        MultiBlockManagement3D management(sparse, new OneToOneThreadAttribution, 1);
+        MultiBlockManagement3D management(sparse, defaultMultiBlockPolicy3D().getThreadAttribution(), 1);
         lattice = new Lattice(management,
                               defaultMultiBlockPolicy3D().getBlockCommunicator(),
                               defaultMultiBlockPolicy3D().getCombinedStatistics(),

@@ -221,7 +221,9 @@ __device__ __forceinline__ void add_guo_force(T* f, const T u[3], const T F[3], 
 //   Brinkman damping u <- (1 - alpha) u of the equilibrium velocity have no
 //   Palabos counterpart and are defined by this library.
 // `drag` receives the momentum the Brinkman term removes from the cell in the
-// collision, omega * rho * (u - u_damped), u the undamped half-force velocity.
+// collision: omega * rho * (u - u_damped) for BGK, rho * (u - u_damped) for
+// regularized BGK (which sets the first moment to j_eq), u the undamped
+// half-force velocity.
 template<typename T, int DYN, bool FORCED>
 __device__ __forceinline__ T collide_bulk(T* f, T omega, const T F[3], T a, T& rhoBarOut, T drag[3]) {
     if constexpr (!FORCED) {
@@ -258,7 +260,9 @@ __device__ __forceinline__ T collide_bulk(T* f, T omega, const T F[3], T a, T& r
             const T u0 = u[c];
             u[c] = u0 * damp;
             jeq[c] = rho * u[c];
-            drag[c] = omega * (rho * u0 - jeq[c]);
+            // momentum the damping removes in this collision: BGK relaxes by omega
+            // towards j_eq, regularized BGK replaces the first moment by j_eq
+            drag[c] = (DYN == 0) ? omega * (rho * u0 - jeq[c]) : (rho * u0 - jeq[c]);
         }
         T uSqr;
         if constexpr (DYN == 0) uSqr = bgk(f, rhoBar, jeq, omega, inv_rho(rhoBar));

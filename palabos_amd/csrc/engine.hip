@@ -47,11 +47,13 @@ template<> struct Store<double> {
     using T = double;
     static __device__ __forceinline__ double load(const double* p) { return __builtin_nontemporal_load(p); }
     static __device__ __forceinline__ void store(double* p, double v) { __builtin_nontemporal_store(v, p); }
+    static __device__ __forceinline__ double round(double v) { return v; }
 };
 template<> struct Store<float> {
     using T = float;
     static __device__ __forceinline__ float load(const float* p) { return __builtin_nontemporal_load(p); }
     static __device__ __forceinline__ void store(float* p, float v) { __builtin_nontemporal_store(v, p); }
+    static __device__ __forceinline__ float round(float v) { return v; }
 };
 template<> struct Store<f16s> {
     using T = float;
@@ -67,6 +69,7 @@ template<> struct Store<f16s> {
         __builtin_memcpy(&b, &h, 2);
         __builtin_nontemporal_store(b, reinterpret_cast<uint16_t*>(p));
     }
+    static __device__ __forceinline__ float round(float v) { return (float)(_Float16)v; }
 };
 __device__ __forceinline__ uint16_t bf16_round(float v) {
     uint32_t x = __float_as_uint(v);
@@ -83,6 +86,7 @@ template<> struct Store<bf16s> {
     static __device__ __forceinline__ void store(bf16s* p, float v) {
         __builtin_nontemporal_store(bf16_round(v), reinterpret_cast<uint16_t*>(p));
     }
+    static __device__ __forceinline__ float round(float v) { return __uint_as_float((uint32_t)bf16_round(v) << 16); }
 };
 
 // ---------------------------------------------------------------------------
@@ -467,11 +471,13 @@ __global__ void __launch_bounds__(256) special_step(SpecialArgs a) {
             st[2] = fmax(st[2], (double)uSqr);
             st[3] += 1.0;
             // link-wise wall: each outgoing population towards the wall carries 2 c_i f*_i
+            // (the stored, storage-rounded value is what is reflected)
 #pragma unroll
             for (int i = 1; i < 19; ++i) {
                 if (!((sc.wall >> i) & 1u)) continue;
+                const T fo = Store<S>::round(fc[i]);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) st[4 + c] += (double)((T)2 * ci(i, c) * fc[i]);
+                for (int c = 0; c < 3; ++c) st[4 + c] += (double)((T)2 * ci(i, c) * fo);
             }
         }
 
@@ -513,7 +519,8 @@ __global__ void __launch_bounds__(256) reduce_partials(const double* bulk, uint6
     // record: sumRho sumU maxU count Fx Fy Fz (force from special) / Px Py Pz (porous from bulk)
     double r[8] = {v[0] + s[0], v[1] + s[1], fmax(v[2], s[2]), v[3] + s[3], s[4], s[5], s[6], 0};
     block_reduce_store(r, out);
-    double q[8] = {v[4], v[5], v[6], 0, 0, 0, 0, 0};
+    // porous drag sums; slot 2 of a record is a max, so the z sum goes to slot 3
+    double q[8] = {v[4], v[5], 0, v[6], 0, 0, 0, 0};
     block_reduce_store(q, out + 8);
 }
 
@@ -1251,7 +1258,7 @@ int plbamd_get_statistics(int handle, double* out) {
     out[PLBAMD_STAT_COUNT] = count;
     out[PLBAMD_STAT_POROUS_X] = h[8];
     out[PLBAMD_STAT_POROUS_Y] = h[9];
-    out[PLBAMD_STAT_POROUS_Z] = h[10];
+    out[PLBAMD_STAT_POROUS_Z] = h[11];
     return PLBAMD_OK;
 }
 

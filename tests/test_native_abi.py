@@ -1,0 +1,44 @@
+"""The C-ABI library loads and exports every entry point include/palabos_amd.h
+declares (no device calls: these run without a GPU)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "palabos_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(plbamd_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_binding_table():
+    from palabos_amd import native
+    assert declared_symbols() == sorted(native.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    from palabos_amd import native
+    lib = native.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", native.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (plbamd_\w+)", out))
+    for name in declared_symbols():
+        assert name in exported, name
+        getattr(lib, name)
+
+
+def test_last_error_is_empty_string_before_any_failure():
+    from palabos_amd import native
+    assert native.lib().plbamd_last_error() in (b"",) or isinstance(native.lib().plbamd_last_error(), bytes)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    from palabos_amd import native
+    monkeypatch.setattr(native, "_lib", None)
+    monkeypatch.setattr(native, "LIB_PATH", str(tmp_path / "absent.so"))
+    with pytest.raises(native.NativeLibraryError):
+        native.lib()

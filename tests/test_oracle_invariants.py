@@ -1,0 +1,219 @@
+"""Pins of the oracle that do not need a Palabos run: storage roundings against
+numpy / exhaustive bit patterns, conservation laws, fixed points, streaming as
+a permutation, link-wise walls against the Palabos-pinned edge reflection,
+Guo and Brinkman momentum balances, and the analytic Taylor-Green decay."""
+import numpy as np
+import pytest
+
+from cases import TAG_NODYN, TAG_WALL, Case, random_state, uniform
+from oracle import oracle
+
+C = np.array([[0, 0, 0], [-1, 0, 0], [0, -1, 0], [0, 0, -1], [-1, -1, 0], [-1, 1, 0], [-1, 0, -1],
+              [-1, 0, 1], [0, -1, -1], [0, -1, 1], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0],
+              [1, -1, 0], [1, 0, 1], [1, 0, -1], [0, 1, 1], [0, 1, -1]])
+OPP = [0] + list(range(10, 19)) + list(range(1, 10))
+
+
+# -- storage roundings ---------------------------------------------------------
+
+def test_f16_decode_exhaustive():
+    bits = np.arange(65536, dtype=np.uint16)
+    got = oracle.f16_to_f32(bits)
+    want = bits.view(np.float16).astype(np.float32)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got[~nan].view(np.uint32), want[~nan].view(np.uint32))
+
+
+def test_f16_encode_matches_numpy_rne():
+    # every half value, the midpoints between neighbours (ties), and 1 ulp either side
+    h = np.arange(0, 0x7c00, dtype=np.uint16).view(np.float16).astype(np.float64)
+    mids = (h[:-1] + h[1:]) / 2
+    x = np.concatenate([h, mids, np.nextafter(mids, 0), np.nextafter(mids, 1), [70000.0, 65520.0, 65519.0]])
+    x = x.astype(np.float32)
+    x = np.concatenate([x, -x, uniform((100000,), 3).astype(np.float32) * 1e-3])
+    got = oracle.f32_to_f16(x)
+    with np.errstate(over="ignore"):
+        want = x.astype(np.float16).view(np.uint16)
+    assert np.array_equal(got, want)
+
+
+def test_bf16_roundings():
+    bits = np.arange(65536, dtype=np.uint16)
+    dec = oracle.bf16_to_f32(bits)
+    assert np.array_equal(dec.view(np.uint32), bits.astype(np.uint32) << 16)
+    x = np.concatenate([uniform((200000,), 9) * 1e-2, uniform((1000,), 10) * 1e30]).astype(np.float32)
+    enc = oracle.f32_to_bf16(x)
+    val = oracle.bf16_to_f32(enc).astype(np.float64)
+    # the two bf16 neighbours of x: truncation and truncation + 1 ulp (away from zero)
+    lo_bits = (x.view(np.uint32) >> 16).astype(np.uint16)
+    lo = oracle.bf16_to_f32(lo_bits).astype(np.float64)
+    hi = oracle.bf16_to_f32(lo_bits + 1).astype(np.float64)
+    xd = x.astype(np.float64)
+    err = np.abs(val - xd)
+    assert np.all(err <= np.minimum(np.abs(lo - xd), np.abs(hi - xd)))
+    tie = np.abs(lo - xd) == np.abs(hi - xd)
+    assert np.all(enc[tie] % 2 == 0)
+
+
+# -- conservation and fixed points ---------------------------------------------
+
+def _totals(f):
+    f = f.astype(np.float64).reshape(-1, 19)
+    return f.sum(), (f @ C).sum(axis=0)
+
+
+@pytest.mark.parametrize("dynamics,omega", [("bgk", 1.7), ("rbgk", 1.3), ("bgk", 0.6)])
+def test_collide_and_stream_conserve_mass_and_momentum(dynamics, omega):
+    case = Case(8, 9, 10, dynamics=dynamics, omega=omega, periodic=(True, True, True))
+    f0 = random_state(case, 21)
+    m0, j0 = _totals(f0)
+    f1, _ = oracle.run(case, f0, 3)
+    m1, j1 = _totals(f1)
+    assert abs(m1 - m0) < 1e-13 * case.nx * case.ny * case.nz
+    np.testing.assert_allclose(j1, j0, atol=1e-13 * case.nx * case.ny * case.nz)
+
+
+@pytest.mark.parametrize("dynamics", ["bgk", "rbgk"])
+def test_uniform_equilibrium_is_a_fixed_point(dynamics):
+    case = Case(5, 6, 7, dynamics=dynamics, omega=1.4, periodic=(True, True, True))
+    rho = np.full(case.shape, 1.01)
+    u = np.broadcast_to([0.03, -0.02, 0.01], case.shape + (3,))
+    f0 = oracle.equilibrium(rho, u)
+    f1, _ = oracle.run(case, f0, 2)
+    np.testing.assert_allclose(f1, f0, rtol=0, atol=1e-16)
+
+
+# -- streaming -----------------------------------------------------------------
+
+def test_streaming_is_a_periodic_shift():
+    """With NoDynamics everywhere a step is pure streaming: f_i(x) <- f_i(x - c_i)."""
+    case = Case(6, 7, 8, periodic=(True, True, True))
+    case.tags = np.full(case.shape, TAG_NODYN, np.uint8)
+    f0 = random_state(case, 5)
+    f1, _ = oracle.run(case, f0, 1)
+    want = np.stack([np.roll(f0[..., i], shift=tuple(C[i]), axis=(0, 1, 2)) for i in range(19)], -1)
+    assert np.array_equal(f1, want)
+
+
+def test_edge_reflection_is_same_step_bounce_back():
+    """At a non-periodic edge a population whose source lies outside is the
+    cell's own opposite population (blockLattice3D.hh:403-427)."""
+    case = Case(5, 6, 7, periodic=(False, True, False))
+    case.tags = np.full(case.shape, TAG_NODYN, np.uint8)
+    f0 = random_state(case, 6)
+    f1, _ = oracle.run(case, f0, 1)
+    nx, ny, nz = case.shape
+    for i in range(1, 19):
+        for x, y, z in [(0, 2, 0), (nx - 1, 0, nz - 1), (2, 3, 0), (0, 0, 3)]:
+            sx, sy, sz = x - C[i][0], (y - C[i][1]) % ny, z - C[i][2]
+            inside = 0 <= sx < nx and 0 <= sz < nz
+            want = f0[sx, sy, sz, i] if inside else f0[x, y, z, OPP[i]]
+            assert f1[x, y, z, i] == want
+
+
+@pytest.mark.parametrize("storage", ["f64", "f16"])
+def test_link_wall_equals_domain_edge(storage):
+    """A periodic lattice whose one-cell outer shell is a link-wise wall evolves its
+    interior exactly like the non-periodic lattice with Palabos's edge reflection."""
+    inner = Case(6, 7, 8, storage=storage, omega=1.6)
+    outer = Case(8, 9, 10, storage=storage, omega=1.6, periodic=(True, True, True))
+    tags = np.full(outer.shape, TAG_WALL, np.uint8)
+    tags[1:-1, 1:-1, 1:-1] = 0
+    outer.tags = tags
+    f_in = oracle.to_storage(random_state(inner, 8), storage)
+    f_out = np.zeros(outer.shape + (19,), dtype=f_in.dtype)
+    f_out[1:-1, 1:-1, 1:-1] = f_in
+    a, sa = oracle.run(inner, f_in, 9)
+    b, sb = oracle.run(outer, f_out, 9)
+    assert np.array_equal(a, b[1:-1, 1:-1, 1:-1])
+    assert sa["avg_rhobar"] == pytest.approx(sb["avg_rhobar"], rel=1e-13)
+
+
+def test_link_wall_force_is_reflected_momentum():
+    """Force on a link-wise wall = total momentum the fluid loses to it per step,
+    measured as the fluid momentum change in a closed periodic box."""
+    case = Case(10, 9, 8, omega=1.5, periodic=(True, True, True))
+    tags = np.zeros(case.shape, np.uint8)
+    tags[4:6, 3:6, 3:5] = TAG_WALL
+    case.tags = tags
+    f0 = random_state(case, 30)
+    f0[tags == TAG_WALL] = 0
+    j0 = _totals(f0)[1]
+    f1, st = oracle.run(case, f0, 1)
+    f1[tags == TAG_WALL] = 0
+    j1 = _totals(f1)[1]
+    np.testing.assert_allclose(j0 - j1, st["force"], atol=1e-13)
+
+
+# -- body force and porosity ---------------------------------------------------
+
+def test_guo_force_momentum_balance():
+    """Guo forcing: the equilibrium is taken at u = j/rho + F/2 and the source adds
+    (1 - omega/2) F, so a collision adds F (1 + omega (rho - 1)/2) to each cell."""
+    F = np.array([2e-5, -1e-5, 5e-6])
+    omega = 1.3
+    case = Case(6, 7, 8, omega=omega, periodic=(True, True, True), force=tuple(F))
+    f0 = random_state(case, 12)
+    j0 = _totals(f0)[1]
+    rho = f0.reshape(-1, 19).sum(1) + 1
+    f1, _ = oracle.run(case, f0, 1)
+    j1 = _totals(f1)[1]
+    want = F * (1 + omega * (rho - 1) / 2).sum()
+    np.testing.assert_allclose(j1 - j0, want, rtol=1e-9, atol=1e-15)
+
+
+@pytest.mark.parametrize("dynamics", ["bgk", "rbgk"])
+def test_brinkman_solid_relaxes_momentum_geometrically(dynamics):
+    """alpha = 1 everywhere: the equilibrium is at rest, so BGK gives j' = (1 - omega) j
+    per collision and the porous drag sums omega * j; regularized BGK sets the first
+    moment to the equilibrium's, j' = 0, and the drag is j."""
+    omega = 1.25
+    case = Case(5, 6, 7, dynamics=dynamics, omega=omega, periodic=(True, True, True),
+                alpha=np.ones((5, 6, 7)))
+    f0 = random_state(case, 14)
+    j0 = _totals(f0)[1]
+    f1, st = oracle.run(case, f0, 1)
+    j1 = _totals(f1)[1]
+    keep = (1 - omega) if dynamics == "bgk" else 0.0
+    removed = omega if dynamics == "bgk" else 1.0
+    np.testing.assert_allclose(j1, keep * j0, rtol=1e-11, atol=1e-14)
+    np.testing.assert_allclose(st["porous"], removed * j0, rtol=1e-11, atol=1e-15)
+
+
+@pytest.mark.parametrize("dynamics", ["bgk", "rbgk"])
+def test_porosity_zero_reduces_to_plain_dynamics(dynamics):
+    base = Case(5, 6, 7, dynamics=dynamics, omega=1.5, periodic=(True, True, True))
+    por = Case(5, 6, 7, dynamics=dynamics, omega=1.5, periodic=(True, True, True), alpha=np.zeros((5, 6, 7)))
+    f0 = random_state(base, 15)
+    a, _ = oracle.run(base, f0, 4)
+    b, st = oracle.run(por, f0, 4)
+    np.testing.assert_allclose(b, a, rtol=0, atol=2e-16)
+    np.testing.assert_allclose(st["porous"], 0, atol=1e-15)
+
+
+# -- analytic ------------------------------------------------------------------
+
+def test_taylor_green_2d_mode_decays_at_the_lattice_viscosity():
+    """u = u0 (sin kx cos ky, -cos kx sin ky, 0) is an exact Navier-Stokes solution
+    with kinetic energy E0 exp(-4 nu k^2 t); nu = (1/omega - 1/2)/3."""
+    n, steps, omega, u0 = 48, 300, 1.2, 0.01
+    case = Case(n, n, 2, omega=omega, periodic=(True, True, True))
+    k = 2 * np.pi / n
+    x, y, _ = np.meshgrid(np.arange(n), np.arange(n), np.arange(2), indexing="ij")
+    u = np.zeros((n, n, 2, 3))
+    u[..., 0] = u0 * np.sin(k * x) * np.cos(k * y)
+    u[..., 1] = -u0 * np.cos(k * x) * np.sin(k * y)
+    f0 = oracle.equilibrium(np.ones((n, n, 2)), u)
+
+    def energy(f):
+        f = f.reshape(-1, 19)
+        rho = f.sum(1) + 1
+        v = (f @ C) / rho[:, None]
+        return 0.5 * (v ** 2).sum()
+
+    e0 = energy(f0)
+    f1, _ = oracle.run(case, f0, steps)
+    nu = (1 / omega - 0.5) / 3
+    measured = -np.log(energy(f1) / e0) / (4 * k * k * steps)
+    assert measured == pytest.approx(nu, rel=5e-3)
