@@ -54,6 +54,12 @@ extern "C" {
 #define PLBAMD_ZOUHE 0        /* ZouHeVelocity/PressureDynamics, boundaryCondition/zouHeDynamics.hh:44-107 */
 #define PLBAMD_REGULARIZED 1  /* RegularizedVelocity/DensityBoundaryDynamics,
                                  boundaryCondition/regularizedBoundaryDynamics.hh:94-106,166-178 */
+#define PLBAMD_WRAPPED 0x100    /* or-ed into the family: the wrapped factories
+                                   (createZouHeBoundaryCondition3D, zouHeBoundary3D.hh:338;
+                                   createLocalBoundaryCondition3D, boundaryCondition3D.hh:1163),
+                                   whose post-stream processor leaves completed populations in
+                                   the lattice (wrappedLocalBoundaryProcessor3D.hh:41-62):
+                                   downloads return the plane cells completed */
 #define PLBAMD_VELOCITY 0
 #define PLBAMD_PRESSURE 1
 
@@ -112,7 +118,9 @@ int plbamd_set_tags(int handle, const uint8_t* tags);
  * (boundaryCondition/boundaryCondition3D.h:40-200) of the factories
  * createDynamicsBasedZouHeBoundaryCondition3D (zouHeBoundary3D.hh:345) and
  * createDynamicsBasedLocalBoundaryCondition3D (boundaryCondition3D.hh:1170):
- * the completion runs inside the cell's collision. Returns the plane id. */
+ * the completion runs inside the cell's collision. With PLBAMD_WRAPPED in
+ * `family` it stands for the wrapped factories (same steps, completed
+ * snapshots). Returns the plane id. */
 int plbamd_add_plane(int handle, int family, int kind, int dir, int ori,
                      const plbamd_box* box, int* plane);
 
@@ -160,15 +168,17 @@ int plbamd_toggle_statistics(int handle, int on);
  * momentum-exchange sums of BlockStatistics (core/blockStatistics.cpp:77-120). */
 int plbamd_get_statistics(int handle, double* out);
 
-/* Per-cell density and velocity of `box` (FP64; box cells and box cells x 3),
- * rho = rhoBar + 1, u = j / rho of the current populations
- * (momentTemplates3D.h:441-449). */
+/* Per-cell density and velocity of `box` (FP64; box cells and box cells x 3) as
+ * Palabos's Dynamics::computeRhoBar / computeVelocity report them: bulk cells
+ * rho = 1 + sum f-bar, u = j / rho (+ F/2 with a body force); boundary-plane
+ * cells the Dirichlet density and velocity; solid nodes rho = 1, u = 0
+ * (momentTemplates3D.h:375-432, boundaryDynamics.hh:643-835, dynamics.hh:824-1058). */
 int plbamd_compute_macroscopic(int handle, const plbamd_box* box, double* rho, double* u);
 
-/* FP64 sums over `box` of the current populations: out[0] = sum rho,
- * out[1] = sum |u|^2 / 2, out[2..4] = sum j. computeAverageDensity /
- * computeAverageEnergy divide these by the cell count
- * (dataProcessors/dataAnalysisWrapper3D.h:628-645). */
+/* FP64 sums over `box` of the same per-cell quantities: out[0] = sum rhoBar,
+ * out[1] = sum |u|^2, out[2..4] = sum u. computeAverageDensity is
+ * 1 + out[0] / cells and computeAverageEnergy out[1] / (2 cells)
+ * (dataProcessors/dataAnalysisWrapper3D.hh:1226-1257). */
 int plbamd_reduce_moments(int handle, const plbamd_box* box, double* out);
 
 /* Number of completed steps. */

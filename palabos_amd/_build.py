@@ -34,5 +34,28 @@ def build_library(verbose=False):
     return LIB
 
 
+CPP_DIR = os.path.join(HERE, "cpp")
+CPP_BIN = os.path.join(CPP_DIR, "bin")
+CPP_EXAMPLES = ["cavity3d", "coprocessor"]
+
+
+def build_cpp_examples(verbose=False):
+    """Compiles the C++ programs written against the Palabos-compatible header
+    (palabos_amd/cpp/palabos_amd.hh) with g++, linked to the native library."""
+    os.makedirs(CPP_BIN, exist_ok=True)
+    cxx = os.environ.get("CXX", "g++")
+    for name in CPP_EXAMPLES:
+        src = os.path.join(CPP_DIR, "examples", name + ".cpp")
+        out = os.path.join(CPP_BIN, name)
+        cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), "-I", CPP_DIR,
+               src, "-L", HERE, "-lpalabos_amd", "-Wl,-rpath,$ORIGIN/../..", "-o", out]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"building {name} failed:\n" + res.stdout + res.stderr)
+        if verbose:
+            print("built", out)
+
+
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_cpp_examples(verbose=True)

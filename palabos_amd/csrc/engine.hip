@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -48,12 +49,14 @@ template<> struct Store<double> {
     static __device__ __forceinline__ double load(const double* p) { return __builtin_nontemporal_load(p); }
     static __device__ __forceinline__ void store(double* p, double v) { __builtin_nontemporal_store(v, p); }
     static __device__ __forceinline__ double round(double v) { return v; }
+    static __device__ __forceinline__ uint16_t bits(double) { return 0; }
 };
 template<> struct Store<float> {
     using T = float;
     static __device__ __forceinline__ float load(const float* p) { return __builtin_nontemporal_load(p); }
     static __device__ __forceinline__ void store(float* p, float v) { __builtin_nontemporal_store(v, p); }
     static __device__ __forceinline__ float round(float v) { return v; }
+    static __device__ __forceinline__ uint16_t bits(float) { return 0; }
 };
 template<> struct Store<f16s> {
     using T = float;
@@ -70,6 +73,12 @@ template<> struct Store<f16s> {
         __builtin_nontemporal_store(b, reinterpret_cast<uint16_t*>(p));
     }
     static __device__ __forceinline__ float round(float v) { return (float)(_Float16)v; }
+    static __device__ __forceinline__ uint16_t bits(float v) {
+        _Float16 h = (_Float16)v;
+        uint16_t b;
+        __builtin_memcpy(&b, &h, 2);
+        return b;
+    }
 };
 __device__ __forceinline__ uint16_t bf16_round(float v) {
     uint32_t x = __float_as_uint(v);
@@ -87,6 +96,7 @@ template<> struct Store<bf16s> {
         __builtin_nontemporal_store(bf16_round(v), reinterpret_cast<uint16_t*>(p));
     }
     static __device__ __forceinline__ float round(float v) { return __uint_as_float((uint32_t)bf16_round(v) << 16); }
+    static __device__ __forceinline__ uint16_t bits(float v) { return bf16_round(v); }
 };
 
 // ---------------------------------------------------------------------------
@@ -103,6 +113,8 @@ struct Geometry {
 struct StepArgs {
     void* f;
     Geometry g;
+    int zChunk;              // z cells per workgroup row segment
+    int nzChunks;            // row segments per row
     double omega;
     double force[3];
     const uint8_t* tags;     // BulkTag per cell or null
@@ -115,7 +127,7 @@ enum SpecialKind : uint32_t { SK_PLANE = 0, SK_MOMENTUM_EXCHANGE = 1, SK_NEAR_WA
 struct SpecialCell {
     uint32_t cell;
     uint16_t x, y, z;
-    uint16_t info;        // kind | family << 2 | bckind << 3 | dir << 4 | (ori > 0) << 6
+    uint16_t info;        // kind | family << 2 | bckind << 3 | dir << 4 | (ori > 0) << 6 | wrapped << 7
     uint32_t value;       // index into the plane value array (velocity: 3 doubles, pressure: 1)
     uint32_t wall;        // bit i: neighbour x + c_i is a link-wise wall
     uint32_t fluid;       // bit i: fluid direction of a momentum-exchange node
@@ -188,6 +200,20 @@ __device__ __forceinline__ void block_reduce_store(double v[8], double* out) {
 
 // ---------------------------------------------------------------------------
 // Bulk kernel: one workgroup per (x, rows y..y+TY-1), threads stride along z.
+//
+// A wavefront always lies in one (x, y) row (the z extent of a workgroup is a
+// multiple of 64), so the nine neighbouring rows (x-1..x+1, y-1..y+1) and
+// their validity are wavefront-uniform and live in scalar registers. A load or
+// store is then a scalar row base (slot + row start) plus one of three 32-bit
+// per-lane byte offsets (z-1, z, z+1): the global_load/store scalar-base form,
+// with no per-population vector address arithmetic. Wavefronts touching a
+// non-periodic edge (or a periodic wrap in x/y that is not a plain row shift)
+// take the general path with per-lane edge selection.
+
+template<typename S>
+__device__ __forceinline__ S* at(S* base, uint32_t byteOffset) {
+    return reinterpret_cast<S*>(reinterpret_cast<char*>(base) + byteOffset);
+}
 
 template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED>
 __global__ void __launch_bounds__(256) bulk_step(StepArgs a) {
@@ -195,13 +221,17 @@ __global__ void __launch_bounds__(256) bulk_step(StepArgs a) {
     S* __restrict__ f = reinterpret_cast<S*>(a.f);
     const Geometry g = a.g;
     const int x = blockIdx.y;
-    const int y = blockIdx.x * blockDim.y + threadIdx.y;
+    const int zChunk = a.zChunk;
+    const int chunk = blockIdx.x % a.nzChunks;
+    const int y = __builtin_amdgcn_readfirstlane((blockIdx.x / a.nzChunks) * blockDim.y + threadIdx.y);
+    const int zBegin = chunk * zChunk;
+    const int zEnd = min(g.nz, zBegin + zChunk);
     const T omega = (T)a.omega;
     const T F[3] = {(T)a.force[0], (T)a.force[1], (T)a.force[2]};
     double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
     if (y < g.ny) {
-        // Row neighbours (uniform along the row): x-1, x+1, y-1, y+1.
+        // Row neighbours (wavefront-uniform): x-1, x+1, y-1, y+1.
         int xm = 0, xp = 0, ym = 0, yp = 0;
         const bool vxm = shift(x, -1, g.nx, g.px, xm), vxp = shift(x, 1, g.nx, g.px, xp);
         const bool vym = shift(y, -1, g.ny, g.py, ym), vyp = shift(y, 1, g.ny, g.py, yp);
@@ -209,9 +239,17 @@ __global__ void __launch_bounds__(256) bulk_step(StepArgs a) {
         const int ys[3] = {ym, y, yp};
         const bool vx[3] = {vxm, true, vxp};
         const bool vy[3] = {vym, true, vyp};
+        const bool rowsValid = vxm && vxp && vym && vyp;
         const uint64_t rowBase = ((uint64_t)x * g.ny + y) * g.nz;
+        // slot-0 pointer of each neighbouring row
+        S* rows[3][3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+            for (int ay = 0; ay < 3; ++ay)
+                rows[ax][ay] = f + ((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz;
 
-        for (int z = threadIdx.x; z < g.nz; z += blockDim.x) {
+        for (int z = zBegin + threadIdx.x; z < zEnd; z += blockDim.x) {
             const uint64_t cell = rowBase + z;
             uint8_t tag = BT_BULK;
             if constexpr (TAGS) {
@@ -220,14 +258,25 @@ __global__ void __launch_bounds__(256) bulk_step(StepArgs a) {
             }
             int zm = 0, zp = 0;
             const bool vzm = shift(z, -1, g.nz, g.pz, zm), vzp = shift(z, 1, g.nz, g.pz, zp);
-            const int zs[3] = {zm, z, zp};
-            const bool vz[3] = {vzm, true, vzp};
+            const uint32_t zb[3] = {(uint32_t)zm * (uint32_t)sizeof(S), (uint32_t)z * (uint32_t)sizeof(S),
+                                    (uint32_t)zp * (uint32_t)sizeof(S)};
+            bool fast = true;
+            if constexpr (ODD) {
+                const uint64_t active = __ballot(1);
+                fast = rowsValid && (__ballot(vzm && vzp) == active);
+            }
 
             T fc[19];
             if constexpr (!ODD) {
 #pragma unroll
-                for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + (uint64_t)i * g.slot + cell);
+                for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(at(rows[1][1] + (uint64_t)i * g.slot, zb[1]));
+            } else if (fast) {
+#pragma unroll
+                for (int i = 0; i < 19; ++i)
+                    fc[i] = Store<S>::load(at(rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot, zb[1 - cz(i)]));
             } else {
+                const int zs[3] = {zm, z, zp};
+                const bool vz[3] = {vzm, true, vzp};
                 fc[0] = Store<S>::load(f + cell);
 #pragma unroll
                 for (int i = 1; i < 19; ++i) {
@@ -264,8 +313,14 @@ __global__ void __launch_bounds__(256) bulk_step(StepArgs a) {
 
             if constexpr (!ODD) {
 #pragma unroll
-                for (int i = 0; i < 19; ++i) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cell, fc[i]);
+                for (int i = 0; i < 19; ++i) Store<S>::store(at(rows[1][1] + (uint64_t)opp(i) * g.slot, zb[1]), fc[i]);
+            } else if (fast) {
+#pragma unroll
+                for (int i = 0; i < 19; ++i)
+                    Store<S>::store(at(rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot, zb[1 + cz(i)]), fc[i]);
             } else {
+                const int zs[3] = {zm, z, zp};
+                const bool vz[3] = {vzm, true, vzp};
                 Store<S>::store(f + cell, fc[0]);
 #pragma unroll
                 for (int i = 1; i < 19; ++i) {
@@ -610,6 +665,29 @@ __global__ void download_box(const void* fv, BoxArgs b, H* host) {
     }
 }
 
+// Completed snapshot of wrapped-factory plane cells inside a downloaded box:
+// the populations the wrapped post-stream processor leaves in Palabos's lattice.
+template<typename S, typename H>
+__global__ void complete_download(const void* fv, BoxArgs b, const double* values, H* host, int raw) {
+    using T = typename Store<S>::T;
+    const S* f = reinterpret_cast<const S*>(fv);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < b.nSpecial; k += gridDim.x * blockDim.x) {
+        const SpecialCell sc = b.special[k];
+        if ((sc.info & 3u) != SK_PLANE || !((sc.info >> 7) & 1u)) continue;
+        const int x = sc.x, y = sc.y, z = sc.z;
+        if (x < b.x0 || x >= b.x0 + b.lx || y < b.y0 || y >= b.y0 + b.ly || z < b.z0 || z >= b.z0 + b.lz) continue;
+        T fc[19];
+        for (int i = 0; i < 19; ++i)
+            fc[i] = Store<S>::load(f + pop_index<S>(b.g, b.parity, x, y, z, i, b.parity ? sc.wall : 0u, sc.cell));
+        complete_plane(fc, sc.info, values + sc.value);
+        const uint64_t o = (((uint64_t)(x - b.x0) * b.ly + (y - b.y0)) * b.lz + (z - b.z0)) * 19;
+        for (int i = 0; i < 19; ++i) {
+            if (raw) reinterpret_cast<uint16_t*>(host)[o + i] = Store<S>::bits(fc[i]);
+            else host[o + i] = (H)Store<S>::round(fc[i]);
+        }
+    }
+}
+
 // Equilibrium initialisation (core/cell.hh:124-132 iniCellAtEquilibrium) in the
 // compute precision of the lattice.
 template<typename S>
@@ -632,10 +710,54 @@ __global__ void equilibrium_box(void* fv, BoxArgs b, const double* rho, const do
     }
 }
 
-// Density and velocity (momentTemplates3D.h:441-449 compute_rho_uLb), and FP64 sums.
+// Per-cell density and velocity as Palabos's analysis functions see them
+// (Dynamics::computeRhoBar / computeVelocity of the cell's dynamics):
+//   bulk            rhoBar = plain sum (momentTemplates3D.h:375-382), u = j / rho
+//                   (compute_uLb, :425-432), plus F/2 with a body force
+//                   (externalForceDynamics.hh:59-74)
+//   velocity plane  rhoBar from the Dirichlet formula, u = imposed velocity
+//                   (boundaryDynamics.hh:643-671, :324-330)
+//   pressure plane  rhoBar imposed, u = j(rhoBar) / rho (boundaryDynamics.hh:776-835)
+//   solid nodes     rhoBar = 0 (rho = 1), u = 0 (core/dynamics.hh:824-829,1004-1058;
+//                   bounceBackModels.hh:170-231)
+template<typename T, int DIR, int ORI>
+__device__ __forceinline__ void plane_macroscopic(const T* f, int bckind, const double* v, T& rhoBar, T u[3]) {
+    T j[3];
+    if (bckind == PLBAMD_VELOCITY) {
+        const T uu[3] = {(T)v[0], (T)v[1], (T)v[2]};
+        velocity_moments<T, DIR, ORI>(f, uu, rhoBar, j);
+        u[0] = uu[0]; u[1] = uu[1]; u[2] = uu[2];
+    } else {
+        rhoBar = (T)v[0] - (T)1;
+        pressure_moments<T, DIR, ORI>(f, rhoBar, j);
+        const T ir = inv_rho(rhoBar);
+        u[0] = j[0] * ir; u[1] = j[1] * ir; u[2] = j[2] * ir;
+    }
+}
+
+template<typename T>
+__device__ void plane_macroscopic_any(const T* f, uint32_t info, const double* v, T& rhoBar, T u[3]) {
+    const int bckind = (info >> 3) & 1, dir = (info >> 4) & 3;
+    const bool pos = (info >> 6) & 1;
+    switch (dir * 2 + (pos ? 1 : 0)) {
+    case 0: plane_macroscopic<T, 0, -1>(f, bckind, v, rhoBar, u); break;
+    case 1: plane_macroscopic<T, 0, 1>(f, bckind, v, rhoBar, u); break;
+    case 2: plane_macroscopic<T, 1, -1>(f, bckind, v, rhoBar, u); break;
+    case 3: plane_macroscopic<T, 1, 1>(f, bckind, v, rhoBar, u); break;
+    case 4: plane_macroscopic<T, 2, -1>(f, bckind, v, rhoBar, u); break;
+    default: plane_macroscopic<T, 2, 1>(f, bckind, v, rhoBar, u); break;
+    }
+}
+
+struct MomentsArgs {
+    const uint8_t* tags;
+    const double* values;
+    double force[3];
+};
+
 template<typename S>
-__global__ void __launch_bounds__(256) moments_box(const void* fv, BoxArgs b, double* rhoOut, double* uOut,
-                                                   double* partials) {
+__global__ void __launch_bounds__(256) moments_box(const void* fv, BoxArgs b, MomentsArgs m, double* rhoOut,
+                                                   double* uOut, double* partials) {
     using T = typename Store<S>::T;
     const S* f = reinterpret_cast<const S*>(fv);
     const uint64_t n = (uint64_t)b.lx * b.ly * b.lz;
@@ -644,27 +766,44 @@ __global__ void __launch_bounds__(256) moments_box(const void* fv, BoxArgs b, do
         const int lz = k % b.lz, ly = (k / b.lz) % b.ly, lx = k / ((uint64_t)b.lz * b.ly);
         const int x = b.x0 + lx, y = b.y0 + ly, z = b.z0 + lz;
         const uint64_t cell = lin(b.g, x, y, z);
-        const uint32_t wall = b.parity ? wall_mask_of(b, cell) : 0;
-        T fc[19];
-        for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + pop_index<S>(b.g, b.parity, x, y, z, i, wall, cell));
-        T rhoBar, j[3];
-        rhobar_j(fc, rhoBar, j);
-        const T ir = inv_rho(rhoBar);
-        const T rho = rhoBar + (T)1;
-        const T u[3] = {j[0] * ir, j[1] * ir, j[2] * ir};
-        if (rhoOut) rhoOut[k] = (double)rho;
+        const uint8_t tag = m.tags ? m.tags[cell] : (uint8_t)BT_BULK;
+        // locate the special record of the cell, if any
+        int32_t sp = -1;
+        if (tag == BT_SKIP) {
+            uint32_t lo = 0, hi = b.nSpecial;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) / 2;
+                if (b.special[mid].cell < cell) lo = mid + 1; else hi = mid;
+            }
+            if (lo < b.nSpecial && b.special[lo].cell == cell) sp = (int32_t)lo;
+        }
+        const uint32_t wall = (b.parity && sp >= 0) ? b.special[sp].wall : 0u;
+        T rhoBar = T(), u[3] = {T(), T(), T()};
+        const bool solid = tag == BT_BOUNCE_BACK || tag == BT_NO_DYNAMICS ||
+                           (tag == BT_SKIP && (sp < 0 || (b.special[sp].info & 3u) == SK_MOMENTUM_EXCHANGE));
+        if (!solid) {
+            T fc[19];
+            for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + pop_index<S>(b.g, b.parity, x, y, z, i, wall, cell));
+            if (sp >= 0 && (b.special[sp].info & 3u) == SK_PLANE) {
+                plane_macroscopic_any(fc, b.special[sp].info, m.values + b.special[sp].value, rhoBar, u);
+            } else {
+                T rb, j[3];
+                rhobar_j(fc, rb, j);
+                rhoBar = rhobar_sum(fc);
+                const T ir = inv_rho(rhoBar);
+                for (int c = 0; c < 3; ++c) u[c] = j[c] * ir + (T)m.force[c] / (T)2;
+            }
+        }
+        if (rhoOut) rhoOut[k] = (double)(rhoBar + (T)1);
         if (uOut) { uOut[3 * k] = (double)u[0]; uOut[3 * k + 1] = (double)u[1]; uOut[3 * k + 2] = (double)u[2]; }
-        st[0] += (double)rho;
-        st[1] += 0.5 * ((double)u[0] * (double)u[0] + (double)u[1] * (double)u[1] + (double)u[2] * (double)u[2]);
-        st[2] += (double)j[0];
-        st[3] += (double)j[1];
-        st[4] += (double)j[2];
+        const T uSqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        st[0] += (double)rhoBar;
+        st[1] += (double)uSqr;
+        st[3] += (double)u[0];
+        st[4] += (double)u[1];
+        st[5] += (double)u[2];
     }
-    if (partials) {
-        // fmax slot 2 of block_reduce_store is a sum here: move j sums past it
-        double r[8] = {st[0], st[1], 0.0, st[2], st[3], st[4], 0, 0};
-        block_reduce_store(r, partials + 8 * (uint64_t)blockIdx.x);
-    }
+    if (partials) block_reduce_store(st, partials + 8 * (uint64_t)blockIdx.x);
 }
 
 __global__ void sum_records(const double* p, uint32_t n, double* out) {
@@ -693,6 +832,7 @@ int fail(int code, const std::string& msg) {
 
 struct Plane {
     int family, kind, dir, ori;
+    bool wrapped;
     plbamd_box box;
     std::vector<double> values;
     bool valuesSet = false;
@@ -714,6 +854,7 @@ struct Lattice {
     std::vector<double> hostAlpha;
 
     bool dirty = true;                // device tag / special structures need a rebuild
+    bool anyWrapped = false;          // some plane stands for a wrapped factory
     uint8_t* dTags = nullptr;
     SpecialCell* dSpecial = nullptr;
     uint32_t nSpecial = 0;
@@ -727,6 +868,7 @@ struct Lattice {
     bool statsValid = false;
 
     dim3 bulkBlock, bulkGrid;
+    int zChunk = 0, nzChunks = 1;
 
     ~Lattice() {
         hipFree(f); hipFree(dTags); hipFree(dSpecial); hipFree(dValues); hipFree(dAlpha);
@@ -794,6 +936,7 @@ int rebuild(Lattice& L) {
         values.insert(values.end(), L.planes[p].values.begin(), L.planes[p].values.end());
     }
     bool anyNonBulk = anyPlane;
+    L.anyWrapped = false;
     for (int x = 0; x < nx; ++x)
         for (int y = 0; y < ny; ++y)
             for (int z = 0; z < nz; ++z) {
@@ -821,7 +964,9 @@ int rebuild(Lattice& L) {
                 s.wall = wall;
                 if (p >= 0) {
                     const Plane& P = L.planes[p];
-                    s.info = SK_PLANE | (P.family << 2) | (P.kind << 3) | (P.dir << 4) | ((P.ori > 0 ? 1 : 0) << 6);
+                    s.info = SK_PLANE | (P.family << 2) | (P.kind << 3) | (P.dir << 4) | ((P.ori > 0 ? 1 : 0) << 6) |
+                             ((P.wrapped ? 1 : 0) << 7);
+                    if (P.wrapped) L.anyWrapped = true;
                     s.value = planeValueBase[p] + boxIndex[c] * (P.kind == PLBAMD_VELOCITY ? 3u : 1u);
                     dev[c] = BT_SKIP;
                     special.push_back(s);
@@ -876,6 +1021,8 @@ int launchStepTyped(Lattice& L, bool odd, bool stats) {
     a.tags = L.dTags;
     a.alpha = L.dAlpha;
     a.partials = L.dBulkPartials;
+    a.zChunk = L.zChunk;
+    a.nzChunks = L.nzChunks;
     const bool forced = L.dAlpha || L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0;
     const bool tags = L.dTags != nullptr;
     // 2 x 2 x 2 x 2 dispatch over (odd, tags, stats, forced)
@@ -983,6 +1130,16 @@ int transfer(Lattice& L, const plbamd_box& b, int elem, void* data, size_t bytes
         else if (elem == PLBAMD_F32) hipLaunchKernelGGL((download_box<S, float>), grid, block, 0, L.stream, L.f, a, (float*)staging);
         else hipLaunchKernelGGL((download_box_raw<S>), grid, block, 0, L.stream, L.f, a, (uint16_t*)staging);
         err = hipGetLastError();
+        if (err == hipSuccess && L.anyWrapped && L.nSpecial) {
+            dim3 g2 = gridFor(L.nSpecial);
+            if (elem == PLBAMD_F64)
+                hipLaunchKernelGGL((complete_download<S, double>), g2, block, 0, L.stream, L.f, a, L.dValues, (double*)staging, 0);
+            else if (elem == PLBAMD_F32)
+                hipLaunchKernelGGL((complete_download<S, float>), g2, block, 0, L.stream, L.f, a, L.dValues, (float*)staging, 0);
+            else
+                hipLaunchKernelGGL((complete_download<S, float>), g2, block, 0, L.stream, L.f, a, L.dValues, (float*)staging, 1);
+            err = hipGetLastError();
+        }
         if (err == hipSuccess) err = hipMemcpyAsync(data, staging, bytes, hipMemcpyDeviceToHost, L.stream);
     }
     if (err == hipSuccess) err = hipStreamSynchronize(L.stream);
@@ -1024,7 +1181,11 @@ int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sum
     double* dU = u ? d + n : nullptr;
     double* dPart = d + fieldBytes / sizeof(double);
     double* dSum = dPart + (size_t)grid.x * 8;
-    hipLaunchKernelGGL((moments_box<S>), grid, dim3(256), 0, L.stream, L.f, boxArgs(L, b), dRho, dU, dPart);
+    MomentsArgs m{};
+    m.tags = L.dTags;
+    m.values = L.dValues;
+    for (int c = 0; c < 3; ++c) m.force[c] = L.desc.force[c];
+    hipLaunchKernelGGL((moments_box<S>), grid, dim3(256), 0, L.stream, L.f, boxArgs(L, b), m, dRho, dU, dPart);
     hipError_t err = hipGetLastError();
     if (err == hipSuccess) {
         hipLaunchKernelGGL(sum_records, dim3(1), dim3(256), 0, L.stream, dPart, grid.x, dSum);
@@ -1038,9 +1199,9 @@ int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sum
     hipFree(d);
     if (err != hipSuccess) return fail(PLBAMD_EHIP, std::string("moments: ") + hipGetErrorString(err));
     if (sums) {
-        sums[0] = host[0];
-        sums[1] = host[1];
-        sums[2] = host[3];
+        sums[0] = host[0];  // sum of rhoBar
+        sums[1] = host[1];  // sum of |u|^2
+        sums[2] = host[3];  // sum of u
         sums[3] = host[4];
         sums[4] = host[5];
     }
@@ -1094,11 +1255,17 @@ int plbamd_lattice_create(const plbamd_lattice_desc* desc, int* handle) {
     PLBAMD_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
     PLBAMD_HIP(hipEventCreate(&L->ev0));
     PLBAMD_HIP(hipEventCreate(&L->ev1));
-    // Bulk launch shape: threads along z (a multiple of the 64-lane wavefront), rows in y.
-    int tx = (int)std::min<int64_t>(256, ((int64_t)desc->nz + 63) / 64 * 64);
-    int ty = 256 / tx;
+    // Bulk launch shape: TX threads along z (a multiple of the 64-lane wavefront),
+    // each covering CPT cells of a row segment; 256/TX... rows in y per workgroup.
+    int txMax = 128, cpt = 1;
+    if (const char* e = getenv("PLBAMD_BULK_TX")) txMax = std::max(64, std::min(256, atoi(e) / 64 * 64));
+    if (const char* e = getenv("PLBAMD_BULK_CPT")) cpt = std::max(1, std::min(16, atoi(e)));
+    int tx = (int)std::min<int64_t>(txMax, ((int64_t)desc->nz + 63) / 64 * 64);
+    int ty = std::max(1, txMax / tx);
+    L->zChunk = tx * cpt;
+    L->nzChunks = (desc->nz + L->zChunk - 1) / L->zChunk;
     L->bulkBlock = dim3(tx, ty);
-    L->bulkGrid = dim3((desc->ny + ty - 1) / ty, desc->nx);
+    L->bulkGrid = dim3((unsigned)(L->nzChunks * ((desc->ny + ty - 1) / ty)), desc->nx);
     L->nBulkPartials = (uint64_t)L->bulkGrid.x * L->bulkGrid.y;
     PLBAMD_HIP(hipMalloc(&L->dBulkPartials, L->nBulkPartials * 8 * sizeof(double)));
     PLBAMD_HIP(hipMalloc(&L->dStats, 16 * sizeof(double)));
@@ -1133,6 +1300,8 @@ int plbamd_add_plane(int handle, int family, int kind, int dir, int ori, const p
     Lattice* L = find(handle);
     if (!L) return fail(PLBAMD_EINVAL, "add_plane: unknown handle");
     if (!box || !boxValid(*L, *box)) return fail(PLBAMD_EINVAL, "add_plane: box outside the lattice");
+    const bool wrapped = (family & PLBAMD_WRAPPED) != 0;
+    family &= ~PLBAMD_WRAPPED;
     if (family != PLBAMD_ZOUHE && family != PLBAMD_REGULARIZED) return fail(PLBAMD_EINVAL, "add_plane: unknown family");
     if (kind != PLBAMD_VELOCITY && kind != PLBAMD_PRESSURE) return fail(PLBAMD_EINVAL, "add_plane: unknown kind");
     if (dir < 0 || dir > 2 || (ori != -1 && ori != 1)) return fail(PLBAMD_EINVAL, "add_plane: bad direction/orientation");
@@ -1141,7 +1310,7 @@ int plbamd_add_plane(int handle, int family, int kind, int dir, int ori, const p
     if (L->desc.force[0] != 0 || L->desc.force[1] != 0 || L->desc.force[2] != 0)
         return fail(PLBAMD_EINVAL, "add_plane: planes are not combined with a body force");
     Plane p;
-    p.family = family; p.kind = kind; p.dir = dir; p.ori = ori; p.box = *box;
+    p.family = family; p.kind = kind; p.dir = dir; p.ori = ori; p.box = *box; p.wrapped = wrapped;
     p.values.assign(boxCells(*box) * (kind == PLBAMD_VELOCITY ? 3 : 1), 0.0);
     if (kind == PLBAMD_PRESSURE) std::fill(p.values.begin(), p.values.end(), 1.0);
     L->planes.push_back(p);

@@ -291,14 +291,14 @@ class Block3D:
         b = _box(box, self.shape)
         out = np.zeros(5, dtype=np.float64)
         check(lib().plbamd_reduce_moments(self.handle, ctypes.byref(b), out.ctypes.data))
-        return {"sum_rho": out[0], "sum_energy": out[1], "sum_j": out[2:5].copy(), "cells": b.cells()}
+        return {"sum_rhobar": out[0], "sum_usqr": out[1], "sum_u": out[2:5].copy(), "cells": b.cells()}
 
     def averageDensity(self, box=None):
-        """computeAverageDensity (dataAnalysisWrapper3D.h:628-631)."""
+        """computeAverageDensity (dataAnalysisWrapper3D.hh:1226-1230)."""
         r = self.reduceMoments(box)
-        return r["sum_rho"] / r["cells"]
+        return 1.0 + r["sum_rhobar"] / r["cells"]
 
     def averageEnergy(self, box=None):
         """computeAverageEnergy (dataAnalysisWrapper3D.hh:1252-1257): mean of |u|^2 / 2."""
         r = self.reduceMoments(box)
-        return r["sum_energy"] / r["cells"]
+        return 0.5 * r["sum_usqr"] / r["cells"]
