@@ -5,7 +5,10 @@ on a 512^3 D3Q19 lattice, BGK, omega = 1.8, u0 = 0.04, synthetic initial state
 f = f_eq(rho=1, u_TG) built on the device. A step is one collide-and-stream
 pass over the whole lattice (all 134,217,728 cells), statistics off (as
 Palabos's own timing runs toggle them off). The primary line is FP64; the FP32
-and FP16-storage runs of the same workload are reported under "variants".
+and FP16-storage runs of the same workload are reported under "variants";
+"adjoint" gives the reverse-sweep / forward time ratio of the hand-written
+adjoint on the Brinkman porosity channel (128 x 64 x 64, 500 steps, checkpoint
+every 50, FP64).
 
 value        MLUPS = cells * steps / wall seconds / 1e6 over the timed steps,
              populations resident in HBM before the timed region starts.
@@ -17,7 +20,7 @@ cpu_baseline the compiled Palabos reference (oracle/_ref/ref_driver, serial,
              restatement when the reference binary is absent.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
-                       [--precision f64] [--no-variants] [--no-cpu]
+                       [--precision f64] [--no-variants] [--no-cpu] [--no-adjoint]
 """
 import argparse
 import json
@@ -79,6 +82,42 @@ def run_precision(precision, n, steps, warmup, omega=1.8, u0=0.04):
         lat.close()
 
 
+def adjoint_ratio(repeats=3):
+    """Brinkman porosity channel 128 x 64 x 64 (Zou-He Poiseuille inlet u_max = 0.04,
+    pressure outlet, omega = 1.6, porosity from 32 seeded Gaussian blobs), 500
+    steps, checkpoint every 50: forward (with checkpoints) and reverse-sweep
+    (recompute + adjoint) times from HIP events, best of `repeats`."""
+    import palabos_amd as plb
+    from palabos_amd import setups
+    nx, ny, nz, u_max = 128, 64, 64, 0.04
+    alpha = setups.brinkman_blobs((nx, ny, nz), n_blobs=32, seed=4)
+    alpha[0] = 0.0
+    alpha[-1] = 0.0
+    prof = setups.poiseuille_profile(ny, nz, u_max)
+    vel = np.zeros(((ny - 2) * (nz - 2), 3))
+    vel[:, 0] = prof.reshape(-1)
+    best = None
+    for _ in range(repeats):
+        lat = plb.Block3D(nx, ny, nz, plb.D3Q19, plb.BGK(1.6), storage="f64")
+        try:
+            p_in = lat.addVelocityBoundary(0, -1, (0, 0, 1, ny - 2, 1, nz - 2))
+            lat.setBoundaryVelocity(p_in, vel)
+            p_out = lat.addPressureBoundary(0, 1, (nx - 1, nx - 1, 1, ny - 2, 1, nz - 2))
+            lat.setBoundaryDensity(p_out, np.ones((ny - 2) * (nz - 2)))
+            lat.setPorosity(alpha)
+            lat.setPopulations(np.zeros((nx, ny, nz, 19)))
+            J, _, _, (ms_f, ms_b) = lat.adjointGradient(500, 50, 1, nx - 2, 1.0, 1.0)
+        finally:
+            lat.close()
+        if best is None or ms_f + ms_b < best[0] + best[1]:
+            best = (ms_f, ms_b, J)
+    ms_f, ms_b, J = best
+    cells_steps = nx * ny * nz * 500
+    return {"workload": "brinkman_channel_128x64x64_500steps_checkpoint50_f64", "forward_ms": ms_f,
+            "reverse_ms": ms_b, "ratio": ms_b / ms_f, "forward_mlups": cells_steps / ms_f / 1e3,
+            "objective": J}
+
+
 def cpu_baseline():
     ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
     n, steps = 128, 60
@@ -124,6 +163,7 @@ def main():
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "f16", "bf16"])
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -159,6 +199,8 @@ def main():
                      "event_ms_per_launch": main_run["event_ms_per_step"]},
         "variants": variants,
     }
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_ratio()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))

@@ -102,6 +102,20 @@ const char* plbamd_last_error(void);
  * Populations start at rest (f-bar = 0). */
 int plbamd_lattice_create(const plbamd_lattice_desc* desc, int* handle);
 
+/* Creates a multi-block lattice: the domain desc->nx x ny x nz tiled by
+ * `nblocks` uneven blocks (inclusive global boxes that must cover the domain
+ * without overlap). Replaces the MultiBlockLattice3D(MultiBlockManagement3D,
+ * ...) constructor over a SparseBlockStructure3D built with addBlock
+ * (multiBlock/multiBlockLattice3D.h:75-84, sparseBlockStructure3D.h:55-57).
+ * Every block is swept by the same single launch per step; blocks exchange a
+ * two-cell envelope (Palabos: one cell, every step) after every second step,
+ * which keeps the bulk bit-identical to a single-block lattice. All other entry
+ * points take global coordinates; populations can be uploaded (and
+ * initialised) only after an even number of steps (PLBAMD_ESTATE otherwise).
+ * At most 65535 blocks. */
+int plbamd_multiblock_create(const plbamd_lattice_desc* desc, const plbamd_box* blocks, int nblocks,
+                             int* handle);
+
 /* Releases a lattice. Replaces BlockLattice3D's destructor (blockLattice3D.hh:120-130). */
 int plbamd_lattice_destroy(int handle);
 
@@ -180,6 +194,32 @@ int plbamd_compute_macroscopic(int handle, const plbamd_box* box, double* rho, d
  * 1 + out[0] / cells and computeAverageEnergy out[1] / (2 cells)
  * (dataProcessors/dataAnalysisWrapper3D.hh:1226-1257). */
 int plbamd_reduce_moments(int handle, const plbamd_box* box, double* out);
+
+/* Objective of plbamd_adjoint_gradient:
+ *   J = (1/T) sum_{t=1..T} [ weight_pressure_drop * cs2 * (<rho>_{x=inlet_x} - <rho>_{x=outlet_x})
+ *                          + weight_drag * omega * sum_cells alpha * j_x ]
+ * evaluated on the states after each step (plane means over all y, z; the drag
+ * term is the momentum the Brinkman porosity removes in the next collision,
+ * x component; FP64 accumulation). */
+typedef struct {
+    int32_t inlet_x, outlet_x;
+    double weight_pressure_drop;
+    double weight_drag;
+} plbamd_objective;
+
+/* Runs nsteps steps from the current state (the lattice advances) and returns J
+ * and its gradient with respect to the porosity of every cell (grad_alpha,
+ * [nx][ny][nz], may be NULL) and to the scale s of every plane's imposed
+ * velocity, u -> s u at s = 1 (grad_plane_scale, one per plane in creation
+ * order; 0 for pressure planes; may be NULL). Hand-written discrete adjoint of
+ * the BGK + Brinkman collide-and-stream with checkpoints every
+ * `checkpoint_every` steps (even) and recomputation; timing_ms (may be NULL)
+ * receives the forward and the backward device time. Single-block FP64 BGK
+ * lattices without a body force or tags (the walls are the domain edges),
+ * started after an even number of steps, porosity zero on the planes.
+ * No Palabos counterpart. */
+int plbamd_adjoint_gradient(int handle, int nsteps, int checkpoint_every, const plbamd_objective* objective,
+                            double* J, double* grad_alpha, double* grad_plane_scale, double* timing_ms);
 
 /* Number of completed steps. */
 int plbamd_get_time(int handle, int64_t* t);

@@ -365,6 +365,8 @@ struct Engine {
     std::vector<int> planeOf;          // per cell, -1 or plane index
     std::vector<long> boxIndexOf;      // per cell, index in the plane box
     std::vector<uint32_t> fluidDirs;   // per cell, bit i set: direction i is a fluid direction (MEBB)
+    const T* alphaT = nullptr;         // porosity in the compute type (derivative-carrying runs)
+    T planeScale = T(1.0);             // scale of every velocity plane's imposed velocity
 
     explicit Engine(const OracleDesc& d_) : d(d_) {
         N = (long)d.nx * d.ny * d.nz;
@@ -420,7 +422,7 @@ struct Engine {
         T rhoBar;
         T j[3];
         bool forced = d.force[0] != 0 || d.force[1] != 0 || d.force[2] != 0;
-        if (d.alpha || forced) {
+        if (d.alpha || alphaT || forced) {
             uSqr = forcedCollide(f, k, rhoBar, st);
         } else if (d.dynamics == 0) {
             get_rhoBar_j(f, rhoBar, j);
@@ -466,7 +468,7 @@ struct Engine {
         // Brinkman damping; the momentum it removes in the collision is summed as
         // the porous drag: omega*rho*(u - u_damped) for BGK, rho*(u - u_damped)
         // for RBGK, whose first moment becomes j_eq.
-        T damp = (T)1 - (d.alpha ? (T)d.alpha[k] : (T)0);
+        T damp = (T)1 - (alphaT ? alphaT[k] : (d.alpha ? (T)d.alpha[k] : (T)0));
         T rho = rhoBar + (T)1;
         T jeq[3];
         for (int c = 0; c < 3; ++c) {
@@ -498,7 +500,8 @@ struct Engine {
         T rhoBar;
         T j[3];
         if (P.kind == 0) {
-            T u[3] = {(T)P.values[3*bi], (T)P.values[3*bi+1], (T)P.values[3*bi+2]};
+            T u[3] = {planeScale * (T)P.values[3*bi], planeScale * (T)P.values[3*bi+1],
+                      planeScale * (T)P.values[3*bi+2]};
             T rhoOnWall = T();
             for (int n = 0; n < nOnWall; ++n) rhoOnWall += f[onWall[n]];
             T rhoNormal = T();
@@ -712,6 +715,71 @@ void completeAll(const OracleDesc& d, S* state) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Forward-mode derivatives for the adjoint check: a dual number with M
+// derivative components, run through the same Engine (the oracle's own
+// collision and streaming), gives exact directional derivatives of the
+// time-averaged objective with respect to seeded porosity cells and the
+// velocity-plane scale. Independent of the hand-derived adjoint.
+
+template<int M>
+struct DualN {
+    double v;
+    double d[M];
+    DualN() : v(0) { for (int k = 0; k < M; ++k) d[k] = 0; }
+    DualN(double x) : v(x) { for (int k = 0; k < M; ++k) d[k] = 0; }
+    explicit operator double() const { return v; }
+};
+template<int M> DualN<M> operator+(const DualN<M>& a, const DualN<M>& b) {
+    DualN<M> r(a.v + b.v); for (int k = 0; k < M; ++k) r.d[k] = a.d[k] + b.d[k]; return r;
+}
+template<int M> DualN<M> operator-(const DualN<M>& a, const DualN<M>& b) {
+    DualN<M> r(a.v - b.v); for (int k = 0; k < M; ++k) r.d[k] = a.d[k] - b.d[k]; return r;
+}
+template<int M> DualN<M> operator-(const DualN<M>& a) {
+    DualN<M> r(-a.v); for (int k = 0; k < M; ++k) r.d[k] = -a.d[k]; return r;
+}
+template<int M> DualN<M> operator*(const DualN<M>& a, const DualN<M>& b) {
+    DualN<M> r(a.v * b.v); for (int k = 0; k < M; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k]; return r;
+}
+template<int M> DualN<M> operator/(const DualN<M>& a, const DualN<M>& b) {
+    DualN<M> r(a.v / b.v);
+    for (int k = 0; k < M; ++k) r.d[k] = (a.d[k] * b.v - a.v * b.d[k]) / (b.v * b.v);
+    return r;
+}
+template<int M> DualN<M> operator*(int a, const DualN<M>& b) { return DualN<M>((double)a) * b; }
+template<int M> DualN<M> operator*(const DualN<M>& a, int b) { return a * DualN<M>((double)b); }
+template<int M> DualN<M> operator-(int a, const DualN<M>& b) { return DualN<M>((double)a) - b; }
+template<int M> DualN<M>& operator+=(DualN<M>& a, const DualN<M>& b) { a = a + b; return a; }
+template<int M> DualN<M>& operator-=(DualN<M>& a, const DualN<M>& b) { a = a - b; return a; }
+template<int M> DualN<M>& operator*=(DualN<M>& a, const DualN<M>& b) { a = a * b; return a; }
+template<int M> bool operator!=(const DualN<M>& a, int b) { return a.v != (double)b; }
+
+constexpr int kSeeds = 16;           // porosity cells carrying a derivative
+typedef DualN<kSeeds + 1> DualO;     // + the velocity-plane scale
+
+// j(f) of the adjoint objective on the natural-layout state (see the C ABI).
+template<typename T>
+T objectiveTerm(const OracleDesc& d, const std::vector<T>& st, const Engine<T>& eng, int xin, int xout,
+                double wdp, double wdrag) {
+    const long plane = (long)d.ny * d.nz;
+    T pin = T(), pout = T(), drag = T();
+    const T omega = (T)d.omega;
+    for (int x = 0; x < d.nx; ++x) for (int y = 0; y < d.ny; ++y) for (int z = 0; z < d.nz; ++z) {
+        const long k = eng.lin(x, y, z);
+        const T* f = &st[k * 19];
+        T rho = (T)1;
+        for (int i = 0; i < 19; ++i) rho = rho + f[i];
+        T jx = T();
+        for (int i = 0; i < 19; ++i) jx = jx + C[i][0] * f[i];
+        if (x == xin) pin = pin + rho;
+        if (x == xout) pout = pout + rho;
+        if (eng.alphaT && eng.tag[k] != TAG_PLANE) drag = drag + omega * eng.alphaT[k] * jx;
+    }
+    const T cs2 = (T)(1.0 / 3.0);
+    return (T)wdp * cs2 * (pin / (T)(double)plane - pout / (T)(double)plane) + (T)wdrag * drag;
+}
+
 extern "C" {
 
 // Advances `state` (natural layout, AoS [x][y][z][19] f-bar in the storage type)
@@ -757,6 +825,48 @@ void oracle_equilibrium(long n, const double* rho, const double* u, double* fOut
         double jSqr = j[0]*j[0] + j[1]*j[1] + j[2]*j[2];
         for (int i = 0; i < 19; ++i) fOut[k*19 + i] = equilibrium(i, rhoBar, invRho, j, jSqr);
     }
+}
+
+// J = (1/T) sum_{t=1..T} j(f^t) of the adjoint objective from FP64 state f0, and
+// its exact derivatives with respect to the porosity of 16 seeded cells and to
+// the scale of the velocity planes' imposed velocity: out = [J, dJ/dalpha[16], dJ/ds].
+int oracle_objective_dual(const OracleDesc* d, const double* f0, int nsteps, int xin, int xout, double wdp,
+                          double wdrag, const long* seeds, double* out) {
+    if (!d || !f0 || nsteps <= 0 || d->storage != 0 || !d->alpha) return -1;
+    Engine<DualO> eng(*d);
+    const long N = eng.N;
+    std::vector<DualO> alpha(N);
+    for (long k = 0; k < N; ++k) alpha[k] = DualO(d->alpha[k]);
+    for (int s = 0; s < kSeeds; ++s) alpha[seeds[s]].d[s] = 1.0;
+    eng.alphaT = alpha.data();
+    eng.planeScale = DualO(1.0);
+    eng.planeScale.d[kSeeds] = 1.0;
+    std::vector<DualO> state(N * 19), post(N * 19);
+    for (long k = 0; k < N * 19; ++k) state[k] = DualO(f0[k]);
+    DualO J = DualO();
+    const DualO invT = DualO(1.0 / nsteps);
+    Stats st;
+    for (int step = 0; step < nsteps; ++step) {
+        for (long k = 0; k < N; ++k) {
+            DualO f[19];
+            for (int i = 0; i < 19; ++i) f[i] = state[k*19 + i];
+            eng.collideCell(f, k, st);
+            for (int i = 0; i < 19; ++i) post[k*19 + i] = f[i];
+        }
+        for (int x = 0; x < d->nx; ++x) for (int y = 0; y < d->ny; ++y) for (int z = 0; z < d->nz; ++z) {
+            long k = eng.lin(x, y, z);
+            state[k*19] = post[k*19];
+            for (int i = 1; i < 19; ++i) {
+                long src = -1;
+                if (!eng.sourceReflects(x, y, z, i, src)) state[k*19 + i] = post[src*19 + i];
+                else state[k*19 + i] = post[k*19 + opposite(i)];
+            }
+        }
+        J = J + invT * objectiveTerm(*d, state, eng, xin, xout, wdp, wdrag);
+    }
+    out[0] = J.v;
+    for (int k = 0; k <= kSeeds; ++k) out[1 + k] = J.d[k];
+    return 0;
 }
 
 void oracle_f32_to_f16(long n, const float* in, uint16_t* out) { for (long k = 0; k < n; ++k) out[k] = f32_to_f16(in[k]); }

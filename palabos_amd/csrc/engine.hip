@@ -1,820 +1,24 @@
-// palabos_amd engine: D3Q19 collide-and-stream on gfx950 (MI355X).
-//
-// Data layout in HBM (one lattice = one Palabos block, envelope included):
-//   populations  SoA, slot-major: f[slot][x][y][z], z fastest, each slot padded
-//                to a multiple of 64 cells; storage type double | float |
-//                f16 bits | bf16 bits (f-bar = f - t_i, roundOffPolicy.h:73-90)
-//   tags         uint8 per cell, only when the lattice has non-bulk cells:
-//                0 bulk, 1 BounceBack node, 2 NoDynamics, 3 handled by the
-//                special-cell list (planes, momentum-exchange nodes, cells next
-//                to link-wise walls) or a link-wise wall cell (never updated)
-//   special list 32-byte records, sorted by cell index
-//
-// Streaming is the in-place AA pattern, one array, no second lattice:
-//   even step  each cell reads its own slots i, collides, writes f*_i to its
-//              own slot opp(i)                        (all accesses local)
-//   odd step   each cell gathers f_i from slot opp(i) of cell x - c_i, collides,
-//              scatters f*_i to slot i of cell x + c_i (back to natural layout)
-// A source or destination outside a non-periodic edge (or a link-wise wall)
-// is replaced by the cell's own opposite slot: that is Palabos's same-step
-// edge reflection (atomicBlock/blockLattice3D.hh:403-427). Every memory slot
-// is read and written by exactly one cell per step, so the update is race-free
-// and the cell order is free, as it is in Palabos.
+// palabos_amd engine: D3Q19 collide-and-stream on gfx950 (MI355X) — host
+// runtime and C ABI (include/palabos_amd.h). The kernels and the data layout
+// are described in kernels.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cstdlib>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/palabos_amd.h"
-#include "d3q19_device.h"
+#include "adjoint.h"
+#include "kernels.h"
 
 namespace plbamd {
-
-// ---------------------------------------------------------------------------
-// Storage types
-
-struct f16s { uint16_t b; };
-struct bf16s { uint16_t b; };
-
-template<typename S> struct Store;
-template<> struct Store<double> {
-    using T = double;
-    static __device__ __forceinline__ double load(const double* p) { return __builtin_nontemporal_load(p); }
-    static __device__ __forceinline__ void store(double* p, double v) { __builtin_nontemporal_store(v, p); }
-    static __device__ __forceinline__ double round(double v) { return v; }
-    static __device__ __forceinline__ uint16_t bits(double) { return 0; }
-};
-template<> struct Store<float> {
-    using T = float;
-    static __device__ __forceinline__ float load(const float* p) { return __builtin_nontemporal_load(p); }
-    static __device__ __forceinline__ void store(float* p, float v) { __builtin_nontemporal_store(v, p); }
-    static __device__ __forceinline__ float round(float v) { return v; }
-    static __device__ __forceinline__ uint16_t bits(float) { return 0; }
-};
-template<> struct Store<f16s> {
-    using T = float;
-    static __device__ __forceinline__ float load(const f16s* p) {
-        uint16_t b = __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
-        _Float16 h;
-        __builtin_memcpy(&h, &b, 2);
-        return (float)h;
-    }
-    static __device__ __forceinline__ void store(f16s* p, float v) {
-        _Float16 h = (_Float16)v;  // round to nearest even
-        uint16_t b;
-        __builtin_memcpy(&b, &h, 2);
-        __builtin_nontemporal_store(b, reinterpret_cast<uint16_t*>(p));
-    }
-    static __device__ __forceinline__ float round(float v) { return (float)(_Float16)v; }
-    static __device__ __forceinline__ uint16_t bits(float v) {
-        _Float16 h = (_Float16)v;
-        uint16_t b;
-        __builtin_memcpy(&b, &h, 2);
-        return b;
-    }
-};
-__device__ __forceinline__ uint16_t bf16_round(float v) {
-    uint32_t x = __float_as_uint(v);
-    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
-    x += 0x7fffu + ((x >> 16) & 1u);
-    return (uint16_t)(x >> 16);
-}
-template<> struct Store<bf16s> {
-    using T = float;
-    static __device__ __forceinline__ float load(const bf16s* p) {
-        uint16_t b = __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
-        return __uint_as_float((uint32_t)b << 16);
-    }
-    static __device__ __forceinline__ void store(bf16s* p, float v) {
-        __builtin_nontemporal_store(bf16_round(v), reinterpret_cast<uint16_t*>(p));
-    }
-    static __device__ __forceinline__ float round(float v) { return __uint_as_float((uint32_t)bf16_round(v) << 16); }
-    static __device__ __forceinline__ uint16_t bits(float v) { return bf16_round(v); }
-};
-
-// ---------------------------------------------------------------------------
-// Kernel arguments
-
-enum BulkTag : uint8_t { BT_BULK = 0, BT_BOUNCE_BACK = 1, BT_NO_DYNAMICS = 2, BT_SKIP = 3 };
-
-struct Geometry {
-    int nx, ny, nz;
-    int px, py, pz;          // periodic flags
-    uint64_t slot;           // elements per population slot
-};
-
-struct StepArgs {
-    void* f;
-    Geometry g;
-    int zChunk;              // z cells per workgroup row segment
-    int nzChunks;            // row segments per row
-    double omega;
-    double force[3];
-    const uint8_t* tags;     // BulkTag per cell or null
-    const double* alpha;     // Brinkman porosity per cell or null
-    double* partials;        // 8 doubles per workgroup when statistics are on
-};
-
-enum SpecialKind : uint32_t { SK_PLANE = 0, SK_MOMENTUM_EXCHANGE = 1, SK_NEAR_WALL = 2 };
-
-struct SpecialCell {
-    uint32_t cell;
-    uint16_t x, y, z;
-    uint16_t info;        // kind | family << 2 | bckind << 3 | dir << 4 | (ori > 0) << 6 | wrapped << 7
-    uint32_t value;       // index into the plane value array (velocity: 3 doubles, pressure: 1)
-    uint32_t wall;        // bit i: neighbour x + c_i is a link-wise wall
-    uint32_t fluid;       // bit i: fluid direction of a momentum-exchange node
-    uint32_t pad[2];
-};
-static_assert(sizeof(SpecialCell) == 32, "special record is 32 bytes");
-
-struct SpecialArgs {
-    void* f;
-    Geometry g;
-    double omega;
-    const SpecialCell* cells;
-    uint32_t count;
-    const double* values;
-    double* partials;     // 8 doubles per workgroup when statistics are on
-};
-
-// ---------------------------------------------------------------------------
-// Neighbour arithmetic
-
-__device__ __forceinline__ bool shift(int v, int d, int n, int periodic, int& out) {
-    int w = v + d;
-    if (w < 0) { if (!periodic) return false; w += n; }
-    else if (w >= n) { if (!periodic) return false; w -= n; }
-    out = w;
-    return true;
-}
-
-__device__ __forceinline__ uint64_t lin(const Geometry& g, int x, int y, int z) {
-    return ((uint64_t)x * g.ny + y) * g.nz + z;
-}
-
-// Position of f_i of cell (x,y,z) in the swapped (after-even-step) layout:
-// slot opp(i) at x - c_i, or slot i at the cell itself when the source reflects.
-__device__ __forceinline__ uint64_t swapped_index(const Geometry& g, int x, int y, int z, int i,
-                                                  uint32_t wallMask, uint64_t cell) {
-    if (i == 0) return cell;
-    int sx, sy, sz;
-    bool ok = shift(x, -cx(i), g.nx, g.px, sx) && shift(y, -cy(i), g.ny, g.py, sy) &&
-              shift(z, -cz(i), g.nz, g.pz, sz) && !((wallMask >> opp(i)) & 1u);
-    if (ok) return (uint64_t)opp(i) * g.slot + lin(g, sx, sy, sz);
-    return (uint64_t)i * g.slot + cell;
-}
-
-// ---------------------------------------------------------------------------
-// Block reduction of the 8-double statistics record (fixed order: deterministic)
-
-__device__ __forceinline__ void block_reduce_store(double v[8], double* out) {
-    __shared__ double red[256][8];
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    const int n = blockDim.x * blockDim.y;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) red[t][k] = v[k];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (t < s && t + s < n) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (k == 2) red[t][k] = fmax(red[t][k], red[t + s][k]);
-                else red[t][k] += red[t + s][k];
-            }
-        }
-        __syncthreads();
-    }
-    if (t == 0) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) out[k] = red[0][k];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Bulk kernel: one workgroup per (x, rows y..y+TY-1), threads stride along z.
-//
-// A wavefront always lies in one (x, y) row (the z extent of a workgroup is a
-// multiple of 64), so the nine neighbouring rows (x-1..x+1, y-1..y+1) and
-// their validity are wavefront-uniform and live in scalar registers. A load or
-// store is then a scalar row base (slot + row start) plus one of three 32-bit
-// per-lane byte offsets (z-1, z, z+1): the global_load/store scalar-base form,
-// with no per-population vector address arithmetic. Wavefronts touching a
-// non-periodic edge (or a periodic wrap in x/y that is not a plain row shift)
-// take the general path with per-lane edge selection.
-
-template<typename S>
-__device__ __forceinline__ S* at(S* base, uint32_t byteOffset) {
-    return reinterpret_cast<S*>(reinterpret_cast<char*>(base) + byteOffset);
-}
-
-template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED>
-__global__ void __launch_bounds__(256) bulk_step(StepArgs a) {
-    using T = typename Store<S>::T;
-    S* __restrict__ f = reinterpret_cast<S*>(a.f);
-    const Geometry g = a.g;
-    const int x = blockIdx.y;
-    const int zChunk = a.zChunk;
-    const int chunk = blockIdx.x % a.nzChunks;
-    const int y = __builtin_amdgcn_readfirstlane((blockIdx.x / a.nzChunks) * blockDim.y + threadIdx.y);
-    const int zBegin = chunk * zChunk;
-    const int zEnd = min(g.nz, zBegin + zChunk);
-    const T omega = (T)a.omega;
-    const T F[3] = {(T)a.force[0], (T)a.force[1], (T)a.force[2]};
-    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-
-    if (y < g.ny) {
-        // Row neighbours (wavefront-uniform): x-1, x+1, y-1, y+1.
-        int xm = 0, xp = 0, ym = 0, yp = 0;
-        const bool vxm = shift(x, -1, g.nx, g.px, xm), vxp = shift(x, 1, g.nx, g.px, xp);
-        const bool vym = shift(y, -1, g.ny, g.py, ym), vyp = shift(y, 1, g.ny, g.py, yp);
-        const int xs[3] = {xm, x, xp};
-        const int ys[3] = {ym, y, yp};
-        const bool vx[3] = {vxm, true, vxp};
-        const bool vy[3] = {vym, true, vyp};
-        const bool rowsValid = vxm && vxp && vym && vyp;
-        const uint64_t rowBase = ((uint64_t)x * g.ny + y) * g.nz;
-        // slot-0 pointer of each neighbouring row
-        S* rows[3][3];
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax)
-#pragma unroll
-            for (int ay = 0; ay < 3; ++ay)
-                rows[ax][ay] = f + ((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz;
-
-        for (int z = zBegin + threadIdx.x; z < zEnd; z += blockDim.x) {
-            const uint64_t cell = rowBase + z;
-            uint8_t tag = BT_BULK;
-            if constexpr (TAGS) {
-                tag = a.tags[cell];
-                if (tag == BT_SKIP) continue;
-            }
-            int zm = 0, zp = 0;
-            const bool vzm = shift(z, -1, g.nz, g.pz, zm), vzp = shift(z, 1, g.nz, g.pz, zp);
-            const uint32_t zb[3] = {(uint32_t)zm * (uint32_t)sizeof(S), (uint32_t)z * (uint32_t)sizeof(S),
-                                    (uint32_t)zp * (uint32_t)sizeof(S)};
-            bool fast = true;
-            if constexpr (ODD) {
-                const uint64_t active = __ballot(1);
-                fast = rowsValid && (__ballot(vzm && vzp) == active);
-            }
-
-            T fc[19];
-            if constexpr (!ODD) {
-#pragma unroll
-                for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(at(rows[1][1] + (uint64_t)i * g.slot, zb[1]));
-            } else if (fast) {
-#pragma unroll
-                for (int i = 0; i < 19; ++i)
-                    fc[i] = Store<S>::load(at(rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot, zb[1 - cz(i)]));
-            } else {
-                const int zs[3] = {zm, z, zp};
-                const bool vz[3] = {vzm, true, vzp};
-                fc[0] = Store<S>::load(f + cell);
-#pragma unroll
-                for (int i = 1; i < 19; ++i) {
-                    // source x - c_i
-                    const int ax = 1 - cx(i), ay = 1 - cy(i), az = 1 - cz(i);
-                    const bool ok = vx[ax] && vy[ay] && vz[az];
-                    const uint64_t src = ok ? (uint64_t)opp(i) * g.slot +
-                                                  (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
-                                            : (uint64_t)i * g.slot + cell;
-                    fc[i] = Store<S>::load(f + src);
-                }
-            }
-
-            if (tag == BT_BULK) {
-                T rhoBar;
-                T drag[3] = {0, 0, 0};
-                const T alpha = (FORCED && a.alpha) ? (T)a.alpha[cell] : (T)0;
-                const T uSqr = collide_bulk<T, DYN, FORCED>(fc, omega, F, alpha, rhoBar, drag);
-                if constexpr (STATS) {
-                    st[0] += (double)rhoBar;
-                    st[1] += (double)uSqr;
-                    st[2] = fmax(st[2], (double)uSqr);
-                    st[3] += 1.0;
-                    if constexpr (FORCED) {
-                        st[4] += (double)drag[0];
-                        st[5] += (double)drag[1];
-                        st[6] += (double)drag[2];
-                    }
-                }
-            } else if (tag == BT_BOUNCE_BACK) {
-#pragma unroll
-                for (int i = 1; i <= 9; ++i) { T t = fc[i]; fc[i] = fc[i + 9]; fc[i + 9] = t; }
-            }
-
-            if constexpr (!ODD) {
-#pragma unroll
-                for (int i = 0; i < 19; ++i) Store<S>::store(at(rows[1][1] + (uint64_t)opp(i) * g.slot, zb[1]), fc[i]);
-            } else if (fast) {
-#pragma unroll
-                for (int i = 0; i < 19; ++i)
-                    Store<S>::store(at(rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot, zb[1 + cz(i)]), fc[i]);
-            } else {
-                const int zs[3] = {zm, z, zp};
-                const bool vz[3] = {vzm, true, vzp};
-                Store<S>::store(f + cell, fc[0]);
-#pragma unroll
-                for (int i = 1; i < 19; ++i) {
-                    // destination x + c_i
-                    const int ax = 1 + cx(i), ay = 1 + cy(i), az = 1 + cz(i);
-                    const bool ok = vx[ax] && vy[ay] && vz[az];
-                    const uint64_t dst = ok ? (uint64_t)i * g.slot +
-                                                  (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
-                                            : (uint64_t)opp(i) * g.slot + cell;
-                    Store<S>::store(f + dst, fc[i]);
-                }
-            }
-        }
-    }
-    if constexpr (STATS) {
-        block_reduce_store(st, a.partials + 8 * (uint64_t)(blockIdx.y * gridDim.x + blockIdx.x));
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Plane completions (dynamics-based composite: completion, then base collision)
-
-// VelocityDirichletBoundaryDynamics::computeRhoBar/computeRhoBarJ
-// (boundaryCondition/boundaryDynamics.hh:643-700), velIsJ() false.
-template<typename T, int DIR, int ORI>
-__device__ __forceinline__ void velocity_moments(const T* f, const T u[3], T& rhoBar, T j[3]) {
-    T rhoOnWall = T();
-#pragma unroll
-    for (int i = 0; i < 19; ++i) if (ci(i, DIR) == 0) rhoOnWall += f[i];
-    T rhoNormal = T();
-#pragma unroll
-    for (int i = 0; i < 19; ++i) if (ci(i, DIR) == ORI) rhoNormal += f[i];
-    const T velNormal = (T)ORI * u[DIR];
-    rhoBar = ((T)2 * rhoNormal + rhoOnWall - velNormal) / ((T)1 + velNormal);
-    const T rho = rhoBar + (T)1;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) j[c] = rho * u[c];
-}
-
-// DensityDirichletBoundaryDynamics::computeJ (boundaryDynamics.hh:776-800).
-template<typename T, int DIR, int ORI>
-__device__ __forceinline__ void pressure_moments(const T* f, T rhoBar, T j[3]) {
-    j[0] = j[1] = j[2] = T();
-    T rhoOnWall = T();
-#pragma unroll
-    for (int i = 0; i < 19; ++i) if (ci(i, DIR) == 0) rhoOnWall += f[i];
-    T rhoNormal = T();
-#pragma unroll
-    for (int i = 0; i < 19; ++i) if (ci(i, DIR) == ORI) rhoNormal += f[i];
-    j[DIR] = (T)ORI * ((T)2 * rhoNormal + rhoOnWall - rhoBar);
-}
-
-// ZouHeClosure (boundaryCondition/zouHeDynamics.hh:44-107).
-template<typename T, int DIR, int ORI>
-__device__ __forceinline__ void zouhe_close(T* f, T rhoBar, const T j[3]) {
-    const T jSqr = j[0] * j[0] + j[1] * j[1] + j[2] * j[2];
-    const T ir = inv_rho(rhoBar);
-    // missing populations: opposite of c[DIR] == ORI, in index order of the latter
-#pragma unroll
-    for (int n = 0; n < 19; ++n) {
-        if (ci(n, DIR) != ORI) continue;
-        const int p = opp(n), o = n;
-        f[p] = f[o] - feq(o, rhoBar, ir, j, jSqr) + feq(p, rhoBar, ir, j, jSqr);
-    }
-    T fRhoBar, fJ[3];
-    rhobar_j(f, fRhoBar, fJ);
-    const T fir = inv_rho(fRhoBar);
-    const T falseRho = fRhoBar + (T)1;
-    T diff[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) diff[c] = (j[c] - falseRho * (fJ[c] * fir)) / (T)4;
-    // the four diagonal missing populations (all but the normal one)
-#pragma unroll
-    for (int n = 0; n < 19; ++n) {
-        if (ci(n, DIR) != ORI) continue;
-        const int p = opp(n);
-        if (cx(p) * cx(p) + cy(p) * cy(p) + cz(p) * cz(p) == 1) continue;
-#pragma unroll
-        for (int iDim = 1; iDim < 3; ++iDim) {
-            const int a2 = (DIR + iDim) % 3;
-            f[p] += ci(p, a2) * diff[a2];
-        }
-    }
-}
-
-// Regularized completion: boundaryTemplates::compute_PiNeq
-// (boundaryCondition/boundaryTemplates.h:42-90) and
-// IsoThermalBulkDynamics::regularize (basicDynamics/isoThermalDynamics.hh:55-69).
-template<typename T, int DIR, int ORI>
-__device__ __forceinline__ void regularized_close(T* f, T rhoBar, const T j[3]) {
-    const T jSqr = j[0] * j[0] + j[1] * j[1] + j[2] * j[2];
-    const T ir = inv_rho(rhoBar);
-    T fNeq[19];
-#pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        fNeq[i] = T();
-        if (ci(i, DIR) == 0) fNeq[i] = f[i] - feq(i, rhoBar, ir, j, jSqr);
-        else if (ci(i, DIR) == ORI) fNeq[i] = f[i] - feq(i, rhoBar, ir, j, jSqr);
-    }
-    T pi[6];
-    int iPi = 0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-#pragma unroll
-        for (int b = a; b < 3; ++b) {
-            pi[iPi] = T();
-#pragma unroll
-            for (int i = 0; i < 19; ++i)
-                if (ci(i, DIR) == 0) pi[iPi] += ci(i, a) * ci(i, b) * fNeq[i];
-#pragma unroll
-            for (int i = 0; i < 19; ++i)
-                if (ci(i, DIR) == ORI) pi[iPi] += (T)2 * ci(i, a) * ci(i, b) * fNeq[i];
-            ++iPi;
-        }
-    }
-    f[0] = feq(0, rhoBar, ir, j, jSqr) + pi_to_fneq_generic(0, pi);
-#pragma unroll
-    for (int i = 1; i <= 9; ++i) {
-        f[i] = feq(i, rhoBar, ir, j, jSqr);
-        f[i + 9] = feq(i + 9, rhoBar, ir, j, jSqr);
-        const T fn = pi_to_fneq_generic(i, pi);
-        f[i] += fn;
-        f[i + 9] += fn;
-    }
-}
-
-template<typename T, int DIR, int ORI>
-__device__ __forceinline__ void complete_plane_dir(T* f, int family, int bckind, const double* v) {
-    T rhoBar, j[3];
-    if (bckind == PLBAMD_VELOCITY) {
-        const T u[3] = {(T)v[0], (T)v[1], (T)v[2]};
-        velocity_moments<T, DIR, ORI>(f, u, rhoBar, j);
-    } else {
-        rhoBar = (T)v[0] - (T)1;  // StoreDensityDynamics::defineDensity, boundaryDynamics.hh:251-255
-        pressure_moments<T, DIR, ORI>(f, rhoBar, j);
-    }
-    if (family == PLBAMD_ZOUHE) zouhe_close<T, DIR, ORI>(f, rhoBar, j);
-    else regularized_close<T, DIR, ORI>(f, rhoBar, j);
-}
-
-template<typename T>
-__device__ void complete_plane(T* f, uint32_t info, const double* v) {
-    const int family = (info >> 2) & 1, bckind = (info >> 3) & 1, dir = (info >> 4) & 3;
-    const bool pos = (info >> 6) & 1;
-    switch (dir * 2 + (pos ? 1 : 0)) {
-    case 0: complete_plane_dir<T, 0, -1>(f, family, bckind, v); break;
-    case 1: complete_plane_dir<T, 0, 1>(f, family, bckind, v); break;
-    case 2: complete_plane_dir<T, 1, -1>(f, family, bckind, v); break;
-    case 3: complete_plane_dir<T, 1, 1>(f, family, bckind, v); break;
-    case 4: complete_plane_dir<T, 2, -1>(f, family, bckind, v); break;
-    default: complete_plane_dir<T, 2, 1>(f, family, bckind, v); break;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Special-cell kernel: planes, momentum-exchange nodes, cells next to link-wise
-// walls. Same AA addressing as the bulk kernel, with the wall mask added to the
-// edge test.
-
-template<typename S, int DYN, bool ODD, bool STATS>
-__global__ void __launch_bounds__(256) special_step(SpecialArgs a) {
-    using T = typename Store<S>::T;
-    S* __restrict__ f = reinterpret_cast<S*>(a.f);
-    const Geometry g = a.g;
-    const T omega = (T)a.omega;
-    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < a.count) {
-        const SpecialCell sc = a.cells[k];
-        const int x = sc.x, y = sc.y, z = sc.z;
-        const uint64_t cell = sc.cell;
-        const uint32_t kind = sc.info & 3u;
-        T fc[19];
-        if constexpr (!ODD) {
-#pragma unroll
-            for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + (uint64_t)i * g.slot + cell);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + swapped_index(g, x, y, z, i, sc.wall, cell));
-        }
-
-        if (kind == SK_MOMENTUM_EXCHANGE) {
-            // bounceBackModels.hh:98-135
-            T mom[3] = {T(), T(), T()};
-#pragma unroll
-            for (int i = 1; i < 19; ++i) {
-                if (!((sc.fluid >> i) & 1u)) continue;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) mom[c] += (T)2 * ci(i, c) * fc[opp(i)];
-            }
-            st[4] += (double)(-mom[0]);
-            st[5] += (double)(-mom[1]);
-            st[6] += (double)(-mom[2]);
-#pragma unroll
-            for (int i = 1; i <= 9; ++i) { T t = fc[i]; fc[i] = fc[i + 9]; fc[i + 9] = t; }
-        } else {
-            if (kind == SK_PLANE) complete_plane(fc, sc.info, a.values + sc.value);
-            T rhoBar, drag[3];
-            const T F0[3] = {0, 0, 0};
-            const T uSqr = collide_bulk<T, DYN, false>(fc, omega, F0, (T)0, rhoBar, drag);
-            st[0] += (double)rhoBar;
-            st[1] += (double)uSqr;
-            st[2] = fmax(st[2], (double)uSqr);
-            st[3] += 1.0;
-            // link-wise wall: each outgoing population towards the wall carries 2 c_i f*_i
-            // (the stored, storage-rounded value is what is reflected)
-#pragma unroll
-            for (int i = 1; i < 19; ++i) {
-                if (!((sc.wall >> i) & 1u)) continue;
-                const T fo = Store<S>::round(fc[i]);
-#pragma unroll
-                for (int c = 0; c < 3; ++c) st[4 + c] += (double)((T)2 * ci(i, c) * fo);
-            }
-        }
-
-        if constexpr (!ODD) {
-#pragma unroll
-            for (int i = 0; i < 19; ++i) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cell, fc[i]);
-        } else {
-            Store<S>::store(f + cell, fc[0]);
-#pragma unroll
-            for (int i = 1; i < 19; ++i) {
-                int dx, dy, dz;
-                const bool ok = shift(x, cx(i), g.nx, g.px, dx) && shift(y, cy(i), g.ny, g.py, dy) &&
-                                shift(z, cz(i), g.nz, g.pz, dz) && !((sc.wall >> i) & 1u);
-                const uint64_t dst = ok ? (uint64_t)i * g.slot + lin(g, dx, dy, dz)
-                                        : (uint64_t)opp(i) * g.slot + cell;
-                Store<S>::store(f + dst, fc[i]);
-            }
-        }
-    }
-    if constexpr (STATS) block_reduce_store(st, a.partials + 8 * (uint64_t)blockIdx.x);
-}
-
-// Final statistics reduction: one workgroup, fixed order.
-__global__ void __launch_bounds__(256) reduce_partials(const double* bulk, uint64_t nBulk,
-                                                       const double* special, uint64_t nSpecial,
-                                                       double* out) {
-    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint64_t b = threadIdx.x; b < nBulk; b += blockDim.x) {
-        const double* p = bulk + 8 * b;
-        v[0] += p[0]; v[1] += p[1]; v[2] = fmax(v[2], p[2]); v[3] += p[3];
-        v[4] += p[4]; v[5] += p[5]; v[6] += p[6];
-    }
-    for (uint64_t b = threadIdx.x; b < nSpecial; b += blockDim.x) {
-        const double* p = special + 8 * b;
-        s[0] += p[0]; s[1] += p[1]; s[2] = fmax(s[2], p[2]); s[3] += p[3];
-        s[4] += p[4]; s[5] += p[5]; s[6] += p[6];
-    }
-    // record: sumRho sumU maxU count Fx Fy Fz (force from special) / Px Py Pz (porous from bulk)
-    double r[8] = {v[0] + s[0], v[1] + s[1], fmax(v[2], s[2]), v[3] + s[3], s[4], s[5], s[6], 0};
-    block_reduce_store(r, out);
-    // porous drag sums; slot 2 of a record is a max, so the z sum goes to slot 3
-    double q[8] = {v[4], v[5], 0, v[6], 0, 0, 0, 0};
-    block_reduce_store(q, out + 8);
-}
-
-// ---------------------------------------------------------------------------
-// Transfers and analysis (natural-layout view of the current parity)
-
-template<typename S>
-__device__ __forceinline__ uint64_t pop_index(const Geometry& g, int parity, int x, int y, int z, int i,
-                                              uint32_t wall, uint64_t cell) {
-    if (parity == 0) return (uint64_t)i * g.slot + cell;
-    return swapped_index(g, x, y, z, i, wall, cell);
-}
-
-struct BoxArgs {
-    Geometry g;
-    int parity;
-    int x0, y0, z0, lx, ly, lz;
-    const SpecialCell* special;   // sorted by cell, to find wall masks
-    uint32_t nSpecial;
-};
-
-__device__ __forceinline__ uint32_t wall_mask_of(const BoxArgs& b, uint64_t cell) {
-    uint32_t lo = 0, hi = b.nSpecial;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) / 2;
-        if (b.special[mid].cell < cell) lo = mid + 1; else hi = mid;
-    }
-    if (lo < b.nSpecial && b.special[lo].cell == cell) return b.special[lo].wall;
-    return 0;
-}
-
-template<typename S, typename H>
-__global__ void upload_box(void* fv, BoxArgs b, const H* host) {
-    using T = typename Store<S>::T;
-    S* f = reinterpret_cast<S*>(fv);
-    const uint64_t n = (uint64_t)b.lx * b.ly * b.lz;
-    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
-        const int lz = k % b.lz, ly = (k / b.lz) % b.ly, lx = k / ((uint64_t)b.lz * b.ly);
-        const int x = b.x0 + lx, y = b.y0 + ly, z = b.z0 + lz;
-        const uint64_t cell = lin(b.g, x, y, z);
-        const uint32_t wall = b.parity ? wall_mask_of(b, cell) : 0;
-        for (int i = 0; i < 19; ++i) {
-            const T v = (T)host[k * 19 + i];
-            Store<S>::store(f + pop_index<S>(b.g, b.parity, x, y, z, i, wall, cell), v);
-        }
-    }
-}
-
-// Raw 16-bit upload/download (bit patterns equal to the storage type).
-template<typename S>
-__global__ void upload_box_raw(void* fv, BoxArgs b, const uint16_t* host) {
-    uint16_t* f = reinterpret_cast<uint16_t*>(fv);
-    const uint64_t n = (uint64_t)b.lx * b.ly * b.lz;
-    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
-        const int lz = k % b.lz, ly = (k / b.lz) % b.ly, lx = k / ((uint64_t)b.lz * b.ly);
-        const int x = b.x0 + lx, y = b.y0 + ly, z = b.z0 + lz;
-        const uint64_t cell = lin(b.g, x, y, z);
-        const uint32_t wall = b.parity ? wall_mask_of(b, cell) : 0;
-        for (int i = 0; i < 19; ++i) f[pop_index<S>(b.g, b.parity, x, y, z, i, wall, cell)] = host[k * 19 + i];
-    }
-}
-
-template<typename S>
-__global__ void download_box_raw(const void* fv, BoxArgs b, uint16_t* host) {
-    const uint16_t* f = reinterpret_cast<const uint16_t*>(fv);
-    const uint64_t n = (uint64_t)b.lx * b.ly * b.lz;
-    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
-        const int lz = k % b.lz, ly = (k / b.lz) % b.ly, lx = k / ((uint64_t)b.lz * b.ly);
-        const int x = b.x0 + lx, y = b.y0 + ly, z = b.z0 + lz;
-        const uint64_t cell = lin(b.g, x, y, z);
-        const uint32_t wall = b.parity ? wall_mask_of(b, cell) : 0;
-        for (int i = 0; i < 19; ++i) host[k * 19 + i] = f[pop_index<S>(b.g, b.parity, x, y, z, i, wall, cell)];
-    }
-}
-
-template<typename S, typename H>
-__global__ void download_box(const void* fv, BoxArgs b, H* host) {
-    const S* f = reinterpret_cast<const S*>(fv);
-    const uint64_t n = (uint64_t)b.lx * b.ly * b.lz;
-    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
-        const int lz = k % b.lz, ly = (k / b.lz) % b.ly, lx = k / ((uint64_t)b.lz * b.ly);
-        const int x = b.x0 + lx, y = b.y0 + ly, z = b.z0 + lz;
-        const uint64_t cell = lin(b.g, x, y, z);
-        const uint32_t wall = b.parity ? wall_mask_of(b, cell) : 0;
-        for (int i = 0; i < 19; ++i)
-            host[k * 19 + i] = (H)Store<S>::load(f + pop_index<S>(b.g, b.parity, x, y, z, i, wall, cell));
-    }
-}
-
-// Completed snapshot of wrapped-factory plane cells inside a downloaded box:
-// the populations the wrapped post-stream processor leaves in Palabos's lattice.
-template<typename S, typename H>
-__global__ void complete_download(const void* fv, BoxArgs b, const double* values, H* host, int raw) {
-    using T = typename Store<S>::T;
-    const S* f = reinterpret_cast<const S*>(fv);
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < b.nSpecial; k += gridDim.x * blockDim.x) {
-        const SpecialCell sc = b.special[k];
-        if ((sc.info & 3u) != SK_PLANE || !((sc.info >> 7) & 1u)) continue;
-        const int x = sc.x, y = sc.y, z = sc.z;
-        if (x < b.x0 || x >= b.x0 + b.lx || y < b.y0 || y >= b.y0 + b.ly || z < b.z0 || z >= b.z0 + b.lz) continue;
-        T fc[19];
-        for (int i = 0; i < 19; ++i)
-            fc[i] = Store<S>::load(f + pop_index<S>(b.g, b.parity, x, y, z, i, b.parity ? sc.wall : 0u, sc.cell));
-        complete_plane(fc, sc.info, values + sc.value);
-        const uint64_t o = (((uint64_t)(x - b.x0) * b.ly + (y - b.y0)) * b.lz + (z - b.z0)) * 19;
-        for (int i = 0; i < 19; ++i) {
-            if (raw) reinterpret_cast<uint16_t*>(host)[o + i] = Store<S>::bits(fc[i]);
-            else host[o + i] = (H)Store<S>::round(fc[i]);
-        }
-    }
-}
-
-// Equilibrium initialisation (core/cell.hh:124-132 iniCellAtEquilibrium) in the
-// compute precision of the lattice.
-template<typename S>
-__global__ void equilibrium_box(void* fv, BoxArgs b, const double* rho, const double* u) {
-    using T = typename Store<S>::T;
-    S* f = reinterpret_cast<S*>(fv);
-    const uint64_t n = (uint64_t)b.lx * b.ly * b.lz;
-    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
-        const int lz = k % b.lz, ly = (k / b.lz) % b.ly, lx = k / ((uint64_t)b.lz * b.ly);
-        const int x = b.x0 + lx, y = b.y0 + ly, z = b.z0 + lz;
-        const uint64_t cell = lin(b.g, x, y, z);
-        const uint32_t wall = b.parity ? wall_mask_of(b, cell) : 0;
-        const T density = (T)rho[k];
-        const T j[3] = {(T)u[3 * k] * density, (T)u[3 * k + 1] * density, (T)u[3 * k + 2] * density};
-        const T jSqr = j[0] * j[0] + j[1] * j[1] + j[2] * j[2];
-        const T rhoBar = density - (T)1;
-        const T ir = inv_rho(rhoBar);
-        for (int i = 0; i < 19; ++i)
-            Store<S>::store(f + pop_index<S>(b.g, b.parity, x, y, z, i, wall, cell), feq(i, rhoBar, ir, j, jSqr));
-    }
-}
-
-// Per-cell density and velocity as Palabos's analysis functions see them
-// (Dynamics::computeRhoBar / computeVelocity of the cell's dynamics):
-//   bulk            rhoBar = plain sum (momentTemplates3D.h:375-382), u = j / rho
-//                   (compute_uLb, :425-432), plus F/2 with a body force
-//                   (externalForceDynamics.hh:59-74)
-//   velocity plane  rhoBar from the Dirichlet formula, u = imposed velocity
-//                   (boundaryDynamics.hh:643-671, :324-330)
-//   pressure plane  rhoBar imposed, u = j(rhoBar) / rho (boundaryDynamics.hh:776-835)
-//   solid nodes     rhoBar = 0 (rho = 1), u = 0 (core/dynamics.hh:824-829,1004-1058;
-//                   bounceBackModels.hh:170-231)
-template<typename T, int DIR, int ORI>
-__device__ __forceinline__ void plane_macroscopic(const T* f, int bckind, const double* v, T& rhoBar, T u[3]) {
-    T j[3];
-    if (bckind == PLBAMD_VELOCITY) {
-        const T uu[3] = {(T)v[0], (T)v[1], (T)v[2]};
-        velocity_moments<T, DIR, ORI>(f, uu, rhoBar, j);
-        u[0] = uu[0]; u[1] = uu[1]; u[2] = uu[2];
-    } else {
-        rhoBar = (T)v[0] - (T)1;
-        pressure_moments<T, DIR, ORI>(f, rhoBar, j);
-        const T ir = inv_rho(rhoBar);
-        u[0] = j[0] * ir; u[1] = j[1] * ir; u[2] = j[2] * ir;
-    }
-}
-
-template<typename T>
-__device__ void plane_macroscopic_any(const T* f, uint32_t info, const double* v, T& rhoBar, T u[3]) {
-    const int bckind = (info >> 3) & 1, dir = (info >> 4) & 3;
-    const bool pos = (info >> 6) & 1;
-    switch (dir * 2 + (pos ? 1 : 0)) {
-    case 0: plane_macroscopic<T, 0, -1>(f, bckind, v, rhoBar, u); break;
-    case 1: plane_macroscopic<T, 0, 1>(f, bckind, v, rhoBar, u); break;
-    case 2: plane_macroscopic<T, 1, -1>(f, bckind, v, rhoBar, u); break;
-    case 3: plane_macroscopic<T, 1, 1>(f, bckind, v, rhoBar, u); break;
-    case 4: plane_macroscopic<T, 2, -1>(f, bckind, v, rhoBar, u); break;
-    default: plane_macroscopic<T, 2, 1>(f, bckind, v, rhoBar, u); break;
-    }
-}
-
-struct MomentsArgs {
-    const uint8_t* tags;
-    const double* values;
-    double force[3];
-};
-
-template<typename S>
-__global__ void __launch_bounds__(256) moments_box(const void* fv, BoxArgs b, MomentsArgs m, double* rhoOut,
-                                                   double* uOut, double* partials) {
-    using T = typename Store<S>::T;
-    const S* f = reinterpret_cast<const S*>(fv);
-    const uint64_t n = (uint64_t)b.lx * b.ly * b.lz;
-    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
-        const int lz = k % b.lz, ly = (k / b.lz) % b.ly, lx = k / ((uint64_t)b.lz * b.ly);
-        const int x = b.x0 + lx, y = b.y0 + ly, z = b.z0 + lz;
-        const uint64_t cell = lin(b.g, x, y, z);
-        const uint8_t tag = m.tags ? m.tags[cell] : (uint8_t)BT_BULK;
-        // locate the special record of the cell, if any
-        int32_t sp = -1;
-        if (tag == BT_SKIP) {
-            uint32_t lo = 0, hi = b.nSpecial;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) / 2;
-                if (b.special[mid].cell < cell) lo = mid + 1; else hi = mid;
-            }
-            if (lo < b.nSpecial && b.special[lo].cell == cell) sp = (int32_t)lo;
-        }
-        const uint32_t wall = (b.parity && sp >= 0) ? b.special[sp].wall : 0u;
-        T rhoBar = T(), u[3] = {T(), T(), T()};
-        const bool solid = tag == BT_BOUNCE_BACK || tag == BT_NO_DYNAMICS ||
-                           (tag == BT_SKIP && (sp < 0 || (b.special[sp].info & 3u) == SK_MOMENTUM_EXCHANGE));
-        if (!solid) {
-            T fc[19];
-            for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + pop_index<S>(b.g, b.parity, x, y, z, i, wall, cell));
-            if (sp >= 0 && (b.special[sp].info & 3u) == SK_PLANE) {
-                plane_macroscopic_any(fc, b.special[sp].info, m.values + b.special[sp].value, rhoBar, u);
-            } else {
-                T rb, j[3];
-                rhobar_j(fc, rb, j);
-                rhoBar = rhobar_sum(fc);
-                const T ir = inv_rho(rhoBar);
-                for (int c = 0; c < 3; ++c) u[c] = j[c] * ir + (T)m.force[c] / (T)2;
-            }
-        }
-        if (rhoOut) rhoOut[k] = (double)(rhoBar + (T)1);
-        if (uOut) { uOut[3 * k] = (double)u[0]; uOut[3 * k + 1] = (double)u[1]; uOut[3 * k + 2] = (double)u[2]; }
-        const T uSqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-        st[0] += (double)rhoBar;
-        st[1] += (double)uSqr;
-        st[3] += (double)u[0];
-        st[4] += (double)u[1];
-        st[5] += (double)u[2];
-    }
-    if (partials) block_reduce_store(st, partials + 8 * (uint64_t)blockIdx.x);
-}
-
-__global__ void sum_records(const double* p, uint32_t n, double* out) {
-    double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t b = threadIdx.x; b < n; b += blockDim.x)
-        for (int k = 0; k < 8; ++k) if (k != 2) r[k] += p[8 * b + k];
-    block_reduce_store(r, out);
-}
-
-// ---------------------------------------------------------------------------
-// Host runtime
 
 thread_local std::string g_lastError;
 
@@ -830,49 +34,66 @@ int fail(int code, const std::string& msg) {
             return fail(PLBAMD_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+#define PLBAMD_TRY(expr)          \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_) return rc_;      \
+    } while (0)
+
+// Envelope width of multi-block lattices: the AA pattern needs two layers
+// refreshed every second step (kernels.h).
+constexpr int kEnvelope = 2;
+
 struct Plane {
     int family, kind, dir, ori;
     bool wrapped;
     plbamd_box box;
     std::vector<double> values;
-    bool valuesSet = false;
 };
 
 struct Lattice {
     plbamd_lattice_desc desc;
-    Geometry g;
-    uint64_t cells = 0;
+    bool multi = false;
+    std::vector<plbamd_box> bulk;     // global bulk box of each block
+    std::vector<BlockGeom> geom;      // device geometry of each block (host copy)
+    BlockGeom* dGeom = nullptr;
+    uint64_t allocCells = 0;          // cells of all blocks (per-cell arrays)
+    uint64_t cells = 0;               // cells of the domain
     size_t elemSize = 0;
-    void* f = nullptr;
+    void* arena = nullptr;
     int64_t time = 0;
     bool statsOn = true;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    std::vector<uint8_t> hostTags;    // PLBAMD_TAG_* or empty
+    std::vector<uint8_t> hostTags;    // PLBAMD_TAG_* per domain cell, or empty
     std::vector<Plane> planes;
-    std::vector<double> hostAlpha;
+    std::vector<double> hostAlpha;    // per domain cell, or empty
 
-    bool dirty = true;                // device tag / special structures need a rebuild
-    bool anyWrapped = false;          // some plane stands for a wrapped factory
+    bool dirty = true;                // device tags / specials / porosity need a rebuild
+    bool anyWrapped = false;
     uint8_t* dTags = nullptr;
     SpecialCell* dSpecial = nullptr;
     uint32_t nSpecial = 0;
     double* dValues = nullptr;
     double* dAlpha = nullptr;
     double* dBulkPartials = nullptr;
-    uint64_t nBulkPartials = 0;
+    uint64_t nBulkPartials = 0;       // = workgroups of the bulk launch
     double* dSpecialPartials = nullptr;
     uint64_t nSpecialPartials = 0;
     double* dStats = nullptr;         // 16 doubles
     bool statsValid = false;
 
-    dim3 bulkBlock, bulkGrid;
-    int zChunk = 0, nzChunks = 1;
+    CopySegment* dSegs = nullptr;
+    int nSegs = 0;
+    uint64_t segCells = 0;
+
+    dim3 bulkBlock;
+    int zChunk = 0;
 
     ~Lattice() {
-        hipFree(f); hipFree(dTags); hipFree(dSpecial); hipFree(dValues); hipFree(dAlpha);
-        hipFree(dBulkPartials); hipFree(dSpecialPartials); hipFree(dStats);
+        hipFree(arena); hipFree(dGeom); hipFree(dTags); hipFree(dSpecial); hipFree(dValues); hipFree(dAlpha);
+        hipFree(dBulkPartials); hipFree(dSpecialPartials); hipFree(dStats); hipFree(dSegs);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
         if (stream) hipStreamDestroy(stream);
@@ -891,22 +112,187 @@ Lattice* find(int handle) {
 
 bool boxValid(const Lattice& L, const plbamd_box& b) {
     return b.x0 >= 0 && b.y0 >= 0 && b.z0 >= 0 && b.x0 <= b.x1 && b.y0 <= b.y1 && b.z0 <= b.z1 &&
-           b.x1 < L.g.nx && b.y1 < L.g.ny && b.z1 < L.g.nz;
+           b.x1 < L.desc.nx && b.y1 < L.desc.ny && b.z1 < L.desc.nz;
 }
 
 uint64_t boxCells(const plbamd_box& b) {
     return (uint64_t)(b.x1 - b.x0 + 1) * (b.y1 - b.y0 + 1) * (b.z1 - b.z0 + 1);
 }
 
-// Builds the device tag array and the special-cell list from the host-side
-// description (tags, planes, walls).
+bool intersect(const plbamd_box& a, const plbamd_box& b, plbamd_box& out) {
+    out.x0 = std::max(a.x0, b.x0); out.x1 = std::min(a.x1, b.x1);
+    out.y0 = std::max(a.y0, b.y0); out.y1 = std::min(a.y1, b.y1);
+    out.z0 = std::max(a.z0, b.z0); out.z1 = std::min(a.z1, b.z1);
+    return out.x0 <= out.x1 && out.y0 <= out.y1 && out.z0 <= out.z1;
+}
+
+size_t storageSize(int storage) {
+    return storage == PLBAMD_F64 ? 8 : (storage == PLBAMD_F32 ? 4 : 2);
+}
+
+dim3 gridFor(uint64_t n) {
+    uint64_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b == 0) b = 1;
+    return dim3((unsigned)b);
+}
+
+int validateDesc(const plbamd_lattice_desc* desc) {
+    if (desc->nx <= 0 || desc->ny <= 0 || desc->nz <= 0 || desc->nx > 65535 || desc->ny > 65535 || desc->nz > 65535)
+        return fail(PLBAMD_EINVAL, "lattice_create: extents must be in [1, 65535]");
+    if (desc->storage < PLBAMD_F64 || desc->storage > PLBAMD_BF16)
+        return fail(PLBAMD_EINVAL, "lattice_create: unknown storage type");
+    if (desc->dynamics != PLBAMD_BGK && desc->dynamics != PLBAMD_RBGK)
+        return fail(PLBAMD_EINVAL, "lattice_create: unknown dynamics");
+    if (!(desc->omega > 0.0 && desc->omega < 2.0)) return fail(PLBAMD_EINVAL, "lattice_create: omega must lie in (0, 2)");
+    return PLBAMD_OK;
+}
+
+// Lays out the blocks in the arena, sets the launch decomposition and the
+// envelope exchange segments, and allocates the device memory.
+int setup(Lattice& L) {
+    const plbamd_lattice_desc& d = L.desc;
+    const int n[3] = {d.nx, d.ny, d.nz};
+    const int per[3] = {d.periodic[0] != 0, d.periodic[1] != 0, d.periodic[2] != 0};
+    // Launch shape: TX threads along z (a multiple of the 64-lane wavefront),
+    // CPT cells each per row segment; TY rows per workgroup when a row is short.
+    int txMax = 128, cpt = 1;
+    if (const char* e = getenv("PLBAMD_BULK_TX")) txMax = std::max(64, std::min(256, atoi(e) / 64 * 64));
+    if (const char* e = getenv("PLBAMD_BULK_CPT")) cpt = std::max(1, std::min(16, atoi(e)));
+    int maxNz = 0;
+    L.geom.clear();
+    for (const plbamd_box& b : L.bulk) {
+        BlockGeom g{};
+        const int lo[3] = {b.x0, b.y0, b.z0}, hi[3] = {b.x1, b.y1, b.z1};
+        int env0[3], env1[3];
+        for (int a = 0; a < 3; ++a) {
+            if (!L.multi) { env0[a] = env1[a] = 0; continue; }
+            env0[a] = (lo[a] > 0 || per[a]) ? kEnvelope : 0;
+            env1[a] = (hi[a] < n[a] - 1 || per[a]) ? kEnvelope : 0;
+        }
+        g.nx = hi[0] - lo[0] + 1 + env0[0] + env1[0];
+        g.ny = hi[1] - lo[1] + 1 + env0[1] + env1[1];
+        g.nz = hi[2] - lo[2] + 1 + env0[2] + env1[2];
+        g.px = L.multi ? 0 : per[0];
+        g.py = L.multi ? 0 : per[1];
+        g.pz = L.multi ? 0 : per[2];
+        g.bx0 = env0[0]; g.by0 = env0[1]; g.bz0 = env0[2];
+        g.bx1 = env0[0] + hi[0] - lo[0]; g.by1 = env0[1] + hi[1] - lo[1]; g.bz1 = env0[2] + hi[2] - lo[2];
+        g.gx0 = lo[0] - env0[0]; g.gy0 = lo[1] - env0[1]; g.gz0 = lo[2] - env0[2];
+        maxNz = std::max(maxNz, g.nz);
+        L.geom.push_back(g);
+    }
+    if (L.multi && (int64_t)L.geom.size() > 65535) return fail(PLBAMD_EINVAL, "multiblock: at most 65535 blocks");
+    const int tx = (int)std::min<int64_t>(txMax, ((int64_t)maxNz + 63) / 64 * 64);
+    const int ty = std::max(1, txMax / tx);
+    L.bulkBlock = dim3(tx, ty);
+    L.zChunk = tx * cpt;
+    uint64_t offset = 0, cellOffset = 0;
+    uint64_t wg = 0;
+    for (BlockGeom& g : L.geom) {
+        const uint64_t cells = (uint64_t)g.nx * g.ny * g.nz;
+        if (cells >= (1ull << 32)) return fail(PLBAMD_EINVAL, "lattice_create: at most 2^32 - 1 cells per block");
+        g.slot = (cells + 63) / 64 * 64;
+        g.offset = offset;
+        g.cellOffset = cellOffset;
+        g.nzChunks = (g.nz + L.zChunk - 1) / L.zChunk;
+        g.nyGroups = (g.ny + ty - 1) / ty;
+        g.wgBegin = (uint32_t)wg;
+        wg += (uint64_t)g.nx * g.nyGroups * g.nzChunks;
+        if (wg >= (1ull << 31)) return fail(PLBAMD_EINVAL, "lattice_create: lattice too large for one launch");
+        offset += g.slot * 19;
+        cellOffset += cells;
+    }
+    L.allocCells = cellOffset;
+    L.nBulkPartials = wg;
+    L.elemSize = storageSize(d.storage);
+    const size_t bytes = offset * L.elemSize;
+    if (hipMalloc(&L.arena, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(PLBAMD_ENOMEM, "lattice_create: cannot allocate " + std::to_string(bytes) + " bytes");
+    }
+    PLBAMD_HIP(hipMemset(L.arena, 0, bytes));
+    PLBAMD_HIP(hipMalloc(&L.dGeom, L.geom.size() * sizeof(BlockGeom)));
+    PLBAMD_HIP(hipMemcpy(L.dGeom, L.geom.data(), L.geom.size() * sizeof(BlockGeom), hipMemcpyHostToDevice));
+    PLBAMD_HIP(hipMalloc(&L.dBulkPartials, L.nBulkPartials * 8 * sizeof(double)));
+    PLBAMD_HIP(hipMalloc(&L.dStats, 16 * sizeof(double)));
+    PLBAMD_HIP(hipMemset(L.dStats, 0, 16 * sizeof(double)));
+
+    // Envelope segments: the allocated box of block b intersected with the bulk
+    // of every block c shifted by every periodic image (not b itself unshifted).
+    if (L.multi) {
+        std::vector<CopySegment> segs;
+        uint64_t total = 0;
+        for (size_t bi = 0; bi < L.geom.size(); ++bi) {
+            const BlockGeom& gb = L.geom[bi];
+            const int alo[3] = {gb.gx0, gb.gy0, gb.gz0};
+            const int ahi[3] = {gb.gx0 + gb.nx - 1, gb.gy0 + gb.ny - 1, gb.gz0 + gb.nz - 1};
+            for (size_t ci = 0; ci < L.geom.size(); ++ci) {
+                const plbamd_box& cb = L.bulk[ci];
+                const BlockGeom& gc = L.geom[ci];
+                for (int sx = -1; sx <= 1; ++sx)
+                    for (int sy = -1; sy <= 1; ++sy)
+                        for (int sz = -1; sz <= 1; ++sz) {
+                            const int s[3] = {sx, sy, sz};
+                            bool okShift = true;
+                            for (int a = 0; a < 3; ++a) if (s[a] != 0 && !per[a]) okShift = false;
+                            if (!okShift || (ci == bi && sx == 0 && sy == 0 && sz == 0)) continue;
+                            const int clo[3] = {cb.x0 + sx * n[0], cb.y0 + sy * n[1], cb.z0 + sz * n[2]};
+                            const int chi[3] = {cb.x1 + sx * n[0], cb.y1 + sy * n[1], cb.z1 + sz * n[2]};
+                            int olo[3], ohi[3];
+                            bool any = true;
+                            for (int a = 0; a < 3; ++a) {
+                                olo[a] = std::max(alo[a], clo[a]);
+                                ohi[a] = std::min(ahi[a], chi[a]);
+                                if (olo[a] > ohi[a]) any = false;
+                            }
+                            if (!any) continue;
+                            CopySegment seg{};
+                            seg.src = (uint16_t)ci;
+                            seg.dst = (uint16_t)bi;
+                            // source: bulk of c; local = global - shift * n - origin of c
+                            seg.sx = olo[0] - sx * n[0] - gc.gx0;
+                            seg.sy = olo[1] - sy * n[1] - gc.gy0;
+                            seg.sz = olo[2] - sz * n[2] - gc.gz0;
+                            seg.dx = olo[0] - gb.gx0;
+                            seg.dy = olo[1] - gb.gy0;
+                            seg.dz = olo[2] - gb.gz0;
+                            seg.lx = ohi[0] - olo[0] + 1;
+                            seg.ly = ohi[1] - olo[1] + 1;
+                            seg.lz = ohi[2] - olo[2] + 1;
+                            seg.cellBegin = total;
+                            total += (uint64_t)seg.lx * seg.ly * seg.lz;
+                            segs.push_back(seg);
+                        }
+            }
+        }
+        L.nSegs = (int)segs.size();
+        L.segCells = total;
+        if (!segs.empty()) {
+            PLBAMD_HIP(hipMalloc(&L.dSegs, segs.size() * sizeof(CopySegment)));
+            PLBAMD_HIP(hipMemcpy(L.dSegs, segs.data(), segs.size() * sizeof(CopySegment), hipMemcpyHostToDevice));
+        }
+    }
+    PLBAMD_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    PLBAMD_HIP(hipEventCreate(&L.ev0));
+    PLBAMD_HIP(hipEventCreate(&L.ev1));
+    return PLBAMD_OK;
+}
+
+// Builds the device tags, the special-cell list and the porosity array of every
+// block from the domain-wide description (tags, planes, porosity).
 int rebuild(Lattice& L) {
     if (!L.dirty) return PLBAMD_OK;
-    const int nx = L.g.nx, ny = L.g.ny, nz = L.g.nz;
+    const int nx = L.desc.nx, ny = L.desc.ny, nz = L.desc.nz;
+    const int per[3] = {L.desc.periodic[0] != 0, L.desc.periodic[1] != 0, L.desc.periodic[2] != 0};
     auto at = [&](int x, int y, int z) { return ((uint64_t)x * ny + y) * nz + z; };
+    auto wrap = [&](int& v, int n, int p) {
+        if (v < 0 || v >= n) { if (!p) return false; v = ((v % n) + n) % n; }
+        return true;
+    };
+    const bool anyPlane = !L.planes.empty();
     std::vector<int> planeOf;
     std::vector<uint32_t> boxIndex;
-    bool anyPlane = !L.planes.empty();
     if (anyPlane) {
         planeOf.assign(L.cells, -1);
         boxIndex.assign(L.cells, 0);
@@ -922,134 +308,192 @@ int rebuild(Lattice& L) {
         }
     }
     const bool haveTags = !L.hostTags.empty();
-    auto tagOf = [&](int x, int y, int z) -> uint8_t { return haveTags ? L.hostTags[at(x, y, z)] : 0; };
-    auto wrap = [&](int& v, int n, int per) {
-        if (v < 0 || v >= n) { if (!per) return false; v = (v + n) % n; }
-        return true;
-    };
-    std::vector<uint8_t> dev(L.cells, BT_BULK);
-    std::vector<SpecialCell> special;
+    auto tagOf = [&](uint64_t c) -> uint8_t { return haveTags ? L.hostTags[c] : (uint8_t)PLBAMD_TAG_BULK; };
     std::vector<uint32_t> planeValueBase(L.planes.size(), 0);
     std::vector<double> values;
     for (size_t p = 0; p < L.planes.size(); ++p) {
         planeValueBase[p] = (uint32_t)values.size();
         values.insert(values.end(), L.planes[p].values.begin(), L.planes[p].values.end());
     }
+    std::vector<uint8_t> dev(L.allocCells, BT_BULK);
+    std::vector<double> alpha;
+    if (!L.hostAlpha.empty()) alpha.assign(L.allocCells, 0.0);
+    std::vector<SpecialCell> special;
     bool anyNonBulk = anyPlane;
     L.anyWrapped = false;
-    for (int x = 0; x < nx; ++x)
-        for (int y = 0; y < ny; ++y)
-            for (int z = 0; z < nz; ++z) {
-                const uint64_t c = at(x, y, z);
-                const uint8_t t = tagOf(x, y, z);
-                const int p = anyPlane ? planeOf[c] : -1;
-                if (t == PLBAMD_TAG_WALL_LINK) { dev[c] = BT_SKIP; anyNonBulk = true; continue; }
-                uint32_t wall = 0, fluid = 0;
-                if (haveTags) {
-                    for (int i = 1; i < 19; ++i) {
-                        int ax = x + cx(i), ay = y + cy(i), az = z + cz(i);
-                        bool in = wrap(ax, nx, L.g.px) && wrap(ay, ny, L.g.py) && wrap(az, nz, L.g.pz);
-                        uint8_t nt = in ? tagOf(ax, ay, az) : PLBAMD_TAG_BULK;
-                        if (in && nt == PLBAMD_TAG_WALL_LINK) wall |= 1u << i;
-                        // bounceBackModels3D.hh:48-65: fluid unless BB / MEBB / NoDynamics;
-                        // beyond a non-periodic edge the envelope holds the background dynamics.
-                        if (nt != PLBAMD_TAG_MOMENTUM_EXCHANGE && nt != PLBAMD_TAG_BOUNCE_BACK &&
-                            nt != PLBAMD_TAG_NO_DYNAMICS && nt != PLBAMD_TAG_WALL_LINK)
-                            fluid |= 1u << i;
+    for (size_t bi = 0; bi < L.geom.size(); ++bi) {
+        const BlockGeom& g = L.geom[bi];
+        for (int lx = 0; lx < g.nx; ++lx)
+            for (int ly = 0; ly < g.ny; ++ly)
+                for (int lz = 0; lz < g.nz; ++lz) {
+                    int x = g.gx0 + lx, y = g.gy0 + ly, z = g.gz0 + lz;
+                    wrap(x, nx, per[0]); wrap(y, ny, per[1]); wrap(z, nz, per[2]);
+                    const uint64_t c = at(x, y, z);
+                    const uint64_t local = ((uint64_t)lx * g.ny + ly) * g.nz + lz;
+                    const uint64_t slot = g.cellOffset + local;
+                    if (!alpha.empty()) alpha[slot] = L.hostAlpha[c];
+                    const uint8_t t = tagOf(c);
+                    const int p = anyPlane ? planeOf[c] : -1;
+                    if (t == PLBAMD_TAG_WALL_LINK) { dev[slot] = BT_SKIP; anyNonBulk = true; continue; }
+                    uint32_t wall = 0, fluid = 0;
+                    if (haveTags) {
+                        for (int i = 1; i < 19; ++i) {
+                            int ax = x + cx(i), ay = y + cy(i), az = z + cz(i);
+                            const bool in = wrap(ax, nx, per[0]) && wrap(ay, ny, per[1]) && wrap(az, nz, per[2]);
+                            const uint8_t nt = in ? tagOf(at(ax, ay, az)) : (uint8_t)PLBAMD_TAG_BULK;
+                            if (in && nt == PLBAMD_TAG_WALL_LINK) wall |= 1u << i;
+                            // bounceBackModels3D.hh:48-65: fluid unless BB / MEBB / NoDynamics;
+                            // beyond a non-periodic edge the envelope holds the background dynamics.
+                            if (nt != PLBAMD_TAG_MOMENTUM_EXCHANGE && nt != PLBAMD_TAG_BOUNCE_BACK &&
+                                nt != PLBAMD_TAG_NO_DYNAMICS && nt != PLBAMD_TAG_WALL_LINK)
+                                fluid |= 1u << i;
+                        }
+                    }
+                    const bool bulk = lx >= g.bx0 && lx <= g.bx1 && ly >= g.by0 && ly <= g.by1 &&
+                                      lz >= g.bz0 && lz <= g.bz1;
+                    SpecialCell s{};
+                    s.cell = (uint32_t)local;
+                    s.x = (uint16_t)lx; s.y = (uint16_t)ly; s.z = (uint16_t)lz;
+                    s.block = (uint16_t)bi;
+                    s.wall = wall;
+                    const uint16_t bulkBit = (uint16_t)((bulk ? 1 : 0) << 8);
+                    if (p >= 0) {
+                        const Plane& P = L.planes[p];
+                        s.info = (uint16_t)(SK_PLANE | (P.family << 2) | (P.kind << 3) | (P.dir << 4) |
+                                            ((P.ori > 0 ? 1 : 0) << 6) | ((P.wrapped ? 1 : 0) << 7)) | bulkBit;
+                        s.value = planeValueBase[p] + boxIndex[c] * (P.kind == PLBAMD_VELOCITY ? 3u : 1u);
+                        if (P.wrapped) L.anyWrapped = true;
+                        dev[slot] = BT_SKIP;
+                        special.push_back(s);
+                    } else if (t == PLBAMD_TAG_MOMENTUM_EXCHANGE) {
+                        s.info = (uint16_t)SK_MOMENTUM_EXCHANGE | bulkBit;
+                        s.fluid = fluid;
+                        dev[slot] = BT_SKIP;
+                        special.push_back(s);
+                        anyNonBulk = true;
+                    } else if (t == PLBAMD_TAG_BOUNCE_BACK || t == PLBAMD_TAG_NO_DYNAMICS) {
+                        // node dynamics keep their full-way semantics next to a link wall
+                        dev[slot] = (t == PLBAMD_TAG_BOUNCE_BACK) ? BT_BOUNCE_BACK : BT_NO_DYNAMICS;
+                        anyNonBulk = true;
+                    } else if (wall) {
+                        s.info = (uint16_t)SK_NEAR_WALL | bulkBit;
+                        dev[slot] = BT_SKIP;
+                        special.push_back(s);
                     }
                 }
-                SpecialCell s{};
-                s.cell = (uint32_t)c;
-                s.x = (uint16_t)x; s.y = (uint16_t)y; s.z = (uint16_t)z;
-                s.wall = wall;
-                if (p >= 0) {
-                    const Plane& P = L.planes[p];
-                    s.info = SK_PLANE | (P.family << 2) | (P.kind << 3) | (P.dir << 4) | ((P.ori > 0 ? 1 : 0) << 6) |
-                             ((P.wrapped ? 1 : 0) << 7);
-                    if (P.wrapped) L.anyWrapped = true;
-                    s.value = planeValueBase[p] + boxIndex[c] * (P.kind == PLBAMD_VELOCITY ? 3u : 1u);
-                    dev[c] = BT_SKIP;
-                    special.push_back(s);
-                } else if (t == PLBAMD_TAG_MOMENTUM_EXCHANGE) {
-                    s.info = SK_MOMENTUM_EXCHANGE;
-                    s.fluid = fluid;
-                    dev[c] = BT_SKIP;
-                    special.push_back(s);
-                    anyNonBulk = true;
-                } else if (t == PLBAMD_TAG_BOUNCE_BACK || t == PLBAMD_TAG_NO_DYNAMICS) {
-                    dev[c] = (t == PLBAMD_TAG_BOUNCE_BACK) ? BT_BOUNCE_BACK : BT_NO_DYNAMICS;
-                    anyNonBulk = true;
-                    if (wall) {  // reflecting node next to a link wall: handled as near-wall bulk is wrong;
-                                 // such cells keep node semantics and ignore the wall
-                    }
-                } else if (wall) {
-                    s.info = SK_NEAR_WALL;
-                    dev[c] = BT_SKIP;
-                    special.push_back(s);
-                }
-            }
+    }
     hipFree(L.dTags); L.dTags = nullptr;
     hipFree(L.dSpecial); L.dSpecial = nullptr;
     hipFree(L.dValues); L.dValues = nullptr;
     hipFree(L.dSpecialPartials); L.dSpecialPartials = nullptr;
+    hipFree(L.dAlpha); L.dAlpha = nullptr;
     L.nSpecial = (uint32_t)special.size();
     if (anyNonBulk) {
-        PLBAMD_HIP(hipMalloc(&L.dTags, L.cells));
-        PLBAMD_HIP(hipMemcpy(L.dTags, dev.data(), L.cells, hipMemcpyHostToDevice));
+        PLBAMD_HIP(hipMalloc(&L.dTags, L.allocCells));
+        PLBAMD_HIP(hipMemcpy(L.dTags, dev.data(), L.allocCells, hipMemcpyHostToDevice));
     }
     if (!special.empty()) {
         PLBAMD_HIP(hipMalloc(&L.dSpecial, special.size() * sizeof(SpecialCell)));
         PLBAMD_HIP(hipMemcpy(L.dSpecial, special.data(), special.size() * sizeof(SpecialCell), hipMemcpyHostToDevice));
         L.nSpecialPartials = (special.size() + 255) / 256;
-        PLBAMD_HIP(hipMalloc(&L.dSpecialPartials, L.nSpecialPartials * 8 * sizeof(double)));
+        PLBAMD_HIP(hipMalloc(&L.dSpecialPartials, L.nSpecialPartials * 16 * sizeof(double)));
     }
     if (!values.empty()) {
         PLBAMD_HIP(hipMalloc(&L.dValues, values.size() * sizeof(double)));
         PLBAMD_HIP(hipMemcpy(L.dValues, values.data(), values.size() * sizeof(double), hipMemcpyHostToDevice));
     }
+    if (!alpha.empty()) {
+        PLBAMD_HIP(hipMalloc(&L.dAlpha, alpha.size() * sizeof(double)));
+        PLBAMD_HIP(hipMemcpy(L.dAlpha, alpha.data(), alpha.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     L.dirty = false;
     return PLBAMD_OK;
 }
 
+template<typename S>
+int exchange(Lattice& L) {
+    if (!L.multi || L.nSegs == 0) return PLBAMD_OK;
+    hipLaunchKernelGGL((exchange_envelopes<S>), gridFor(L.segCells), dim3(256), 0, L.stream, L.arena, L.dGeom,
+                       L.dSegs, L.nSegs, L.segCells);
+    PLBAMD_HIP(hipGetLastError());
+    return PLBAMD_OK;
+}
+
+// One collide-and-stream step of every block. With `record` (FP64 BGK only,
+// statistics off), the kernels also write the pre-collision moments of every
+// cell and the pre-completion populations of every special record.
+struct Recording {
+    double* moments = nullptr;
+    uint64_t momStride = 0;
+    double* specialF = nullptr;
+};
+
 template<typename S, int DYN>
-int launchStepTyped(Lattice& L, bool odd, bool stats) {
+int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = nullptr) {
     StepArgs a{};
-    a.f = L.f;
-    a.g = L.g;
+    a.arena = L.arena;
+    a.blocks = L.dGeom;
+    a.nBlocks = (int)L.geom.size();
+    a.zChunk = L.zChunk;
     a.omega = L.desc.omega;
     for (int c = 0; c < 3; ++c) a.force[c] = L.desc.force[c];
     a.tags = L.dTags;
     a.alpha = L.dAlpha;
     a.partials = L.dBulkPartials;
-    a.zChunk = L.zChunk;
-    a.nzChunks = L.nzChunks;
     const bool forced = L.dAlpha || L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0;
     const bool tags = L.dTags != nullptr;
-    // 2 x 2 x 2 x 2 dispatch over (odd, tags, stats, forced)
+    const dim3 grid((unsigned)L.nBulkPartials);
+    if (record) {
+        if constexpr (std::is_same<S, double>::value && DYN == 0) {
+            if (stats) return fail(PLBAMD_EINVAL, "recording step: statistics must be off");
+            a.moments = record->moments;
+            a.momStride = record->momStride;
+#define PLBAMD_REC(O, TG, FO) \
+    hipLaunchKernelGGL((bulk_step<double, 0, O, TG, false, FO, true>), grid, L.bulkBlock, 0, L.stream, a)
+#define PLBAMD_REC_F(O, TG) if (forced) PLBAMD_REC(O, TG, true); else PLBAMD_REC(O, TG, false)
+#define PLBAMD_REC_T(O) if (tags) { PLBAMD_REC_F(O, true); } else { PLBAMD_REC_F(O, false); }
+            if (odd) { PLBAMD_REC_T(true) } else { PLBAMD_REC_T(false) }
+#undef PLBAMD_REC_T
+#undef PLBAMD_REC_F
+#undef PLBAMD_REC
+        } else {
+            return fail(PLBAMD_EINVAL, "recording step: FP64 BGK only");
+        }
+    } else {
 #define PLBAMD_BULK(O, TG, ST, FO) \
-    hipLaunchKernelGGL((bulk_step<S, DYN, O, TG, ST, FO>), L.bulkGrid, L.bulkBlock, 0, L.stream, a)
+    hipLaunchKernelGGL((bulk_step<S, DYN, O, TG, ST, FO>), grid, L.bulkBlock, 0, L.stream, a)
 #define PLBAMD_BULK_F(O, TG, ST) if (forced) PLBAMD_BULK(O, TG, ST, true); else PLBAMD_BULK(O, TG, ST, false)
 #define PLBAMD_BULK_S(O, TG) if (stats) { PLBAMD_BULK_F(O, TG, true); } else { PLBAMD_BULK_F(O, TG, false); }
 #define PLBAMD_BULK_T(O) if (tags) { PLBAMD_BULK_S(O, true) } else { PLBAMD_BULK_S(O, false) }
-    if (odd) { PLBAMD_BULK_T(true) } else { PLBAMD_BULK_T(false) }
+        if (odd) { PLBAMD_BULK_T(true) } else { PLBAMD_BULK_T(false) }
 #undef PLBAMD_BULK_T
 #undef PLBAMD_BULK_S
 #undef PLBAMD_BULK_F
 #undef PLBAMD_BULK
+    }
     PLBAMD_HIP(hipGetLastError());
     if (L.nSpecial) {
         SpecialArgs s{};
-        s.f = L.f; s.g = L.g; s.omega = L.desc.omega;
+        s.arena = L.arena; s.blocks = L.dGeom; s.omega = L.desc.omega;
+        for (int c = 0; c < 3; ++c) s.force[c] = L.desc.force[c];
+        s.alpha = L.dAlpha;
         s.cells = L.dSpecial; s.count = L.nSpecial; s.values = L.dValues;
         s.partials = L.dSpecialPartials;
-        dim3 grid((L.nSpecial + 255) / 256), block(256);
-        if (odd) {
-            if (stats) hipLaunchKernelGGL((special_step<S, DYN, true, true>), grid, block, 0, L.stream, s);
-            else hipLaunchKernelGGL((special_step<S, DYN, true, false>), grid, block, 0, L.stream, s);
-        } else {
-            if (stats) hipLaunchKernelGGL((special_step<S, DYN, false, true>), grid, block, 0, L.stream, s);
-            else hipLaunchKernelGGL((special_step<S, DYN, false, false>), grid, block, 0, L.stream, s);
+        if (record) {
+            s.moments = record->moments;
+            s.momStride = record->momStride;
+            s.dumpF = record->specialF;
         }
+        dim3 sg((L.nSpecial + 255) / 256), sb(256);
+#define PLBAMD_SPECIAL(O, ST, FO) hipLaunchKernelGGL((special_step<S, DYN, O, ST, FO>), sg, sb, 0, L.stream, s)
+#define PLBAMD_SPECIAL_F(O, ST) if (forced) PLBAMD_SPECIAL(O, ST, true); else PLBAMD_SPECIAL(O, ST, false)
+        if (odd) {
+            if (stats) { PLBAMD_SPECIAL_F(true, true); } else { PLBAMD_SPECIAL_F(true, false); }
+        } else {
+            if (stats) { PLBAMD_SPECIAL_F(false, true); } else { PLBAMD_SPECIAL_F(false, false); }
+        }
+#undef PLBAMD_SPECIAL_F
+#undef PLBAMD_SPECIAL
         PLBAMD_HIP(hipGetLastError());
     }
     if (stats) {
@@ -1057,6 +501,7 @@ int launchStepTyped(Lattice& L, bool odd, bool stats) {
                            L.dSpecialPartials, L.nSpecial ? L.nSpecialPartials : 0, L.dStats);
         PLBAMD_HIP(hipGetLastError());
     }
+    if (odd) PLBAMD_TRY(exchange<S>(L));
     return PLBAMD_OK;
 }
 
@@ -1067,11 +512,11 @@ int launchStep(Lattice& L, bool odd, bool stats) {
 }
 
 int step(Lattice& L, int nsteps) {
-    int rc = rebuild(L);
-    if (rc) return rc;
+    PLBAMD_TRY(rebuild(L));
     for (int s = 0; s < nsteps; ++s) {
         const bool odd = (L.time & 1) != 0;
         const bool stats = L.statsOn && s == nsteps - 1;
+        int rc;
         switch (L.desc.storage) {
         case PLBAMD_F64: rc = launchStep<double>(L, odd, stats); break;
         case PLBAMD_F32: rc = launchStep<float>(L, odd, stats); break;
@@ -1085,22 +530,22 @@ int step(Lattice& L, int nsteps) {
     return PLBAMD_OK;
 }
 
-BoxArgs boxArgs(const Lattice& L, const plbamd_box& b) {
-    BoxArgs a{};
-    a.g = L.g;
+// Transfer arguments for block bi and the part of the global box `g` it owns.
+bool boxArgs(const Lattice& L, size_t bi, const plbamd_box& g, BoxArgs& a) {
+    plbamd_box I;
+    if (!intersect(g, L.bulk[bi], I)) return false;
+    const BlockGeom& bg = L.geom[bi];
+    a = BoxArgs{};
+    a.g = bg;
+    a.block = (int)bi;
     a.parity = (int)(L.time & 1);
-    a.x0 = b.x0; a.y0 = b.y0; a.z0 = b.z0;
-    a.lx = b.x1 - b.x0 + 1; a.ly = b.y1 - b.y0 + 1; a.lz = b.z1 - b.z0 + 1;
+    a.ix0 = I.x0 - bg.gx0; a.iy0 = I.y0 - bg.gy0; a.iz0 = I.z0 - bg.gz0;
+    a.ilx = I.x1 - I.x0 + 1; a.ily = I.y1 - I.y0 + 1; a.ilz = I.z1 - I.z0 + 1;
+    a.ox = I.x0 - g.x0; a.oy = I.y0 - g.y0; a.oz = I.z0 - g.z0;
+    a.sly = g.y1 - g.y0 + 1; a.slz = g.z1 - g.z0 + 1;
     a.special = L.dSpecial;
     a.nSpecial = L.nSpecial;
-    return a;
-}
-
-dim3 gridFor(uint64_t n) {
-    uint64_t b = (n + 255) / 256;
-    if (b > 65536) b = 65536;
-    if (b == 0) b = 1;
-    return dim3((unsigned)b);
+    return true;
 }
 
 template<typename S>
@@ -1110,38 +555,40 @@ int transfer(Lattice& L, const plbamd_box& b, int elem, void* data, size_t bytes
     if (bytes != n * 19 * hostElem) return fail(PLBAMD_EINVAL, "transfer: byte count does not match the box");
     if ((elem == PLBAMD_F16 || elem == PLBAMD_BF16) && elem != L.desc.storage)
         return fail(PLBAMD_EINVAL, "transfer: 16-bit host data must match the storage type");
-    int rc = rebuild(L);
-    if (rc) return rc;
+    if (up && L.multi && (L.time & 1))
+        return fail(PLBAMD_ESTATE, "upload: a multi-block lattice accepts populations after an even number of steps");
+    PLBAMD_TRY(rebuild(L));
     void* staging = nullptr;
     PLBAMD_HIP(hipMalloc(&staging, bytes));
-    BoxArgs a = boxArgs(L, b);
-    dim3 grid = gridFor(n), block(256);
     hipError_t err = hipSuccess;
-    if (up) {
-        err = hipMemcpyAsync(staging, data, bytes, hipMemcpyHostToDevice, L.stream);
-        if (err == hipSuccess) {
-            if (elem == PLBAMD_F64) hipLaunchKernelGGL((upload_box<S, double>), grid, block, 0, L.stream, L.f, a, (const double*)staging);
-            else if (elem == PLBAMD_F32) hipLaunchKernelGGL((upload_box<S, float>), grid, block, 0, L.stream, L.f, a, (const float*)staging);
-            else hipLaunchKernelGGL((upload_box_raw<S>), grid, block, 0, L.stream, L.f, a, (const uint16_t*)staging);
+    if (up) err = hipMemcpyAsync(staging, data, bytes, hipMemcpyHostToDevice, L.stream);
+    for (size_t bi = 0; bi < L.geom.size() && err == hipSuccess; ++bi) {
+        BoxArgs a;
+        if (!boxArgs(L, bi, b, a)) continue;
+        const dim3 grid = gridFor((uint64_t)a.ilx * a.ily * a.ilz), block(256);
+        if (up) {
+            if (elem == PLBAMD_F64) hipLaunchKernelGGL((upload_box<S, double>), grid, block, 0, L.stream, L.arena, a, (const double*)staging);
+            else if (elem == PLBAMD_F32) hipLaunchKernelGGL((upload_box<S, float>), grid, block, 0, L.stream, L.arena, a, (const float*)staging);
+            else hipLaunchKernelGGL((upload_box_raw<S>), grid, block, 0, L.stream, L.arena, a, (const uint16_t*)staging);
+        } else {
+            if (elem == PLBAMD_F64) hipLaunchKernelGGL((download_box<S, double>), grid, block, 0, L.stream, L.arena, a, (double*)staging);
+            else if (elem == PLBAMD_F32) hipLaunchKernelGGL((download_box<S, float>), grid, block, 0, L.stream, L.arena, a, (float*)staging);
+            else hipLaunchKernelGGL((download_box_raw<S>), grid, block, 0, L.stream, L.arena, a, (uint16_t*)staging);
             err = hipGetLastError();
+            if (err == hipSuccess && L.anyWrapped && L.nSpecial) {
+                const dim3 g2 = gridFor(L.nSpecial);
+                if (elem == PLBAMD_F64)
+                    hipLaunchKernelGGL((complete_download<S, double>), g2, block, 0, L.stream, L.arena, a, L.dValues, (double*)staging, 0);
+                else if (elem == PLBAMD_F32)
+                    hipLaunchKernelGGL((complete_download<S, float>), g2, block, 0, L.stream, L.arena, a, L.dValues, (float*)staging, 0);
+                else
+                    hipLaunchKernelGGL((complete_download<S, float>), g2, block, 0, L.stream, L.arena, a, L.dValues, (float*)staging, 1);
+            }
         }
-    } else {
-        if (elem == PLBAMD_F64) hipLaunchKernelGGL((download_box<S, double>), grid, block, 0, L.stream, L.f, a, (double*)staging);
-        else if (elem == PLBAMD_F32) hipLaunchKernelGGL((download_box<S, float>), grid, block, 0, L.stream, L.f, a, (float*)staging);
-        else hipLaunchKernelGGL((download_box_raw<S>), grid, block, 0, L.stream, L.f, a, (uint16_t*)staging);
-        err = hipGetLastError();
-        if (err == hipSuccess && L.anyWrapped && L.nSpecial) {
-            dim3 g2 = gridFor(L.nSpecial);
-            if (elem == PLBAMD_F64)
-                hipLaunchKernelGGL((complete_download<S, double>), g2, block, 0, L.stream, L.f, a, L.dValues, (double*)staging, 0);
-            else if (elem == PLBAMD_F32)
-                hipLaunchKernelGGL((complete_download<S, float>), g2, block, 0, L.stream, L.f, a, L.dValues, (float*)staging, 0);
-            else
-                hipLaunchKernelGGL((complete_download<S, float>), g2, block, 0, L.stream, L.f, a, L.dValues, (float*)staging, 1);
-            err = hipGetLastError();
-        }
-        if (err == hipSuccess) err = hipMemcpyAsync(data, staging, bytes, hipMemcpyDeviceToHost, L.stream);
+        if (err == hipSuccess) err = hipGetLastError();
     }
+    if (err == hipSuccess && up && exchange<S>(L) != PLBAMD_OK) err = hipErrorUnknown;
+    if (err == hipSuccess && !up) err = hipMemcpyAsync(data, staging, bytes, hipMemcpyDeviceToHost, L.stream);
     if (err == hipSuccess) err = hipStreamSynchronize(L.stream);
     hipFree(staging);
     if (err != hipSuccess) return fail(PLBAMD_EHIP, std::string("transfer: ") + hipGetErrorString(err));
@@ -1151,16 +598,21 @@ int transfer(Lattice& L, const plbamd_box& b, int elem, void* data, size_t bytes
 template<typename S>
 int equilibrium(Lattice& L, const plbamd_box& b, const double* rho, const double* u) {
     const uint64_t n = boxCells(b);
-    int rc = rebuild(L);
-    if (rc) return rc;
+    if (L.multi && (L.time & 1))
+        return fail(PLBAMD_ESTATE, "initialize_at_equilibrium: a multi-block lattice accepts populations after an even number of steps");
+    PLBAMD_TRY(rebuild(L));
     double* d = nullptr;
     PLBAMD_HIP(hipMalloc(&d, n * 4 * sizeof(double)));
     hipError_t err = hipMemcpyAsync(d, rho, n * sizeof(double), hipMemcpyHostToDevice, L.stream);
     if (err == hipSuccess) err = hipMemcpyAsync(d + n, u, 3 * n * sizeof(double), hipMemcpyHostToDevice, L.stream);
-    if (err == hipSuccess) {
-        hipLaunchKernelGGL((equilibrium_box<S>), gridFor(n), dim3(256), 0, L.stream, L.f, boxArgs(L, b), d, d + n);
+    for (size_t bi = 0; bi < L.geom.size() && err == hipSuccess; ++bi) {
+        BoxArgs a;
+        if (!boxArgs(L, bi, b, a)) continue;
+        hipLaunchKernelGGL((equilibrium_box<S>), gridFor((uint64_t)a.ilx * a.ily * a.ilz), dim3(256), 0, L.stream,
+                           L.arena, a, d, d + n);
         err = hipGetLastError();
     }
+    if (err == hipSuccess && exchange<S>(L) != PLBAMD_OK) err = hipErrorUnknown;
     if (err == hipSuccess) err = hipStreamSynchronize(L.stream);
     hipFree(d);
     if (err != hipSuccess) return fail(PLBAMD_EHIP, std::string("equilibrium: ") + hipGetErrorString(err));
@@ -1170,25 +622,38 @@ int equilibrium(Lattice& L, const plbamd_box& b, const double* rho, const double
 template<typename S>
 int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sums) {
     const uint64_t n = boxCells(b);
-    int rc = rebuild(L);
-    if (rc) return rc;
-    dim3 grid = gridFor(n);
+    PLBAMD_TRY(rebuild(L));
+    std::vector<BoxArgs> parts;
+    uint64_t nRecords = 0;
+    for (size_t bi = 0; bi < L.geom.size(); ++bi) {
+        BoxArgs a;
+        if (boxArgs(L, bi, b, a)) {
+            parts.push_back(a);
+            nRecords += gridFor((uint64_t)a.ilx * a.ily * a.ilz).x;
+        }
+    }
+    const size_t fieldDoubles = (rho || u) ? n * 4 : 0;
     double* d = nullptr;
-    const size_t fieldBytes = (rho || u) ? n * 4 * sizeof(double) : 0;
-    const size_t partBytes = (size_t)grid.x * 8 * sizeof(double) + 8 * sizeof(double);
-    PLBAMD_HIP(hipMalloc(&d, fieldBytes + partBytes));
+    PLBAMD_HIP(hipMalloc(&d, (fieldDoubles + nRecords * 8 + 8) * sizeof(double)));
     double* dRho = rho ? d : nullptr;
     double* dU = u ? d + n : nullptr;
-    double* dPart = d + fieldBytes / sizeof(double);
-    double* dSum = dPart + (size_t)grid.x * 8;
+    double* dPart = d + fieldDoubles;
+    double* dSum = dPart + nRecords * 8;
     MomentsArgs m{};
     m.tags = L.dTags;
     m.values = L.dValues;
     for (int c = 0; c < 3; ++c) m.force[c] = L.desc.force[c];
-    hipLaunchKernelGGL((moments_box<S>), grid, dim3(256), 0, L.stream, L.f, boxArgs(L, b), m, dRho, dU, dPart);
-    hipError_t err = hipGetLastError();
+    hipError_t err = hipSuccess;
+    uint64_t rec = 0;
+    for (const BoxArgs& a : parts) {
+        const dim3 grid = gridFor((uint64_t)a.ilx * a.ily * a.ilz);
+        hipLaunchKernelGGL((moments_box<S>), grid, dim3(256), 0, L.stream, L.arena, a, m, dRho, dU, dPart + rec * 8);
+        rec += grid.x;
+        err = hipGetLastError();
+        if (err != hipSuccess) break;
+    }
     if (err == hipSuccess) {
-        hipLaunchKernelGGL(sum_records, dim3(1), dim3(256), 0, L.stream, dPart, grid.x, dSum);
+        hipLaunchKernelGGL(sum_records, dim3(1), dim3(256), 0, L.stream, dPart, (uint32_t)nRecords, dSum);
         err = hipGetLastError();
     }
     double host[8];
@@ -1208,8 +673,201 @@ int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sum
     return PLBAMD_OK;
 }
 
-size_t storageSize(int storage) {
-    return storage == PLBAMD_F64 ? 8 : (storage == PLBAMD_F32 ? 4 : 2);
+
+// ---------------------------------------------------------------------------
+// Adjoint gradient (adjoint.h): forward T steps with a checkpoint every K
+// steps, then segments backwards: recompute the segment from its checkpoint
+// with the forward kernels, recording per step the pre-collision moments
+// (rhoBar, j) of every cell and the populations of the plane cells, then K
+// reverse steps.
+
+int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, double* Jout, double* gradAlpha,
+                    double* gradScale, double* timing) {
+    if (L.multi) return fail(PLBAMD_EINVAL, "adjoint: single-block lattices only");
+    if (L.desc.storage != PLBAMD_F64) return fail(PLBAMD_EINVAL, "adjoint: FP64 storage only");
+    if (L.desc.dynamics != PLBAMD_BGK) return fail(PLBAMD_EINVAL, "adjoint: BGK dynamics only");
+    if (L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0)
+        return fail(PLBAMD_EINVAL, "adjoint: no body force");
+    if (!L.hostTags.empty()) return fail(PLBAMD_EINVAL, "adjoint: no per-cell tags (walls are the domain edges)");
+    if (T <= 0 || K <= 0 || (K & 1)) return fail(PLBAMD_EINVAL, "adjoint: steps > 0 and an even checkpoint interval");
+    if (L.time & 1) return fail(PLBAMD_ESTATE, "adjoint: start after an even number of steps");
+    if (obj.inlet_x < 0 || obj.inlet_x >= L.desc.nx || obj.outlet_x < 0 || obj.outlet_x >= L.desc.nx)
+        return fail(PLBAMD_EINVAL, "adjoint: measuring planes outside the lattice");
+    if (!L.hostAlpha.empty())
+        for (const Plane& p : L.planes) {
+            const plbamd_box& b = p.box;
+            for (int x = b.x0; x <= b.x1; ++x)
+                for (int y = b.y0; y <= b.y1; ++y)
+                    for (int z = b.z0; z <= b.z1; ++z)
+                        if (L.hostAlpha[((uint64_t)x * L.desc.ny + y) * L.desc.nz + z] != 0.0)
+                            return fail(PLBAMD_EINVAL, "adjoint: porosity must vanish on boundary planes");
+        }
+    PLBAMD_TRY(rebuild(L));
+    const BlockGeom& g = L.geom[0];
+    const uint64_t stateElems = g.slot * 19;
+    const size_t stateBytes = stateElems * sizeof(double);
+    const int nCk = (T + K - 1) / K;
+    const uint64_t cells = (uint64_t)g.nx * g.ny * g.nz;
+    const uint64_t momElems = 4 * cells;                       // per recorded step
+    const uint64_t pfElems = 19 * (uint64_t)std::max<uint32_t>(1, L.nSpecial);
+    const dim3 ablock(64, 4);
+    const dim3 agrid((unsigned)((g.nz + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)g.nx);
+    const uint64_t cellWGs = (uint64_t)agrid.x * agrid.y * agrid.z;
+    const uint32_t planeWGs = (uint32_t)((20 * (uint64_t)L.nSpecial + 255) / 256);
+    const uint64_t recordsPerStep = cellWGs + planeWGs;
+    // device memory: checkpoints, a working state, the recorded moments and plane
+    // populations of one segment, two adjoint states, gradients, partial sums
+    double *ck = nullptr, *work = nullptr, *mom = nullptr, *pf = nullptr, *lamA = nullptr, *lamB = nullptr,
+           *gA = nullptr, *gS = nullptr, *part = nullptr, *sum = nullptr;
+    auto release = [&]() {
+        hipFree(ck); hipFree(work); hipFree(mom); hipFree(pf); hipFree(lamA); hipFree(lamB); hipFree(gA);
+        hipFree(gS); hipFree(part); hipFree(sum);
+    };
+    const uint64_t nRecords = recordsPerStep * (uint64_t)(T + 1);
+    hipError_t err = hipMalloc(&ck, stateBytes * nCk);
+    if (err == hipSuccess) err = hipMalloc(&work, stateBytes);
+    if (err == hipSuccess) err = hipMalloc(&mom, momElems * (K + 1) * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc(&pf, pfElems * (K + 1) * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc(&lamA, stateBytes);
+    if (err == hipSuccess) err = hipMalloc(&lamB, stateBytes);
+    if (err == hipSuccess) err = hipMalloc(&gA, cells * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc(&gS, std::max<uint32_t>(1, L.nSpecial) * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc(&part, nRecords * 8 * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc(&sum, 8 * sizeof(double));
+    if (err != hipSuccess) {
+        (void)hipGetLastError();
+        release();
+        return fail(PLBAMD_ENOMEM, "adjoint: cannot allocate the checkpoint and adjoint buffers");
+    }
+    if (err == hipSuccess) err = hipMemsetAsync(gA, 0, cells * sizeof(double), L.stream);
+    if (err == hipSuccess) err = hipMemsetAsync(gS, 0, std::max<uint32_t>(1, L.nSpecial) * sizeof(double), L.stream);
+    if (err == hipSuccess) err = hipMemsetAsync(part, 0, nRecords * 8 * sizeof(double), L.stream);
+    const bool saveStats = L.statsOn;
+    L.statsOn = false;
+    float msF = 0, msB = 0;
+    // forward, with checkpoints (even time: the AA array is in natural layout)
+    if (err == hipSuccess) err = hipEventRecord(L.ev0, L.stream);
+    for (int c = 0; c < nCk && err == hipSuccess; ++c) {
+        err = hipMemcpyAsync(ck + (uint64_t)c * stateElems, L.arena, stateBytes, hipMemcpyDeviceToDevice, L.stream);
+        if (err != hipSuccess) break;
+        if (step(L, std::min(K, T - c * K)) != PLBAMD_OK) err = hipErrorUnknown;
+    }
+    if (err == hipSuccess) err = hipEventRecord(L.ev1, L.stream);
+    if (err == hipSuccess) err = hipEventSynchronize(L.ev1);
+    if (err == hipSuccess) err = hipEventElapsedTime(&msF, L.ev0, L.ev1);
+    // backward
+    AdjointArgs a{};
+    a.momStride = cells;
+    a.alpha = L.dAlpha;
+    a.gAlpha = gA;
+    a.tags = L.dTags;
+    a.g = g;
+    a.omega = L.desc.omega;
+    a.invT = 1.0 / T;
+    a.wdp = obj.weight_pressure_drop;
+    a.wdrag = obj.weight_drag;
+    a.inW = (1.0 / 3.0) / ((double)g.ny * g.nz);
+    a.outW = (1.0 / 3.0) / ((double)g.ny * g.nz);
+    a.xin = obj.inlet_x;
+    a.xout = obj.outlet_x;
+    uint64_t rec = 0;
+    double* lamNext = lamA;
+    double* lamCur = lamB;
+    void* const arena = L.arena;
+    const int64_t time0 = L.time;
+    L.arena = work;   // the recompute runs the step kernels on the working state
+    if (err == hipSuccess) err = hipEventRecord(L.ev0, L.stream);
+    for (int c = nCk - 1; c >= 0 && err == hipSuccess; --c) {
+        const int t0 = c * K, len = std::min(K, T - t0);
+        err = hipMemcpyAsync(work, ck + (uint64_t)c * stateElems, stateBytes, hipMemcpyDeviceToDevice, L.stream);
+        // steps t0 .. t0 + len - 1 (and, in the last segment, one more for f^T)
+        const int nRec = (c == nCk - 1) ? len + 1 : len;
+        for (int m = 0; m < nRec && err == hipSuccess; ++m) {
+            Recording r;
+            r.moments = mom + (uint64_t)m * momElems;
+            r.momStride = cells;
+            r.specialF = L.nSpecial ? pf + (uint64_t)m * pfElems : nullptr;
+            if (launchStepTyped<double, 0>(L, (m & 1) != 0, false, &r) != PLBAMD_OK) err = hipErrorUnknown;
+        }
+        if (err == hipSuccess && c == nCk - 1) {
+            a.mom = mom + (uint64_t)len * momElems;
+            a.lam = lamNext;
+            a.partials = part + rec * 8;
+            hipLaunchKernelGGL(adjoint_final, agrid, ablock, 0, L.stream, a);
+            rec += cellWGs;
+            err = hipGetLastError();
+        }
+        for (int m = len - 1; m >= 0 && err == hipSuccess; --m) {
+            const int t = t0 + m;
+            a.mom = mom + (uint64_t)m * momElems;
+            a.planeF = pf + (uint64_t)m * pfElems;
+            a.lamNext = lamNext;
+            a.lam = lamCur;
+            a.explicitTerm = t >= 1 ? 1 : 0;
+            a.partials = part + rec * 8;
+            hipLaunchKernelGGL(adjoint_bulk, agrid, ablock, 0, L.stream, a);
+            rec += cellWGs;
+            err = hipGetLastError();
+            if (err == hipSuccess && L.nSpecial) {
+                AdjointSpecialArgs sa{};
+                sa.base = a;
+                sa.cells = L.dSpecial;
+                sa.count = L.nSpecial;
+                sa.values = L.dValues;
+                sa.partials = part + rec * 8;
+                sa.gScale = gS;
+                hipLaunchKernelGGL(adjoint_planes, dim3(planeWGs), dim3(256), 0, L.stream, sa);
+                rec += planeWGs;
+                err = hipGetLastError();
+            }
+            std::swap(lamNext, lamCur);
+        }
+    }
+    L.arena = arena;
+    L.time = time0;
+    if (err == hipSuccess) err = hipEventRecord(L.ev1, L.stream);
+    if (err == hipSuccess) err = hipEventSynchronize(L.ev1);
+    if (err == hipSuccess) err = hipEventElapsedTime(&msB, L.ev0, L.ev1);
+    L.statsOn = saveStats;
+    if (err == hipSuccess) {
+        hipLaunchKernelGGL(sum_records, dim3(1), dim3(256), 0, L.stream, part, (uint32_t)rec, sum);
+        err = hipGetLastError();
+    }
+    double hsum[8] = {0};
+    if (err == hipSuccess) err = hipMemcpyAsync(hsum, sum, sizeof(hsum), hipMemcpyDeviceToHost, L.stream);
+    std::vector<double> hgA(cells), hgS(std::max<uint32_t>(1, L.nSpecial));
+    if (err == hipSuccess && gradAlpha) err = hipMemcpyAsync(gradAlpha, gA, cells * sizeof(double), hipMemcpyDeviceToHost, L.stream);
+    if (err == hipSuccess) err = hipMemcpyAsync(hgS.data(), gS, hgS.size() * sizeof(double), hipMemcpyDeviceToHost, L.stream);
+    std::vector<SpecialCell> hs(L.nSpecial);
+    if (err == hipSuccess && L.nSpecial)
+        err = hipMemcpyAsync(hs.data(), L.dSpecial, L.nSpecial * sizeof(SpecialCell), hipMemcpyDeviceToHost, L.stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(L.stream);
+    release();
+    if (err != hipSuccess) return fail(PLBAMD_EHIP, std::string("adjoint: ") + hipGetErrorString(err));
+    if (Jout) *Jout = hsum[0];
+    if (gradScale) {
+        // plane of each special record from its value index
+        std::vector<uint32_t> base(L.planes.size() + 1, 0);
+        for (size_t p = 0; p < L.planes.size(); ++p) base[p + 1] = base[p] + (uint32_t)L.planes[p].values.size();
+        for (size_t p = 0; p < L.planes.size(); ++p) gradScale[p] = 0.0;
+        for (uint32_t k = 0; k < L.nSpecial; ++k) {
+            if ((hs[k].info & 3u) != SK_PLANE) continue;
+            const size_t p = std::upper_bound(base.begin(), base.end(), hs[k].value) - base.begin() - 1;
+            gradScale[p] += hgS[k];
+        }
+    }
+    if (timing) {
+        timing[0] = msF;
+        timing[1] = msB;
+    }
+    return PLBAMD_OK;
+}
+
+int registerLattice(std::unique_ptr<Lattice> L, int* handle) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    *handle = g_nextHandle++;
+    g_lattices[*handle] = std::move(L);
+    return PLBAMD_OK;
 }
 
 }  // namespace plbamd
@@ -1230,57 +888,52 @@ const char* plbamd_last_error(void) { return g_lastError.c_str(); }
 
 int plbamd_lattice_create(const plbamd_lattice_desc* desc, int* handle) {
     if (!desc || !handle) return fail(PLBAMD_EINVAL, "lattice_create: null argument");
-    if (desc->nx <= 0 || desc->ny <= 0 || desc->nz <= 0 || desc->nx > 65535 || desc->ny > 65535 || desc->nz > 65535)
-        return fail(PLBAMD_EINVAL, "lattice_create: extents must be in [1, 65535]");
-    if (desc->storage < PLBAMD_F64 || desc->storage > PLBAMD_BF16)
-        return fail(PLBAMD_EINVAL, "lattice_create: unknown storage type");
-    if (desc->dynamics != PLBAMD_BGK && desc->dynamics != PLBAMD_RBGK)
-        return fail(PLBAMD_EINVAL, "lattice_create: unknown dynamics");
-    if (!(desc->omega > 0.0 && desc->omega < 2.0)) return fail(PLBAMD_EINVAL, "lattice_create: omega must lie in (0, 2)");
-    const uint64_t cells = (uint64_t)desc->nx * desc->ny * desc->nz;
-    if (cells >= (1ull << 32)) return fail(PLBAMD_EINVAL, "lattice_create: at most 2^32 - 1 cells");
+    PLBAMD_TRY(validateDesc(desc));
     auto L = std::make_unique<Lattice>();
     L->desc = *desc;
-    L->g.nx = desc->nx; L->g.ny = desc->ny; L->g.nz = desc->nz;
-    L->g.px = desc->periodic[0] != 0; L->g.py = desc->periodic[1] != 0; L->g.pz = desc->periodic[2] != 0;
-    L->cells = cells;
-    L->g.slot = (cells + 63) / 64 * 64;
-    L->elemSize = storageSize(desc->storage);
-    const size_t bytes = L->g.slot * 19 * L->elemSize;
-    if (hipMalloc(&L->f, bytes) != hipSuccess) {
-        hipGetLastError();
-        return fail(PLBAMD_ENOMEM, "lattice_create: cannot allocate " + std::to_string(bytes) + " bytes");
+    L->multi = false;
+    L->cells = (uint64_t)desc->nx * desc->ny * desc->nz;
+    plbamd_box all{0, desc->nx - 1, 0, desc->ny - 1, 0, desc->nz - 1};
+    L->bulk.push_back(all);
+    PLBAMD_TRY(setup(*L));
+    return registerLattice(std::move(L), handle);
+}
+
+int plbamd_multiblock_create(const plbamd_lattice_desc* desc, const plbamd_box* blocks, int nblocks, int* handle) {
+    if (!desc || !handle || !blocks || nblocks <= 0) return fail(PLBAMD_EINVAL, "multiblock_create: bad argument");
+    PLBAMD_TRY(validateDesc(desc));
+    const uint64_t cells = (uint64_t)desc->nx * desc->ny * desc->nz;
+    // the blocks must tile the domain: inside, no overlap, every cell covered
+    std::vector<uint8_t> cover(cells, 0);
+    for (int b = 0; b < nblocks; ++b) {
+        const plbamd_box& B = blocks[b];
+        if (B.x0 < 0 || B.y0 < 0 || B.z0 < 0 || B.x0 > B.x1 || B.y0 > B.y1 || B.z0 > B.z1 || B.x1 >= desc->nx ||
+            B.y1 >= desc->ny || B.z1 >= desc->nz)
+            return fail(PLBAMD_EINVAL, "multiblock_create: block " + std::to_string(b) + " leaves the domain");
+        for (int x = B.x0; x <= B.x1; ++x)
+            for (int y = B.y0; y <= B.y1; ++y)
+                for (int z = B.z0; z <= B.z1; ++z) {
+                    uint8_t& c = cover[((uint64_t)x * desc->ny + y) * desc->nz + z];
+                    if (c) return fail(PLBAMD_EINVAL, "multiblock_create: blocks overlap");
+                    c = 1;
+                }
     }
-    PLBAMD_HIP(hipMemset(L->f, 0, bytes));
-    PLBAMD_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
-    PLBAMD_HIP(hipEventCreate(&L->ev0));
-    PLBAMD_HIP(hipEventCreate(&L->ev1));
-    // Bulk launch shape: TX threads along z (a multiple of the 64-lane wavefront),
-    // each covering CPT cells of a row segment; 256/TX... rows in y per workgroup.
-    int txMax = 128, cpt = 1;
-    if (const char* e = getenv("PLBAMD_BULK_TX")) txMax = std::max(64, std::min(256, atoi(e) / 64 * 64));
-    if (const char* e = getenv("PLBAMD_BULK_CPT")) cpt = std::max(1, std::min(16, atoi(e)));
-    int tx = (int)std::min<int64_t>(txMax, ((int64_t)desc->nz + 63) / 64 * 64);
-    int ty = std::max(1, txMax / tx);
-    L->zChunk = tx * cpt;
-    L->nzChunks = (desc->nz + L->zChunk - 1) / L->zChunk;
-    L->bulkBlock = dim3(tx, ty);
-    L->bulkGrid = dim3((unsigned)(L->nzChunks * ((desc->ny + ty - 1) / ty)), desc->nx);
-    L->nBulkPartials = (uint64_t)L->bulkGrid.x * L->bulkGrid.y;
-    PLBAMD_HIP(hipMalloc(&L->dBulkPartials, L->nBulkPartials * 8 * sizeof(double)));
-    PLBAMD_HIP(hipMalloc(&L->dStats, 16 * sizeof(double)));
-    PLBAMD_HIP(hipMemset(L->dStats, 0, 16 * sizeof(double)));
-    std::lock_guard<std::mutex> lock(g_mutex);
-    *handle = g_nextHandle++;
-    g_lattices[*handle] = std::move(L);
-    return PLBAMD_OK;
+    for (uint64_t k = 0; k < cells; ++k)
+        if (!cover[k]) return fail(PLBAMD_EINVAL, "multiblock_create: the blocks do not cover the domain");
+    auto L = std::make_unique<Lattice>();
+    L->desc = *desc;
+    L->multi = true;
+    L->cells = cells;
+    L->bulk.assign(blocks, blocks + nblocks);
+    PLBAMD_TRY(setup(*L));
+    return registerLattice(std::move(L), handle);
 }
 
 int plbamd_lattice_destroy(int handle) {
     std::lock_guard<std::mutex> lock(g_mutex);
     auto it = g_lattices.find(handle);
     if (it == g_lattices.end()) return fail(PLBAMD_EINVAL, "lattice_destroy: unknown handle");
-    hipStreamSynchronize(it->second->stream);
+    (void)hipStreamSynchronize(it->second->stream);
     g_lattices.erase(it);
     return PLBAMD_OK;
 }
@@ -1333,11 +986,9 @@ int plbamd_set_plane_values(int handle, int plane, const double* values, size_t 
 int plbamd_set_porosity(int handle, const double* alpha) {
     Lattice* L = find(handle);
     if (!L) return fail(PLBAMD_EINVAL, "set_porosity: unknown handle");
-    hipFree(L->dAlpha);
-    L->dAlpha = nullptr;
-    if (!alpha) return PLBAMD_OK;
-    PLBAMD_HIP(hipMalloc(&L->dAlpha, L->cells * sizeof(double)));
-    PLBAMD_HIP(hipMemcpy(L->dAlpha, alpha, L->cells * sizeof(double), hipMemcpyHostToDevice));
+    if (alpha) L->hostAlpha.assign(alpha, alpha + L->cells);
+    else L->hostAlpha.clear();
+    L->dirty = true;
     return PLBAMD_OK;
 }
 
@@ -1379,8 +1030,7 @@ int plbamd_collide_and_stream(int handle, int nsteps) {
     Lattice* L = find(handle);
     if (!L) return fail(PLBAMD_EINVAL, "collide_and_stream: unknown handle");
     if (nsteps < 0) return fail(PLBAMD_EINVAL, "collide_and_stream: negative step count");
-    int rc = step(*L, nsteps);
-    if (rc) return rc;
+    PLBAMD_TRY(step(*L, nsteps));
     PLBAMD_HIP(hipStreamSynchronize(L->stream));
     return PLBAMD_OK;
 }
@@ -1389,11 +1039,9 @@ int plbamd_collide_and_stream_timed(int handle, int nsteps, double* ms) {
     Lattice* L = find(handle);
     if (!L) return fail(PLBAMD_EINVAL, "collide_and_stream_timed: unknown handle");
     if (nsteps < 0 || !ms) return fail(PLBAMD_EINVAL, "collide_and_stream_timed: bad argument");
-    int rc = rebuild(*L);
-    if (rc) return rc;
+    PLBAMD_TRY(rebuild(*L));
     PLBAMD_HIP(hipEventRecord(L->ev0, L->stream));
-    rc = step(*L, nsteps);
-    if (rc) return rc;
+    PLBAMD_TRY(step(*L, nsteps));
     PLBAMD_HIP(hipEventRecord(L->ev1, L->stream));
     PLBAMD_HIP(hipEventSynchronize(L->ev1));
     float elapsed = 0.f;
@@ -1451,6 +1099,13 @@ int plbamd_compute_macroscopic(int handle, const plbamd_box* box, double* rho, d
 int plbamd_reduce_moments(int handle, const plbamd_box* box, double* out) {
     if (!out) return fail(PLBAMD_EINVAL, "reduce_moments: null output");
     return dispatchMoments(handle, box, nullptr, nullptr, out);
+}
+
+int plbamd_adjoint_gradient(int handle, int nsteps, int checkpoint_every, const plbamd_objective* objective,
+                            double* J, double* grad_alpha, double* grad_plane_scale, double* timing_ms) {
+    Lattice* L = find(handle);
+    if (!L || !objective) return fail(PLBAMD_EINVAL, "adjoint_gradient: bad argument");
+    return adjointGradient(*L, nsteps, checkpoint_every, *objective, J, grad_alpha, grad_plane_scale, timing_ms);
 }
 
 int plbamd_get_time(int handle, int64_t* t) {

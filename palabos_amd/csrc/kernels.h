@@ -1,0 +1,702 @@
+// Device kernels of the engine: AA-pattern collide-and-stream over a table of
+// blocks, the special-cell kernel, the envelope exchange, transfers, analysis.
+//
+// Data layout in HBM. A lattice is one or more blocks in one arena. A block
+// stores its cells (bulk plus envelope) slot-major, f[slot][x][y][z], z
+// fastest, each slot padded to a multiple of 64 cells, in the storage type
+// (double | float | f16 bits | bf16 bits) of shifted populations f - t_i
+// (roundOffPolicy.h:73-90). Per-cell tags (uint8) and porosity (FP64) follow
+// the same cell order. A single-block lattice has no envelope and wraps its
+// periodic axes in place; a multi-block lattice gives every block a two-cell
+// envelope towards its neighbours and periodic images, refreshed by the
+// exchange kernel every second step.
+//
+// Streaming is the in-place AA pattern, one array:
+//   even step  each cell reads its own slots i, collides, writes f*_i to its
+//              own slot opp(i)                        (all accesses local)
+//   odd step   each cell gathers f_i from slot opp(i) of cell x - c_i, collides,
+//              scatters f*_i to slot i of cell x + c_i (natural layout again)
+// A source or destination outside a non-periodic edge of the block (or a
+// link-wise wall) is replaced by the cell's own opposite slot: Palabos's
+// same-step edge reflection (atomicBlock/blockLattice3D.hh:403-427). Every
+// memory slot is read and written by exactly one cell per step, so the update
+// is race-free and the cell order is free, as it is in Palabos.
+//
+// Envelopes. An even step is local, so it corrupts nothing; an odd step
+// corrupts the outermost layer (its gathers leave the block) and, through its
+// scatters, the layer inside it. A two-cell envelope refreshed after every odd
+// step therefore keeps every bulk cell bit-identical to a single-block run, as
+// Palabos's one-cell envelope refreshed every step does
+// (multiBlock/multiBlockLattice3D.hh:318-360).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/palabos_amd.h"
+#include "boundary_device.h"
+#include "d3q19_device.h"
+#include "storage.h"
+
+namespace plbamd {
+
+enum BulkTag : uint8_t { BT_BULK = 0, BT_BOUNCE_BACK = 1, BT_NO_DYNAMICS = 2, BT_SKIP = 3 };
+
+// One block of a lattice. Coordinates are block-local (0..n-1 over the
+// allocated cells); the bulk (cells that own their values and take
+// statistics) is [b0, b1] inclusive.
+struct BlockGeom {
+    int nx, ny, nz;           // allocated extent
+    int px, py, pz;           // wrap inside the block (single-block periodic axes)
+    int bx0, by0, bz0;        // bulk begin, local
+    int bx1, by1, bz1;        // bulk end, local, inclusive
+    int gx0, gy0, gz0;        // global coordinate of local (0,0,0), before periodic wrap
+    int nzChunks, nyGroups;   // bulk-launch decomposition: row segments, y groups
+    uint64_t offset;          // element offset of slot 0 in the arena
+    uint64_t slot;            // elements per slot
+    uint64_t cellOffset;      // offset of the block's cells in per-cell arrays
+    uint32_t wgBegin;         // first workgroup of the block in the bulk launch
+    uint32_t pad;
+};
+
+__device__ __forceinline__ uint64_t lin(const BlockGeom& g, int x, int y, int z) {
+    return ((uint64_t)x * g.ny + y) * g.nz + z;
+}
+
+__device__ __forceinline__ bool shift(int v, int d, int n, int periodic, int& out) {
+    int w = v + d;
+    if (w < 0) { if (!periodic) return false; w += n; }
+    else if (w >= n) { if (!periodic) return false; w -= n; }
+    out = w;
+    return true;
+}
+
+__device__ __forceinline__ bool in_bulk(const BlockGeom& g, int x, int y, int z) {
+    return x >= g.bx0 && x <= g.bx1 && y >= g.by0 && y <= g.by1 && z >= g.bz0 && z <= g.bz1;
+}
+
+// Position of f_i of cell (x,y,z) in the swapped (after-even-step) layout:
+// slot opp(i) at x - c_i, or slot i at the cell itself when the source reflects.
+__device__ __forceinline__ uint64_t swapped_index(const BlockGeom& g, int x, int y, int z, int i,
+                                                  uint32_t wallMask, uint64_t cell) {
+    if (i == 0) return cell;
+    int sx, sy, sz;
+    bool ok = shift(x, -cx(i), g.nx, g.px, sx) && shift(y, -cy(i), g.ny, g.py, sy) &&
+              shift(z, -cz(i), g.nz, g.pz, sz) && !((wallMask >> opp(i)) & 1u);
+    if (ok) return (uint64_t)opp(i) * g.slot + lin(g, sx, sy, sz);
+    return (uint64_t)i * g.slot + cell;
+}
+
+__device__ __forceinline__ uint64_t pop_index(const BlockGeom& g, int parity, int x, int y, int z, int i,
+                                              uint32_t wall, uint64_t cell) {
+    if (parity == 0) return (uint64_t)i * g.slot + cell;
+    return swapped_index(g, x, y, z, i, wall, cell);
+}
+
+// Block of a workgroup of the bulk launch (binary search over wgBegin).
+__device__ __forceinline__ int find_block(const BlockGeom* blocks, int nBlocks, uint32_t wg) {
+    int lo = 0, hi = nBlocks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (blocks[mid].wgBegin <= wg) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Block reduction of an 8-double record (fixed order: deterministic). Slot 2
+// is a maximum, the others are sums.
+__device__ __forceinline__ void block_reduce_store(double v[8], double* out) {
+    __shared__ double red[256][8];
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    const int n = blockDim.x * blockDim.y;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) red[t][k] = v[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s && t + s < n) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (k == 2) red[t][k] = fmax(red[t][k], red[t + s][k]);
+                else red[t][k] += red[t + s][k];
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) out[k] = red[0][k];
+    }
+    __syncthreads();
+}
+
+template<typename S>
+__device__ __forceinline__ S* at(S* base, uint32_t byteOffset) {
+    return reinterpret_cast<S*>(reinterpret_cast<char*>(base) + byteOffset);
+}
+
+// ---------------------------------------------------------------------------
+// Bulk kernel. One launch covers every block: a workgroup takes one segment
+// of zChunk cells of a row (x, y) of one block (TY rows when the row is
+// shorter than a workgroup). A wavefront always lies in one (x, y) row, so
+// the nine neighbouring rows and their validity are wavefront-uniform and
+// live in scalar registers: a load or store is a row base plus one of three
+// per-lane z offsets. Wavefronts touching an edge of the block take the
+// general path with per-lane edge selection.
+
+struct StepArgs {
+    void* arena;
+    const BlockGeom* blocks;
+    int nBlocks;
+    int zChunk;
+    double omega;
+    double force[3];
+    const uint8_t* tags;     // BulkTag per cell (all blocks) or null
+    const double* alpha;     // Brinkman porosity per cell (all blocks) or null
+    double* partials;        // 8 doubles per workgroup when statistics are on
+    double* moments;         // MOMENTS: rhoBar, jx, jy, jz of the pre-collision state,
+    uint64_t momStride;      //          [4][momStride] by cell (all blocks)
+};
+
+// Pre-collision moments rhoBar = sum_i f_i, j = sum_i c_i f_i (plain order),
+// recorded for the adjoint's reverse sweep.
+template<typename T>
+__device__ __forceinline__ void store_moments(const T* fc, double* m, uint64_t stride, uint64_t idx) {
+    double rb = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const double v = (double)fc[i];
+        rb += v;
+        jx += cx(i) * v;
+        jy += cy(i) * v;
+        jz += cz(i) * v;
+    }
+    m[idx] = rb;
+    m[stride + idx] = jx;
+    m[2 * stride + idx] = jy;
+    m[3 * stride + idx] = jz;
+}
+
+template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
+__global__ void __launch_bounds__(256) bulk_step(StepArgs a) {
+    using T = typename Store<S>::T;
+    const uint32_t wg = blockIdx.x;
+    const int b = find_block(a.blocks, a.nBlocks, wg);
+    const BlockGeom g = a.blocks[b];
+    S* __restrict__ f = reinterpret_cast<S*>(a.arena) + g.offset;
+    const uint32_t local = wg - g.wgBegin;
+    const int chunk = (int)(local % (uint32_t)g.nzChunks);
+    const uint32_t rest = local / (uint32_t)g.nzChunks;
+    const int yGroup = (int)(rest % (uint32_t)g.nyGroups);
+    const int x = (int)(rest / (uint32_t)g.nyGroups);
+    const int y = __builtin_amdgcn_readfirstlane(yGroup * (int)blockDim.y + (int)threadIdx.y);
+    const int zBegin = chunk * a.zChunk;
+    const int zEnd = min(g.nz, zBegin + a.zChunk);
+    const T omega = (T)a.omega;
+    const T F[3] = {(T)a.force[0], (T)a.force[1], (T)a.force[2]};
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    if (y < g.ny) {
+        int xm = 0, xp = 0, ym = 0, yp = 0;
+        const bool vxm = shift(x, -1, g.nx, g.px, xm), vxp = shift(x, 1, g.nx, g.px, xp);
+        const bool vym = shift(y, -1, g.ny, g.py, ym), vyp = shift(y, 1, g.ny, g.py, yp);
+        const int xs[3] = {xm, x, xp};
+        const int ys[3] = {ym, y, yp};
+        const bool vx[3] = {vxm, true, vxp};
+        const bool vy[3] = {vym, true, vyp};
+        const bool rowsValid = vxm && vxp && vym && vyp;
+        const bool rowInBulkXY = x >= g.bx0 && x <= g.bx1 && y >= g.by0 && y <= g.by1;
+        const uint64_t rowBase = ((uint64_t)x * g.ny + y) * g.nz;
+        S* rows[3][3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+            for (int ay = 0; ay < 3; ++ay)
+                rows[ax][ay] = f + ((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz;
+
+        for (int z = zBegin + (int)threadIdx.x; z < zEnd; z += blockDim.x) {
+            const uint64_t cell = rowBase + z;
+            uint8_t tag = BT_BULK;
+            if constexpr (TAGS) {
+                tag = a.tags[g.cellOffset + cell];
+                if (tag == BT_SKIP) continue;
+            }
+            int zm = 0, zp = 0;
+            const bool vzm = shift(z, -1, g.nz, g.pz, zm), vzp = shift(z, 1, g.nz, g.pz, zp);
+            const uint32_t zb[3] = {(uint32_t)zm * (uint32_t)sizeof(S), (uint32_t)z * (uint32_t)sizeof(S),
+                                    (uint32_t)zp * (uint32_t)sizeof(S)};
+            bool fast = true;
+            if constexpr (ODD) {
+                const uint64_t active = __ballot(1);
+                fast = rowsValid && (__ballot(vzm && vzp) == active);
+            }
+
+            T fc[19];
+            if constexpr (!ODD) {
+#pragma unroll
+                for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(at(rows[1][1] + (uint64_t)i * g.slot, zb[1]));
+            } else if (fast) {
+#pragma unroll
+                for (int i = 0; i < 19; ++i)
+                    fc[i] = Store<S>::load(at(rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot, zb[1 - cz(i)]));
+            } else {
+                const int zs[3] = {zm, z, zp};
+                const bool vz[3] = {vzm, true, vzp};
+                fc[0] = Store<S>::load(f + cell);
+#pragma unroll
+                for (int i = 1; i < 19; ++i) {
+                    const int ax = 1 - cx(i), ay = 1 - cy(i), az = 1 - cz(i);
+                    const bool ok = vx[ax] && vy[ay] && vz[az];
+                    const uint64_t src = ok ? (uint64_t)opp(i) * g.slot +
+                                                  (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
+                                            : (uint64_t)i * g.slot + cell;
+                    fc[i] = Store<S>::load(f + src);
+                }
+            }
+
+            if constexpr (MOMENTS) {
+                if (tag == BT_BULK) store_moments(fc, a.moments, a.momStride, g.cellOffset + cell);
+            }
+            if (tag == BT_BULK) {
+                T rhoBar;
+                T drag[3] = {0, 0, 0};
+                const T alpha = (FORCED && a.alpha) ? (T)a.alpha[g.cellOffset + cell] : (T)0;
+                const T uSqr = collide_bulk<T, DYN, FORCED>(fc, omega, F, alpha, rhoBar, drag);
+                if constexpr (STATS) {
+                    if (rowInBulkXY && z >= g.bz0 && z <= g.bz1) {
+                        st[0] += (double)rhoBar;
+                        st[1] += (double)uSqr;
+                        st[2] = fmax(st[2], (double)uSqr);
+                        st[3] += 1.0;
+                        if constexpr (FORCED) {
+                            st[4] += (double)drag[0];
+                            st[5] += (double)drag[1];
+                            st[6] += (double)drag[2];
+                        }
+                    }
+                }
+            } else if (tag == BT_BOUNCE_BACK) {
+#pragma unroll
+                for (int i = 1; i <= 9; ++i) { T t = fc[i]; fc[i] = fc[i + 9]; fc[i + 9] = t; }
+            }
+
+            if constexpr (!ODD) {
+#pragma unroll
+                for (int i = 0; i < 19; ++i) Store<S>::store(at(rows[1][1] + (uint64_t)opp(i) * g.slot, zb[1]), fc[i]);
+            } else if (fast) {
+#pragma unroll
+                for (int i = 0; i < 19; ++i)
+                    Store<S>::store(at(rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot, zb[1 + cz(i)]), fc[i]);
+            } else {
+                const int zs[3] = {zm, z, zp};
+                const bool vz[3] = {vzm, true, vzp};
+                Store<S>::store(f + cell, fc[0]);
+#pragma unroll
+                for (int i = 1; i < 19; ++i) {
+                    const int ax = 1 + cx(i), ay = 1 + cy(i), az = 1 + cz(i);
+                    const bool ok = vx[ax] && vy[ay] && vz[az];
+                    const uint64_t dst = ok ? (uint64_t)i * g.slot +
+                                                  (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
+                                            : (uint64_t)opp(i) * g.slot + cell;
+                    Store<S>::store(f + dst, fc[i]);
+                }
+            }
+        }
+    }
+    if constexpr (STATS) block_reduce_store(st, a.partials + 8 * (uint64_t)wg);
+}
+
+// ---------------------------------------------------------------------------
+// Special-cell kernel: boundary planes, momentum-exchange nodes, cells next to
+// link-wise walls. Same AA addressing as the bulk kernel, with the wall mask
+// added to the edge test.
+
+enum SpecialKind : uint32_t { SK_PLANE = 0, SK_MOMENTUM_EXCHANGE = 1, SK_NEAR_WALL = 2 };
+
+struct SpecialCell {
+    uint32_t cell;        // block-local linear cell index
+    uint16_t x, y, z;     // block-local coordinates
+    uint16_t info;        // kind | family << 2 | bckind << 3 | dir << 4 | (ori > 0) << 6 | wrapped << 7
+                          //      | bulk << 8 (the cell takes statistics)
+    uint32_t value;       // index into the plane value array (velocity: 3 doubles, pressure: 1)
+    uint32_t wall;        // bit i: neighbour x + c_i is a link-wise wall
+    uint32_t fluid;       // bit i: fluid direction of a momentum-exchange node
+    uint16_t block;       // block index
+    uint16_t pad0;
+    uint32_t pad1;
+};
+static_assert(sizeof(SpecialCell) == 32, "special record is 32 bytes");
+
+struct SpecialArgs {
+    void* arena;
+    const BlockGeom* blocks;
+    double omega;
+    double force[3];
+    const double* alpha;  // Brinkman porosity per cell (all blocks) or null
+    const SpecialCell* cells;
+    uint32_t count;
+    const double* values;
+    double* partials;     // 16 doubles per workgroup when statistics are on:
+                          // rhoBar uSqr maxUSqr count Fx Fy Fz -, then Px Py - Pz - - - -
+    double* moments;      // or null: pre-collision moments (layout of StepArgs::moments)
+    uint64_t momStride;
+    double* dumpF;        // or null: pre-completion populations, 19 per record
+};
+
+template<typename S, int DYN, bool ODD, bool STATS, bool FORCED>
+__global__ void __launch_bounds__(256) special_step(SpecialArgs a) {
+    using T = typename Store<S>::T;
+    const T omega = (T)a.omega;
+    const T F[3] = {(T)a.force[0], (T)a.force[1], (T)a.force[2]};
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double sp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < a.count) {
+        const SpecialCell sc = a.cells[k];
+        const BlockGeom g = a.blocks[sc.block];
+        S* __restrict__ f = reinterpret_cast<S*>(a.arena) + g.offset;
+        const int x = sc.x, y = sc.y, z = sc.z;
+        const uint64_t cell = sc.cell;
+        const uint32_t kind = sc.info & 3u;
+        const bool bulk = (sc.info >> 8) & 1u;
+        T fc[19];
+        if constexpr (!ODD) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + (uint64_t)i * g.slot + cell);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + swapped_index(g, x, y, z, i, sc.wall, cell));
+        }
+        if (a.moments) store_moments(fc, a.moments, a.momStride, g.cellOffset + cell);
+        if (a.dumpF) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) a.dumpF[19 * (uint64_t)k + i] = (double)fc[i];
+        }
+
+        if (kind == SK_MOMENTUM_EXCHANGE) {
+            // bounceBackModels.hh:98-135
+            T mom[3] = {T(), T(), T()};
+#pragma unroll
+            for (int i = 1; i < 19; ++i) {
+                if (!((sc.fluid >> i) & 1u)) continue;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) mom[c] += (T)2 * ci(i, c) * fc[opp(i)];
+            }
+            if (bulk) {
+                st[4] += (double)(-mom[0]);
+                st[5] += (double)(-mom[1]);
+                st[6] += (double)(-mom[2]);
+            }
+#pragma unroll
+            for (int i = 1; i <= 9; ++i) { T t = fc[i]; fc[i] = fc[i + 9]; fc[i + 9] = t; }
+        } else {
+            if (kind == SK_PLANE) complete_plane(fc, sc.info, a.values + sc.value);
+            T rhoBar, drag[3] = {0, 0, 0};
+            const T alpha = (FORCED && a.alpha) ? (T)a.alpha[g.cellOffset + cell] : (T)0;
+            const T uSqr = collide_bulk<T, DYN, FORCED>(fc, omega, F, alpha, rhoBar, drag);
+            if (bulk) {
+                st[0] += (double)rhoBar;
+                st[1] += (double)uSqr;
+                st[2] = fmax(st[2], (double)uSqr);
+                st[3] += 1.0;
+                if constexpr (FORCED) {
+                    sp[0] += (double)drag[0];
+                    sp[1] += (double)drag[1];
+                    sp[3] += (double)drag[2];
+                }
+                // link-wise wall: each outgoing population towards the wall carries
+                // 2 c_i f*_i (the stored, storage-rounded value is what is reflected)
+#pragma unroll
+                for (int i = 1; i < 19; ++i) {
+                    if (!((sc.wall >> i) & 1u)) continue;
+                    const T fo = Store<S>::round(fc[i]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) st[4 + c] += (double)((T)2 * ci(i, c) * fo);
+                }
+            }
+        }
+
+        if constexpr (!ODD) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cell, fc[i]);
+        } else {
+            Store<S>::store(f + cell, fc[0]);
+#pragma unroll
+            for (int i = 1; i < 19; ++i) {
+                int dx, dy, dz;
+                const bool ok = shift(x, cx(i), g.nx, g.px, dx) && shift(y, cy(i), g.ny, g.py, dy) &&
+                                shift(z, cz(i), g.nz, g.pz, dz) && !((sc.wall >> i) & 1u);
+                const uint64_t dst = ok ? (uint64_t)i * g.slot + lin(g, dx, dy, dz)
+                                        : (uint64_t)opp(i) * g.slot + cell;
+                Store<S>::store(f + dst, fc[i]);
+            }
+        }
+    }
+    if constexpr (STATS) {
+        block_reduce_store(st, a.partials + 16 * (uint64_t)blockIdx.x);
+        block_reduce_store(sp, a.partials + 16 * (uint64_t)blockIdx.x + 8);
+    }
+}
+
+// Final statistics reduction: one workgroup, fixed order. out[0..7]:
+// sumRhoBar sumUSqr maxUSqr count Fx Fy Fz -, out[8..15]: Px Py - Pz (slot 2 is a max).
+__global__ void __launch_bounds__(256) reduce_partials(const double* bulk, uint64_t nBulk,
+                                                       const double* special, uint64_t nSpecial,
+                                                       double* out) {
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint64_t b = threadIdx.x; b < nBulk; b += blockDim.x) {
+        const double* p = bulk + 8 * b;
+        v[0] += p[0]; v[1] += p[1]; v[2] = fmax(v[2], p[2]); v[3] += p[3];
+        v[4] += p[4]; v[5] += p[5]; v[6] += p[6];
+    }
+    double sp[3] = {0, 0, 0};
+    for (uint64_t b = threadIdx.x; b < nSpecial; b += blockDim.x) {
+        const double* p = special + 16 * b;
+        s[0] += p[0]; s[1] += p[1]; s[2] = fmax(s[2], p[2]); s[3] += p[3];
+        s[4] += p[4]; s[5] += p[5]; s[6] += p[6];
+        sp[0] += p[8]; sp[1] += p[9]; sp[2] += p[11];
+    }
+    double r[8] = {v[0] + s[0], v[1] + s[1], fmax(v[2], s[2]), v[3] + s[3], s[4], s[5], s[6], 0};
+    block_reduce_store(r, out);
+    double q[8] = {v[4] + sp[0], v[5] + sp[1], 0, v[6] + sp[2], 0, 0, 0, 0};
+    block_reduce_store(q, out + 8);
+}
+
+// ---------------------------------------------------------------------------
+// Envelope exchange (multi-block): copies the natural-layout populations of
+// the bulk cells that make up each envelope, segment by segment
+// (SerialBlockCommunicator3D::duplicateOverlaps, serialBlockCommunicator3D.cpp:43-95,
+// BlockLatticeDataTransfer3D::attribute_static, blockLattice3D.hh:873-885).
+
+struct CopySegment {
+    uint16_t src, dst;            // block indices
+    uint16_t pad[2];
+    int sx, sy, sz;               // block-local origin in the source
+    int dx, dy, dz;               // block-local origin in the destination
+    int lx, ly, lz;               // extent
+    uint64_t cellBegin;           // first cell of the segment in the launch
+};
+
+template<typename S>
+__global__ void __launch_bounds__(256) exchange_envelopes(void* arenaV, const BlockGeom* blocks,
+                                                          const CopySegment* segs, int nSegs, uint64_t total) {
+    S* arena = reinterpret_cast<S*>(arenaV);
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < total;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nSegs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (segs[mid].cellBegin <= k) lo = mid; else hi = mid - 1;
+        }
+        const CopySegment s = segs[lo];
+        const uint64_t r = k - s.cellBegin;
+        const int lz = (int)(r % (uint64_t)s.lz), ly = (int)((r / s.lz) % (uint64_t)s.ly);
+        const int lx = (int)(r / ((uint64_t)s.lz * s.ly));
+        const BlockGeom& gs = blocks[s.src];
+        const BlockGeom& gd = blocks[s.dst];
+        const S* src = arena + gs.offset + lin(gs, s.sx + lx, s.sy + ly, s.sz + lz);
+        S* dst = arena + gd.offset + lin(gd, s.dx + lx, s.dy + ly, s.dz + lz);
+#pragma unroll
+        for (int i = 0; i < 19; ++i) dst[(uint64_t)i * gd.slot] = src[(uint64_t)i * gs.slot];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Transfers and analysis on one block: the cells of the global box I that the
+// block owns, staged at their position in the caller's box G.
+
+struct BoxArgs {
+    BlockGeom g;
+    int block;
+    int parity;
+    int ix0, iy0, iz0, ilx, ily, ilz;   // iteration box, block-local
+    int ox, oy, oz;                     // offset of the iteration box inside the staging box
+    int sly, slz;                       // staging box extents along y, z
+    const SpecialCell* special;         // sorted by (block, cell)
+    uint32_t nSpecial;
+};
+
+__device__ __forceinline__ int find_special(const BoxArgs& b, uint64_t cell) {
+    const uint64_t key = ((uint64_t)b.block << 32) | cell;
+    uint32_t lo = 0, hi = b.nSpecial;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        const uint64_t mk = ((uint64_t)b.special[mid].block << 32) | b.special[mid].cell;
+        if (mk < key) lo = mid + 1; else hi = mid;
+    }
+    if (lo < b.nSpecial && b.special[lo].block == b.block && b.special[lo].cell == cell) return (int)lo;
+    return -1;
+}
+
+__device__ __forceinline__ uint32_t wall_mask_of(const BoxArgs& b, uint64_t cell) {
+    const int s = find_special(b, cell);
+    return s >= 0 ? b.special[s].wall : 0u;
+}
+
+struct BoxCell {
+    int x, y, z;          // block-local
+    uint64_t cell;        // block-local linear
+    uint64_t staged;      // index in the staging box
+};
+
+__device__ __forceinline__ BoxCell box_cell(const BoxArgs& b, uint64_t k) {
+    BoxCell c;
+    const int lz = (int)(k % (uint64_t)b.ilz), ly = (int)((k / b.ilz) % (uint64_t)b.ily);
+    const int lx = (int)(k / ((uint64_t)b.ilz * b.ily));
+    c.x = b.ix0 + lx; c.y = b.iy0 + ly; c.z = b.iz0 + lz;
+    c.cell = lin(b.g, c.x, c.y, c.z);
+    c.staged = ((uint64_t)(b.ox + lx) * b.sly + (b.oy + ly)) * b.slz + (b.oz + lz);
+    return c;
+}
+
+#define PLBAMD_BOX_LOOP(b)                                                                      \
+    const uint64_t n_ = (uint64_t)(b).ilx * (b).ily * (b).ilz;                                  \
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n_;                  \
+         k += (uint64_t)gridDim.x * blockDim.x)
+
+template<typename S, typename H>
+__global__ void upload_box(void* arena, BoxArgs b, const H* host) {
+    using T = typename Store<S>::T;
+    S* f = reinterpret_cast<S*>(arena) + b.g.offset;
+    PLBAMD_BOX_LOOP(b) {
+        const BoxCell c = box_cell(b, k);
+        const uint32_t wall = b.parity ? wall_mask_of(b, c.cell) : 0;
+        for (int i = 0; i < 19; ++i)
+            Store<S>::store(f + pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell), (T)host[c.staged * 19 + i]);
+    }
+}
+
+template<typename S>
+__global__ void upload_box_raw(void* arena, BoxArgs b, const uint16_t* host) {
+    uint16_t* f = reinterpret_cast<uint16_t*>(reinterpret_cast<S*>(arena) + b.g.offset);
+    PLBAMD_BOX_LOOP(b) {
+        const BoxCell c = box_cell(b, k);
+        const uint32_t wall = b.parity ? wall_mask_of(b, c.cell) : 0;
+        for (int i = 0; i < 19; ++i) f[pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell)] = host[c.staged * 19 + i];
+    }
+}
+
+template<typename S, typename H>
+__global__ void download_box(const void* arena, BoxArgs b, H* host) {
+    const S* f = reinterpret_cast<const S*>(arena) + b.g.offset;
+    PLBAMD_BOX_LOOP(b) {
+        const BoxCell c = box_cell(b, k);
+        const uint32_t wall = b.parity ? wall_mask_of(b, c.cell) : 0;
+        for (int i = 0; i < 19; ++i)
+            host[c.staged * 19 + i] = (H)Store<S>::load(f + pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell));
+    }
+}
+
+template<typename S>
+__global__ void download_box_raw(const void* arena, BoxArgs b, uint16_t* host) {
+    const uint16_t* f = reinterpret_cast<const uint16_t*>(reinterpret_cast<const S*>(arena) + b.g.offset);
+    PLBAMD_BOX_LOOP(b) {
+        const BoxCell c = box_cell(b, k);
+        const uint32_t wall = b.parity ? wall_mask_of(b, c.cell) : 0;
+        for (int i = 0; i < 19; ++i) host[c.staged * 19 + i] = f[pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell)];
+    }
+}
+
+// Completed snapshot of wrapped-factory plane cells: the populations the
+// wrapped post-stream processor leaves in Palabos's lattice
+// (wrappedLocalBoundaryProcessor3D.hh:41-62).
+template<typename S, typename H>
+__global__ void complete_download(const void* arena, BoxArgs b, const double* values, H* host, int raw) {
+    using T = typename Store<S>::T;
+    const S* f = reinterpret_cast<const S*>(arena) + b.g.offset;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < b.nSpecial; k += gridDim.x * blockDim.x) {
+        const SpecialCell sc = b.special[k];
+        if (sc.block != b.block || (sc.info & 3u) != SK_PLANE || !((sc.info >> 7) & 1u)) continue;
+        const int x = sc.x, y = sc.y, z = sc.z;
+        if (x < b.ix0 || x >= b.ix0 + b.ilx || y < b.iy0 || y >= b.iy0 + b.ily || z < b.iz0 || z >= b.iz0 + b.ilz)
+            continue;
+        T fc[19];
+        for (int i = 0; i < 19; ++i)
+            fc[i] = Store<S>::load(f + pop_index(b.g, b.parity, x, y, z, i, b.parity ? sc.wall : 0u, sc.cell));
+        complete_plane(fc, sc.info, values + sc.value);
+        const uint64_t o = (((uint64_t)(b.ox + x - b.ix0) * b.sly + (b.oy + y - b.iy0)) * b.slz + (b.oz + z - b.iz0)) * 19;
+        for (int i = 0; i < 19; ++i) {
+            if (raw) reinterpret_cast<uint16_t*>(host)[o + i] = Store<S>::bits(fc[i]);
+            else host[o + i] = (H)Store<S>::round(fc[i]);
+        }
+    }
+}
+
+// Equilibrium initialisation (core/cell.hh:124-132 iniCellAtEquilibrium) in the
+// compute precision of the lattice.
+template<typename S>
+__global__ void equilibrium_box(void* arena, BoxArgs b, const double* rho, const double* u) {
+    using T = typename Store<S>::T;
+    S* f = reinterpret_cast<S*>(arena) + b.g.offset;
+    PLBAMD_BOX_LOOP(b) {
+        const BoxCell c = box_cell(b, k);
+        const uint32_t wall = b.parity ? wall_mask_of(b, c.cell) : 0;
+        const T density = (T)rho[c.staged];
+        const T j[3] = {(T)u[3 * c.staged] * density, (T)u[3 * c.staged + 1] * density, (T)u[3 * c.staged + 2] * density};
+        const T jSqr = j[0] * j[0] + j[1] * j[1] + j[2] * j[2];
+        const T rhoBar = density - (T)1;
+        const T ir = inv_rho(rhoBar);
+        for (int i = 0; i < 19; ++i)
+            Store<S>::store(f + pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell), feq(i, rhoBar, ir, j, jSqr));
+    }
+}
+
+struct MomentsArgs {
+    const uint8_t* tags;
+    const double* values;
+    double force[3];
+};
+
+// Per-cell density and velocity as Palabos's analysis functions see them
+// (plane_macroscopic_any and the rules listed there), and their FP64 sums.
+template<typename S>
+__global__ void __launch_bounds__(256) moments_box(const void* arena, BoxArgs b, MomentsArgs m, double* rhoOut,
+                                                   double* uOut, double* partials) {
+    using T = typename Store<S>::T;
+    const S* f = reinterpret_cast<const S*>(arena) + b.g.offset;
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    PLBAMD_BOX_LOOP(b) {
+        const BoxCell c = box_cell(b, k);
+        const uint8_t tag = m.tags ? m.tags[b.g.cellOffset + c.cell] : (uint8_t)BT_BULK;
+        const int sp = (tag == BT_SKIP) ? find_special(b, c.cell) : -1;
+        const uint32_t wall = (b.parity && sp >= 0) ? b.special[sp].wall : 0u;
+        T rhoBar = T(), u[3] = {T(), T(), T()};
+        const bool solid = tag == BT_BOUNCE_BACK || tag == BT_NO_DYNAMICS ||
+                           (tag == BT_SKIP && (sp < 0 || (b.special[sp].info & 3u) == SK_MOMENTUM_EXCHANGE));
+        if (!solid) {
+            T fc[19];
+            for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell));
+            if (sp >= 0 && (b.special[sp].info & 3u) == SK_PLANE) {
+                plane_macroscopic_any(fc, b.special[sp].info, m.values + b.special[sp].value, rhoBar, u);
+            } else {
+                T rb, j[3];
+                rhobar_j(fc, rb, j);
+                rhoBar = rhobar_sum(fc);
+                const T ir = inv_rho(rhoBar);
+                for (int a2 = 0; a2 < 3; ++a2) u[a2] = j[a2] * ir + (T)m.force[a2] / (T)2;
+            }
+        }
+        if (rhoOut) rhoOut[c.staged] = (double)(rhoBar + (T)1);
+        if (uOut) {
+            uOut[3 * c.staged] = (double)u[0];
+            uOut[3 * c.staged + 1] = (double)u[1];
+            uOut[3 * c.staged + 2] = (double)u[2];
+        }
+        const T uSqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        st[0] += (double)rhoBar;
+        st[1] += (double)uSqr;
+        st[3] += (double)u[0];
+        st[4] += (double)u[1];
+        st[5] += (double)u[2];
+    }
+    if (partials) block_reduce_store(st, partials + 8 * (uint64_t)blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) sum_records(const double* p, uint32_t n, double* out) {
+    double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < n; b += blockDim.x)
+        for (int k = 0; k < 8; ++k) if (k != 2) r[k] += p[8 * b + k];
+    block_reduce_store(r, out);
+}
+
+}  // namespace plbamd

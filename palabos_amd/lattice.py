@@ -70,10 +70,12 @@ def _box(b, shape):
 
 
 class Block3D:
-    """One lattice block (a Palabos BlockLattice3D / single-block MultiBlockLattice3D)."""
+    """A lattice: one block (a Palabos BlockLattice3D / single-block
+    MultiBlockLattice3D) or, with `blocks`, a multi-block lattice tiled by uneven
+    inclusive boxes (MultiBlockLattice3D over a SparseBlockStructure3D)."""
 
     def __init__(self, nx, ny, nz, descriptor=D3Q19, dynamics=None, storage="f64",
-                 periodic=False, force=None, device=0):
+                 periodic=False, force=None, device=0, blocks=None):
         if descriptor != D3Q19:
             raise ValueError("only the D3Q19 descriptor is supported")
         if dynamics is None:
@@ -92,7 +94,12 @@ class Block3D:
             for a in range(3):
                 d.force[a] = float(force[a])
         h = ctypes.c_int(0)
-        check(lib().plbamd_lattice_create(ctypes.byref(d), ctypes.byref(h)))
+        if blocks is None:
+            check(lib().plbamd_lattice_create(ctypes.byref(d), ctypes.byref(h)))
+        else:
+            arr = (Box * len(blocks))(*[Box.of(b) for b in blocks])
+            check(lib().plbamd_multiblock_create(ctypes.byref(d), arr, len(blocks), ctypes.byref(h)))
+        self.blocks = None if blocks is None else [tuple(b) for b in blocks]
         self.handle = h.value
         self.shape = (int(nx), int(ny), int(nz))
         self.storage = storage
@@ -247,6 +254,23 @@ class Block3D:
         ms = ctypes.c_double(0.0)
         check(lib().plbamd_collide_and_stream_timed(self.handle, int(nsteps), ctypes.byref(ms)))
         return ms.value
+
+    def adjointGradient(self, nsteps, checkpoint_every, inlet_x, outlet_x, weight_pressure_drop=1.0,
+                        weight_drag=1.0):
+        """Runs nsteps (the lattice advances) and returns the time-averaged objective
+        J = mean_t [w_dp cs2 (<rho>_inlet_x - <rho>_outlet_x) + w_drag omega sum alpha j_x]
+        with its gradient with respect to the porosity field and to the scale of each
+        plane's imposed velocity (hand-written adjoint, checkpoints every
+        `checkpoint_every` steps). Returns (J, grad_alpha [nx,ny,nz], grad_plane_scale
+        [planes], (forward_ms, backward_ms))."""
+        obj = native.Objective(int(inlet_x), int(outlet_x), float(weight_pressure_drop), float(weight_drag))
+        J = ctypes.c_double(0.0)
+        ga = np.zeros(self.shape, dtype=np.float64)
+        gs = np.zeros(max(1, len(self._planes)), dtype=np.float64)
+        tm = np.zeros(2, dtype=np.float64)
+        check(lib().plbamd_adjoint_gradient(self.handle, int(nsteps), int(checkpoint_every), ctypes.byref(obj),
+                                            ctypes.byref(J), ga.ctypes.data, gs.ctypes.data, tm.ctypes.data))
+        return J.value, ga, gs[:len(self._planes)], (tm[0], tm[1])
 
     def toggleInternalStatistics(self, on):
         check(lib().plbamd_toggle_statistics(self.handle, int(bool(on))))

@@ -25,12 +25,13 @@ STORAGE_NUMPY = {"f64": np.float64, "f32": np.float32, "f16": np.uint16, "bf16":
 
 # Every symbol include/palabos_amd.h declares.
 EXPORTS = (
-    "plbamd_init", "plbamd_last_error", "plbamd_lattice_create", "plbamd_lattice_destroy",
+    "plbamd_init", "plbamd_last_error", "plbamd_lattice_create", "plbamd_multiblock_create",
+    "plbamd_lattice_destroy",
     "plbamd_set_tags", "plbamd_add_plane", "plbamd_set_plane_values", "plbamd_set_porosity",
     "plbamd_upload", "plbamd_download", "plbamd_initialize_at_equilibrium",
     "plbamd_collide_and_stream", "plbamd_collide_and_stream_timed", "plbamd_toggle_statistics",
     "plbamd_get_statistics", "plbamd_compute_macroscopic", "plbamd_reduce_moments",
-    "plbamd_get_time",
+    "plbamd_adjoint_gradient", "plbamd_get_time",
 )
 
 
@@ -66,6 +67,11 @@ class LatticeDesc(ctypes.Structure):
                 ("force", ctypes.c_double * 3)]
 
 
+class Objective(ctypes.Structure):
+    _fields_ = [("inlet_x", ctypes.c_int32), ("outlet_x", ctypes.c_int32),
+                ("weight_pressure_drop", ctypes.c_double), ("weight_drag", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -86,6 +92,7 @@ def lib():
     sig = {
         "plbamd_init": [i32],
         "plbamd_lattice_create": [ctypes.POINTER(LatticeDesc), ctypes.POINTER(ctypes.c_int)],
+        "plbamd_multiblock_create": [ctypes.POINTER(LatticeDesc), pbox, i32, ctypes.POINTER(ctypes.c_int)],
         "plbamd_lattice_destroy": [i32],
         "plbamd_set_tags": [i32, vp],
         "plbamd_add_plane": [i32, i32, i32, i32, i32, pbox, ctypes.POINTER(ctypes.c_int)],
@@ -101,6 +108,7 @@ def lib():
         "plbamd_compute_macroscopic": [i32, pbox, vp, vp],
         "plbamd_reduce_moments": [i32, pbox, vp],
         "plbamd_get_time": [i32, ctypes.POINTER(ctypes.c_int64)],
+        "plbamd_adjoint_gradient": [i32, i32, i32, ctypes.POINTER(Objective), vp, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -182,3 +182,32 @@ def gpu_run(case, f_storage, nsteps, stats=True):
         return out, st
     finally:
         lat.close()
+
+
+def uneven_blocks(shape, cuts):
+    """Tiles a domain with boxes: cuts[a] lists the interior cut positions along
+    axis a (a block ends at cut - 1). Returns inclusive (x0,x1,y0,y1,z0,z1) boxes."""
+    edges = []
+    for a in range(3):
+        e = [0] + sorted(cuts[a]) + [shape[a]]
+        edges.append([(e[k], e[k + 1] - 1) for k in range(len(e) - 1)])
+    return [(x0, x1, y0, y1, z0, z1) for (x0, x1) in edges[0] for (y0, y1) in edges[1] for (z0, z1) in edges[2]]
+
+
+def seeded_uneven_blocks(shape, counts, lo, hi, seed):
+    """`counts[a]` pieces per axis with lengths in [lo, hi] summing to the extent,
+    by rejection sampling from a seeded generator; returns the tiling boxes."""
+    rng = np.random.RandomState(seed)
+    cuts = []
+    for a in range(3):
+        n, k = shape[a], counts[a]
+        if not (k * lo <= n <= k * hi):
+            raise ValueError(f"axis {a}: {k} pieces in [{lo}, {hi}] cannot sum to {n}")
+        while True:
+            lengths = list(rng.randint(lo, hi + 1, size=k - 1))
+            last = n - sum(lengths)
+            if lo <= last <= hi:
+                lengths.append(last)
+                break
+        cuts.append([int(c) for c in np.cumsum(lengths)[:-1]])
+    return uneven_blocks(shape, cuts)

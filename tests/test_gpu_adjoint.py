@@ -1,0 +1,150 @@
+"""Hand-written adjoint of the BGK + Brinkman collide-and-stream (config:
+Brinkman porosity channel). Checked against the oracle's forward-mode
+(dual-number) derivatives on a reduced channel, and against central finite
+differences (eps = 1e-4) of the GPU forward objective on the full
+128 x 64 x 64, 500-step case, checkpointing every 50 steps."""
+import copy
+
+import numpy as np
+import pytest
+
+import palabos_amd
+from cases import channel_case, gpu_lattice, rest_state
+from oracle import oracle
+from palabos_amd import setups
+
+pytestmark = pytest.mark.gpu
+
+
+def brinkman_channel(nx, ny, nz, omega, u_max, seed=4, n_blobs=32):
+    case = channel_case(nx, ny, nz, omega, u_max)
+    alpha = setups.brinkman_blobs(case.shape, n_blobs=n_blobs, seed=seed)
+    alpha[0] = 0.0
+    alpha[-1] = 0.0      # no porosity on the inlet / outlet planes
+    case.alpha = alpha
+    return case
+
+
+def seeded_cells(alpha, seed=7, lo=0.2, hi=0.8):
+    rng = np.random.RandomState(seed)
+    flat = alpha.ravel()
+    pool = np.flatnonzero((flat > lo) & (flat < hi))
+    return np.sort(rng.choice(pool, 16, replace=False))
+
+
+def gpu_gradient(case, T, K, xin, xout):
+    lat = gpu_lattice(case)
+    try:
+        lat.setPopulations(rest_state(case))
+        return lat.adjointGradient(T, K, xin, xout, 1.0, 1.0)
+    finally:
+        lat.close()
+
+
+def test_adjoint_matches_oracle_dual_numbers(gpu):
+    case = brinkman_channel(24, 12, 12, 1.6, 0.04, n_blobs=6)
+    T, K, xin, xout = 40, 10, 1, case.nx - 2
+    seeds = seeded_cells(case.alpha)
+    J, ga, gs, _ = gpu_gradient(case, T, K, xin, xout)
+    Jd, dJa, dJs = oracle.objective_dual(case, rest_state(case), T, xin, xout, 1.0, 1.0, seeds)
+    assert J == pytest.approx(Jd, rel=1e-12)
+    np.testing.assert_allclose(ga.ravel()[seeds], dJa, rtol=1e-8, atol=1e-14)
+    assert gs[0] == pytest.approx(dJs, rel=1e-9)
+    assert gs[1] == 0.0   # the pressure plane has no velocity to scale
+
+
+def test_adjoint_config4_against_finite_differences(gpu):
+    """128 x 64 x 64 channel, Brinkman porosity from 32 seeded Gaussian blobs through
+    a sigmoid, inlet u_max = 0.04, omega = 1.6, 500 steps, checkpoint every 50;
+    J = mean over steps of (pressure drop + FP64 Brinkman drag). Gradients with
+    respect to alpha on 16 seeded cells and to u_max agree with central finite
+    differences (eps = 1e-4) to 1e-3.
+
+    J is about 680 (the drag sums over the whole lattice), so FP64 rounding in the
+    two perturbed runs leaves about 1e-9 of noise in a finite-difference dJ/dalpha:
+    the 16 cells are drawn from those where the adjoint gradient is at least 1e-6
+    (partly porous cells, 0.05 < alpha < 0.95), and four more cells drawn from all
+    partly porous ones are checked to 1e-3 relative or 5e-9 absolute."""
+    case = brinkman_channel(128, 64, 64, 1.6, 0.04)
+    T, K, xin, xout = 500, 50, 1, case.nx - 2
+    J, ga, gs, (ms_f, ms_b) = gpu_gradient(case, T, K, xin, xout)
+    print(f"[adjoint] J={J:.12e} forward {ms_f:.1f} ms backward {ms_b:.1f} ms ratio {ms_b / ms_f:.2f}", flush=True)
+    flat_a, flat_g = case.alpha.ravel(), ga.ravel()
+    porous = (flat_a > 0.05) & (flat_a < 0.95)
+    rng = np.random.RandomState(7)
+    strong = np.sort(rng.choice(np.flatnonzero(porous & (np.abs(flat_g) >= 1e-6)), 16, replace=False))
+    any_cells = np.sort(rng.choice(np.flatnonzero(porous), 4, replace=False))
+    eps = 1e-4
+
+    def J_of(c):
+        return gpu_gradient(c, T, K, xin, xout)[0]
+
+    def fd_alpha(s):
+        cp, cm = copy.deepcopy(case), copy.deepcopy(case)
+        cp.alpha.ravel()[s] += eps
+        cm.alpha.ravel()[s] -= eps
+        return (J_of(cp) - J_of(cm)) / (2 * eps)
+
+    worst = 0.0
+    for s in strong:
+        fd = fd_alpha(s)
+        err = abs(flat_g[s] - fd) / abs(fd)
+        worst = max(worst, err)
+        assert err <= 1e-3, (s, flat_g[s], fd)
+    for s in any_cells:
+        fd = fd_alpha(s)
+        assert abs(flat_g[s] - fd) <= max(1e-3 * abs(fd), 5e-9), (s, flat_g[s], fd)
+    # inlet velocity: u_max -> u_max (1 +- eps), dJ/du_max = dJ/ds / u_max
+    cp, cm = copy.deepcopy(case), copy.deepcopy(case)
+    cp.planes[0].values = case.planes[0].values * (1 + eps)
+    cm.planes[0].values = case.planes[0].values * (1 - eps)
+    fd_s = (J_of(cp) - J_of(cm)) / (2 * eps)
+    assert abs(gs[0] - fd_s) / abs(fd_s) <= 1e-3
+    print(f"[adjoint] worst alpha relative error {worst:.2e}; dJ/du_max adjoint {gs[0] / 0.04:.10e} "
+          f"fd {fd_s / 0.04:.10e}", flush=True)
+
+
+def test_adjoint_refuses_unsupported_lattices(gpu):
+    case = brinkman_channel(16, 8, 8, 1.6, 0.04, n_blobs=2)
+    for mutate in ("f32", "tags", "odd"):
+        c = copy.deepcopy(case)
+        if mutate == "f32":
+            c.storage = "f32"
+        if mutate == "tags":
+            c.tags = np.zeros(c.shape, np.uint8)
+            c.tags[5, 4, 4] = 1
+        lat = gpu_lattice(c)
+        try:
+            if mutate == "odd":
+                lat.collideAndStream(1)
+            with pytest.raises(palabos_amd.PlbError):
+                lat.adjointGradient(10, 4, 1, 14)
+        finally:
+            lat.close()
+
+
+def test_autograd_function_chains_the_adjoint(gpu):
+    """torch.autograd through BrinkmanObjective: d(J^2)/d(alpha), d(J^2)/ds equal
+    2 J times the oracle's dual-number derivatives (inlet scale s = 1.25)."""
+    import torch
+    from palabos_amd.adjoint import AdjointConfig, BrinkmanObjective
+    case = brinkman_channel(20, 10, 10, 1.6, 0.04, n_blobs=4)
+    T, K, xin, xout = 24, 8, 1, case.nx - 2
+    s = 1.25
+    base = case.planes[0].values.copy()
+
+    def factory():
+        lat = gpu_lattice(case)
+        return lat, rest_state(case), 0, base
+
+    alpha = torch.tensor(case.alpha, dtype=torch.float64, requires_grad=True)
+    scale = torch.tensor(s, dtype=torch.float64, requires_grad=True)
+    J = BrinkmanObjective.apply(alpha, scale, factory, AdjointConfig(T, K, xin, xout))
+    (J * J).backward()
+    seeds = seeded_cells(case.alpha, seed=3)
+    scaled = copy.deepcopy(case)
+    scaled.planes[0].values = base * s
+    Jd, dJa, dJs = oracle.objective_dual(scaled, rest_state(case), T, xin, xout, 1.0, 1.0, seeds)
+    assert float(J) == pytest.approx(Jd, rel=1e-12)
+    np.testing.assert_allclose(alpha.grad.numpy().ravel()[seeds], 2 * Jd * dJa, rtol=1e-8, atol=1e-14)
+    assert float(scale.grad) == pytest.approx(2 * Jd * dJs / s, rel=1e-9)

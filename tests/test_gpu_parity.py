@@ -11,7 +11,8 @@ import os
 import numpy as np
 import pytest
 
-from cases import Case, TAG_BB, TAG_WALL, cavity_case, channel_case, gpu_lattice, gpu_run, random_state
+from cases import (Case, TAG_BB, TAG_WALL, cavity_case, channel_case, gpu_lattice, gpu_run, random_state,
+                   sphere_tags)
 from golden_cases import GOLDEN, all_faces as _all_faces, load, walls_case as _walls
 from oracle import oracle
 
@@ -43,7 +44,28 @@ CASES = {
     "brinkman_rbgk": lambda s: (Case(8, 9, 10, storage=s, dynamics="rbgk", omega=1.6, periodic=(True, False, True),
                                      force=(2e-5, 0, 0),
                                      alpha=0.5 + 0.5 * np.sin(np.arange(720).reshape(8, 9, 10) * 0.37)), 11),
+    "channel_porous_walls_rbgk": lambda s: (_porous_channel(s), 14),
+    "forced_porous_walls_rbgk": lambda s: (_forced_walls(s), 14),
 }
+
+
+def _porous_channel(storage):
+    """Zou-He planes, a link-wise wall sphere, RBGK with porosity: plane cells and
+    cells next to the wall take the damped collision too."""
+    case = channel_case(16, 10, 11, 1.8, 0.05, storage=storage, sphere=((6, 5, 5), 2.2), sphere_tag=TAG_WALL,
+                        dynamics="rbgk")
+    case.alpha = 0.2 + 0.2 * np.cos(np.arange(16 * 10 * 11).reshape(16, 10, 11) * 0.23)
+    return case
+
+
+def _forced_walls(storage):
+    """Body force, porosity and a link-wise wall sphere (cells next to the wall
+    take the forced collision), periodic in x and y."""
+    case = Case(12, 10, 11, storage=storage, dynamics="rbgk", omega=1.7, periodic=(True, True, False),
+                force=(1e-5, 0.0, 2e-6))
+    case.tags = sphere_tags(case.shape, (6, 5, 5), 2.6, TAG_WALL)
+    case.alpha = 0.2 + 0.2 * np.cos(np.arange(12 * 10 * 11).reshape(12, 10, 11) * 0.23)
+    return case
 
 
 def _state(case, seed=7):

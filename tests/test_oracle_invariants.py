@@ -217,3 +217,45 @@ def test_taylor_green_2d_mode_decays_at_the_lattice_viscosity():
     nu = (1 / omega - 0.5) / 3
     measured = -np.log(energy(f1) / e0) / (4 * k * k * steps)
     assert measured == pytest.approx(nu, rel=5e-3)
+
+
+# -- forward-mode derivatives (the adjoint's CPU reference) --------------------
+
+def test_dual_number_derivatives_match_finite_differences():
+    """The oracle's dual-number derivatives of the adjoint objective equal central
+    finite differences of plain FP64 oracle runs (porosity cells and inlet scale)."""
+    import copy
+    from cases import channel_case, rest_state
+    from palabos_amd import setups
+    case = channel_case(16, 8, 8, 1.6, 0.04)
+    alpha = setups.brinkman_blobs(case.shape, n_blobs=4, sigma=(2, 3), seed=1)
+    alpha[0] = 0
+    alpha[-1] = 0
+    case.alpha = alpha
+    f0 = rest_state(case)
+    seeds = np.random.RandomState(0).choice(np.flatnonzero(alpha.ravel() > 0.1), 16, replace=False)
+    T, xin, xout = 12, 1, 14
+    J, dJa, dJs = oracle.objective_dual(case, f0, T, xin, xout, 1.0, 1.0, seeds)
+
+    def J_of(c):
+        f, tot = f0.copy(), 0.0
+        for _ in range(T):
+            f, _ = oracle.run(c, f, 1)
+            rho = f.sum(-1) + 1
+            jx = f @ C[:, 0]
+            plane = c.ny * c.nz
+            tot += ((rho[xin].sum() - rho[xout].sum()) / (3 * plane) + c.omega * (c.alpha * jx).sum()) / T
+        return tot
+
+    assert J == pytest.approx(J_of(case), rel=1e-13)
+    eps = 1e-6
+    for k in range(3):
+        cp, cm = copy.deepcopy(case), copy.deepcopy(case)
+        cp.alpha.ravel()[seeds[k]] += eps
+        cm.alpha.ravel()[seeds[k]] -= eps
+        fd = (J_of(cp) - J_of(cm)) / (2 * eps)
+        assert dJa[k] == pytest.approx(fd, rel=2e-3, abs=1e-11)
+    cp, cm = copy.deepcopy(case), copy.deepcopy(case)
+    cp.planes[0].values = case.planes[0].values * (1 + eps)
+    cm.planes[0].values = case.planes[0].values * (1 - eps)
+    assert dJs == pytest.approx((J_of(cp) - J_of(cm)) / (2 * eps), rel=1e-7)
