@@ -154,6 +154,19 @@ def traffic_from_profile(precision, n):
     return None
 
 
+def copy_calibration(n_doubles=1 << 29, repeats=3):
+    """A plain device-to-device copy of known size (torch's copy kernel): the
+    reference that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    src = torch.ones(n_doubles, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(repeats):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n_doubles * 8, "copy_calibration_bytes_written": n_doubles * 8,
+                      "launches": repeats}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,12 +177,20 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true", help="print only the adjoint timing object")
+    ap.add_argument("--copy-calibration", action="store_true", help="run only the counter-calibration copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the lattice would shard into x-slabs with per-face halo exchange over RCCL; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_ratio()}))
+        return
 
     main_run = run_precision(args.precision, args.n, args.steps, args.warmup)
     variants = []

@@ -79,6 +79,9 @@ struct Lattice {
     double* dAlpha = nullptr;
     double* dBulkPartials = nullptr;
     uint64_t nBulkPartials = 0;       // = workgroups of the bulk launch
+    int xcdRemap = 0;
+    bool staged = false;           // odd steps through bulk_odd_staged
+    size_t stagedLds = 0;          // its dynamic LDS bytes
     double* dSpecialPartials = nullptr;
     uint64_t nSpecialPartials = 0;
     double* dStats = nullptr;         // 16 doubles
@@ -156,8 +159,13 @@ int setup(Lattice& L) {
     const int per[3] = {d.periodic[0] != 0, d.periodic[1] != 0, d.periodic[2] != 0};
     // Launch shape: TX threads along z (a multiple of the 64-lane wavefront),
     // CPT cells each per row segment; TY rows per workgroup when a row is short.
-    int txMax = 128, cpt = 1;
-    if (const char* e = getenv("PLBAMD_BULK_TX")) txMax = std::max(64, std::min(256, atoi(e) / 64 * 64));
+    // Measured on MI355X (512^3 periodic): 256 threads for FP64, 512 for the
+    // 4- and 2-byte storages (fewer, larger workgroups for the cheaper cells),
+    // workgroups remapped so that neighbouring row segments share an XCD.
+    int txMax = storageSize(d.storage) >= 8 ? 256 : 512, cpt = 1;
+    L.xcdRemap = 1;
+    if (const char* e = getenv("PLBAMD_XCD_REMAP")) L.xcdRemap = atoi(e) != 0;
+    if (const char* e = getenv("PLBAMD_BULK_TX")) txMax = std::max(64, std::min(1024, atoi(e) / 64 * 64));
     if (const char* e = getenv("PLBAMD_BULK_CPT")) cpt = std::max(1, std::min(16, atoi(e)));
     int maxNz = 0;
     L.geom.clear();
@@ -183,10 +191,21 @@ int setup(Lattice& L) {
         L.geom.push_back(g);
     }
     if (L.multi && (int64_t)L.geom.size() > 65535) return fail(PLBAMD_EINVAL, "multiblock: at most 65535 blocks");
+    // Odd steps of FP64 and FP32 lattices go through the LDS-staged kernel (one
+    // row segment per workgroup; measured 9 % and 13 % faster at 512^3); 16-bit
+    // storage keeps the unstaged kernel, which is faster there (its loads are
+    // narrow enough that the extra LDS traffic costs more than the misaligned
+    // lines). PLBAMD_STAGED=0/1 overrides. The unstaged kernel packs several
+    // short rows into one workgroup.
+    bool staged = cpt == 1 && storageSize(d.storage) >= 4;
+    if (const char* e = getenv("PLBAMD_STAGED")) staged = cpt == 1 && atoi(e) != 0;
     const int tx = (int)std::min<int64_t>(txMax, ((int64_t)maxNz + 63) / 64 * 64);
-    const int ty = std::max(1, txMax / tx);
+    const int ty = staged ? 1 : std::max(1, txMax / tx);
     L.bulkBlock = dim3(tx, ty);
     L.zChunk = tx * cpt;
+    L.staged = staged;
+    L.stagedLds = (size_t)kZMoving * (tx + 2) * std::max<size_t>(4, storageSize(d.storage)) + (size_t)tx;
+    if (L.stagedLds > 65536) L.staged = false;   // stay within the default LDS limit per workgroup
     uint64_t offset = 0, cellOffset = 0;
     uint64_t wg = 0;
     for (BlockGeom& g : L.geom) {
@@ -440,6 +459,7 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
     a.tags = L.dTags;
     a.alpha = L.dAlpha;
     a.partials = L.dBulkPartials;
+    a.xcdRemap = L.xcdRemap;
     const bool forced = L.dAlpha || L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0;
     const bool tags = L.dTags != nullptr;
     const dim3 grid((unsigned)L.nBulkPartials);
@@ -448,8 +468,14 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
             if (stats) return fail(PLBAMD_EINVAL, "recording step: statistics must be off");
             a.moments = record->moments;
             a.momStride = record->momStride;
-#define PLBAMD_REC(O, TG, FO) \
-    hipLaunchKernelGGL((bulk_step<double, 0, O, TG, false, FO, true>), grid, L.bulkBlock, 0, L.stream, a)
+#define PLBAMD_REC(O, TG, FO)                                                                                   \
+    do {                                                                                                        \
+        if (O && L.staged)                                                                                      \
+            hipLaunchKernelGGL((bulk_odd_staged<double, 0, TG, false, FO, true>), grid, L.bulkBlock,             \
+                               L.stagedLds, L.stream, a);                                                       \
+        else                                                                                                    \
+            hipLaunchKernelGGL((bulk_step<double, 0, O, TG, false, FO, true>), grid, L.bulkBlock, 0, L.stream, a); \
+    } while (0)
 #define PLBAMD_REC_F(O, TG) if (forced) PLBAMD_REC(O, TG, true); else PLBAMD_REC(O, TG, false)
 #define PLBAMD_REC_T(O) if (tags) { PLBAMD_REC_F(O, true); } else { PLBAMD_REC_F(O, false); }
             if (odd) { PLBAMD_REC_T(true) } else { PLBAMD_REC_T(false) }
@@ -460,8 +486,14 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
             return fail(PLBAMD_EINVAL, "recording step: FP64 BGK only");
         }
     } else {
-#define PLBAMD_BULK(O, TG, ST, FO) \
-    hipLaunchKernelGGL((bulk_step<S, DYN, O, TG, ST, FO>), grid, L.bulkBlock, 0, L.stream, a)
+#define PLBAMD_BULK(O, TG, ST, FO)                                                                              \
+    do {                                                                                                        \
+        if (O && L.staged)                                                                                      \
+            hipLaunchKernelGGL((bulk_odd_staged<S, DYN, TG, ST, FO>), grid, L.bulkBlock, L.stagedLds, L.stream,  \
+                               a);                                                                              \
+        else                                                                                                    \
+            hipLaunchKernelGGL((bulk_step<S, DYN, O, TG, ST, FO>), grid, L.bulkBlock, 0, L.stream, a);         \
+    } while (0)
 #define PLBAMD_BULK_F(O, TG, ST) if (forced) PLBAMD_BULK(O, TG, ST, true); else PLBAMD_BULK(O, TG, ST, false)
 #define PLBAMD_BULK_S(O, TG) if (stats) { PLBAMD_BULK_F(O, TG, true); } else { PLBAMD_BULK_F(O, TG, false); }
 #define PLBAMD_BULK_T(O) if (tags) { PLBAMD_BULK_S(O, true) } else { PLBAMD_BULK_S(O, false) }

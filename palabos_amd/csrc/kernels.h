@@ -30,6 +30,8 @@
 // (multiBlock/multiBlockLattice3D.hh:318-360).
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -103,28 +105,37 @@ __device__ __forceinline__ int find_block(const BlockGeom* blocks, int nBlocks, 
     return lo;
 }
 
-// Block reduction of an 8-double record (fixed order: deterministic). Slot 2
-// is a maximum, the others are sums.
+// Block reduction of an 8-double record (fixed order: deterministic): a
+// shuffle tree inside each wavefront, then the wavefront sums in order. Slot 2
+// is a maximum, the others are sums. Up to 1024 threads per workgroup.
 __device__ __forceinline__ void block_reduce_store(double v[8], double* out) {
-    __shared__ double red[256][8];
+    __shared__ double red[16][8];
     const int t = threadIdx.x + threadIdx.y * blockDim.x;
     const int n = blockDim.x * blockDim.y;
+    const int lane = t & 63, wave = t >> 6, nWaves = (n + 63) >> 6;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) red[t][k] = v[k];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (t < s && t + s < n) {
+    for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (k == 2) red[t][k] = fmax(red[t][k], red[t + s][k]);
-                else red[t][k] += red[t + s][k];
-            }
+        for (int k = 0; k < 8; ++k) {
+            const double o = __shfl_down(v[k], off, 64);
+            if (lane + off < 64 && t + off < n) v[k] = (k == 2) ? fmax(v[k], o) : v[k] + o;
         }
-        __syncthreads();
     }
-    if (t == 0) {
+    if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) out[k] = red[0][k];
+        for (int k = 0; k < 8; ++k) red[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (t == 0) {
+        double r[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r[k] = red[0][k];
+        for (int w = 1; w < nWaves; ++w) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) r[k] = (k == 2) ? fmax(r[k], red[w][k]) : r[k] + red[w][k];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) out[k] = r[k];
     }
     __syncthreads();
 }
@@ -155,7 +166,18 @@ struct StepArgs {
     double* partials;        // 8 doubles per workgroup when statistics are on
     double* moments;         // MOMENTS: rhoBar, jx, jy, jz of the pre-collision state,
     uint64_t momStride;      //          [4][momStride] by cell (all blocks)
+    int xcdRemap;            // 1: consecutive work items on the same XCD
 };
+
+// The dispatcher hands workgroup b to XCD b % 8. With the remap, XCD k takes
+// the k-th contiguous eighth of the work items (the z-chunks of a row and the
+// neighbouring rows share that XCD's L2); the tail beyond a multiple of 8 maps
+// to itself. A bijection on [0, n).
+__device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
+    const uint32_t q = n / 8;
+    if (b >= 8 * q) return b;
+    return (b % 8) * q + b / 8;
+}
 
 // Pre-collision moments rhoBar = sum_i f_i, j = sum_i c_i f_i (plain order),
 // recorded for the adjoint's reverse sweep.
@@ -176,10 +198,45 @@ __device__ __forceinline__ void store_moments(const T* fc, double* m, uint64_t s
     m[3 * stride + idx] = jz;
 }
 
-template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
-__global__ void __launch_bounds__(256) bulk_step(StepArgs a) {
+// Collision of one cell by its bulk tag: background dynamics (with moments
+// recording and statistics), bounce-back swap, or nothing (NoDynamics).
+template<typename S, int DYN, bool STATS, bool FORCED, bool MOMENTS>
+__device__ __forceinline__ void collide_cell(const StepArgs& a, const BlockGeom& g, uint64_t cell, uint8_t tag,
+                                             bool inBulk, typename Store<S>::T fc[19], double st[8]) {
     using T = typename Store<S>::T;
-    const uint32_t wg = blockIdx.x;
+    if constexpr (MOMENTS) {
+        if (tag == BT_BULK) store_moments(fc, a.moments, a.momStride, g.cellOffset + cell);
+    }
+    if (tag == BT_BULK) {
+        const T omega = (T)a.omega;
+        const T F[3] = {(T)a.force[0], (T)a.force[1], (T)a.force[2]};
+        T rhoBar;
+        T drag[3] = {0, 0, 0};
+        const T alpha = (FORCED && a.alpha) ? (T)a.alpha[g.cellOffset + cell] : (T)0;
+        const T uSqr = collide_bulk<T, DYN, FORCED>(fc, omega, F, alpha, rhoBar, drag);
+        if constexpr (STATS) {
+            if (inBulk) {
+                st[0] += (double)rhoBar;
+                st[1] += (double)uSqr;
+                st[2] = fmax(st[2], (double)uSqr);
+                st[3] += 1.0;
+                if constexpr (FORCED) {
+                    st[4] += (double)drag[0];
+                    st[5] += (double)drag[1];
+                    st[6] += (double)drag[2];
+                }
+            }
+        }
+    } else if (tag == BT_BOUNCE_BACK) {
+#pragma unroll
+        for (int i = 1; i <= 9; ++i) { T t = fc[i]; fc[i] = fc[i + 9]; fc[i + 9] = t; }
+    }
+}
+
+template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
+__global__ void __launch_bounds__(1024) bulk_step(StepArgs a) {
+    using T = typename Store<S>::T;
+    const uint32_t wg = a.xcdRemap ? xcd_item(blockIdx.x, gridDim.x) : blockIdx.x;
     const int b = find_block(a.blocks, a.nBlocks, wg);
     const BlockGeom g = a.blocks[b];
     S* __restrict__ f = reinterpret_cast<S*>(a.arena) + g.offset;
@@ -191,8 +248,6 @@ __global__ void __launch_bounds__(256) bulk_step(StepArgs a) {
     const int y = __builtin_amdgcn_readfirstlane(yGroup * (int)blockDim.y + (int)threadIdx.y);
     const int zBegin = chunk * a.zChunk;
     const int zEnd = min(g.nz, zBegin + a.zChunk);
-    const T omega = (T)a.omega;
-    const T F[3] = {(T)a.force[0], (T)a.force[1], (T)a.force[2]};
     double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
     if (y < g.ny) {
@@ -253,31 +308,8 @@ __global__ void __launch_bounds__(256) bulk_step(StepArgs a) {
                 }
             }
 
-            if constexpr (MOMENTS) {
-                if (tag == BT_BULK) store_moments(fc, a.moments, a.momStride, g.cellOffset + cell);
-            }
-            if (tag == BT_BULK) {
-                T rhoBar;
-                T drag[3] = {0, 0, 0};
-                const T alpha = (FORCED && a.alpha) ? (T)a.alpha[g.cellOffset + cell] : (T)0;
-                const T uSqr = collide_bulk<T, DYN, FORCED>(fc, omega, F, alpha, rhoBar, drag);
-                if constexpr (STATS) {
-                    if (rowInBulkXY && z >= g.bz0 && z <= g.bz1) {
-                        st[0] += (double)rhoBar;
-                        st[1] += (double)uSqr;
-                        st[2] = fmax(st[2], (double)uSqr);
-                        st[3] += 1.0;
-                        if constexpr (FORCED) {
-                            st[4] += (double)drag[0];
-                            st[5] += (double)drag[1];
-                            st[6] += (double)drag[2];
-                        }
-                    }
-                }
-            } else if (tag == BT_BOUNCE_BACK) {
-#pragma unroll
-                for (int i = 1; i <= 9; ++i) { T t = fc[i]; fc[i] = fc[i + 9]; fc[i + 9] = t; }
-            }
+            collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, rowInBulkXY && z >= g.bz0 && z <= g.bz1,
+                                                       fc, st);
 
             if constexpr (!ODD) {
 #pragma unroll
@@ -299,6 +331,207 @@ __global__ void __launch_bounds__(256) bulk_step(StepArgs a) {
                                             : (uint64_t)opp(i) * g.slot + cell;
                     Store<S>::store(f + dst, fc[i]);
                 }
+            }
+        }
+    }
+    if constexpr (STATS) block_reduce_store(st, a.partials + 8 * (uint64_t)wg);
+}
+
+// ---------------------------------------------------------------------------
+// Odd (streaming) step with the z-moving populations staged through LDS.
+//
+// In the AA odd step a cell gathers slot opp(i) from x - c_i and scatters to
+// slot i at x + c_i. For the 10 directions with c_z != 0 those accesses are
+// shifted by one element along z, so every wavefront access straddles one
+// more cache line than it uses (one line in two for 16-bit storage). Here a
+// workgroup takes one z-segment of one row (one cell per thread): it loads
+// the segment of each z-moving source row aligned into LDS (plus one halo
+// element at each end), reads the shifted values from LDS, and writes the
+// shifted outputs back through LDS so that the global stores are aligned too.
+// The 9 other directions are aligned already and go straight to registers.
+// Rows at a non-periodic x/y edge of the block take the per-lane general path.
+
+// Index of direction i among the z-moving directions, or -1.
+__host__ __device__ constexpr int zmove_index(int i) {
+    int k = 0;
+    for (int j = 0; j < i; ++j) k += cz(j) != 0 ? 1 : 0;
+    return cz(i) != 0 ? k : -1;
+}
+constexpr int kZMoving = 10;
+
+template<typename S, int DYN, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
+__global__ void __launch_bounds__(1024) bulk_odd_staged(StepArgs a) {
+    using T = typename Store<S>::T;
+    using R = typename Store<S>::R;
+    // LDS words of at least 32 bits: 16-bit values would put two lanes on one bank word
+    using L = typename std::conditional<(sizeof(R) < 4), uint32_t, R>::type;
+    extern __shared__ __align__(16) unsigned char ldsRaw[];
+    const int W = (int)blockDim.x;            // segment length: one cell per thread
+    const int P = W + 2;                      // pitch: low halo, segment, high halo
+    L* buf = reinterpret_cast<L*>(ldsRaw);
+    uint8_t* skipped = ldsRaw + (size_t)kZMoving * P * sizeof(L);   // TAGS only: W flags
+
+    const uint32_t wg = a.xcdRemap ? xcd_item(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int b = find_block(a.blocks, a.nBlocks, wg);
+    const BlockGeom g = a.blocks[b];
+    S* __restrict__ f = reinterpret_cast<S*>(a.arena) + g.offset;
+    const uint32_t local = wg - g.wgBegin;
+    const int chunk = (int)(local % (uint32_t)g.nzChunks);
+    const uint32_t rest = local / (uint32_t)g.nzChunks;
+    const int y = (int)(rest % (uint32_t)g.nyGroups);
+    const int x = (int)(rest / (uint32_t)g.nyGroups);
+    const int z0 = chunk * W;
+    const int len = min(g.nz - z0, W);
+    const int t = (int)threadIdx.x;
+    const int z = z0 + t;
+    const bool active = t < len;
+    const uint64_t rowBase = ((uint64_t)x * g.ny + y) * g.nz;
+    const uint64_t cell = rowBase + z;
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    int xm = 0, xp = 0, ym = 0, yp = 0;
+    const bool vxm = shift(x, -1, g.nx, g.px, xm), vxp = shift(x, 1, g.nx, g.px, xp);
+    const bool vym = shift(y, -1, g.ny, g.py, ym), vyp = shift(y, 1, g.ny, g.py, yp);
+    const int xs[3] = {xm, x, xp};
+    const int ys[3] = {ym, y, yp};
+    const bool vx[3] = {vxm, true, vxp};
+    const bool vy[3] = {vym, true, vyp};
+    const bool rowsValid = vxm && vxp && vym && vyp;
+    const bool inBulk = x >= g.bx0 && x <= g.bx1 && y >= g.by0 && y <= g.by1 && z >= g.bz0 && z <= g.bz1;
+
+    uint8_t tag = BT_BULK;
+    if constexpr (TAGS) {
+        if (active) tag = a.tags[g.cellOffset + cell];
+    }
+    const bool compute = active && tag != BT_SKIP;
+    T fc[19];
+
+    if (!rowsValid) {
+        // general path: per-lane neighbour validity in x, y and z
+        if (compute) {
+            int zm = 0, zp = 0;
+            const bool vzm = shift(z, -1, g.nz, g.pz, zm), vzp = shift(z, 1, g.nz, g.pz, zp);
+            const int zs[3] = {zm, z, zp};
+            const bool vz[3] = {vzm, true, vzp};
+            fc[0] = Store<S>::load(f + cell);
+#pragma unroll
+            for (int i = 1; i < 19; ++i) {
+                const int ax = 1 - cx(i), ay = 1 - cy(i), az = 1 - cz(i);
+                const bool ok = vx[ax] && vy[ay] && vz[az];
+                const uint64_t src = ok ? (uint64_t)opp(i) * g.slot + (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
+                                        : (uint64_t)i * g.slot + cell;
+                fc[i] = Store<S>::load(f + src);
+            }
+            collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, inBulk, fc, st);
+            Store<S>::store(f + cell, fc[0]);
+#pragma unroll
+            for (int i = 1; i < 19; ++i) {
+                const int ax = 1 + cx(i), ay = 1 + cy(i), az = 1 + cz(i);
+                const bool ok = vx[ax] && vy[ay] && vz[az];
+                const uint64_t dst = ok ? (uint64_t)i * g.slot + (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
+                                        : (uint64_t)opp(i) * g.slot + cell;
+                Store<S>::store(f + dst, fc[i]);
+            }
+        }
+    } else {
+        S* rows[3][3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+            for (int ay = 0; ay < 3; ++ay)
+                rows[ax][ay] = f + ((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz;
+        int zlo = 0, zhi = 0;
+        const bool vlo = shift(z0, -1, g.nz, g.pz, zlo);
+        const bool vhi = shift(z0 + len - 1, 1, g.nz, g.pz, zhi);
+
+        // 1. loads, all issued before any is consumed: the 9 aligned directions
+        // to registers, the 10 z-moving source-row elements to registers and then
+        // to LDS. Inactive lanes (past the row end) reload the segment's first
+        // cell and keep their results to themselves.
+        const int zc = active ? z : z0;
+        R raw[kZMoving];
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            const S* src = rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot;
+            if (cz(i) == 0) fc[i] = Store<S>::load(src + zc);
+            else raw[zmove_index(i)] = Store<S>::load_raw(src + zc);
+        }
+        R halo[kZMoving];
+        const bool loLane = t == 0 && vlo, hiLane = t == W - 1 && vhi;
+        if (loLane || hiLane) {
+            const int zh = loLane ? zlo : zhi;
+#pragma unroll
+            for (int i = 0; i < 19; ++i)
+                if (cz(i) != 0)
+                    halo[zmove_index(i)] = Store<S>::load_raw(rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot + zh);
+        }
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < kZMoving; ++k) buf[k * P + t + 1] = raw[k];
+        }
+        if (loLane || hiLane) {
+            const int hp = loLane ? 0 : len + 1;
+#pragma unroll
+            for (int k = 0; k < kZMoving; ++k) buf[k * P + hp] = halo[k];
+        }
+        if constexpr (TAGS) {
+            if (active) skipped[t] = compute ? 0 : 1;
+        }
+        __syncthreads();
+        if (compute) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                if (cz(i) != 0) {
+                    const int k = zmove_index(i);
+                    int zs = 0;
+                    const bool ok = shift(z, -cz(i), g.nz, g.pz, zs);
+                    fc[i] = ok ? Store<S>::from_raw((R)buf[k * P + t + 1 - cz(i)])
+                               : Store<S>::load(f + (uint64_t)i * g.slot + cell);   // edge reflection
+                }
+            }
+        }
+        __syncthreads();   // every LDS input is read before the outputs reuse the buffer
+
+        // 2. collision, aligned outputs to global, z-moving outputs to LDS
+        if (compute) {
+            collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, inBulk, fc, st);
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                if (cz(i) == 0) {
+                    Store<S>::store(rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot + z, fc[i]);
+                } else {
+                    const int k = zmove_index(i);
+                    int zd = 0;
+                    if (shift(z, cz(i), g.nz, g.pz, zd)) buf[k * P + t + 1 + cz(i)] = Store<S>::to_raw(fc[i]);
+                    else Store<S>::store(f + (uint64_t)opp(i) * g.slot + cell, fc[i]);   // edge reflection
+                }
+            }
+        }
+        __syncthreads();
+
+        // 3. aligned stores of the z-moving destination rows; position q takes the
+        // output of cell q - c_z when that cell is in this segment and was computed
+#pragma unroll
+        for (int k = 0; k < kZMoving; ++k) raw[k] = (R)buf[k * P + t + 1];
+        if (loLane || hiLane) {
+            const int hp = loLane ? 0 : len + 1;
+#pragma unroll
+            for (int k = 0; k < kZMoving; ++k) halo[k] = (R)buf[k * P + hp];
+        }
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            if (cz(i) != 0) {
+                const int k = zmove_index(i);
+                S* dst = rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot;
+                const int p = t - cz(i);
+                bool ok = active && p >= 0 && p < len;
+                if constexpr (TAGS) ok = ok && !skipped[p < 0 ? 0 : (p >= len ? 0 : p)];
+                if (ok) Store<S>::store_raw(dst + z, raw[k]);
+                // halo position: the low one takes cell 0's output for c_z < 0,
+                // the high one cell len-1's output for c_z > 0
+                bool h = cz(i) < 0 ? loLane : hiLane;
+                if constexpr (TAGS) h = h && !skipped[cz(i) < 0 ? 0 : len - 1];
+                if (h) Store<S>::store_raw(dst + (cz(i) < 0 ? zlo : zhi), halo[k]);
             }
         }
     }

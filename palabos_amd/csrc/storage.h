@@ -14,6 +14,11 @@ struct bf16s { uint16_t b; };
 template<typename S> struct Store;
 template<> struct Store<double> {
     using T = double;
+    using R = double;   // raw stored value
+    static __device__ __forceinline__ R load_raw(const double* p) { return __builtin_nontemporal_load(p); }
+    static __device__ __forceinline__ void store_raw(double* p, R v) { __builtin_nontemporal_store(v, p); }
+    static __device__ __forceinline__ T from_raw(R v) { return v; }
+    static __device__ __forceinline__ R to_raw(T v) { return v; }
     static __device__ __forceinline__ double load(const double* p) { return __builtin_nontemporal_load(p); }
     static __device__ __forceinline__ void store(double* p, double v) { __builtin_nontemporal_store(v, p); }
     static __device__ __forceinline__ double round(double v) { return v; }
@@ -21,6 +26,11 @@ template<> struct Store<double> {
 };
 template<> struct Store<float> {
     using T = float;
+    using R = float;
+    static __device__ __forceinline__ R load_raw(const float* p) { return __builtin_nontemporal_load(p); }
+    static __device__ __forceinline__ void store_raw(float* p, R v) { __builtin_nontemporal_store(v, p); }
+    static __device__ __forceinline__ T from_raw(R v) { return v; }
+    static __device__ __forceinline__ R to_raw(T v) { return v; }
     static __device__ __forceinline__ float load(const float* p) { return __builtin_nontemporal_load(p); }
     static __device__ __forceinline__ void store(float* p, float v) { __builtin_nontemporal_store(v, p); }
     static __device__ __forceinline__ float round(float v) { return v; }
@@ -28,6 +38,19 @@ template<> struct Store<float> {
 };
 template<> struct Store<f16s> {
     using T = float;
+    using R = uint16_t;
+    static __device__ __forceinline__ R load_raw(const f16s* p) {
+        return __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
+    }
+    static __device__ __forceinline__ void store_raw(f16s* p, R v) {
+        __builtin_nontemporal_store(v, reinterpret_cast<uint16_t*>(p));
+    }
+    static __device__ __forceinline__ T from_raw(R b) {
+        _Float16 h;
+        __builtin_memcpy(&h, &b, 2);
+        return (float)h;
+    }
+    static __device__ __forceinline__ R to_raw(T v) { return bits(v); }
     static __device__ __forceinline__ float load(const f16s* p) {
         uint16_t b = __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
         _Float16 h;
@@ -56,6 +79,15 @@ __device__ __forceinline__ uint16_t bf16_round(float v) {
 }
 template<> struct Store<bf16s> {
     using T = float;
+    using R = uint16_t;
+    static __device__ __forceinline__ R load_raw(const bf16s* p) {
+        return __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
+    }
+    static __device__ __forceinline__ void store_raw(bf16s* p, R v) {
+        __builtin_nontemporal_store(v, reinterpret_cast<uint16_t*>(p));
+    }
+    static __device__ __forceinline__ T from_raw(R b) { return __uint_as_float((uint32_t)b << 16); }
+    static __device__ __forceinline__ R to_raw(T v) { return bf16_round(v); }
     static __device__ __forceinline__ float load(const bf16s* p) {
         uint16_t b = __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
         return __uint_as_float((uint32_t)b << 16);

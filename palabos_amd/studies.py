@@ -1,0 +1,220 @@
+"""The benchmark configurations as runnable studies on the engine.
+
+    python -m palabos_amd.studies taylor_green [--n 512 --steps 2000]
+    python -m palabos_amd.studies sphere_drag  [--steps 5000 --storage f64]
+    python -m palabos_amd.studies porous       [--steps 3000 --storage f16]
+
+Each prints one JSON object. Everything runs on the device through the
+native library; the host only builds the geometry, reads back reductions and
+velocity slabs, and does the FP64 bookkeeping of the error norms.
+"""
+import argparse
+import json
+import math
+import time
+
+import numpy as np
+
+from . import lattice as _lat
+from . import setups
+
+
+def _slabs(n, cells_per_slab=1 << 22, ny=None, nz=None):
+    ny = ny or n
+    nz = nz or n
+    step = max(1, min(n, cells_per_slab // (ny * nz)))
+    for x0 in range(0, n, step):
+        yield x0, min(n, x0 + step)
+
+
+# ---------------------------------------------------------------------------
+# Taylor-Green vortex: precision study
+
+def taylor_green_2d_velocity(n, u0, x0, x1, nz):
+    """u = u0 (sin kx cos ky, -cos kx sin ky, 0), k = 2 pi / n: an exact
+    Navier-Stokes solution whose velocity decays as exp(-2 nu k^2 t)."""
+    k = 2.0 * np.pi / n
+    c = np.arange(n) * k
+    s, co = np.sin(c), np.cos(c)
+    u = np.zeros((x1 - x0, n, nz, 3))
+    u[..., 0] = u0 * s[x0:x1, None, None] * co[None, :, None]
+    u[..., 1] = -u0 * co[x0:x1, None, None] * s[None, :, None]
+    return u
+
+
+def _tg_lattice(n, storage, omega, u0, mode):
+    lat = _lat.Block3D(n, n, n, _lat.D3Q19, _lat.BGK(omega), storage=storage, periodic=True)
+    for x0, x1 in _slabs(n):
+        u = (setups.taylor_green_velocity(n, u0, x0, x1) if mode == "3d"
+             else taylor_green_2d_velocity(n, u0, x0, x1, n))
+        lat.initializeAtEquilibrium(1.0, u, box=(x0, x1 - 1, 0, n - 1, 0, n - 1))
+    lat.toggleInternalStatistics(False)
+    return lat
+
+
+def taylor_green_precision(n=512, steps=2000, omega=1.8, u0=0.04, storages=("f32", "f16", "bf16"),
+                           modes=("3d", "2d")):
+    """Runs the periodic Taylor-Green vortex in FP64 and in each reduced storage;
+    reports, per storage, the relative velocity L2 error against the FP64 run
+    and (all storages) against the analytic decay: the exact 2D-mode solution
+    u(t) = u(0) exp(-2 nu k^2 t), and for the 3D vortex the early-time kinetic
+    energy E0 exp(-6 nu k^2 t) (an approximation: the 3D vortex has no closed form)."""
+    nu = (1.0 / omega - 0.5) / 3.0
+    k = 2.0 * np.pi / n
+    out = {"lattice": [n, n, n], "steps": steps, "omega": omega, "u0": u0, "nu": nu, "modes": {}}
+    for mode in modes:
+        ref = _tg_lattice(n, "f64", omega, u0, mode)
+        res = {}
+        try:
+            e0 = ref.averageEnergy()
+            t0 = time.perf_counter()
+            ref.collideAndStream(steps)
+            e64 = ref.averageEnergy()
+            res["f64"] = {"seconds": time.perf_counter() - t0, "energy_ratio": e64 / e0}
+            decay_u = math.exp(-2.0 * nu * k * k * steps)
+            decay_e = math.exp((-4.0 if mode == "2d" else -6.0) * nu * k * k * steps)
+            for storage in ("f64",) + tuple(storages):
+                lat = ref if storage == "f64" else _tg_lattice(n, storage, omega, u0, mode)
+                try:
+                    if storage != "f64":
+                        lat.collideAndStream(steps)
+                        res[storage] = {"energy_ratio": lat.averageEnergy() / e0}
+                    d_ref = s_ref = d_an = s_an = 0.0
+                    for x0, x1 in _slabs(n):
+                        box = (x0, x1 - 1, 0, n - 1, 0, n - 1)
+                        u = lat.velocity(box)
+                        if storage != "f64":
+                            u64 = ref.velocity(box)
+                            d_ref += float(((u - u64) ** 2).sum())
+                            s_ref += float((u64 ** 2).sum())
+                        if mode == "2d":
+                            ua = taylor_green_2d_velocity(n, u0, x0, x1, n).reshape(-1, 3) * decay_u
+                            d_an += float(((u - ua) ** 2).sum())
+                            s_an += float((ua ** 2).sum())
+                    r = res[storage]
+                    if storage != "f64":
+                        r["velocity_l2_vs_f64"] = math.sqrt(d_ref / s_ref)
+                    if mode == "2d":
+                        r["velocity_l2_vs_analytic"] = math.sqrt(d_an / s_an)
+                    r["energy_error_vs_analytic"] = r["energy_ratio"] / decay_e - 1.0
+                finally:
+                    if storage != "f64":
+                        lat.close()
+            res["analytic_energy_ratio"] = decay_e
+        finally:
+            ref.close()
+        out["modes"][mode] = res
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Channel flow past a sphere: drag coefficient
+
+def sphere_drag(nx=256, ny=128, nz=128, radius=16.0, center=(64.0, 64.0, 64.0), u_max=0.05, re=100.0,
+                steps=5000, storage="f64", average_last=500, sample_every=10):
+    """Zou-He Poiseuille inlet (x = 0, u_max), Zou-He pressure outlet (rho = 1),
+    half-way bounce-back side walls (the domain edges), a momentum-exchange
+    sphere. Re = u_max 2 r / nu. C_d = 2 F_x / (rho u^2 pi r^2) with u = u_max
+    and with the mean inlet velocity; F_x averaged over the last steps."""
+    nu = u_max * 2.0 * radius / re
+    omega = 1.0 / (3.0 * nu + 0.5)
+    lat = _lat.Block3D(nx, ny, nz, _lat.D3Q19, _lat.BGK(omega), storage=storage)
+    try:
+        tags = np.zeros((nx, ny, nz), np.uint8)
+        tags[setups.sphere_mask((nx, ny, nz), center, radius)] = _lat.native.TAG_MOMENTUM_EXCHANGE
+        lat.setTags(tags)
+        prof = setups.poiseuille_profile(ny, nz, u_max)
+        vel = np.zeros(((ny - 2) * (nz - 2), 3))
+        vel[:, 0] = prof.reshape(-1)
+        p_in = lat.addVelocityBoundary(0, -1, (0, 0, 1, ny - 2, 1, nz - 2))
+        lat.setBoundaryVelocity(p_in, vel)
+        p_out = lat.addPressureBoundary(0, 1, (nx - 1, nx - 1, 1, ny - 2, 1, nz - 2))
+        lat.setBoundaryDensity(p_out, np.ones((ny - 2) * (nz - 2)))
+        lat.toggleInternalStatistics(False)
+        main = steps - average_last
+        ms = lat.collideAndStreamTimed(main)
+        lat.toggleInternalStatistics(True)
+        fx = []
+        done = main
+        while done < steps:
+            m = min(sample_every, steps - done)
+            lat.collideAndStream(m)
+            done += m
+            fx.append(lat.getStatistics()["force"][0])
+        fx = np.array(fx)
+        u_mean = float(prof.sum()) / ((ny - 2) * (nz - 2))
+        area = math.pi * radius * radius
+        return {"lattice": [nx, ny, nz], "storage": storage, "steps": steps, "re": re, "omega": omega,
+                "sphere_cells": int((tags != 0).sum()), "fx_mean": float(fx.mean()), "fx_std": float(fx.std()),
+                "cd_u_max": 2.0 * fx.mean() / (u_max ** 2 * area),
+                "cd_u_mean": 2.0 * fx.mean() / (u_mean ** 2 * area),
+                "mlups": nx * ny * nz * main / (ms * 1e-3) / 1e6, "timed_steps": main}
+    finally:
+        lat.close()
+
+
+# ---------------------------------------------------------------------------
+# Porous medium of random spheres, 60 uneven blocks: permeability
+
+def porous_geometry(shape=(384, 256, 256), count=400, r_range=(6.0, 14.0), seed=5, scale=1.0):
+    centres, radii = setups.random_spheres(shape, count, r_range, seed, periodic=(True, False, False))
+    return setups.spheres_mask(shape, centres, radii, scale=scale, periodic=(True, False, False))
+
+
+def porous_permeability(shape=(384, 256, 256), count=400, steps=3000, storage="f16", omega=1.7, g=1e-6,
+                        scale=1.0, blocks=None, block_seed=11, check_every=500):
+    """Regularized BGK driven by a Guo body force g along x through spheres made
+    of link-wise walls (half-way bounce-back links), periodic in x, walls on the
+    y and z faces, 60 uneven blocks (5 x 4 x 3 pieces of 32-128 cells). Darcy
+    permeability k = nu <u_x> / g with <u_x> the superficial velocity (mean over
+    all cells, solid cells counting zero)."""
+    nu = (1.0 / omega - 0.5) / 3.0
+    solid = porous_geometry(shape, count, scale=scale)
+    if blocks is None:
+        blocks = setups.uneven_blocks(shape, (5, 4, 3), 32, 128, block_seed)
+    lat = _lat.Block3D(*shape, _lat.D3Q19, _lat.RegularizedBGK(omega), storage=storage,
+                       periodic=(True, False, False), force=(g, 0.0, 0.0), blocks=blocks)
+    try:
+        tags = np.zeros(shape, np.uint8)
+        tags[solid] = _lat.native.TAG_WALL_LINK
+        lat.setTags(tags)
+        lat.toggleInternalStatistics(False)
+        history = []
+        done = 0
+        ms = 0.0
+        while done < steps:
+            m = min(check_every, steps - done)
+            ms += lat.collideAndStreamTimed(m)
+            done += m
+            mom = lat.reduceMoments()
+            history.append(float(mom["sum_u"][0]) / mom["cells"])
+        ux = history[-1]
+        return {"lattice": list(shape), "storage": storage, "blocks": len(blocks), "steps": steps,
+                "porosity": float(1.0 - solid.mean()), "omega": omega, "g": g, "radius_scale": scale,
+                "superficial_ux": ux, "permeability": nu * ux / g,
+                "ux_history": history, "mlups": float(np.prod(shape)) * steps / (ms * 1e-3) / 1e6}
+    finally:
+        lat.close()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("study", choices=["taylor_green", "sphere_drag", "porous"])
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--storage", default=None)
+    ap.add_argument("--scale", type=float, default=1.0)
+    args = ap.parse_args()
+    from . import native
+    native.init(0)
+    if args.study == "taylor_green":
+        r = taylor_green_precision(n=args.n, steps=args.steps or 2000)
+    elif args.study == "sphere_drag":
+        r = sphere_drag(steps=args.steps or 5000, storage=args.storage or "f64")
+    else:
+        r = porous_permeability(steps=args.steps or 3000, storage=args.storage or "f16", scale=args.scale)
+    print(json.dumps(r))
+
+
+if __name__ == "__main__":
+    main()

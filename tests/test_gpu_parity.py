@@ -104,6 +104,35 @@ def test_gpu_matches_oracle(gpu, name, storage):
     np.testing.assert_allclose(gst["porous"], wst["porous"], rtol=1e-11, atol=1e-14)
 
 
+LAUNCH_VARIANTS = {
+    "unstaged": {"PLBAMD_STAGED": "0"},
+    "staged": {"PLBAMD_STAGED": "1"},
+    "segments64": {"PLBAMD_BULK_TX": "64"},
+    "no_xcd_remap": {"PLBAMD_XCD_REMAP": "0"},
+}
+
+
+@pytest.mark.parametrize("storage", ["f64", "f16"])
+@pytest.mark.parametrize("name", ["periodic_rbgk", "periodic_odd_shape", "walls_bb_nodyn", "channel_zouhe_mebb",
+                                  "channel_wall_link", "faces_regularized", "forced_porous_walls_rbgk"])
+@pytest.mark.parametrize("variant", sorted(LAUNCH_VARIANTS))
+def test_gpu_launch_variants(gpu, monkeypatch, variant, name, storage):
+    """The same results with the odd step unstaged (one kernel for both
+    parities) or staged through LDS whatever the storage, with 64-cell row
+    segments (several segments per row: halo elements between workgroups), and
+    without the XCD-aware workgroup order."""
+    for k, v in LAUNCH_VARIANTS[variant].items():
+        monkeypatch.setenv(k, v)
+    case, steps = CASES[name](storage)
+    print(f"[case] {variant} {name} {storage} {case.shape} steps={steps}", flush=True)
+    f0 = _state(case)
+    want, wst = oracle.run(case, f0, steps)
+    got, gst = gpu_run(case, f0, steps)
+    _compare(case, got, want)
+    np.testing.assert_allclose(gst["avg_rhobar"], wst["avg_rhobar"], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(gst["force"], wst["force"], rtol=1e-11, atol=1e-14)
+
+
 @pytest.mark.parametrize("steps", [0, 1, 2, 3])
 def test_transfer_roundtrip_every_parity(gpu, steps):
     """Upload/download work in both AA layouts: download after an even or odd step
