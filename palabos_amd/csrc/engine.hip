@@ -81,6 +81,8 @@ struct Lattice {
     uint64_t nBulkPartials = 0;       // = workgroups of the bulk launch
     int xcdRemap = 0;
     bool staged = false;           // odd steps through bulk_odd_staged
+    bool pairs = false;            // 16-bit storage through bulk_pairs
+    size_t pairsLds = 0;           // its dynamic LDS bytes (odd steps)
     size_t stagedLds = 0;          // its dynamic LDS bytes
     double* dSpecialPartials = nullptr;
     uint64_t nSpecialPartials = 0;
@@ -199,11 +201,23 @@ int setup(Lattice& L) {
     // short rows into one workgroup.
     bool staged = cpt == 1 && storageSize(d.storage) >= 4;
     if (const char* e = getenv("PLBAMD_STAGED")) staged = cpt == 1 && atoi(e) != 0;
-    const int tx = (int)std::min<int64_t>(txMax, ((int64_t)maxNz + 63) / 64 * 64);
-    const int ty = staged ? 1 : std::max(1, txMax / tx);
+    // 16-bit storage with every block's nz even: two cells per thread
+    // (bulk_pairs, 32-bit pair accesses, its own LDS staging of odd steps).
+    bool pairs = cpt == 1 && storageSize(d.storage) == 2;
+    for (const BlockGeom& g : L.geom) pairs = pairs && (g.nz % 2 == 0);
+    if (const char* e = getenv("PLBAMD_PAIRS")) pairs = pairs && atoi(e) != 0;
+    if (pairs) {
+        staged = false;
+        if (!getenv("PLBAMD_BULK_TX")) txMax = 256;
+    }
+    const int cellsPerThread = pairs ? 2 : 1;
+    const int tx = (int)std::min<int64_t>(txMax, ((int64_t)(maxNz + cellsPerThread - 1) / cellsPerThread + 63) / 64 * 64);
+    const int ty = (staged || pairs) ? 1 : std::max(1, txMax / tx);
     L.bulkBlock = dim3(tx, ty);
-    L.zChunk = tx * cpt;
+    L.zChunk = tx * (pairs ? 2 : cpt);
     L.staged = staged;
+    L.pairs = pairs;
+    L.pairsLds = (size_t)kZMoving * (tx + 2) * 4 + 2 * (size_t)tx;
     L.stagedLds = (size_t)kZMoving * (tx + 2) * std::max<size_t>(4, storageSize(d.storage)) + (size_t)tx;
     if (L.stagedLds > 65536) L.staged = false;   // stay within the default LDS limit per workgroup
     uint64_t offset = 0, cellOffset = 0;
@@ -484,6 +498,21 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
 #undef PLBAMD_REC
         } else {
             return fail(PLBAMD_EINVAL, "recording step: FP64 BGK only");
+        }
+    } else if (L.pairs) {
+        if constexpr (sizeof(typename Store<S>::R) == 2) {
+#define PLBAMD_PAIRS(O, TG, ST, FO) \
+    hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO>), grid, L.bulkBlock, O ? L.pairsLds : 0, L.stream, a)
+#define PLBAMD_PAIRS_F(O, TG, ST) if (forced) PLBAMD_PAIRS(O, TG, ST, true); else PLBAMD_PAIRS(O, TG, ST, false)
+#define PLBAMD_PAIRS_S(O, TG) if (stats) { PLBAMD_PAIRS_F(O, TG, true); } else { PLBAMD_PAIRS_F(O, TG, false); }
+#define PLBAMD_PAIRS_T(O) if (tags) { PLBAMD_PAIRS_S(O, true) } else { PLBAMD_PAIRS_S(O, false) }
+            if (odd) { PLBAMD_PAIRS_T(true) } else { PLBAMD_PAIRS_T(false) }
+#undef PLBAMD_PAIRS_T
+#undef PLBAMD_PAIRS_S
+#undef PLBAMD_PAIRS_F
+#undef PLBAMD_PAIRS
+        } else {
+            return fail(PLBAMD_EINVAL, "pair kernel: 16-bit storage only");
         }
     } else {
 #define PLBAMD_BULK(O, TG, ST, FO)                                                                              \

@@ -539,6 +539,272 @@ __global__ void __launch_bounds__(1024) bulk_odd_staged(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// 16-bit storage, two cells per thread. A thread owns the cell pair (z, z+1),
+// z even, and moves both 16-bit values of a slot with one aligned 32-bit
+// access: half the memory instructions of one cell per thread. Even steps are
+// purely local. Odd steps stage the z-moving source and destination rows as
+// 32-bit pair words in LDS (the neighbouring pair supplies the shifted half),
+// so every global access stays a 32-bit aligned pair, except single halves at
+// a segment's ends and where one cell of a pair belongs to the special-cell
+// kernel. Requires nz even for every block. Rows at a non-periodic x/y edge
+// take the per-cell general path.
+
+__device__ __forceinline__ uint32_t ld_pair(const void* p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
+}
+__device__ __forceinline__ void st_pair(void* p, uint32_t v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<uint32_t*>(p));
+}
+__device__ __forceinline__ uint16_t lo16(uint32_t v) { return (uint16_t)(v & 0xffffu); }
+__device__ __forceinline__ uint16_t hi16(uint32_t v) { return (uint16_t)(v >> 16); }
+__device__ __forceinline__ uint32_t pack16(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
+
+// One cell of an odd step by per-lane neighbour tests (rows at block edges).
+template<typename S, int DYN, bool STATS, bool FORCED>
+__device__ __forceinline__ void odd_cell_general(const StepArgs& a, const BlockGeom& g, S* f, int z, uint64_t cell,
+                                                 uint8_t tag, bool inBulk, const int xs[3], const int ys[3],
+                                                 const bool vx[3], const bool vy[3], double st[8]) {
+    using T = typename Store<S>::T;
+    int zm = 0, zp = 0;
+    const bool vzm = shift(z, -1, g.nz, g.pz, zm), vzp = shift(z, 1, g.nz, g.pz, zp);
+    const int zs[3] = {zm, z, zp};
+    const bool vz[3] = {vzm, true, vzp};
+    T fc[19];
+    fc[0] = Store<S>::load(f + cell);
+#pragma unroll
+    for (int i = 1; i < 19; ++i) {
+        const int ax = 1 - cx(i), ay = 1 - cy(i), az = 1 - cz(i);
+        const bool ok = vx[ax] && vy[ay] && vz[az];
+        const uint64_t src = ok ? (uint64_t)opp(i) * g.slot + (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
+                                : (uint64_t)i * g.slot + cell;
+        fc[i] = Store<S>::load(f + src);
+    }
+    collide_cell<S, DYN, STATS, FORCED, false>(a, g, cell, tag, inBulk, fc, st);
+    Store<S>::store(f + cell, fc[0]);
+#pragma unroll
+    for (int i = 1; i < 19; ++i) {
+        const int ax = 1 + cx(i), ay = 1 + cy(i), az = 1 + cz(i);
+        const bool ok = vx[ax] && vy[ay] && vz[az];
+        const uint64_t dst = ok ? (uint64_t)i * g.slot + (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
+                                : (uint64_t)opp(i) * g.slot + cell;
+        Store<S>::store(f + dst, fc[i]);
+    }
+}
+
+template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED>
+__global__ void __launch_bounds__(1024) bulk_pairs(StepArgs a) {
+    static_assert(sizeof(typename Store<S>::R) == 2, "pairs of 16-bit values");
+    using T = typename Store<S>::T;
+    extern __shared__ __align__(16) unsigned char ldsRaw[];
+    const int W = (int)blockDim.x;            // pairs per segment
+    const int P = W + 2;                      // pitch in pair words: low halo, segment, high halo
+    uint32_t* buf = reinterpret_cast<uint32_t*>(ldsRaw);
+    uint8_t* skipped = ldsRaw + (size_t)kZMoving * P * 4;   // ODD && TAGS: 2W flags
+
+    const uint32_t wg = a.xcdRemap ? xcd_item(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int b = find_block(a.blocks, a.nBlocks, wg);
+    const BlockGeom g = a.blocks[b];
+    S* __restrict__ f = reinterpret_cast<S*>(a.arena) + g.offset;
+    const uint32_t local = wg - g.wgBegin;
+    const int chunk = (int)(local % (uint32_t)g.nzChunks);
+    const uint32_t rest = local / (uint32_t)g.nzChunks;
+    const int y = (int)(rest % (uint32_t)g.nyGroups);
+    const int x = (int)(rest / (uint32_t)g.nyGroups);
+    const int z0 = chunk * 2 * W;
+    const int len = min(g.nz - z0, 2 * W);    // even
+    const int nPairs = len / 2;
+    const int t = (int)threadIdx.x;
+    const bool active = t < nPairs;
+    const int zA = z0 + 2 * t, zB = zA + 1;
+    const uint64_t rowBase = ((uint64_t)x * g.ny + y) * g.nz;
+    const uint64_t cellA = rowBase + zA, cellB = cellA + 1;
+    const bool rowInBulk = x >= g.bx0 && x <= g.bx1 && y >= g.by0 && y <= g.by1;
+    const bool inBulkA = rowInBulk && zA >= g.bz0 && zA <= g.bz1;
+    const bool inBulkB = rowInBulk && zB >= g.bz0 && zB <= g.bz1;
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    uint8_t tagA = BT_BULK, tagB = BT_BULK;
+    if constexpr (TAGS) {
+        if (active) {
+            tagA = a.tags[g.cellOffset + cellA];
+            tagB = a.tags[g.cellOffset + cellB];
+        }
+    }
+    const bool compA = active && tagA != BT_SKIP, compB = active && tagB != BT_SKIP;
+    T fa[19], fb[19];
+
+    if constexpr (!ODD) {
+        // even step: each cell reads its own slots and writes slot opp(i)
+        S* own = f + cellA;
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                const uint32_t v = ld_pair(own + (uint64_t)i * g.slot);
+                fa[i] = Store<S>::from_raw(lo16(v));
+                fb[i] = Store<S>::from_raw(hi16(v));
+            }
+        }
+        if (compA) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellA, tagA, inBulkA, fa, st);
+        if (compB) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellB, tagB, inBulkB, fb, st);
+        if (compA && compB) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i)
+                st_pair(own + (uint64_t)opp(i) * g.slot, pack16(Store<S>::to_raw(fa[i]), Store<S>::to_raw(fb[i])));
+        } else if (compA || compB) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                if (compA) Store<S>::store(own + (uint64_t)opp(i) * g.slot, fa[i]);
+                if (compB) Store<S>::store(own + (uint64_t)opp(i) * g.slot + 1, fb[i]);
+            }
+        }
+    } else {
+        int xm = 0, xp = 0, ym = 0, yp = 0;
+        const bool vxm = shift(x, -1, g.nx, g.px, xm), vxp = shift(x, 1, g.nx, g.px, xp);
+        const bool vym = shift(y, -1, g.ny, g.py, ym), vyp = shift(y, 1, g.ny, g.py, yp);
+        const int xs[3] = {xm, x, xp};
+        const int ys[3] = {ym, y, yp};
+        const bool vx[3] = {vxm, true, vxp};
+        const bool vy[3] = {vym, true, vyp};
+        if (!(vxm && vxp && vym && vyp)) {
+            if (compA) odd_cell_general<S, DYN, STATS, FORCED>(a, g, f, zA, cellA, tagA, inBulkA, xs, ys, vx, vy, st);
+            if (compB) odd_cell_general<S, DYN, STATS, FORCED>(a, g, f, zB, cellB, tagB, inBulkB, xs, ys, vx, vy, st);
+        } else {
+            S* rows[3][3];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+                for (int ay = 0; ay < 3; ++ay)
+                    rows[ax][ay] = f + ((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz;
+            // halo pair words: (z0-2, z0-1) and (z0+len, z0+len+1)
+            int zlo = 0, zhi = 0;
+            const bool vlo = shift(z0, -2, g.nz, g.pz, zlo);
+            const bool vhi = shift(z0 + len - 1, 1, g.nz, g.pz, zhi);
+            const int zc = active ? zA : z0;
+            uint32_t raw[kZMoving];
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                const S* src = rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot;
+                if (cz(i) == 0) {
+                    const uint32_t v = ld_pair(src + zc);
+                    fa[i] = Store<S>::from_raw(lo16(v));
+                    fb[i] = Store<S>::from_raw(hi16(v));
+                } else {
+                    raw[zmove_index(i)] = ld_pair(src + zc);
+                }
+            }
+            uint32_t halo[kZMoving];
+            const bool loLane = t == 0 && vlo, hiLane = t == W - 1 && vhi;
+            if (loLane || hiLane) {
+                const int zh = loLane ? zlo : zhi;
+#pragma unroll
+                for (int i = 0; i < 19; ++i)
+                    if (cz(i) != 0)
+                        halo[zmove_index(i)] = ld_pair(rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot + zh);
+            }
+            if (active) {
+#pragma unroll
+                for (int k = 0; k < kZMoving; ++k) buf[k * P + t + 1] = raw[k];
+            }
+            if (loLane || hiLane) {
+                const int hp = loLane ? 0 : nPairs + 1;
+#pragma unroll
+                for (int k = 0; k < kZMoving; ++k) buf[k * P + hp] = halo[k];
+            }
+            if constexpr (TAGS) {
+                if (active) {
+                    skipped[2 * t] = compA ? 0 : 1;
+                    skipped[2 * t + 1] = compB ? 0 : 1;
+                }
+            }
+            __syncthreads();
+            // shifted inputs: c_z = +1 reads z - 1, c_z = -1 reads z + 1
+            const bool vzA = zA > 0 || g.pz, vzB = zB < g.nz - 1 || g.pz;
+            if (active) {
+#pragma unroll
+                for (int i = 0; i < 19; ++i) {
+                    if (cz(i) == 0) continue;
+                    const uint32_t* row = buf + zmove_index(i) * P;
+                    if (cz(i) > 0) {
+                        fa[i] = vzA ? Store<S>::from_raw(hi16(row[t])) : Store<S>::load(f + (uint64_t)i * g.slot + cellA);
+                        fb[i] = Store<S>::from_raw(lo16(row[t + 1]));
+                    } else {
+                        fa[i] = Store<S>::from_raw(hi16(row[t + 1]));
+                        fb[i] = vzB ? Store<S>::from_raw(lo16(row[t + 2])) : Store<S>::load(f + (uint64_t)i * g.slot + cellB);
+                    }
+                }
+            }
+            __syncthreads();   // every LDS input is read before the outputs reuse the buffer
+            if (compA) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellA, tagA, inBulkA, fa, st);
+            if (compB) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellB, tagB, inBulkB, fb, st);
+            if (active) {
+#pragma unroll
+                for (int i = 0; i < 19; ++i) {
+                    if (cz(i) == 0) {
+                        S* dst = rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot + zA;
+                        if (compA && compB) st_pair(dst, pack16(Store<S>::to_raw(fa[i]), Store<S>::to_raw(fb[i])));
+                        else {
+                            if (compA) Store<S>::store(dst, fa[i]);
+                            if (compB) Store<S>::store(dst + 1, fb[i]);
+                        }
+                    } else {
+                        buf[zmove_index(i) * P + t + 1] = pack16(Store<S>::to_raw(fa[i]), Store<S>::to_raw(fb[i]));
+                        // edge reflection where the destination leaves a non-periodic z edge
+                        if (cz(i) > 0 && !vzB && compB) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellB, fb[i]);
+                        if (cz(i) < 0 && !vzA && compA) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellA, fa[i]);
+                    }
+                }
+            }
+            __syncthreads();
+            // destination pair words: position (zA, zB) takes, for c_z = +1, the
+            // outputs of cells zA - 1 and zA; for c_z = -1, of cells zB and zB + 1
+            auto computed = [&](int c) {   // segment-local cell c was collided here
+                if (c < 0 || c >= len) return false;
+                if constexpr (TAGS) return skipped[c] == 0;
+                return true;
+            };
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                if (cz(i) == 0) continue;
+                const uint32_t* row = buf + zmove_index(i) * P;
+                S* dst = rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot;
+                if (active) {
+                    uint16_t lo, hi;
+                    bool okLo, okHi;
+                    if (cz(i) > 0) {
+                        lo = hi16(row[t]);
+                        okLo = computed(2 * t - 1);
+                        hi = lo16(row[t + 1]);
+                        okHi = computed(2 * t);
+                    } else {
+                        lo = hi16(row[t + 1]);
+                        okLo = computed(2 * t + 1);
+                        hi = lo16(row[t + 2]);
+                        okHi = computed(2 * t + 2);
+                    }
+                    if (okLo && okHi) st_pair(dst + zA, pack16(lo, hi));
+                    else {
+                        if (okLo) Store<S>::store_raw(dst + zA, lo);
+                        if (okHi) Store<S>::store_raw(dst + zB, hi);
+                    }
+                }
+                // single halves beyond the segment: c_z = +1 sends the last cell's
+                // output to z0 + len; c_z = -1 sends the first cell's to z0 - 1
+                if (cz(i) > 0) {
+                    int zd = 0;
+                    if (t == W - 1 && shift(z0 + len - 1, 1, g.nz, g.pz, zd) && computed(len - 1))
+                        Store<S>::store_raw(dst + zd, hi16(row[nPairs]));
+                } else {
+                    int zd = 0;
+                    if (t == 0 && shift(z0, -1, g.nz, g.pz, zd) && computed(0))
+                        Store<S>::store_raw(dst + zd, lo16(row[1]));
+                }
+            }
+        }
+    }
+    if constexpr (STATS) block_reduce_store(st, a.partials + 8 * (uint64_t)wg);
+}
+
+// ---------------------------------------------------------------------------
 // Special-cell kernel: boundary planes, momentum-exchange nodes, cells next to
 // link-wise walls. Same AA addressing as the bulk kernel, with the wall mask
 // added to the edge test.

@@ -74,6 +74,17 @@ def test_periodic_uneven_blocks(gpu, storage):
     _check(case, blocks, [1, 2, 5, 8])
 
 
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_even_z_blocks_16bit(gpu, storage):
+    """Every block (with its envelope) even along z: the two-cells-per-thread
+    kernel runs every block, with link walls and a body force across blocks."""
+    case = Case(20, 18, 16, storage=storage, dynamics="rbgk", omega=1.6, periodic=(True, True, False),
+                force=(1e-5, 2e-6, 0))
+    case.tags = sphere_tags(case.shape, (9.5, 8.5, 7.5), 3.4, TAG_WALL)
+    blocks = uneven_blocks(case.shape, [[7, 13], [5], [8]])
+    _check(case, blocks, [1, 4, 7])
+
+
 def test_mixed_periodicity_rbgk_force_porosity(gpu):
     rng = np.random.RandomState(2)
     case = Case(18, 14, 12, storage="f64", dynamics="rbgk", omega=1.6, periodic=(True, False, True),

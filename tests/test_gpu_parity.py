@@ -109,6 +109,7 @@ LAUNCH_VARIANTS = {
     "staged": {"PLBAMD_STAGED": "1"},
     "segments64": {"PLBAMD_BULK_TX": "64"},
     "no_xcd_remap": {"PLBAMD_XCD_REMAP": "0"},
+    "no_pairs": {"PLBAMD_PAIRS": "0"},
 }
 
 
