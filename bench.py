@@ -12,9 +12,12 @@ every 50, FP64).
 
 value        MLUPS = cells * steps / wall seconds / 1e6 over the timed steps,
              populations resident in HBM before the timed region starts.
-roofline     the bulk kernel: algorithmic bytes per launch (read + write 19
-             populations per cell) / average launch duration, measured with
-             HIP events on the stream the kernel runs on.
+roofline     the collide-and-stream launches: algorithmic bytes per launch
+             (read + write 19 populations per cell) / average launch duration,
+             measured with HIP events on the stream the kernels run on; one
+             launch per step (even and odd steps alternate two kernels).
+             "traffic" is the HBM bytes per launch from rocprofv3 counters
+             (profiles/bulk_step_tg512_counters.txt).
 cpu_baseline the compiled Palabos reference (oracle/_ref/ref_driver, serial,
              1 core) on a bounded sample (periodic BGK 128^3), or the oracle
              restatement when the reference binary is absent.
@@ -37,6 +40,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"f64": 19 * 2 * 8, "f32": 19 * 2 * 4, "f16": 19 * 2 * 2, "bf16": 19 * 2 * 2}
+# The launches that make one step at this workload (even / odd steps alternate;
+# the roofline figure is their average launch duration).
+KERNELS = {"f64": "plbamd::bulk_step (even) / plbamd::bulk_odd_staged (odd)",
+           "f32": "plbamd::bulk_step (even) / plbamd::bulk_odd_staged (odd)",
+           "f16": "plbamd::bulk_pairs (even and odd)", "bf16": "plbamd::bulk_pairs (even and odd)"}
 
 
 def taylor_green_init(lat, n, u0):
@@ -198,6 +206,9 @@ def main():
         for p in ("f64", "f32", "f16"):
             if p != args.precision:
                 variants.append(run_precision(p, args.n, args.steps, args.warmup))
+    for v in variants:
+        v["kernel"] = KERNELS[v["dtype"]]
+        v["traffic"] = traffic_from_profile(v["dtype"], args.n)
     traffic = traffic_from_profile(args.precision, args.n)
     line = {
         "metric": "D3Q19 MLUPS (FP64/FP32/FP16-store) & % HBM peak; adjoint/forward ratio",
@@ -216,7 +227,7 @@ def main():
                    "omega": 1.8, "u0": 0.04, "precision": args.precision, "statistics": False},
         "roofline": {"bound": "hbm", "achieved": main_run["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": main_run["frac"], "traffic": traffic,
-                     "kernel": "plbamd::bulk_step", "algorithmic_bytes_per_cell": BYTES_PER_CELL[args.precision],
+                     "kernel": KERNELS[args.precision], "algorithmic_bytes_per_cell": BYTES_PER_CELL[args.precision],
                      "event_ms_per_launch": main_run["event_ms_per_step"]},
         "variants": variants,
     }

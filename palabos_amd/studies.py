@@ -2,7 +2,7 @@
 
     python -m palabos_amd.studies taylor_green [--n 512 --steps 2000]
     python -m palabos_amd.studies sphere_drag  [--steps 5000 --storage f64]
-    python -m palabos_amd.studies porous       [--steps 3000 --storage f16]
+    python -m palabos_amd.studies porous       [--steps 20000 --storage f16]
 
 Each prints one JSON object. Everything runs on the device through the
 native library; the host only builds the geometry, reads back reductions and
@@ -161,7 +161,7 @@ def porous_geometry(shape=(384, 256, 256), count=400, r_range=(6.0, 14.0), seed=
     return setups.spheres_mask(shape, centres, radii, scale=scale, periodic=(True, False, False))
 
 
-def porous_permeability(shape=(384, 256, 256), count=400, steps=3000, storage="f16", omega=1.7, g=1e-6,
+def porous_permeability(shape=(384, 256, 256), count=400, steps=20000, storage="f16", omega=1.7, g=1e-6,
                         scale=1.0, blocks=None, block_seed=11, check_every=500):
     """Regularized BGK driven by a Guo body force g along x through spheres made
     of link-wise walls (half-way bounce-back links), periodic in x, walls on the
@@ -212,7 +212,7 @@ def main():
     elif args.study == "sphere_drag":
         r = sphere_drag(steps=args.steps or 5000, storage=args.storage or "f64")
     else:
-        r = porous_permeability(steps=args.steps or 3000, storage=args.storage or "f16", scale=args.scale)
+        r = porous_permeability(steps=args.steps or 20000, storage=args.storage or "f16", scale=args.scale)
     print(json.dumps(r))
 
 
