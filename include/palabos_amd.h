@@ -196,15 +196,20 @@ int plbamd_compute_macroscopic(int handle, const plbamd_box* box, double* rho, d
 int plbamd_reduce_moments(int handle, const plbamd_box* box, double* out);
 
 /* Objective of plbamd_adjoint_gradient:
- *   J = (1/T) sum_{t=1..T} [ weight_pressure_drop * cs2 * (<rho>_{x=inlet_x} - <rho>_{x=outlet_x})
- *                          + weight_drag * omega * sum_cells alpha * j_x ]
- * evaluated on the states after each step (plane means over all y, z; the drag
- * term is the momentum the Brinkman porosity removes in the next collision,
- * x component; FP64 accumulation). */
+ *   J = 1/(T - t0) sum_{t=t0+1..T} [ weight_pressure_drop * cs2 * (<rho>_{x=inlet_x} - <rho>_{x=outlet_x})
+ *                                  + weight_drag * omega * sum_cells alpha * j_x
+ *                                  + weight_flux * <j_x> ]
+ * evaluated on the states after each step, t0 = first_step (plane means over
+ * all y, z; the drag term is the momentum the Brinkman porosity removes in the
+ * next collision, x component; <j_x> the mean x momentum over all cells, whose
+ * time average times nu / g is the Darcy permeability of a force-driven
+ * porous medium; FP64 accumulation). */
 typedef struct {
     int32_t inlet_x, outlet_x;
     double weight_pressure_drop;
     double weight_drag;
+    double weight_flux;
+    int32_t first_step;
 } plbamd_objective;
 
 /* Runs nsteps steps from the current state (the lattice advances) and returns J
@@ -212,11 +217,15 @@ typedef struct {
  * [nx][ny][nz], may be NULL) and to the scale s of every plane's imposed
  * velocity, u -> s u at s = 1 (grad_plane_scale, one per plane in creation
  * order; 0 for pressure planes; may be NULL). Hand-written discrete adjoint of
- * the BGK + Brinkman collide-and-stream with checkpoints every
- * `checkpoint_every` steps (even) and recomputation; timing_ms (may be NULL)
- * receives the forward and the backward device time. Single-block FP64 BGK
- * lattices without a body force or tags (the walls are the domain edges),
- * started after an even number of steps, porosity zero on the planes.
+ * the collide-and-stream (BGK or regularized BGK, Guo body force, Brinkman
+ * porosity) with checkpoints every `checkpoint_every` steps (even) and
+ * recomputation; timing_ms (may be NULL) receives the forward and the backward
+ * device time. FP64 or 16-bit storage (the 16-bit forward is differentiated
+ * through its FP32 arithmetic, the storage rounding taken as the identity;
+ * every block even along z); no per-cell tags (the walls are the domain
+ * edges); planes only on single-block BGK lattices without a body force, with
+ * zero porosity on them; started after an even number of steps. Multi-block
+ * lattices refresh the envelopes of the adjoint state after every reverse step.
  * No Palabos counterpart. */
 int plbamd_adjoint_gradient(int handle, int nsteps, int checkpoint_every, const plbamd_objective* objective,
                             double* J, double* grad_alpha, double* grad_plane_scale, double* timing_ms);

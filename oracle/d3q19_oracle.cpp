@@ -761,9 +761,9 @@ typedef DualN<kSeeds + 1> DualO;     // + the velocity-plane scale
 // j(f) of the adjoint objective on the natural-layout state (see the C ABI).
 template<typename T>
 T objectiveTerm(const OracleDesc& d, const std::vector<T>& st, const Engine<T>& eng, int xin, int xout,
-                double wdp, double wdrag) {
+                double wdp, double wdrag, double wflux) {
     const long plane = (long)d.ny * d.nz;
-    T pin = T(), pout = T(), drag = T();
+    T pin = T(), pout = T(), drag = T(), flux = T();
     const T omega = (T)d.omega;
     for (int x = 0; x < d.nx; ++x) for (int y = 0; y < d.ny; ++y) for (int z = 0; z < d.nz; ++z) {
         const long k = eng.lin(x, y, z);
@@ -775,9 +775,12 @@ T objectiveTerm(const OracleDesc& d, const std::vector<T>& st, const Engine<T>& 
         if (x == xin) pin = pin + rho;
         if (x == xout) pout = pout + rho;
         if (eng.alphaT && eng.tag[k] != TAG_PLANE) drag = drag + omega * eng.alphaT[k] * jx;
+        flux = flux + jx;
     }
     const T cs2 = (T)(1.0 / 3.0);
-    return (T)wdp * cs2 * (pin / (T)(double)plane - pout / (T)(double)plane) + (T)wdrag * drag;
+    const double cells = (double)d.nx * d.ny * d.nz;
+    return (T)wdp * cs2 * (pin / (T)(double)plane - pout / (T)(double)plane) + (T)wdrag * drag +
+           (T)(wflux / cells) * flux;
 }
 
 extern "C" {
@@ -827,12 +830,13 @@ void oracle_equilibrium(long n, const double* rho, const double* u, double* fOut
     }
 }
 
-// J = (1/T) sum_{t=1..T} j(f^t) of the adjoint objective from FP64 state f0, and
-// its exact derivatives with respect to the porosity of 16 seeded cells and to
-// the scale of the velocity planes' imposed velocity: out = [J, dJ/dalpha[16], dJ/ds].
+// J = 1/(T - t0) sum_{t=t0+1..T} j(f^t) of the adjoint objective (pressure drop,
+// Brinkman drag, mean x momentum) from FP64 state f0, and its exact derivatives
+// with respect to the porosity of 16 seeded cells and to the scale of the
+// velocity planes' imposed velocity: out = [J, dJ/dalpha[16], dJ/ds].
 int oracle_objective_dual(const OracleDesc* d, const double* f0, int nsteps, int xin, int xout, double wdp,
-                          double wdrag, const long* seeds, double* out) {
-    if (!d || !f0 || nsteps <= 0 || d->storage != 0 || !d->alpha) return -1;
+                          double wdrag, double wflux, int firstStep, const long* seeds, double* out) {
+    if (!d || !f0 || nsteps <= 0 || firstStep < 0 || firstStep >= nsteps || d->storage != 0 || !d->alpha) return -1;
     Engine<DualO> eng(*d);
     const long N = eng.N;
     std::vector<DualO> alpha(N);
@@ -844,7 +848,7 @@ int oracle_objective_dual(const OracleDesc* d, const double* f0, int nsteps, int
     std::vector<DualO> state(N * 19), post(N * 19);
     for (long k = 0; k < N * 19; ++k) state[k] = DualO(f0[k]);
     DualO J = DualO();
-    const DualO invT = DualO(1.0 / nsteps);
+    const DualO invT = DualO(1.0 / (nsteps - firstStep));
     Stats st;
     for (int step = 0; step < nsteps; ++step) {
         for (long k = 0; k < N; ++k) {
@@ -862,7 +866,7 @@ int oracle_objective_dual(const OracleDesc* d, const double* f0, int nsteps, int
                 else state[k*19 + i] = post[k*19 + opposite(i)];
             }
         }
-        J = J + invT * objectiveTerm(*d, state, eng, xin, xout, wdp, wdrag);
+        if (step + 1 > firstStep) J = J + invT * objectiveTerm(*d, state, eng, xin, xout, wdp, wdrag, wflux);
     }
     out[0] = J.v;
     for (int k = 0; k <= kSeeds; ++k) out[1 + k] = J.d[k];

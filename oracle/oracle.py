@@ -67,8 +67,8 @@ def lib():
         _lib.oracle_equilibrium.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p]
         _lib.oracle_objective_dual.argtypes = [ctypes.POINTER(_Desc), ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
-                                               ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
-                                               ctypes.c_void_p]
+                                               ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         _lib.oracle_objective_dual.restype = ctypes.c_int
         for name in ("oracle_f32_to_f16", "oracle_f16_to_f32", "oracle_f32_to_bf16",
                      "oracle_bf16_to_f32"):
@@ -150,10 +150,10 @@ def complete(case, f):
     return state
 
 
-def objective_dual(case, f0, nsteps, xin, xout, w_dp, w_drag, seeds):
-    """J = (1/T) sum_t j(f^t) of the adjoint objective and its exact derivatives
-    (forward-mode dual numbers through the oracle) with respect to the porosity of
-    16 seeded cells (flat cell indices) and to the velocity-plane scale.
+def objective_dual(case, f0, nsteps, xin, xout, w_dp, w_drag, seeds, w_flux=0.0, first_step=0):
+    """J = 1/(T - t0) sum_{t > t0} j(f^t) of the adjoint objective and its exact
+    derivatives (forward-mode dual numbers through the oracle) with respect to the
+    porosity of 16 seeded cells (flat cell indices) and to the velocity-plane scale.
     Returns (J, dJ/dalpha[16], dJ/ds)."""
     f0 = np.ascontiguousarray(f0, dtype=np.float64)
     seeds = np.ascontiguousarray(seeds, dtype=np.int64)
@@ -162,7 +162,8 @@ def objective_dual(case, f0, nsteps, xin, xout, w_dp, w_drag, seeds):
     d = _desc(case, keep)
     out = np.zeros(18)
     rc = lib().oracle_objective_dual(ctypes.byref(d), f0.ctypes.data, int(nsteps), int(xin), int(xout),
-                                     float(w_dp), float(w_drag), seeds.ctypes.data, out.ctypes.data)
+                                     float(w_dp), float(w_drag), float(w_flux), int(first_step),
+                                     seeds.ctypes.data, out.ctypes.data)
     if rc != 0:
         raise ValueError("oracle_objective_dual rejected the case")
     return out[0], out[1:17].copy(), out[17]

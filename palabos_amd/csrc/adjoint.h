@@ -1,32 +1,48 @@
-// Discrete adjoint of the FP64 BGK collide-and-stream with a Brinkman porosity
-// term, for gradients of a time-averaged objective with respect to the
-// porosity field and the scale of each velocity plane. No Palabos counterpart.
+// Discrete adjoint of the collide-and-stream step (BGK or regularized BGK, an
+// optional Guo body force, a Brinkman porosity term), for gradients of a
+// time-averaged objective with respect to the porosity field and the scale of
+// each velocity plane. No Palabos counterpart.
 //
 // Forward step t (state f^t: post-stream populations, natural layout):
 //   f*(y)      = C(f^t(y); alpha(y), planes)      collision (plane cells:
 //                                                 completion, then BGK)
 //   f^{t+1}_i(x) = f*_i(x - c_i)  or  f*_opp(i)(x) when x - c_i is outside
-// Objective: J = (1/T) sum_{t=1..T} j(f^t), with
+// Objective: J = 1/(T - t0) sum_{t = t0+1..T} j(f^t), with
 //   j(f) = w_dp cs2 (<rho>_{x = xin} - <rho>_{x = xout}) + w_drag omega sum_x alpha(x) j_x(x)
-// (the second term is the momentum the porosity removes in the next collision).
+//          + w_flux <j_x>                  (<.> over the lattice's cells)
+// (the drag term is the momentum the porosity removes in the next collision).
 //
 // Reverse step (lambda^t = dJ/df^t):
 //   mu_i(y)    = lambda^{t+1}_i(y + c_i)  or  lambda^{t+1}_opp(i)(y) when y + c_i is outside
-//   lambda^t(y) = (dC/df)^T mu(y) + (1/T) dj/df(f^t)(y)          (t >= 1)
-//   g_alpha(y) += mu(y) . dC/dalpha + (1/T) w_drag omega j_x(f^t(y))
+//   lambda^t(y) = (dC/df)^T mu(y) + dj/df(f^t)(y) / (T - t0)          (t > t0)
+//   g_alpha(y) += mu(y) . dC/dalpha + w_drag omega j_x(f^t(y)) / (T - t0)
 //   g_s(plane) += mu(y) . dC/ds        (s scales the plane's imposed velocity)
-// Bulk cells, closed form (f' = (1-w) f + w feq(rhoBar, J), J = (1-alpha) j,
-// rho = 1 + rhoBar, feq_i = t_i (rhoBar + 3 c_i.J + (4.5 (c_i.J)^2 - 1.5 J^2) / rho)):
-//   A = sum_i mu_i t_i (1 - (4.5 (c_i.J)^2 - 1.5 J^2) / rho^2)
-//   B = sum_i mu_i t_i (3 c_i + (9 (c_i.J) c_i - 3 J) / rho)
-//   (dC/df)^T mu = (1-w) mu + w (A + (1-alpha) B.c_k),   mu . dC/dalpha = -w B.j
+//
+// Bulk cells, closed form. With rho = 1 + rhoBar, u_d = (1 - alpha)(j / rho + F/2),
+// J = rho u_d, feq_i(rhoBar, J) = t_i (rhoBar + 3 c_i.J + (4.5 (c_i.J)^2 - 1.5 J^2) / rho),
+// Pi = sum c c f - rhoBar/3 I - j j^T / rho, fneq_i(Pi) = 4.5 t_i (c_i c_i^T - I/3) : Pi and
+// the Guo source G_i(u_d) = (1 - w/2) t_i (3 (c_i - u_d).F + 9 (c_i.u_d)(c_i.F)):
+//   BGK:  f' = (1 - w) f + w feq(rhoBar, J) + G(u_d)
+//   RBGK: f' = feq(rhoBar, J) + (1 - w) fneq(Pi) + G(u_d)
+// and the contractions of mu
+//   A = sum_i mu_i t_i (1 - (4.5 (c_i.J)^2 - 1.5 J^2) / rho^2)      (d feq / d rhoBar)
+//   B = sum_i mu_i t_i (3 c_i + (9 (c_i.J) c_i - 3 J) / rho)       (d feq / d J)
+//   M = sum_i mu_i 4.5 t_i (c_i c_i^T - I/3)                       (d fneq / d Pi)
+//   g = (1 - w/2) sum_i mu_i t_i (-3 F + 9 (c_i.F) c_i)            (d G / d u_d)
+// the reverse collision is lambda_k = s + v.c_k (+ (1 - w) c_k^T M c_k for RBGK) with
+//   BGK:  lambda_k += (1 - w) mu_k,  s = w (A + (1-alpha) B.F/2) - (1-alpha) g.j / rho^2,
+//         v = w (1-alpha) B + (1-alpha) g / rho
+//   RBGK: s = A + (1-alpha) B.F/2 + (1-w)(j^T M j / rho^2 - tr M / 3) - (1-alpha) g.j / rho^2,
+//         v = (1-alpha) B - 2 (1-w) M j / rho + (1-alpha) g / rho
+//   mu . dC/dalpha = -(w_B B.(j + rho F/2) + g.(j / rho + F/2)), w_B = w (BGK) or 1 (RBGK).
 // Plane cells use the device completion and collision routines evaluated on
-// dual numbers, one direction per pass (the Jacobian-transpose product column
-// by column).
+// dual numbers, one thread per (cell, direction).
 //
 // The reverse sweep needs of f^t only rhoBar and j for bulk cells (32 bytes
-// per cell instead of 152): the recompute runs the forward step kernels with
-// moment recording on, and keeps the full f^t of the plane cells only.
+// per cell instead of 19 populations): the recompute runs the forward step
+// kernels with moment recording on, and keeps the full f^t of the plane cells
+// only. Multi-block lattices run the reverse step block by block and refresh
+// the envelopes of lambda from their owners after every reverse step.
 #pragma once
 
 #include "kernels.h"
@@ -55,28 +71,30 @@ __host__ __device__ inline Dual& operator-=(Dual& a, Dual b) { a = a - b; return
 __host__ __device__ inline Dual& operator*=(Dual& a, Dual b) { a = a * b; return a; }
 
 struct AdjointArgs {
-    const double* mom;        // moments of f^t: rhoBar, jx, jy, jz, [4][momStride]
+    const double* mom;        // moments of f^t: rhoBar, jx, jy, jz, [4][momStride] by cellOffset + cell
     uint64_t momStride;
     const double* planeF;     // f^t of the special (plane) records, 19 per record
-    const double* lamNext;    // lambda^{t+1}
-    double* lam;              // lambda^t
-    const double* alpha;      // porosity per cell
-    double* gAlpha;           // gradient accumulator per cell
-    const uint8_t* tags;      // BulkTag per cell (BT_SKIP: plane cells)
+    const double* lamNext;    // lambda^{t+1} of this block (slot stride g.slot)
+    double* lam;              // lambda^t of this block
+    const double* alpha;      // porosity by cellOffset + cell, or null
+    double* gAlpha;           // gradient accumulator by cellOffset + cell
+    const uint8_t* tags;      // BulkTag by cellOffset + cell (BT_SKIP: plane cells), or null
     BlockGeom g;
     double omega;
-    double invT;              // 1 / T
+    double force[3];
+    double invT;              // 1 / (T - t0)
     double wdp, wdrag;        // objective weights
+    double wflux;             // w_flux / (number of lattice cells)
     double inW, outW;         // cs2 / |plane| of the inlet and outlet measuring planes
-    int xin, xout;
-    int explicitTerm;         // 1 when j(f^t) is part of J (t >= 1)
+    int xin, xout;            // global x of the measuring planes
+    int explicitTerm;         // 1 when j(f^t) is part of J (t > t0)
     double* partials;         // per workgroup: J contribution (slot 0)
 };
 
-// dj/df_k(y) for the objective (plane-mean pressure difference and drag).
-__device__ __forceinline__ void objective_gradient(const AdjointArgs& a, int x, double alphaY, double djdf[19]) {
-    const double p = (x == a.xin ? a.inW : 0.0) - (x == a.xout ? a.outW : 0.0);
-    for (int k = 0; k < 19; ++k) djdf[k] = a.wdp * p + a.wdrag * a.omega * alphaY * (double)cx(k);
+// dj/df_k(y) for the objective (plane-mean pressure difference, drag, mean flux).
+__device__ __forceinline__ void objective_gradient(const AdjointArgs& a, int gx, double alphaY, double djdf[19]) {
+    const double p = (gx == a.xin ? a.inW : 0.0) - (gx == a.xout ? a.outW : 0.0);
+    for (int k = 0; k < 19; ++k) djdf[k] = a.wdp * p + (a.wdrag * a.omega * alphaY + a.wflux) * (double)cx(k);
 }
 
 // mu(y) gathered from lambda^{t+1} (transpose of streaming with edge reflection).
@@ -100,6 +118,7 @@ __device__ __forceinline__ uint64_t adjoint_wg() {
 }
 
 // Reverse step for bulk (non-plane) cells; plane cells are left to adjoint_planes.
+template<int DYN, bool FORCED>
 __global__ void __launch_bounds__(256) adjoint_bulk(AdjointArgs a) {
     const BlockGeom g = a.g;
     const int z = blockIdx.x * blockDim.x + threadIdx.x;
@@ -107,40 +126,95 @@ __global__ void __launch_bounds__(256) adjoint_bulk(AdjointArgs a) {
     const int x = blockIdx.z;
     double J = 0.0;
     const uint64_t cell = ((uint64_t)x * g.ny + y) * g.nz + z;
-    if (z < g.nz && y < g.ny && !(a.tags && a.tags[cell] == BT_SKIP)) {
+    const uint64_t idx = g.cellOffset + cell;
+    if (z < g.nz && y < g.ny && !(a.tags && a.tags[idx] == BT_SKIP)) {
         double mu[19];
         gather_mu(a, x, y, z, cell, mu);
-        const double al = a.alpha ? a.alpha[cell] : 0.0;
-        const double rhoBar = a.mom[cell];
-        const double j[3] = {a.mom[a.momStride + cell], a.mom[2 * a.momStride + cell], a.mom[3 * a.momStride + cell]};
+        const double al = a.alpha ? a.alpha[idx] : 0.0;
+        const double dm = 1.0 - al;
+        const double rhoBar = a.mom[idx];
+        const double j[3] = {a.mom[a.momStride + idx], a.mom[2 * a.momStride + idx], a.mom[3 * a.momStride + idx]};
         const double rho = 1.0 + rhoBar;
-        const double Jv[3] = {(1 - al) * j[0], (1 - al) * j[1], (1 - al) * j[2]};
+        const double F[3] = {FORCED ? a.force[0] : 0.0, FORCED ? a.force[1] : 0.0, FORCED ? a.force[2] : 0.0};
+        const double Jv[3] = {dm * (j[0] + rho * F[0] / 2), dm * (j[1] + rho * F[1] / 2), dm * (j[2] + rho * F[2] / 2)};
         const double J2 = Jv[0] * Jv[0] + Jv[1] * Jv[1] + Jv[2] * Jv[2];
-        double A = 0, B[3] = {0, 0, 0};
+        const double w = a.omega;
+        double A = 0, B[3] = {0, 0, 0}, gG[3] = {0, 0, 0};
+        double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
             const double t = weight<double>(i);
             const double cJ = cx(i) * Jv[0] + cy(i) * Jv[1] + cz(i) * Jv[2];
-            A += mu[i] * t * (1.0 - (4.5 * cJ * cJ - 1.5 * J2) / (rho * rho));
+            const double mt = mu[i] * t;
+            A += mt * (1.0 - (4.5 * cJ * cJ - 1.5 * J2) / (rho * rho));
 #pragma unroll
-            for (int c = 0; c < 3; ++c) B[c] += mu[i] * t * (3.0 * ci(i, c) + (9.0 * cJ * ci(i, c) - 3.0 * Jv[c]) / rho);
+            for (int c = 0; c < 3; ++c) B[c] += mt * (3.0 * ci(i, c) + (9.0 * cJ * ci(i, c) - 3.0 * Jv[c]) / rho);
+            if constexpr (DYN == 1) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int e = 0; e < 3; ++e)
+                        M[c][e] += 4.5 * mt * (ci(i, c) * ci(i, e) - (c == e ? 1.0 / 3.0 : 0.0));
+            }
+            if constexpr (FORCED) {
+                const double cF = cx(i) * F[0] + cy(i) * F[1] + cz(i) * F[2];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) gG[c] += mt * (-3.0 * F[c] + 9.0 * cF * ci(i, c));
+            }
         }
-        const double w = a.omega;
+        if constexpr (FORCED) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gG[c] *= (1.0 - w / 2.0);
+        }
+        const double BF = B[0] * F[0] + B[1] * F[1] + B[2] * F[2];
+        const double gj = gG[0] * j[0] + gG[1] * j[1] + gG[2] * j[2];
+        double s, v[3];
+        if constexpr (DYN == 0) {
+            s = w * (A + dm * BF / 2) - dm * gj / (rho * rho);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] = w * dm * B[c] + dm * gG[c] / rho;
+        } else {
+            double Mj[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Mj[c] = M[c][0] * j[0] + M[c][1] * j[1] + M[c][2] * j[2];
+            const double jMj = j[0] * Mj[0] + j[1] * Mj[1] + j[2] * Mj[2];
+            const double trM = M[0][0] + M[1][1] + M[2][2];
+            s = A + dm * BF / 2 + (1 - w) * (jMj / (rho * rho) - trM / 3.0) - dm * gj / (rho * rho);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] = dm * B[c] - 2.0 * (1 - w) * Mj[c] / rho + dm * gG[c] / rho;
+        }
+        const int gx = g.gx0 + x;
         double djdf[19];
-        objective_gradient(a, x, al, djdf);
+        objective_gradient(a, gx, al, djdf);
 #pragma unroll
         for (int k = 0; k < 19; ++k) {
-            double v = (1 - w) * mu[k] + w * (A + (1 - al) * (B[0] * cx(k) + B[1] * cy(k) + B[2] * cz(k)));
-            if (a.explicitTerm) v += a.invT * djdf[k];
-            a.lam[(uint64_t)k * g.slot + cell] = v;
+            double val = s + v[0] * cx(k) + v[1] * cy(k) + v[2] * cz(k);
+            if constexpr (DYN == 0) {
+                val += (1 - w) * mu[k];
+            } else {
+                double cMc = 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) cMc += ci(k, c) * M[c][e] * ci(k, e);
+                val += (1 - w) * cMc;
+            }
+            if (a.explicitTerm) val += a.invT * djdf[k];
+            a.lam[(uint64_t)k * g.slot + cell] = val;
         }
-        double ga = -w * (B[0] * j[0] + B[1] * j[1] + B[2] * j[2]);
-        if (a.explicitTerm) {
-            ga += a.invT * a.wdrag * w * j[0];
-            const double p = (x == a.xin ? a.inW : 0.0) - (x == a.xout ? a.outW : 0.0);
-            J += a.invT * (a.wdp * p * rho + a.wdrag * w * al * j[0]);
+        if (in_bulk(g, x, y, z)) {
+            const double wB = DYN == 0 ? w : 1.0;
+            double ga = -(wB * (B[0] * (j[0] + rho * F[0] / 2) + B[1] * (j[1] + rho * F[1] / 2) +
+                                B[2] * (j[2] + rho * F[2] / 2)) +
+                          gG[0] * (j[0] / rho + F[0] / 2) + gG[1] * (j[1] / rho + F[1] / 2) +
+                          gG[2] * (j[2] / rho + F[2] / 2));
+            if (a.explicitTerm) {
+                ga += a.invT * a.wdrag * w * j[0];
+                const double p = (gx == a.xin ? a.inW : 0.0) - (gx == a.xout ? a.outW : 0.0);
+                J += a.invT * (a.wdp * p * rho + a.wdrag * w * al * j[0] + a.wflux * j[0]);
+            }
+            if (a.gAlpha) a.gAlpha[idx] += ga;
         }
-        if (a.gAlpha) a.gAlpha[cell] += ga;
     }
     double r[8] = {J, 0, 0, 0, 0, 0, 0, 0};
     block_reduce_store(r, a.partials + 8 * adjoint_wg());
@@ -190,13 +264,18 @@ __global__ void __launch_bounds__(256) adjoint_planes(AdjointSpecialArgs s) {
             collide_bulk<Dual, 0, false>(fd, Dual(a.omega), F0, Dual(0.0), rhoBar, drag);
             double dot = 0;
             for (int i = 0; i < 19; ++i) dot += mu[i] * fd[i].d;
-            const double p = (x == a.xin ? a.inW : 0.0) - (x == a.xout ? a.outW : 0.0);
+            const int gx = g.gx0 + x;
+            const double p = (gx == a.xin ? a.inW : 0.0) - (gx == a.xout ? a.outW : 0.0);
             if (dirn < 19) {
                 double djdf[19];
-                objective_gradient(a, x, 0.0, djdf);
+                objective_gradient(a, gx, 0.0, djdf);
                 if (a.explicitTerm) dot += a.invT * djdf[dirn];
                 a.lam[(uint64_t)dirn * g.slot + cell] = dot;
-                if (dirn == 0 && a.explicitTerm) J += a.invT * a.wdp * p * (1.0 + rhoBar0);
+                if (dirn == 0 && a.explicitTerm) {
+                    double jx0 = 0;
+                    for (int i = 0; i < 19; ++i) jx0 += cx(i) * a.planeF[19 * (uint64_t)k + i];
+                    J += a.invT * (a.wdp * p * (1.0 + rhoBar0) + a.wflux * jx0);
+                }
             } else {
                 s.gScale[k] += dot;
             }
@@ -206,7 +285,8 @@ __global__ void __launch_bounds__(256) adjoint_planes(AdjointSpecialArgs s) {
     block_reduce_store(r, s.partials + 8 * (uint64_t)blockIdx.x);
 }
 
-// lambda^T = (1/T) dj/df(f^T); J += (1/T) j(f^T); g_alpha += (1/T) w_drag omega j_x(f^T).
+// lambda^T = dj/df(f^T) / (T - t0); J += j(f^T) / (T - t0);
+// g_alpha += w_drag omega j_x(f^T) / (T - t0).
 __global__ void __launch_bounds__(256) adjoint_final(AdjointArgs a) {
     const BlockGeom g = a.g;
     const int z = blockIdx.x * blockDim.x + threadIdx.x;
@@ -214,17 +294,21 @@ __global__ void __launch_bounds__(256) adjoint_final(AdjointArgs a) {
     const int x = blockIdx.z;
     double J = 0.0;
     const uint64_t cell = ((uint64_t)x * g.ny + y) * g.nz + z;
+    const uint64_t idx = g.cellOffset + cell;
     if (z < g.nz && y < g.ny) {
-        const bool plane = a.tags && a.tags[cell] == BT_SKIP;
-        const double al = (a.alpha && !plane) ? a.alpha[cell] : 0.0;
-        const double rhoBar = a.mom[cell];
-        const double jx = a.mom[a.momStride + cell];
+        const bool plane = a.tags && a.tags[idx] == BT_SKIP;
+        const double al = (a.alpha && !plane) ? a.alpha[idx] : 0.0;
+        const double rhoBar = a.mom[idx];
+        const double jx = a.mom[a.momStride + idx];
+        const int gx = g.gx0 + x;
         double djdf[19];
-        objective_gradient(a, x, al, djdf);
+        objective_gradient(a, gx, al, djdf);
         for (int k = 0; k < 19; ++k) a.lam[(uint64_t)k * g.slot + cell] = a.invT * djdf[k];
-        if (a.gAlpha && !plane) a.gAlpha[cell] += a.invT * a.wdrag * a.omega * jx;
-        const double p = (x == a.xin ? a.inW : 0.0) - (x == a.xout ? a.outW : 0.0);
-        J += a.invT * (a.wdp * p * (1.0 + rhoBar) + a.wdrag * a.omega * al * jx);
+        if (in_bulk(g, x, y, z)) {
+            if (a.gAlpha && !plane) a.gAlpha[idx] += a.invT * a.wdrag * a.omega * jx;
+            const double p = (gx == a.xin ? a.inW : 0.0) - (gx == a.xout ? a.outW : 0.0);
+            J += a.invT * (a.wdp * p * (1.0 + rhoBar) + a.wdrag * a.omega * al * jx + a.wflux * jx);
+        }
     }
     double r[8] = {J, 0, 0, 0, 0, 0, 0, 0};
     block_reduce_store(r, a.partials + 8 * adjoint_wg());

@@ -180,6 +180,12 @@ int setup(Lattice& L) {
             env0[a] = (lo[a] > 0 || per[a]) ? kEnvelope : 0;
             env1[a] = (hi[a] < n[a] - 1 || per[a]) ? kEnvelope : 0;
         }
+        // 16-bit storage runs two cells per thread when every block is even along
+        // z: widen one z envelope to three cells where that makes the block even.
+        if (L.multi && storageSize(d.storage) == 2 && (hi[2] - lo[2] + 1 + env0[2] + env1[2]) % 2 != 0) {
+            if (env1[2] > 0) env1[2] += 1;
+            else if (env0[2] > 0) env0[2] += 1;
+        }
         g.nx = hi[0] - lo[0] + 1 + env0[0] + env1[0];
         g.ny = hi[1] - lo[1] + 1 + env0[1] + env1[1];
         g.nz = hi[2] - lo[2] + 1 + env0[2] + env1[2];
@@ -478,17 +484,17 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
     const bool tags = L.dTags != nullptr;
     const dim3 grid((unsigned)L.nBulkPartials);
     if (record) {
-        if constexpr (std::is_same<S, double>::value && DYN == 0) {
-            if (stats) return fail(PLBAMD_EINVAL, "recording step: statistics must be off");
-            a.moments = record->moments;
-            a.momStride = record->momStride;
+        if (stats) return fail(PLBAMD_EINVAL, "recording step: statistics must be off");
+        a.moments = record->moments;
+        a.momStride = record->momStride;
+        if constexpr (std::is_same<S, double>::value) {
 #define PLBAMD_REC(O, TG, FO)                                                                                   \
     do {                                                                                                        \
         if (O && L.staged)                                                                                      \
-            hipLaunchKernelGGL((bulk_odd_staged<double, 0, TG, false, FO, true>), grid, L.bulkBlock,             \
+            hipLaunchKernelGGL((bulk_odd_staged<double, DYN, TG, false, FO, true>), grid, L.bulkBlock,           \
                                L.stagedLds, L.stream, a);                                                       \
         else                                                                                                    \
-            hipLaunchKernelGGL((bulk_step<double, 0, O, TG, false, FO, true>), grid, L.bulkBlock, 0, L.stream, a); \
+            hipLaunchKernelGGL((bulk_step<double, DYN, O, TG, false, FO, true>), grid, L.bulkBlock, 0, L.stream, a); \
     } while (0)
 #define PLBAMD_REC_F(O, TG) if (forced) PLBAMD_REC(O, TG, true); else PLBAMD_REC(O, TG, false)
 #define PLBAMD_REC_T(O) if (tags) { PLBAMD_REC_F(O, true); } else { PLBAMD_REC_F(O, false); }
@@ -496,8 +502,18 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
 #undef PLBAMD_REC_T
 #undef PLBAMD_REC_F
 #undef PLBAMD_REC
+        } else if constexpr (sizeof(typename Store<S>::R) == 2) {
+            if (!L.pairs) return fail(PLBAMD_EINVAL, "recording step: 16-bit storage needs every block even along z");
+#define PLBAMD_RECP(O, TG, FO) \
+    hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, false, FO, true>), grid, L.bulkBlock, O ? L.pairsLds : 0, L.stream, a)
+#define PLBAMD_RECP_F(O, TG) if (forced) PLBAMD_RECP(O, TG, true); else PLBAMD_RECP(O, TG, false)
+#define PLBAMD_RECP_T(O) if (tags) { PLBAMD_RECP_F(O, true); } else { PLBAMD_RECP_F(O, false); }
+            if (odd) { PLBAMD_RECP_T(true) } else { PLBAMD_RECP_T(false) }
+#undef PLBAMD_RECP_T
+#undef PLBAMD_RECP_F
+#undef PLBAMD_RECP
         } else {
-            return fail(PLBAMD_EINVAL, "recording step: FP64 BGK only");
+            return fail(PLBAMD_EINVAL, "recording step: FP64 or 16-bit storage");
         }
     } else if (L.pairs) {
         if constexpr (sizeof(typename Store<S>::R) == 2) {
@@ -742,44 +758,65 @@ int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sum
 // (rhoBar, j) of every cell and the populations of the plane cells, then K
 // reverse steps.
 
+int recordStep(Lattice& L, bool odd, const Recording& r) {
+    const bool bgk = L.desc.dynamics == PLBAMD_BGK;
+    switch (L.desc.storage) {
+    case PLBAMD_F64: return bgk ? launchStepTyped<double, 0>(L, odd, false, &r) : launchStepTyped<double, 1>(L, odd, false, &r);
+    case PLBAMD_F16: return bgk ? launchStepTyped<f16s, 0>(L, odd, false, &r) : launchStepTyped<f16s, 1>(L, odd, false, &r);
+    case PLBAMD_BF16: return bgk ? launchStepTyped<bf16s, 0>(L, odd, false, &r) : launchStepTyped<bf16s, 1>(L, odd, false, &r);
+    default: return fail(PLBAMD_EINVAL, "adjoint: FP64 or 16-bit storage");
+    }
+}
+
 int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, double* Jout, double* gradAlpha,
                     double* gradScale, double* timing) {
-    if (L.multi) return fail(PLBAMD_EINVAL, "adjoint: single-block lattices only");
-    if (L.desc.storage != PLBAMD_F64) return fail(PLBAMD_EINVAL, "adjoint: FP64 storage only");
-    if (L.desc.dynamics != PLBAMD_BGK) return fail(PLBAMD_EINVAL, "adjoint: BGK dynamics only");
-    if (L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0)
-        return fail(PLBAMD_EINVAL, "adjoint: no body force");
+    const int st = L.desc.storage;
+    if (st != PLBAMD_F64 && st != PLBAMD_F16 && st != PLBAMD_BF16)
+        return fail(PLBAMD_EINVAL, "adjoint: FP64 or 16-bit storage");
+    const bool forced = L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0;
     if (!L.hostTags.empty()) return fail(PLBAMD_EINVAL, "adjoint: no per-cell tags (walls are the domain edges)");
     if (T <= 0 || K <= 0 || (K & 1)) return fail(PLBAMD_EINVAL, "adjoint: steps > 0 and an even checkpoint interval");
+    if (obj.first_step < 0 || obj.first_step >= T) return fail(PLBAMD_EINVAL, "adjoint: first_step in [0, nsteps)");
     if (L.time & 1) return fail(PLBAMD_ESTATE, "adjoint: start after an even number of steps");
     if (obj.inlet_x < 0 || obj.inlet_x >= L.desc.nx || obj.outlet_x < 0 || obj.outlet_x >= L.desc.nx)
         return fail(PLBAMD_EINVAL, "adjoint: measuring planes outside the lattice");
-    if (!L.hostAlpha.empty())
-        for (const Plane& p : L.planes) {
-            const plbamd_box& b = p.box;
-            for (int x = b.x0; x <= b.x1; ++x)
-                for (int y = b.y0; y <= b.y1; ++y)
-                    for (int z = b.z0; z <= b.z1; ++z)
-                        if (L.hostAlpha[((uint64_t)x * L.desc.ny + y) * L.desc.nz + z] != 0.0)
-                            return fail(PLBAMD_EINVAL, "adjoint: porosity must vanish on boundary planes");
-        }
+    if (!L.planes.empty()) {
+        if (L.multi) return fail(PLBAMD_EINVAL, "adjoint: boundary planes on single-block lattices only");
+        if (L.desc.dynamics != PLBAMD_BGK) return fail(PLBAMD_EINVAL, "adjoint: boundary planes with BGK only");
+        if (!L.hostAlpha.empty())
+            for (const Plane& p : L.planes) {
+                const plbamd_box& b = p.box;
+                for (int x = b.x0; x <= b.x1; ++x)
+                    for (int y = b.y0; y <= b.y1; ++y)
+                        for (int z = b.z0; z <= b.z1; ++z)
+                            if (L.hostAlpha[((uint64_t)x * L.desc.ny + y) * L.desc.nz + z] != 0.0)
+                                return fail(PLBAMD_EINVAL, "adjoint: porosity must vanish on boundary planes");
+            }
+    }
     PLBAMD_TRY(rebuild(L));
-    const BlockGeom& g = L.geom[0];
-    const uint64_t stateElems = g.slot * 19;
-    const size_t stateBytes = stateElems * sizeof(double);
+    if (st != PLBAMD_F64 && !L.pairs) return fail(PLBAMD_EINVAL, "adjoint: 16-bit storage needs every block even along z");
+    const BlockGeom& last = L.geom.back();
+    const uint64_t arenaElems = last.offset + last.slot * 19;
+    const size_t stateBytes = arenaElems * L.elemSize;
     const int nCk = (T + K - 1) / K;
-    const uint64_t cells = (uint64_t)g.nx * g.ny * g.nz;
+    const uint64_t cells = L.allocCells;
     const uint64_t momElems = 4 * cells;                       // per recorded step
     const uint64_t pfElems = 19 * (uint64_t)std::max<uint32_t>(1, L.nSpecial);
     const dim3 ablock(64, 4);
-    const dim3 agrid((unsigned)((g.nz + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)g.nx);
-    const uint64_t cellWGs = (uint64_t)agrid.x * agrid.y * agrid.z;
+    std::vector<dim3> agrid(L.geom.size());
+    uint64_t cellWGs = 0;
+    for (size_t b = 0; b < L.geom.size(); ++b) {
+        const BlockGeom& g = L.geom[b];
+        agrid[b] = dim3((unsigned)((g.nz + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)g.nx);
+        cellWGs += (uint64_t)agrid[b].x * agrid[b].y * agrid[b].z;
+    }
     const uint32_t planeWGs = (uint32_t)((20 * (uint64_t)L.nSpecial + 255) / 256);
     const uint64_t recordsPerStep = cellWGs + planeWGs;
     // device memory: checkpoints, a working state, the recorded moments and plane
     // populations of one segment, two adjoint states, gradients, partial sums
-    double *ck = nullptr, *work = nullptr, *mom = nullptr, *pf = nullptr, *lamA = nullptr, *lamB = nullptr,
-           *gA = nullptr, *gS = nullptr, *part = nullptr, *sum = nullptr;
+    char *ck = nullptr, *work = nullptr;
+    double *mom = nullptr, *pf = nullptr, *lamA = nullptr, *lamB = nullptr, *gA = nullptr, *gS = nullptr,
+           *part = nullptr, *sum = nullptr;
     auto release = [&]() {
         hipFree(ck); hipFree(work); hipFree(mom); hipFree(pf); hipFree(lamA); hipFree(lamB); hipFree(gA);
         hipFree(gS); hipFree(part); hipFree(sum);
@@ -789,8 +826,8 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     if (err == hipSuccess) err = hipMalloc(&work, stateBytes);
     if (err == hipSuccess) err = hipMalloc(&mom, momElems * (K + 1) * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&pf, pfElems * (K + 1) * sizeof(double));
-    if (err == hipSuccess) err = hipMalloc(&lamA, stateBytes);
-    if (err == hipSuccess) err = hipMalloc(&lamB, stateBytes);
+    if (err == hipSuccess) err = hipMalloc(&lamA, arenaElems * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc(&lamB, arenaElems * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&gA, cells * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&gS, std::max<uint32_t>(1, L.nSpecial) * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&part, nRecords * 8 * sizeof(double));
@@ -809,7 +846,7 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     // forward, with checkpoints (even time: the AA array is in natural layout)
     if (err == hipSuccess) err = hipEventRecord(L.ev0, L.stream);
     for (int c = 0; c < nCk && err == hipSuccess; ++c) {
-        err = hipMemcpyAsync(ck + (uint64_t)c * stateElems, L.arena, stateBytes, hipMemcpyDeviceToDevice, L.stream);
+        err = hipMemcpyAsync(ck + (uint64_t)c * stateBytes, L.arena, stateBytes, hipMemcpyDeviceToDevice, L.stream);
         if (err != hipSuccess) break;
         if (step(L, std::min(K, T - c * K)) != PLBAMD_OK) err = hipErrorUnknown;
     }
@@ -822,15 +859,31 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     a.alpha = L.dAlpha;
     a.gAlpha = gA;
     a.tags = L.dTags;
-    a.g = g;
     a.omega = L.desc.omega;
-    a.invT = 1.0 / T;
+    for (int c = 0; c < 3; ++c) a.force[c] = L.desc.force[c];
+    a.invT = 1.0 / (T - obj.first_step);
     a.wdp = obj.weight_pressure_drop;
     a.wdrag = obj.weight_drag;
-    a.inW = (1.0 / 3.0) / ((double)g.ny * g.nz);
-    a.outW = (1.0 / 3.0) / ((double)g.ny * g.nz);
+    a.wflux = obj.weight_flux / ((double)L.desc.nx * L.desc.ny * L.desc.nz);
+    a.inW = (1.0 / 3.0) / ((double)L.desc.ny * L.desc.nz);
+    a.outW = a.inW;
     a.xin = obj.inlet_x;
     a.xout = obj.outlet_x;
+    const bool rbgk = L.desc.dynamics == PLBAMD_RBGK;
+    auto reverseKernel = [&](const AdjointArgs& args, const dim3& gr) {
+        if (rbgk) {
+            if (forced) hipLaunchKernelGGL((adjoint_bulk<1, true>), gr, ablock, 0, L.stream, args);
+            else hipLaunchKernelGGL((adjoint_bulk<1, false>), gr, ablock, 0, L.stream, args);
+        } else {
+            if (forced) hipLaunchKernelGGL((adjoint_bulk<0, true>), gr, ablock, 0, L.stream, args);
+            else hipLaunchKernelGGL((adjoint_bulk<0, false>), gr, ablock, 0, L.stream, args);
+        }
+    };
+    auto exchangeLam = [&](double* lam) {
+        if (L.multi && L.nSegs)
+            hipLaunchKernelGGL((exchange_envelopes<double>), gridFor(L.segCells), dim3(256), 0, L.stream,
+                               (void*)lam, L.dGeom, L.dSegs, L.nSegs, L.segCells);
+    };
     uint64_t rec = 0;
     double* lamNext = lamA;
     double* lamCur = lamB;
@@ -840,7 +893,7 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     if (err == hipSuccess) err = hipEventRecord(L.ev0, L.stream);
     for (int c = nCk - 1; c >= 0 && err == hipSuccess; --c) {
         const int t0 = c * K, len = std::min(K, T - t0);
-        err = hipMemcpyAsync(work, ck + (uint64_t)c * stateElems, stateBytes, hipMemcpyDeviceToDevice, L.stream);
+        err = hipMemcpyAsync(work, ck + (uint64_t)c * stateBytes, stateBytes, hipMemcpyDeviceToDevice, L.stream);
         // steps t0 .. t0 + len - 1 (and, in the last segment, one more for f^T)
         const int nRec = (c == nCk - 1) ? len + 1 : len;
         for (int m = 0; m < nRec && err == hipSuccess; ++m) {
@@ -848,29 +901,40 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
             r.moments = mom + (uint64_t)m * momElems;
             r.momStride = cells;
             r.specialF = L.nSpecial ? pf + (uint64_t)m * pfElems : nullptr;
-            if (launchStepTyped<double, 0>(L, (m & 1) != 0, false, &r) != PLBAMD_OK) err = hipErrorUnknown;
+            if (recordStep(L, (m & 1) != 0, r) != PLBAMD_OK) err = hipErrorUnknown;
         }
         if (err == hipSuccess && c == nCk - 1) {
             a.mom = mom + (uint64_t)len * momElems;
-            a.lam = lamNext;
-            a.partials = part + rec * 8;
-            hipLaunchKernelGGL(adjoint_final, agrid, ablock, 0, L.stream, a);
-            rec += cellWGs;
-            err = hipGetLastError();
+            a.explicitTerm = 1;
+            for (size_t b = 0; b < L.geom.size() && err == hipSuccess; ++b) {
+                a.g = L.geom[b];
+                a.lam = lamNext + L.geom[b].offset;
+                a.partials = part + rec * 8;
+                hipLaunchKernelGGL(adjoint_final, agrid[b], ablock, 0, L.stream, a);
+                rec += (uint64_t)agrid[b].x * agrid[b].y * agrid[b].z;
+                err = hipGetLastError();
+            }
+            exchangeLam(lamNext);
         }
         for (int m = len - 1; m >= 0 && err == hipSuccess; --m) {
             const int t = t0 + m;
             a.mom = mom + (uint64_t)m * momElems;
             a.planeF = pf + (uint64_t)m * pfElems;
-            a.lamNext = lamNext;
-            a.lam = lamCur;
-            a.explicitTerm = t >= 1 ? 1 : 0;
-            a.partials = part + rec * 8;
-            hipLaunchKernelGGL(adjoint_bulk, agrid, ablock, 0, L.stream, a);
-            rec += cellWGs;
-            err = hipGetLastError();
+            a.explicitTerm = t > obj.first_step ? 1 : 0;
+            for (size_t b = 0; b < L.geom.size() && err == hipSuccess; ++b) {
+                a.g = L.geom[b];
+                a.lamNext = lamNext + L.geom[b].offset;
+                a.lam = lamCur + L.geom[b].offset;
+                a.partials = part + rec * 8;
+                reverseKernel(a, agrid[b]);
+                rec += (uint64_t)agrid[b].x * agrid[b].y * agrid[b].z;
+                err = hipGetLastError();
+            }
             if (err == hipSuccess && L.nSpecial) {
                 AdjointSpecialArgs sa{};
+                a.g = L.geom[0];
+                a.lamNext = lamNext;
+                a.lam = lamCur;
                 sa.base = a;
                 sa.cells = L.dSpecial;
                 sa.count = L.nSpecial;
@@ -881,6 +945,7 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
                 rec += planeWGs;
                 err = hipGetLastError();
             }
+            exchangeLam(lamCur);
             std::swap(lamNext, lamCur);
         }
     }
@@ -896,8 +961,9 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     }
     double hsum[8] = {0};
     if (err == hipSuccess) err = hipMemcpyAsync(hsum, sum, sizeof(hsum), hipMemcpyDeviceToHost, L.stream);
-    std::vector<double> hgA(cells), hgS(std::max<uint32_t>(1, L.nSpecial));
-    if (err == hipSuccess && gradAlpha) err = hipMemcpyAsync(gradAlpha, gA, cells * sizeof(double), hipMemcpyDeviceToHost, L.stream);
+    std::vector<double> hgA(gradAlpha ? cells : 0), hgS(std::max<uint32_t>(1, L.nSpecial));
+    if (err == hipSuccess && gradAlpha)
+        err = hipMemcpyAsync(hgA.data(), gA, cells * sizeof(double), hipMemcpyDeviceToHost, L.stream);
     if (err == hipSuccess) err = hipMemcpyAsync(hgS.data(), gS, hgS.size() * sizeof(double), hipMemcpyDeviceToHost, L.stream);
     std::vector<SpecialCell> hs(L.nSpecial);
     if (err == hipSuccess && L.nSpecial)
@@ -906,6 +972,18 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     release();
     if (err != hipSuccess) return fail(PLBAMD_EHIP, std::string("adjoint: ") + hipGetErrorString(err));
     if (Jout) *Jout = hsum[0];
+    if (gradAlpha) {
+        // bulk cells of every block to the global [nx][ny][nz] array
+        const int ny = L.desc.ny, nz = L.desc.nz;
+        for (const BlockGeom& g : L.geom)
+            for (int x = g.bx0; x <= g.bx1; ++x)
+                for (int y = g.by0; y <= g.by1; ++y)
+                    for (int z = g.bz0; z <= g.bz1; ++z) {
+                        const uint64_t local = ((uint64_t)x * g.ny + y) * g.nz + z;
+                        const uint64_t glob = ((uint64_t)(g.gx0 + x) * ny + (g.gy0 + y)) * nz + (g.gz0 + z);
+                        gradAlpha[glob] = hgA[g.cellOffset + local];
+                    }
+    }
     if (gradScale) {
         // plane of each special record from its value index
         std::vector<uint32_t> base(L.planes.size() + 1, 0);

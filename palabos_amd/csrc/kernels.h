@@ -560,7 +560,7 @@ __device__ __forceinline__ uint16_t hi16(uint32_t v) { return (uint16_t)(v >> 16
 __device__ __forceinline__ uint32_t pack16(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
 
 // One cell of an odd step by per-lane neighbour tests (rows at block edges).
-template<typename S, int DYN, bool STATS, bool FORCED>
+template<typename S, int DYN, bool STATS, bool FORCED, bool MOMENTS>
 __device__ __forceinline__ void odd_cell_general(const StepArgs& a, const BlockGeom& g, S* f, int z, uint64_t cell,
                                                  uint8_t tag, bool inBulk, const int xs[3], const int ys[3],
                                                  const bool vx[3], const bool vy[3], double st[8]) {
@@ -579,7 +579,7 @@ __device__ __forceinline__ void odd_cell_general(const StepArgs& a, const BlockG
                                 : (uint64_t)i * g.slot + cell;
         fc[i] = Store<S>::load(f + src);
     }
-    collide_cell<S, DYN, STATS, FORCED, false>(a, g, cell, tag, inBulk, fc, st);
+    collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, inBulk, fc, st);
     Store<S>::store(f + cell, fc[0]);
 #pragma unroll
     for (int i = 1; i < 19; ++i) {
@@ -591,7 +591,7 @@ __device__ __forceinline__ void odd_cell_general(const StepArgs& a, const BlockG
     }
 }
 
-template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED>
+template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
 __global__ void __launch_bounds__(1024) bulk_pairs(StepArgs a) {
     static_assert(sizeof(typename Store<S>::R) == 2, "pairs of 16-bit values");
     using T = typename Store<S>::T;
@@ -644,8 +644,8 @@ __global__ void __launch_bounds__(1024) bulk_pairs(StepArgs a) {
                 fb[i] = Store<S>::from_raw(hi16(v));
             }
         }
-        if (compA) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellA, tagA, inBulkA, fa, st);
-        if (compB) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellB, tagB, inBulkB, fb, st);
+        if (compA) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellA, tagA, inBulkA, fa, st);
+        if (compB) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellB, tagB, inBulkB, fb, st);
         if (compA && compB) {
 #pragma unroll
             for (int i = 0; i < 19; ++i)
@@ -666,8 +666,8 @@ __global__ void __launch_bounds__(1024) bulk_pairs(StepArgs a) {
         const bool vx[3] = {vxm, true, vxp};
         const bool vy[3] = {vym, true, vyp};
         if (!(vxm && vxp && vym && vyp)) {
-            if (compA) odd_cell_general<S, DYN, STATS, FORCED>(a, g, f, zA, cellA, tagA, inBulkA, xs, ys, vx, vy, st);
-            if (compB) odd_cell_general<S, DYN, STATS, FORCED>(a, g, f, zB, cellB, tagB, inBulkB, xs, ys, vx, vy, st);
+            if (compA) odd_cell_general<S, DYN, STATS, FORCED, MOMENTS>(a, g, f, zA, cellA, tagA, inBulkA, xs, ys, vx, vy, st);
+            if (compB) odd_cell_general<S, DYN, STATS, FORCED, MOMENTS>(a, g, f, zB, cellB, tagB, inBulkB, xs, ys, vx, vy, st);
         } else {
             S* rows[3][3];
 #pragma unroll
@@ -734,8 +734,8 @@ __global__ void __launch_bounds__(1024) bulk_pairs(StepArgs a) {
                 }
             }
             __syncthreads();   // every LDS input is read before the outputs reuse the buffer
-            if (compA) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellA, tagA, inBulkA, fa, st);
-            if (compB) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellB, tagB, inBulkB, fb, st);
+            if (compA) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellA, tagA, inBulkA, fa, st);
+            if (compB) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellB, tagB, inBulkB, fb, st);
             if (active) {
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {

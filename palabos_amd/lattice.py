@@ -255,15 +255,17 @@ class Block3D:
         check(lib().plbamd_collide_and_stream_timed(self.handle, int(nsteps), ctypes.byref(ms)))
         return ms.value
 
-    def adjointGradient(self, nsteps, checkpoint_every, inlet_x, outlet_x, weight_pressure_drop=1.0,
-                        weight_drag=1.0):
-        """Runs nsteps (the lattice advances) and returns the time-averaged objective
-        J = mean_t [w_dp cs2 (<rho>_inlet_x - <rho>_outlet_x) + w_drag omega sum alpha j_x]
-        with its gradient with respect to the porosity field and to the scale of each
+    def adjointGradient(self, nsteps, checkpoint_every, inlet_x=0, outlet_x=0, weight_pressure_drop=1.0,
+                        weight_drag=1.0, weight_flux=0.0, first_step=0):
+        """Runs nsteps (the lattice advances) and returns the objective
+        J = mean over steps t0+1..T of [w_dp cs2 (<rho>_inlet_x - <rho>_outlet_x)
+        + w_drag omega sum alpha j_x + w_flux <j_x>] (t0 = first_step) with its
+        gradient with respect to the porosity field and to the scale of each
         plane's imposed velocity (hand-written adjoint, checkpoints every
         `checkpoint_every` steps). Returns (J, grad_alpha [nx,ny,nz], grad_plane_scale
         [planes], (forward_ms, backward_ms))."""
-        obj = native.Objective(int(inlet_x), int(outlet_x), float(weight_pressure_drop), float(weight_drag))
+        obj = native.Objective(int(inlet_x), int(outlet_x), float(weight_pressure_drop), float(weight_drag),
+                               float(weight_flux), int(first_step))
         J = ctypes.c_double(0.0)
         ga = np.zeros(self.shape, dtype=np.float64)
         gs = np.zeros(max(1, len(self._planes)), dtype=np.float64)

@@ -69,7 +69,8 @@ class LatticeDesc(ctypes.Structure):
 
 class Objective(ctypes.Structure):
     _fields_ = [("inlet_x", ctypes.c_int32), ("outlet_x", ctypes.c_int32),
-                ("weight_pressure_drop", ctypes.c_double), ("weight_drag", ctypes.c_double)]
+                ("weight_pressure_drop", ctypes.c_double), ("weight_drag", ctypes.c_double),
+                ("weight_flux", ctypes.c_double), ("first_step", ctypes.c_int32)]
 
 
 _lib = None

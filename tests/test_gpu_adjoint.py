@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import palabos_amd
-from cases import channel_case, gpu_lattice, rest_state
+from cases import Case, channel_case, gpu_lattice, random_state, rest_state, uneven_blocks
 from oracle import oracle
 from palabos_amd import setups
 
@@ -148,3 +148,63 @@ def test_autograd_function_chains_the_adjoint(gpu):
     assert float(J) == pytest.approx(Jd, rel=1e-12)
     np.testing.assert_allclose(alpha.grad.numpy().ravel()[seeds], 2 * Jd * dJa, rtol=1e-8, atol=1e-14)
     assert float(scale.grad) == pytest.approx(2 * Jd * dJs / s, rel=1e-9)
+
+
+def _forced_porous(dynamics, storage="f64", shape=(16, 10, 12)):
+    case = Case(*shape, storage=storage, dynamics=dynamics, omega=1.7, periodic=(True, False, True),
+                force=(1e-4, 0.0, 2e-5))
+    case.alpha = setups.brinkman_blobs(case.shape, n_blobs=5, sigma=(2, 3), seed=2)
+    return case
+
+
+def _lattice(case, blocks=None):
+    dyn = palabos_amd.BGK(case.omega) if case.dynamics == "bgk" else palabos_amd.RegularizedBGK(case.omega)
+    lat = palabos_amd.Block3D(case.nx, case.ny, case.nz, palabos_amd.D3Q19, dyn, storage=case.storage,
+                              periodic=case.periodic, force=case.force, blocks=blocks)
+    lat.setPorosity(case.alpha)
+    return lat
+
+
+@pytest.mark.parametrize("dynamics", ["bgk", "rbgk"])
+@pytest.mark.parametrize("tiling", ["single", "blocks"])
+def test_forced_adjoint_matches_oracle(gpu, dynamics, tiling):
+    """Guo-forced BGK / regularized BGK with porosity, periodic in x and z, walls
+    in y; objective: pressure drop + drag + mean x momentum over the last 6 of 14
+    steps. Single block and 6 uneven blocks (the adjoint state's envelopes are
+    refreshed after every reverse step) against the oracle's dual numbers."""
+    case = _forced_porous(dynamics)
+    f0 = random_state(case, 3) * 0.3
+    seeds = np.sort(np.random.RandomState(0).choice(np.flatnonzero(case.alpha.ravel() > 0.1), 16, replace=False))
+    T, K, t0 = 14, 4, 8
+    blocks = uneven_blocks(case.shape, [[7], [4], [5]]) if tiling == "blocks" else None
+    lat = _lattice(case, blocks)
+    try:
+        lat.setPopulations(f0)
+        J, ga, _, _ = lat.adjointGradient(T, K, 1, case.nx - 2, 0.5, 1.0, weight_flux=2.0, first_step=t0)
+    finally:
+        lat.close()
+    Jd, dJa, _ = oracle.objective_dual(case, f0, T, 1, case.nx - 2, 0.5, 1.0, seeds, w_flux=2.0, first_step=t0)
+    assert J == pytest.approx(Jd, rel=1e-11)
+    np.testing.assert_allclose(ga.ravel()[seeds], dJa, rtol=1e-8, atol=1e-15)
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_sixteen_bit_adjoint_close_to_fp64(gpu, storage):
+    """The adjoint of the 16-bit-storage forward (FP32 arithmetic, rounding taken
+    as the identity) against the FP64 adjoint of the same problem, on 4 blocks."""
+    case = _forced_porous("rbgk", shape=(16, 10, 12))
+    blocks = uneven_blocks(case.shape, [[8], [], [6]])
+    out = {}
+    for st in ("f64", storage):
+        c = copy.deepcopy(case)
+        c.storage = st
+        lat = _lattice(c, blocks)
+        try:
+            J, ga, _, _ = lat.adjointGradient(20, 4, 0, 0, 0.0, 0.0, weight_flux=1.0, first_step=10)
+        finally:
+            lat.close()
+        out[st] = (J, ga)
+    (J64, g64), (J16, g16) = out["f64"], out[storage]
+    tol = 2e-3 if storage == "f16" else 2e-2
+    assert abs(J16 - J64) <= tol * abs(J64)
+    assert np.linalg.norm(g16 - g64) <= tol * np.linalg.norm(g64)
