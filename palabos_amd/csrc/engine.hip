@@ -80,6 +80,7 @@ struct Lattice {
     double* dBulkPartials = nullptr;
     uint64_t nBulkPartials = 0;       // = workgroups of the bulk launch
     int xcdRemap = 0;
+    std::vector<uint32_t> planeValueBase;   // first value of each plane in dValues
     bool staged = false;           // odd steps through bulk_odd_staged
     bool pairs = false;            // 16-bit storage through bulk_pairs
     size_t pairsLds = 0;           // its dynamic LDS bytes (odd steps)
@@ -348,7 +349,8 @@ int rebuild(Lattice& L) {
     }
     const bool haveTags = !L.hostTags.empty();
     auto tagOf = [&](uint64_t c) -> uint8_t { return haveTags ? L.hostTags[c] : (uint8_t)PLBAMD_TAG_BULK; };
-    std::vector<uint32_t> planeValueBase(L.planes.size(), 0);
+    std::vector<uint32_t>& planeValueBase = L.planeValueBase;
+    planeValueBase.assign(L.planes.size(), 0);
     std::vector<double> values;
     for (size_t p = 0; p < L.planes.size(); ++p) {
         planeValueBase[p] = (uint32_t)values.size();
@@ -1118,6 +1120,13 @@ int plbamd_set_plane_values(int handle, int plane, const double* values, size_t 
     Plane& p = L->planes[plane];
     if (!values || count != p.values.size()) return fail(PLBAMD_EINVAL, "set_plane_values: wrong value count");
     std::copy(values, values + count, p.values.begin());
+    if (!L->dirty && L->dValues && plane < (int)L->planeValueBase.size()) {
+        // built already: refresh the plane's values on the device (after the queued steps)
+        PLBAMD_HIP(hipStreamSynchronize(L->stream));
+        PLBAMD_HIP(hipMemcpy(L->dValues + L->planeValueBase[plane], values, count * sizeof(double),
+                             hipMemcpyHostToDevice));
+        return PLBAMD_OK;
+    }
     L->dirty = true;
     return PLBAMD_OK;
 }

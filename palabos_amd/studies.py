@@ -3,6 +3,7 @@
     python -m palabos_amd.studies taylor_green [--n 512 --steps 2000]
     python -m palabos_amd.studies sphere_drag  [--steps 5000 --storage f64]
     python -m palabos_amd.studies porous       [--steps 20000 --storage f16]
+    python -m palabos_amd.studies porous_adjoint [--steps 2000 --storage f16]
 
 Each prints one JSON object. Everything runs on the device through the
 native library; the host only builds the geometry, reads back reductions and
@@ -113,10 +114,13 @@ def taylor_green_precision(n=512, steps=2000, omega=1.8, u0=0.04, storages=("f32
 # Channel flow past a sphere: drag coefficient
 
 def sphere_drag(nx=256, ny=128, nz=128, radius=16.0, center=(64.0, 64.0, 64.0), u_max=0.05, re=100.0,
-                steps=5000, storage="f64", average_last=2000, sample_every=10):
+                steps=5000, storage="f64", average_last=2000, sample_every=10, ramp=1500):
     """Zou-He Poiseuille inlet (x = 0, u_max), Zou-He pressure outlet (rho = 1),
     half-way bounce-back side walls (the domain edges), a momentum-exchange
-    sphere. Re = u_max 2 r / nu. C_d = 2 F_x / (rho u^2 pi r^2) with u = u_max
+    sphere. Re = u_max 2 r / nu. The inlet velocity rises as sin^2 over the
+    first `ramp` steps (an impulsive start fills the channel with pressure
+    waves that the velocity and pressure planes reflect, and that dominate the
+    force for thousands of steps). C_d = 2 F_x / (rho u^2 pi r^2) with u = u_max
     and with the mean inlet velocity; F_x averaged over the last steps."""
     nu = u_max * 2.0 * radius / re
     omega = 1.0 / (3.0 * nu + 0.5)
@@ -134,7 +138,15 @@ def sphere_drag(nx=256, ny=128, nz=128, radius=16.0, center=(64.0, 64.0, 64.0), 
         lat.setBoundaryDensity(p_out, np.ones((ny - 2) * (nz - 2)))
         lat.toggleInternalStatistics(False)
         main = steps - average_last
-        ms = lat.collideAndStreamTimed(main)
+        ms = 0.0
+        done = 0
+        while done < min(ramp, main):
+            m = min(10, ramp - done)
+            lat.setBoundaryVelocity(p_in, vel * math.sin(0.5 * math.pi * (done + m) / ramp) ** 2)
+            ms += lat.collideAndStreamTimed(m)
+            done += m
+        lat.setBoundaryVelocity(p_in, vel)
+        ms += lat.collideAndStreamTimed(main - done)
         lat.toggleInternalStatistics(True)
         fx = []
         done = main
@@ -146,7 +158,7 @@ def sphere_drag(nx=256, ny=128, nz=128, radius=16.0, center=(64.0, 64.0, 64.0), 
         fx = np.array(fx)
         u_mean = float(prof.sum()) / ((ny - 2) * (nz - 2))
         area = math.pi * radius * radius
-        return {"lattice": [nx, ny, nz], "storage": storage, "steps": steps, "re": re, "omega": omega,
+        return {"lattice": [nx, ny, nz], "storage": storage, "steps": steps, "re": re, "omega": omega, "ramp": ramp,
                 "sphere_cells": int((tags != 0).sum()), "fx_mean": float(fx.mean()), "fx_std": float(fx.std()),
                 "cd_u_max": 2.0 * fx.mean() / (u_max ** 2 * area),
                 "cd_u_mean": 2.0 * fx.mean() / (u_mean ** 2 * area),
@@ -199,9 +211,84 @@ def porous_permeability(shape=(384, 256, 256), count=400, steps=20000, storage="
         lat.close()
 
 
+def brinkman_spheres(shape, centres, radii, scale=1.0, width=1.0, periodic=(True, False, False)):
+    """Smooth porosity of a set of spheres: alpha = 1 - prod_b (1 - sigma((s r_b - d_b(x)) / w)),
+    sigma the logistic function, and its derivative d alpha / d s =
+    (1 - alpha) sum_b sigma_b r_b / w (each sphere evaluated on the box where
+    sigma exceeds ~1e-6)."""
+    log_open = np.zeros(shape)      # sum_b log(1 - sigma_b) = -sum_b softplus(z_b)
+    dsum = np.zeros(shape)          # sum_b sigma_b r_b / w
+    margin = 14.0 * width
+    for c, r in zip(centres, radii):
+        rs = r * scale
+        lo = np.floor(c - rs - margin).astype(int)
+        hi = np.ceil(c + rs + margin).astype(int)
+        idx = []
+        for a in range(3):
+            v = np.arange(lo[a], hi[a] + 1)
+            if periodic[a]:
+                d = v - c[a]
+                v = v % shape[a]
+            else:
+                keep = (v >= 0) & (v < shape[a])
+                v = v[keep]
+                d = v - c[a]
+            idx.append((v, d))
+        (vx, dx), (vy, dy), (vz, dz) = idx
+        dist = np.sqrt(dx[:, None, None] ** 2 + dy[None, :, None] ** 2 + dz[None, None, :] ** 2)
+        z = (rs - dist) / width
+        sig = 1.0 / (1.0 + np.exp(-z))
+        ix = np.ix_(vx, vy, vz)
+        log_open[ix] -= np.logaddexp(0.0, z)
+        dsum[ix] += sig * r / width
+    open_frac = np.exp(log_open)
+    return 1.0 - open_frac, open_frac * dsum
+
+
+def porous_adjoint(shape=(384, 256, 256), count=400, r_range=(6.0, 14.0), seed=5, steps=2000, checkpoint_every=50,
+                   window=1000, storage="f16", omega=1.7, g=1e-5, width=1.0, blocks=None, block_seed=11, fd_eps=1e-2,
+                   check_fd=True):
+    """Darcy permeability of the porous medium with the spheres as a smooth
+    Brinkman porosity, and its gradient with respect to a global sphere-radius
+    scale s by the adjoint: K = nu <j_x> / g averaged over the last `window` of
+    `steps` steps from rest, dK/ds = nu / g sum_x dJ/dalpha(x) dalpha/ds(x);
+    checked against central finite differences in s (K(s +- eps) from two more
+    runs). Regularized BGK with a Guo force, periodic in x, walls on the y and z
+    faces, 60 uneven blocks, checkpoints every `checkpoint_every` steps."""
+    nu = (1.0 / omega - 0.5) / 3.0
+    centres, radii = setups.random_spheres(shape, count, r_range, seed, periodic=(True, False, False))
+    if blocks is None:
+        blocks = setups.uneven_blocks(shape, (5, 4, 3), 32, 128, block_seed)
+
+    def run(scale, want_gradient):
+        alpha, dads = brinkman_spheres(shape, centres, radii, scale=scale, width=width)
+        lat = _lat.Block3D(*shape, _lat.D3Q19, _lat.RegularizedBGK(omega), storage=storage,
+                           periodic=(True, False, False), force=(g, 0.0, 0.0), blocks=blocks)
+        try:
+            lat.setPorosity(alpha)
+            J, ga, _, (ms_f, ms_b) = lat.adjointGradient(steps, checkpoint_every, 0, 0, 0.0, 0.0,
+                                                         weight_flux=1.0, first_step=steps - window)
+        finally:
+            lat.close()
+        dK = nu / g * float((ga * dads).sum()) if want_gradient else None
+        return nu * J / g, dK, ms_f, ms_b, float(1.0 - alpha.mean())
+
+    K, dK, ms_f, ms_b, porosity = run(1.0, True)
+    out = {"lattice": list(shape), "storage": storage, "blocks": len(blocks), "steps": steps,
+           "checkpoint_every": checkpoint_every, "window": window, "omega": omega, "g": g, "width": width,
+           "porosity": porosity, "permeability": K, "dK_dscale_adjoint": dK,
+           "forward_ms": ms_f, "reverse_ms": ms_b, "reverse_over_forward": ms_b / ms_f}
+    if check_fd:
+        Kp = run(1.0 + fd_eps, False)[0]
+        Km = run(1.0 - fd_eps, False)[0]
+        fd = (Kp - Km) / (2 * fd_eps)
+        out.update({"dK_dscale_fd": fd, "fd_eps": fd_eps, "relative_difference": abs(dK - fd) / abs(fd)})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["taylor_green", "sphere_drag", "porous"])
+    ap.add_argument("study", choices=["taylor_green", "sphere_drag", "porous", "porous_adjoint"])
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--storage", default=None)
@@ -213,8 +300,10 @@ def main():
         r = taylor_green_precision(n=args.n, steps=args.steps or 2000)
     elif args.study == "sphere_drag":
         r = sphere_drag(steps=args.steps or 5000, storage=args.storage or "f64")
-    else:
+    elif args.study == "porous":
         r = porous_permeability(steps=args.steps or 20000, storage=args.storage or "f16", scale=args.scale)
+    else:
+        r = porous_adjoint(steps=args.steps or 2000, storage=args.storage or "f16")
     print(json.dumps(r))
 
 

@@ -5,6 +5,7 @@ order, both compiled without contraction) and for FP16/BF16 storage (same FP32
 arithmetic, same round-to-nearest-even conversions). Statistics are FP64 sums
 taken in a different order, so they agree to 1e-12 relative.
 """
+import copy
 import hashlib
 import os
 
@@ -132,6 +133,29 @@ def test_gpu_launch_variants(gpu, monkeypatch, variant, name, storage):
     _compare(case, got, want)
     np.testing.assert_allclose(gst["avg_rhobar"], wst["avg_rhobar"], rtol=1e-12, atol=1e-15)
     np.testing.assert_allclose(gst["force"], wst["force"], rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f16"])
+def test_plane_values_changed_between_runs(gpu, storage):
+    """New inlet values set after the lattice has run (refreshed on the device
+    without a rebuild) act from the next step on, as in the oracle."""
+    case = channel_case(16, 10, 11, 1.8, 0.05, storage=storage, sphere=((6, 5, 5), 2.2))
+    f0 = _state(case)
+    lat = gpu_lattice(case)
+    try:
+        lat.setPopulations(f0)
+        lat.collideAndStream(5)
+        slower = case.planes[0].values * 0.5
+        lat.setBoundaryVelocity(0, slower)
+        lat.collideAndStream(6)
+        got = lat.getPopulations(dtype=np.uint16 if storage == "f16" else np.float64)
+    finally:
+        lat.close()
+    want, _ = oracle.run(case, f0, 5)
+    case2 = copy.deepcopy(case)
+    case2.planes[0].values = slower
+    want, _ = oracle.run(case2, want, 6)
+    _compare(case, got, want)
 
 
 @pytest.mark.parametrize("steps", [0, 1, 2, 3])

@@ -28,11 +28,11 @@ def test_taylor_green_precision_small(gpu):
 
 def test_sphere_drag_small(gpu):
     r = studies.sphere_drag(nx=64, ny=32, nz=32, radius=4.0, center=(16.0, 16.0, 16.0), u_max=0.05, re=20.0,
-                            steps=1500, average_last=200)
+                            steps=1500, average_last=200, ramp=500)
     print(r, flush=True)
     assert r["sphere_cells"] > 0
     assert r["fx_mean"] > 0
-    assert r["fx_std"] < 0.3 * r["fx_mean"]   # acoustic waves between the Zou-He planes
+    assert r["fx_std"] < 0.1 * r["fx_mean"]   # the ramped start keeps the pressure waves small
     assert 1.0 < r["cd_u_mean"] < 100.0
 
 
@@ -48,3 +48,16 @@ def test_porous_permeability_small(gpu):
     assert np.all(h > 0) and r["permeability"] > 0
     # approaching a steady Darcy flow: increments shrink
     assert abs(h[-1] - h[-2]) < abs(h[1] - h[0])
+
+
+def test_porous_adjoint_small_fd(gpu):
+    """Radius-scale gradient of the permeability of a smooth-sphere porous medium
+    (regularized BGK, Guo force, 8 uneven blocks, FP64) by the adjoint against
+    central finite differences in the scale."""
+    shape = (48, 32, 32)
+    blocks = studies.setups.uneven_blocks(shape, (2, 2, 2), 12, 36, 3)
+    r = studies.porous_adjoint(shape=shape, count=6, steps=200, checkpoint_every=20, window=100, storage="f64",
+                               g=1e-4, blocks=blocks, fd_eps=1e-3)
+    print(r, flush=True)
+    assert r["dK_dscale_adjoint"] < 0      # larger spheres, lower permeability
+    assert r["relative_difference"] < 1e-4
