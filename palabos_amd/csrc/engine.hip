@@ -57,6 +57,9 @@ struct Lattice {
     std::vector<plbamd_box> bulk;     // global bulk box of each block
     std::vector<BlockGeom> geom;      // device geometry of each block (host copy)
     BlockGeom* dGeom = nullptr;
+    uint16_t* dWgBlock = nullptr;   // block of every bulk work item (multi-block)
+    CopyTask* dTasks = nullptr;     // envelope exchange: one workgroup per task
+    uint32_t nTasks = 0;
     uint64_t allocCells = 0;          // cells of all blocks (per-cell arrays)
     uint64_t cells = 0;               // cells of the domain
     size_t elemSize = 0;
@@ -100,6 +103,7 @@ struct Lattice {
     ~Lattice() {
         hipFree(arena); hipFree(dGeom); hipFree(dTags); hipFree(dSpecial); hipFree(dValues); hipFree(dAlpha);
         hipFree(dBulkPartials); hipFree(dSpecialPartials); hipFree(dStats); hipFree(dSegs);
+        hipFree(dWgBlock); hipFree(dTasks);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
         if (stream) hipStreamDestroy(stream);
@@ -218,7 +222,22 @@ int setup(Lattice& L) {
         if (!getenv("PLBAMD_BULK_TX")) txMax = 256;
     }
     const int cellsPerThread = pairs ? 2 : 1;
-    const int tx = (int)std::min<int64_t>(txMax, ((int64_t)(maxNz + cellsPerThread - 1) / cellsPerThread + 63) / 64 * 64);
+    int tx = (int)std::min<int64_t>(txMax, ((int64_t)(maxNz + cellsPerThread - 1) / cellsPerThread + 63) / 64 * 64);
+    if ((staged || pairs) && !getenv("PLBAMD_BULK_TX")) {
+        // one row segment per workgroup: take the width (a multiple of 64 up to the
+        // maximum) that leaves the fewest idle threads over every block's rows,
+        // the widest among equals
+        uint64_t best = UINT64_MAX;
+        for (int w = 64; w <= tx; w += 64) {
+            uint64_t threads = 0;
+            for (const BlockGeom& g : L.geom) {
+                const int units = (g.nz + cellsPerThread - 1) / cellsPerThread;
+                threads += (uint64_t)g.nx * g.ny * ((units + w - 1) / w) * w;
+            }
+            if (threads <= best) { best = threads; L.bulkBlock.x = w; }
+        }
+        tx = (int)L.bulkBlock.x;
+    }
     const int ty = (staged || pairs) ? 1 : std::max(1, txMax / tx);
     L.bulkBlock = dim3(tx, ty);
     L.zChunk = tx * (pairs ? 2 : cpt);
@@ -254,6 +273,15 @@ int setup(Lattice& L) {
     PLBAMD_HIP(hipMemset(L.arena, 0, bytes));
     PLBAMD_HIP(hipMalloc(&L.dGeom, L.geom.size() * sizeof(BlockGeom)));
     PLBAMD_HIP(hipMemcpy(L.dGeom, L.geom.data(), L.geom.size() * sizeof(BlockGeom), hipMemcpyHostToDevice));
+    if (L.multi) {
+        std::vector<uint16_t> wgBlock(L.nBulkPartials);
+        for (size_t b = 0; b < L.geom.size(); ++b) {
+            const uint64_t end = b + 1 < L.geom.size() ? L.geom[b + 1].wgBegin : L.nBulkPartials;
+            std::fill(wgBlock.begin() + L.geom[b].wgBegin, wgBlock.begin() + end, (uint16_t)b);
+        }
+        PLBAMD_HIP(hipMalloc(&L.dWgBlock, wgBlock.size() * sizeof(uint16_t)));
+        PLBAMD_HIP(hipMemcpy(L.dWgBlock, wgBlock.data(), wgBlock.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    }
     PLBAMD_HIP(hipMalloc(&L.dBulkPartials, L.nBulkPartials * 8 * sizeof(double)));
     PLBAMD_HIP(hipMalloc(&L.dStats, 16 * sizeof(double)));
     PLBAMD_HIP(hipMemset(L.dStats, 0, 16 * sizeof(double)));
@@ -311,6 +339,14 @@ int setup(Lattice& L) {
         if (!segs.empty()) {
             PLBAMD_HIP(hipMalloc(&L.dSegs, segs.size() * sizeof(CopySegment)));
             PLBAMD_HIP(hipMemcpy(L.dSegs, segs.data(), segs.size() * sizeof(CopySegment), hipMemcpyHostToDevice));
+            std::vector<CopyTask> tasks;
+            for (size_t k = 0; k < segs.size(); ++k) {
+                const uint64_t n = (uint64_t)segs[k].lx * segs[k].ly * segs[k].lz;
+                for (uint64_t f = 0; f < n; f += 256) tasks.push_back(CopyTask{(uint32_t)k, (uint32_t)f});
+            }
+            L.nTasks = (uint32_t)tasks.size();
+            PLBAMD_HIP(hipMalloc(&L.dTasks, tasks.size() * sizeof(CopyTask)));
+            PLBAMD_HIP(hipMemcpy(L.dTasks, tasks.data(), tasks.size() * sizeof(CopyTask), hipMemcpyHostToDevice));
         }
     }
     PLBAMD_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
@@ -454,8 +490,8 @@ int rebuild(Lattice& L) {
 template<typename S>
 int exchange(Lattice& L) {
     if (!L.multi || L.nSegs == 0) return PLBAMD_OK;
-    hipLaunchKernelGGL((exchange_envelopes<S>), gridFor(L.segCells), dim3(256), 0, L.stream, L.arena, L.dGeom,
-                       L.dSegs, L.nSegs, L.segCells);
+    hipLaunchKernelGGL((exchange_envelopes<S>), dim3(L.nTasks), dim3(256), 0, L.stream, L.arena, L.dGeom, L.dSegs,
+                       L.dTasks);
     PLBAMD_HIP(hipGetLastError());
     return PLBAMD_OK;
 }
@@ -482,6 +518,7 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
     a.alpha = L.dAlpha;
     a.partials = L.dBulkPartials;
     a.xcdRemap = L.xcdRemap;
+    a.wgBlock = L.dWgBlock;
     const bool forced = L.dAlpha || L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0;
     const bool tags = L.dTags != nullptr;
     const dim3 grid((unsigned)L.nBulkPartials);
@@ -842,6 +879,8 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     if (err == hipSuccess) err = hipMemsetAsync(gA, 0, cells * sizeof(double), L.stream);
     if (err == hipSuccess) err = hipMemsetAsync(gS, 0, std::max<uint32_t>(1, L.nSpecial) * sizeof(double), L.stream);
     if (err == hipSuccess) err = hipMemsetAsync(part, 0, nRecords * 8 * sizeof(double), L.stream);
+    // cells the odd steps skip (outermost envelope rows) keep zero moments
+    if (err == hipSuccess) err = hipMemsetAsync(mom, 0, momElems * (K + 1) * sizeof(double), L.stream);
     const bool saveStats = L.statsOn;
     L.statsOn = false;
     float msF = 0, msB = 0;
@@ -883,8 +922,8 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     };
     auto exchangeLam = [&](double* lam) {
         if (L.multi && L.nSegs)
-            hipLaunchKernelGGL((exchange_envelopes<double>), gridFor(L.segCells), dim3(256), 0, L.stream,
-                               (void*)lam, L.dGeom, L.dSegs, L.nSegs, L.segCells);
+            hipLaunchKernelGGL((exchange_envelopes<double>), dim3(L.nTasks), dim3(256), 0, L.stream, (void*)lam,
+                               L.dGeom, L.dSegs, L.dTasks);
     };
     uint64_t rec = 0;
     double* lamNext = lamA;

@@ -73,6 +73,13 @@ __device__ __forceinline__ bool shift(int v, int d, int n, int periodic, int& ou
     return true;
 }
 
+// A row on the outermost layer of a block's envelope along x or y (not a
+// domain edge: those blocks have no envelope on that side).
+__device__ __forceinline__ bool outer_envelope_row(const BlockGeom& g, int x, int y) {
+    return (g.bx0 > 0 && x == 0) || (g.bx1 < g.nx - 1 && x == g.nx - 1) ||
+           (g.by0 > 0 && y == 0) || (g.by1 < g.ny - 1 && y == g.ny - 1);
+}
+
 __device__ __forceinline__ bool in_bulk(const BlockGeom& g, int x, int y, int z) {
     return x >= g.bx0 && x <= g.bx1 && y >= g.by0 && y <= g.by1 && z >= g.bz0 && z <= g.bz1;
 }
@@ -167,7 +174,14 @@ struct StepArgs {
     double* moments;         // MOMENTS: rhoBar, jx, jy, jz of the pre-collision state,
     uint64_t momStride;      //          [4][momStride] by cell (all blocks)
     int xcdRemap;            // 1: consecutive work items on the same XCD
+    const uint16_t* wgBlock; // block of every work item (multi-block), or null (one block)
 };
+
+// Block of a work item: one table load (the binary search over the block table
+// costs a chain of dependent loads at the start of every workgroup).
+__device__ __forceinline__ int block_of(const StepArgs& a, uint32_t wg) {
+    return a.wgBlock ? (int)a.wgBlock[wg] : 0;
+}
 
 // The dispatcher hands workgroup b to XCD b % 8. With the remap, XCD k takes
 // the k-th contiguous eighth of the work items (the z-chunks of a row and the
@@ -237,7 +251,7 @@ template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool
 __global__ void __launch_bounds__(1024) bulk_step(StepArgs a) {
     using T = typename Store<S>::T;
     const uint32_t wg = a.xcdRemap ? xcd_item(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int b = find_block(a.blocks, a.nBlocks, wg);
+    const int b = block_of(a, wg);
     const BlockGeom g = a.blocks[b];
     S* __restrict__ f = reinterpret_cast<S*>(a.arena) + g.offset;
     const uint32_t local = wg - g.wgBegin;
@@ -372,7 +386,7 @@ __global__ void __launch_bounds__(1024) bulk_odd_staged(StepArgs a) {
     uint8_t* skipped = ldsRaw + (size_t)kZMoving * P * sizeof(L);   // TAGS only: W flags
 
     const uint32_t wg = a.xcdRemap ? xcd_item(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int b = find_block(a.blocks, a.nBlocks, wg);
+    const int b = block_of(a, wg);
     const BlockGeom g = a.blocks[b];
     S* __restrict__ f = reinterpret_cast<S*>(a.arena) + g.offset;
     const uint32_t local = wg - g.wgBegin;
@@ -407,8 +421,10 @@ __global__ void __launch_bounds__(1024) bulk_odd_staged(StepArgs a) {
     T fc[19];
 
     if (!rowsValid) {
-        // general path: per-lane neighbour validity in x, y and z
-        if (compute) {
+        // general path: per-lane neighbour validity in x, y and z. Rows in the
+        // outermost envelope layer of a block are skipped: nothing they would write
+        // reaches the bulk, and the envelope exchange after this step replaces them.
+        if (compute && !outer_envelope_row(g, x, y)) {
             int zm = 0, zp = 0;
             const bool vzm = shift(z, -1, g.nz, g.pz, zm), vzp = shift(z, 1, g.nz, g.pz, zp);
             const int zs[3] = {zm, z, zp};
@@ -602,7 +618,7 @@ __global__ void __launch_bounds__(1024) bulk_pairs(StepArgs a) {
     uint8_t* skipped = ldsRaw + (size_t)kZMoving * P * 4;   // ODD && TAGS: 2W flags
 
     const uint32_t wg = a.xcdRemap ? xcd_item(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int b = find_block(a.blocks, a.nBlocks, wg);
+    const int b = block_of(a, wg);
     const BlockGeom g = a.blocks[b];
     S* __restrict__ f = reinterpret_cast<S*>(a.arena) + g.offset;
     const uint32_t local = wg - g.wgBegin;
@@ -666,8 +682,12 @@ __global__ void __launch_bounds__(1024) bulk_pairs(StepArgs a) {
         const bool vx[3] = {vxm, true, vxp};
         const bool vy[3] = {vym, true, vyp};
         if (!(vxm && vxp && vym && vyp)) {
+            if (outer_envelope_row(g, x, y)) {
+                // outermost envelope layer: see bulk_odd_staged
+            } else {
             if (compA) odd_cell_general<S, DYN, STATS, FORCED, MOMENTS>(a, g, f, zA, cellA, tagA, inBulkA, xs, ys, vx, vy, st);
             if (compB) odd_cell_general<S, DYN, STATS, FORCED, MOMENTS>(a, g, f, zB, cellB, tagB, inBulkB, xs, ys, vx, vy, st);
+            }
         } else {
             S* rows[3][3];
 #pragma unroll
@@ -976,27 +996,33 @@ struct CopySegment {
     uint64_t cellBegin;           // first cell of the segment in the launch
 };
 
+// One workgroup per task: 256 consecutive cells of one segment.
+struct CopyTask {
+    uint32_t seg;
+    uint32_t first;
+};
+
 template<typename S>
 __global__ void __launch_bounds__(256) exchange_envelopes(void* arenaV, const BlockGeom* blocks,
-                                                          const CopySegment* segs, int nSegs, uint64_t total) {
+                                                          const CopySegment* segs, const CopyTask* tasks) {
     S* arena = reinterpret_cast<S*>(arenaV);
-    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < total;
-         k += (uint64_t)gridDim.x * blockDim.x) {
-        int lo = 0, hi = nSegs - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) / 2;
-            if (segs[mid].cellBegin <= k) lo = mid; else hi = mid - 1;
-        }
-        const CopySegment s = segs[lo];
-        const uint64_t r = k - s.cellBegin;
+    const CopyTask task = tasks[blockIdx.x];
+    const CopySegment s = segs[task.seg];
+    const uint64_t r = (uint64_t)task.first + threadIdx.x;
+    if (r < (uint64_t)s.lx * s.ly * s.lz) {
         const int lz = (int)(r % (uint64_t)s.lz), ly = (int)((r / s.lz) % (uint64_t)s.ly);
         const int lx = (int)(r / ((uint64_t)s.lz * s.ly));
         const BlockGeom& gs = blocks[s.src];
         const BlockGeom& gd = blocks[s.dst];
         const S* src = arena + gs.offset + lin(gs, s.sx + lx, s.sy + ly, s.sz + lz);
         S* dst = arena + gd.offset + lin(gd, s.dx + lx, s.dy + ly, s.dz + lz);
+        // plain (cached) accesses: the source cells were just written by the step
+        // kernel and the next step reads the envelope back
+        S v[19];
 #pragma unroll
-        for (int i = 0; i < 19; ++i) dst[(uint64_t)i * gd.slot] = src[(uint64_t)i * gs.slot];
+        for (int i = 0; i < 19; ++i) v[i] = src[(uint64_t)i * gs.slot];
+#pragma unroll
+        for (int i = 0; i < 19; ++i) dst[(uint64_t)i * gd.slot] = v[i];
     }
 }
 
