@@ -74,8 +74,8 @@ struct AdjointArgs {
     const double* mom;        // moments of f^t: rhoBar, jx, jy, jz, [4][momStride] by cellOffset + cell
     uint64_t momStride;
     const double* planeF;     // f^t of the special (plane) records, 19 per record
-    const double* lamNext;    // lambda^{t+1} of this block (slot stride g.slot)
-    double* lam;              // lambda^t of this block
+    const void* lamNext;      // lambda^{t+1} of this block (slot stride g.slot), FP64 or FP32
+    void* lam;                // lambda^t of this block
     const double* alpha;      // porosity by cellOffset + cell, or null
     double* gAlpha;           // gradient accumulator by cellOffset + cell
     const uint8_t* tags;      // BulkTag by cellOffset + cell (BT_SKIP: plane cells), or null
@@ -88,8 +88,39 @@ struct AdjointArgs {
     double inW, outW;         // cs2 / |plane| of the inlet and outlet measuring planes
     int xin, xout;            // global x of the measuring planes
     int explicitTerm;         // 1 when j(f^t) is part of J (t > t0)
-    double* partials;         // per workgroup: J contribution (slot 0)
+    // every block in one launch: block b takes workgroups [wgBegin[b], wgBegin[b+1])
+    // of 64 (z) x 4 (y) threads over its (ceil(nz/64), ceil(ny/4), nx) tiles
+    const BlockGeom* blocks;
+    const uint16_t* wgBlock;  // block of every workgroup
+    const uint32_t* wgBegin;  // first workgroup of every block
+    double* acc;              // per workgroup: J accumulated over the reverse sweep
 };
+
+// Cell of this thread (one cell per thread) and the per-workgroup J
+// accumulation (fixed order: each workgroup adds its own record once per step).
+struct AdjointCell {
+    BlockGeom g;
+    int x, y, z;
+};
+__device__ __forceinline__ AdjointCell adjoint_cell_of(const AdjointArgs& a) {
+    AdjointCell c;
+    const uint32_t wg = blockIdx.x;
+    const int b = (int)a.wgBlock[wg];
+    c.g = a.blocks[b];
+    const uint32_t local = wg - a.wgBegin[b];
+    const uint32_t gz = (uint32_t)(c.g.nz + 63) / 64, gy = (uint32_t)(c.g.ny + 3) / 4;
+    c.z = (int)(local % gz) * 64 + (int)threadIdx.x;
+    const uint32_t rest = local / gz;
+    c.y = (int)(rest % gy) * 4 + (int)threadIdx.y;
+    c.x = (int)(rest / gy);
+    return c;
+}
+__device__ __forceinline__ void accumulate_J(double J, double* acc) {
+    __shared__ double total[8];
+    double r[8] = {J, 0, 0, 0, 0, 0, 0, 0};
+    block_reduce_store(r, total);
+    if (threadIdx.x == 0 && threadIdx.y == 0) acc[blockIdx.x] += total[0];
+}
 
 // dj/df_k(y) for the objective (plane-mean pressure difference, drag, mean flux).
 __device__ __forceinline__ void objective_gradient(const AdjointArgs& a, int gx, double alphaY, double djdf[19]) {
@@ -98,38 +129,39 @@ __device__ __forceinline__ void objective_gradient(const AdjointArgs& a, int gx,
 }
 
 // mu(y) gathered from lambda^{t+1} (transpose of streaming with edge reflection).
-__device__ __forceinline__ void gather_mu(const AdjointArgs& a, int x, int y, int z, uint64_t cell, double mu[19]) {
-    const BlockGeom& g = a.g;
-    mu[0] = a.lamNext[cell];
+template<typename LT>
+__device__ __forceinline__ void gather_mu(const BlockGeom& g, const LT* lamNext, int x, int y, int z, uint64_t cell,
+                                          double mu[19]) {
+    mu[0] = (double)lamNext[cell];
 #pragma unroll
     for (int i = 1; i < 19; ++i) {
         int dx, dy, dz;
         const bool ok = shift(x, cx(i), g.nx, g.px, dx) && shift(y, cy(i), g.ny, g.py, dy) &&
                         shift(z, cz(i), g.nz, g.pz, dz);
-        mu[i] = ok ? a.lamNext[(uint64_t)i * g.slot + lin(g, dx, dy, dz)]
-                   : a.lamNext[(uint64_t)opp(i) * g.slot + cell];
+        mu[i] = (double)(ok ? lamNext[(uint64_t)i * g.slot + lin(g, dx, dy, dz)]
+                            : lamNext[(uint64_t)opp(i) * g.slot + cell]);
     }
 }
 
-// Launch shape of the cell-wise adjoint kernels: z along threadIdx.x (64),
-// four y rows per workgroup, one x plane per blockIdx.z.
-__device__ __forceinline__ uint64_t adjoint_wg() {
-    return ((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+// c_x of a direction known only at run time, from bit masks (a lookup into a
+// local table would live in scratch memory).
+__device__ __forceinline__ int cx_of(uint32_t i) {
+    constexpr uint32_t neg = (1u << 1) | (1u << 4) | (1u << 5) | (1u << 6) | (1u << 7);
+    constexpr uint32_t pos = (1u << 10) | (1u << 13) | (1u << 14) | (1u << 15) | (1u << 16);
+    return (int)((pos >> i) & 1u) - (int)((neg >> i) & 1u);
 }
 
-// Reverse step for bulk (non-plane) cells; plane cells are left to adjoint_planes.
-template<int DYN, bool FORCED>
-__global__ void __launch_bounds__(256) adjoint_bulk(AdjointArgs a) {
-    const BlockGeom g = a.g;
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int x = blockIdx.z;
-    double J = 0.0;
+// Reverse step of one bulk (non-plane) cell; plane cells are left to adjoint_planes.
+template<typename LT, int DYN, bool FORCED>
+__device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGeom& g, int x, int y, int z,
+                                             double& J) {
     const uint64_t cell = ((uint64_t)x * g.ny + y) * g.nz + z;
     const uint64_t idx = g.cellOffset + cell;
-    if (z < g.nz && y < g.ny && !(a.tags && a.tags[idx] == BT_SKIP)) {
+    const LT* lamNext = reinterpret_cast<const LT*>(a.lamNext) + g.offset;
+    LT* lam = reinterpret_cast<LT*>(a.lam) + g.offset;
+    if (!(a.tags && a.tags[idx] == BT_SKIP)) {
         double mu[19];
-        gather_mu(a, x, y, z, cell, mu);
+        gather_mu<LT>(g, lamNext, x, y, z, cell, mu);
         const double al = a.alpha ? a.alpha[idx] : 0.0;
         const double dm = 1.0 - al;
         const double rhoBar = a.mom[idx];
@@ -200,7 +232,7 @@ __global__ void __launch_bounds__(256) adjoint_bulk(AdjointArgs a) {
                 val += (1 - w) * cMc;
             }
             if (a.explicitTerm) val += a.invT * djdf[k];
-            a.lam[(uint64_t)k * g.slot + cell] = val;
+            lam[(uint64_t)k * g.slot + cell] = (LT)val;
         }
         if (in_bulk(g, x, y, z)) {
             const double wB = DYN == 0 ? w : 1.0;
@@ -216,8 +248,15 @@ __global__ void __launch_bounds__(256) adjoint_bulk(AdjointArgs a) {
             if (a.gAlpha) a.gAlpha[idx] += ga;
         }
     }
-    double r[8] = {J, 0, 0, 0, 0, 0, 0, 0};
-    block_reduce_store(r, a.partials + 8 * adjoint_wg());
+}
+
+// Reverse step of every bulk cell of every block (64 x 4 cells per workgroup).
+template<typename LT, int DYN, bool FORCED>
+__global__ void __launch_bounds__(256) adjoint_bulk(AdjointArgs a) {
+    const AdjointCell c = adjoint_cell_of(a);
+    double J = 0.0;
+    if (c.z < c.g.nz && c.y < c.g.ny) adjoint_cell<LT, DYN, FORCED>(a, c.g, c.x, c.y, c.z, J);
+    accumulate_J(J, a.acc);
 }
 
 struct AdjointSpecialArgs {
@@ -225,13 +264,14 @@ struct AdjointSpecialArgs {
     const SpecialCell* cells;
     uint32_t count;
     const double* values;
-    double* partials;           // per workgroup: slot 0 J
+    double* partials;           // per workgroup: J accumulated over the reverse sweep
     double* gScale;             // per special cell: d/ds contribution (summed per plane on the host)
 };
 
 // Reverse step for plane cells: the collision Jacobian by dual numbers. One
 // thread per (record, direction): direction k < 19 seeds df_k and yields
 // lambda_k = mu . dC/df_k; direction 19 seeds the plane's velocity scale.
+template<typename LT>
 __global__ void __launch_bounds__(256) adjoint_planes(AdjointSpecialArgs s) {
     const AdjointArgs& a = s.base;
     const BlockGeom g = a.g;
@@ -244,15 +284,10 @@ __global__ void __launch_bounds__(256) adjoint_planes(AdjointSpecialArgs s) {
         if (dirn < 19 || velocity) {
             const int x = sc.x, y = sc.y, z = sc.z;
             const uint64_t cell = sc.cell;
-            double mu[19];
-            gather_mu(a, x, y, z, cell, mu);
+            const double* f0 = a.planeF + 19 * (uint64_t)k;
             Dual fd[19];
-            double rhoBar0 = 0;
-            for (int i = 0; i < 19; ++i) {
-                const double fi = a.planeF[19 * (uint64_t)k + i];
-                fd[i] = Dual(fi, (uint32_t)i == dirn ? 1.0 : 0.0);
-                rhoBar0 += fi;
-            }
+#pragma unroll
+            for (int i = 0; i < 19; ++i) fd[i] = Dual(f0[i], (uint32_t)i == dirn ? 1.0 : 0.0);
             const double* v0 = s.values + sc.value;
             const double vals[3] = {v0[0], velocity ? v0[1] : 0.0, velocity ? v0[2] : 0.0};
             // the velocity-scale direction: d(s u)/ds = u
@@ -262,18 +297,31 @@ __global__ void __launch_bounds__(256) adjoint_planes(AdjointSpecialArgs s) {
             Dual rhoBar, drag[3];
             const Dual F0[3] = {Dual(0.0), Dual(0.0), Dual(0.0)};
             collide_bulk<Dual, 0, false>(fd, Dual(a.omega), F0, Dual(0.0), rhoBar, drag);
-            double dot = 0;
-            for (int i = 0; i < 19; ++i) dot += mu[i] * fd[i].d;
+            // mu . d f' / d(direction), mu gathered from lambda^{t+1} as it is consumed
+            const LT* lamNext = reinterpret_cast<const LT*>(a.lamNext);
+            double dot = (double)lamNext[cell] * fd[0].d;
+#pragma unroll
+            for (int i = 1; i < 19; ++i) {
+                int dx, dy, dz;
+                const bool ok = shift(x, cx(i), g.nx, g.px, dx) && shift(y, cy(i), g.ny, g.py, dy) &&
+                                shift(z, cz(i), g.nz, g.pz, dz);
+                const double m = (double)(ok ? lamNext[(uint64_t)i * g.slot + lin(g, dx, dy, dz)]
+                                             : lamNext[(uint64_t)opp(i) * g.slot + cell]);
+                dot += m * fd[i].d;
+            }
             const int gx = g.gx0 + x;
             const double p = (gx == a.xin ? a.inW : 0.0) - (gx == a.xout ? a.outW : 0.0);
             if (dirn < 19) {
-                double djdf[19];
-                objective_gradient(a, gx, 0.0, djdf);
-                if (a.explicitTerm) dot += a.invT * djdf[dirn];
-                a.lam[(uint64_t)dirn * g.slot + cell] = dot;
+                // dj/df_dirn of a plane cell (no porosity there), without a local array
+                if (a.explicitTerm) dot += a.invT * (a.wdp * p + a.wflux * (double)cx_of(dirn));
+                reinterpret_cast<LT*>(a.lam)[(uint64_t)dirn * g.slot + cell] = (LT)dot;
                 if (dirn == 0 && a.explicitTerm) {
-                    double jx0 = 0;
-                    for (int i = 0; i < 19; ++i) jx0 += cx(i) * a.planeF[19 * (uint64_t)k + i];
+                    double rhoBar0 = 0, jx0 = 0;
+#pragma unroll
+                    for (int i = 0; i < 19; ++i) {
+                        rhoBar0 += f0[i];
+                        jx0 += cx(i) * f0[i];
+                    }
                     J += a.invT * (a.wdp * p * (1.0 + rhoBar0) + a.wflux * jx0);
                 }
             } else {
@@ -281,21 +329,21 @@ __global__ void __launch_bounds__(256) adjoint_planes(AdjointSpecialArgs s) {
             }
         }
     }
-    double r[8] = {J, 0, 0, 0, 0, 0, 0, 0};
-    block_reduce_store(r, s.partials + 8 * (uint64_t)blockIdx.x);
+    accumulate_J(J, s.partials);
 }
 
 // lambda^T = dj/df(f^T) / (T - t0); J += j(f^T) / (T - t0);
 // g_alpha += w_drag omega j_x(f^T) / (T - t0).
+template<typename LT>
 __global__ void __launch_bounds__(256) adjoint_final(AdjointArgs a) {
-    const BlockGeom g = a.g;
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int x = blockIdx.z;
+    const AdjointCell c = adjoint_cell_of(a);
+    const BlockGeom& g = c.g;
+    const int x = c.x, y = c.y, z = c.z;
     double J = 0.0;
-    const uint64_t cell = ((uint64_t)x * g.ny + y) * g.nz + z;
-    const uint64_t idx = g.cellOffset + cell;
     if (z < g.nz && y < g.ny) {
+        LT* lam = reinterpret_cast<LT*>(a.lam) + g.offset;
+        const uint64_t cell = ((uint64_t)x * g.ny + y) * g.nz + z;
+        const uint64_t idx = g.cellOffset + cell;
         const bool plane = a.tags && a.tags[idx] == BT_SKIP;
         const double al = (a.alpha && !plane) ? a.alpha[idx] : 0.0;
         const double rhoBar = a.mom[idx];
@@ -303,15 +351,26 @@ __global__ void __launch_bounds__(256) adjoint_final(AdjointArgs a) {
         const int gx = g.gx0 + x;
         double djdf[19];
         objective_gradient(a, gx, al, djdf);
-        for (int k = 0; k < 19; ++k) a.lam[(uint64_t)k * g.slot + cell] = a.invT * djdf[k];
+#pragma unroll
+        for (int k = 0; k < 19; ++k) lam[(uint64_t)k * g.slot + cell] = (LT)(a.invT * djdf[k]);
         if (in_bulk(g, x, y, z)) {
             if (a.gAlpha && !plane) a.gAlpha[idx] += a.invT * a.wdrag * a.omega * jx;
             const double p = (gx == a.xin ? a.inW : 0.0) - (gx == a.xout ? a.outW : 0.0);
             J += a.invT * (a.wdp * p * (1.0 + rhoBar) + a.wdrag * a.omega * al * jx + a.wflux * jx);
         }
     }
-    double r[8] = {J, 0, 0, 0, 0, 0, 0, 0};
-    block_reduce_store(r, a.partials + 8 * adjoint_wg());
+    accumulate_J(J, a.acc);
+}
+
+// Sum of the per-workgroup accumulators of the bulk and plane kernels (one
+// workgroup, fixed order).
+__global__ void __launch_bounds__(256) sum_values(const double* a, uint64_t na, const double* b, uint64_t nb,
+                                                  double* out) {
+    double v = 0.0;
+    for (uint64_t i = threadIdx.x; i < na; i += blockDim.x) v += a[i];
+    for (uint64_t i = threadIdx.x; i < nb; i += blockDim.x) v += b[i];
+    double r[8] = {v, 0, 0, 0, 0, 0, 0, 0};
+    block_reduce_store(r, out);
 }
 
 }  // namespace plbamd

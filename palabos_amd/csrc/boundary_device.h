@@ -128,7 +128,7 @@ __device__ __forceinline__ void complete_plane_dir(T* f, int family, int bckind,
 }
 
 template<typename T, typename V = double>
-__device__ void complete_plane(T* f, uint32_t info, const V* v) {
+__device__ __forceinline__ void complete_plane(T* f, uint32_t info, const V* v) {
     const int family = (info >> 2) & 1, bckind = (info >> 3) & 1, dir = (info >> 4) & 3;
     const bool pos = (info >> 6) & 1;
     switch (dir * 2 + (pos ? 1 : 0)) {
@@ -167,7 +167,7 @@ __device__ __forceinline__ void plane_macroscopic(const T* f, int bckind, const 
 }
 
 template<typename T>
-__device__ void plane_macroscopic_any(const T* f, uint32_t info, const double* v, T& rhoBar, T u[3]) {
+__device__ __forceinline__ void plane_macroscopic_any(const T* f, uint32_t info, const double* v, T& rhoBar, T u[3]) {
     const int bckind = (info >> 3) & 1, dir = (info >> 4) & 3;
     const bool pos = (info >> 6) & 1;
     switch (dir * 2 + (pos ? 1 : 0)) {

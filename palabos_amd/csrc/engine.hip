@@ -172,7 +172,9 @@ int setup(Lattice& L) {
     int txMax = storageSize(d.storage) >= 8 ? 256 : 512, cpt = 1;
     L.xcdRemap = 1;
     if (const char* e = getenv("PLBAMD_XCD_REMAP")) L.xcdRemap = atoi(e) != 0;
-    if (const char* e = getenv("PLBAMD_BULK_TX")) txMax = std::max(64, std::min(1024, atoi(e) / 64 * 64));
+    // at most 512 threads: the step kernels are compiled for 512-thread workgroups
+    // (a 1024-thread bound caps them at 128 registers and the heavier variants spill)
+    if (const char* e = getenv("PLBAMD_BULK_TX")) txMax = std::max(64, std::min(512, atoi(e) / 64 * 64));
     if (const char* e = getenv("PLBAMD_BULK_CPT")) cpt = std::max(1, std::min(16, atoi(e)));
     int maxNz = 0;
     L.geom.clear();
@@ -841,36 +843,49 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     const uint64_t cells = L.allocCells;
     const uint64_t momElems = 4 * cells;                       // per recorded step
     const uint64_t pfElems = 19 * (uint64_t)std::max<uint32_t>(1, L.nSpecial);
+    // reverse kernels: one launch over every block, 64 (z) x 4 (y) cells per workgroup
     const dim3 ablock(64, 4);
-    std::vector<dim3> agrid(L.geom.size());
-    uint64_t cellWGs = 0;
+    std::vector<uint32_t> adjBegin(L.geom.size() + 1, 0);
     for (size_t b = 0; b < L.geom.size(); ++b) {
         const BlockGeom& g = L.geom[b];
-        agrid[b] = dim3((unsigned)((g.nz + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)g.nx);
-        cellWGs += (uint64_t)agrid[b].x * agrid[b].y * agrid[b].z;
+        adjBegin[b + 1] = adjBegin[b] + (uint32_t)(((g.nz + 63) / 64) * ((g.ny + 3) / 4) * g.nx);
     }
+    const uint32_t adjWGs = adjBegin.back();
+    std::vector<uint16_t> adjBlock(adjWGs);
+    for (size_t b = 0; b < L.geom.size(); ++b)
+        std::fill(adjBlock.begin() + adjBegin[b], adjBlock.begin() + adjBegin[b + 1], (uint16_t)b);
+    const dim3 agrid(adjWGs);
     const uint32_t planeWGs = (uint32_t)((20 * (uint64_t)L.nSpecial + 255) / 256);
-    const uint64_t recordsPerStep = cellWGs + planeWGs;
     // device memory: checkpoints, a working state, the recorded moments and plane
     // populations of one segment, two adjoint states, gradients, partial sums
-    char *ck = nullptr, *work = nullptr;
-    double *mom = nullptr, *pf = nullptr, *lamA = nullptr, *lamB = nullptr, *gA = nullptr, *gS = nullptr,
-           *part = nullptr, *sum = nullptr;
+    // the adjoint state is FP64 for an FP64 forward, FP32 for a 16-bit one
+    const bool lam64 = st == PLBAMD_F64;
+    const size_t lamSize = lam64 ? sizeof(double) : sizeof(float);
+    char *ck = nullptr, *work = nullptr, *lamA = nullptr, *lamB = nullptr;
+    double *mom = nullptr, *pf = nullptr, *gA = nullptr, *gS = nullptr, *acc = nullptr, *sum = nullptr;
+    uint16_t* dAdjBlock = nullptr;
+    uint32_t* dAdjBegin = nullptr;
     auto release = [&]() {
         hipFree(ck); hipFree(work); hipFree(mom); hipFree(pf); hipFree(lamA); hipFree(lamB); hipFree(gA);
-        hipFree(gS); hipFree(part); hipFree(sum);
+        hipFree(gS); hipFree(acc); hipFree(sum); hipFree(dAdjBlock); hipFree(dAdjBegin);
     };
-    const uint64_t nRecords = recordsPerStep * (uint64_t)(T + 1);
+    const uint64_t nAcc = (uint64_t)adjWGs + planeWGs;   // J accumulators: bulk workgroups, then plane ones
     hipError_t err = hipMalloc(&ck, stateBytes * nCk);
     if (err == hipSuccess) err = hipMalloc(&work, stateBytes);
     if (err == hipSuccess) err = hipMalloc(&mom, momElems * (K + 1) * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&pf, pfElems * (K + 1) * sizeof(double));
-    if (err == hipSuccess) err = hipMalloc(&lamA, arenaElems * sizeof(double));
-    if (err == hipSuccess) err = hipMalloc(&lamB, arenaElems * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc(&lamA, arenaElems * lamSize);
+    if (err == hipSuccess) err = hipMalloc(&lamB, arenaElems * lamSize);
     if (err == hipSuccess) err = hipMalloc(&gA, cells * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&gS, std::max<uint32_t>(1, L.nSpecial) * sizeof(double));
-    if (err == hipSuccess) err = hipMalloc(&part, nRecords * 8 * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc(&acc, nAcc * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&sum, 8 * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc(&dAdjBlock, adjBlock.size() * sizeof(uint16_t));
+    if (err == hipSuccess) err = hipMalloc(&dAdjBegin, adjBegin.size() * sizeof(uint32_t));
+    if (err == hipSuccess)
+        err = hipMemcpy(dAdjBlock, adjBlock.data(), adjBlock.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemcpy(dAdjBegin, adjBegin.data(), adjBegin.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (err != hipSuccess) {
         (void)hipGetLastError();
         release();
@@ -878,7 +893,7 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     }
     if (err == hipSuccess) err = hipMemsetAsync(gA, 0, cells * sizeof(double), L.stream);
     if (err == hipSuccess) err = hipMemsetAsync(gS, 0, std::max<uint32_t>(1, L.nSpecial) * sizeof(double), L.stream);
-    if (err == hipSuccess) err = hipMemsetAsync(part, 0, nRecords * 8 * sizeof(double), L.stream);
+    if (err == hipSuccess) err = hipMemsetAsync(acc, 0, nAcc * sizeof(double), L.stream);
     // cells the odd steps skip (outermost envelope rows) keep zero moments
     if (err == hipSuccess) err = hipMemsetAsync(mom, 0, momElems * (K + 1) * sizeof(double), L.stream);
     const bool saveStats = L.statsOn;
@@ -910,24 +925,45 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     a.outW = a.inW;
     a.xin = obj.inlet_x;
     a.xout = obj.outlet_x;
+    a.blocks = L.dGeom;
+    a.wgBlock = dAdjBlock;
+    a.wgBegin = dAdjBegin;
+    a.acc = acc;
     const bool rbgk = L.desc.dynamics == PLBAMD_RBGK;
+#define PLBAMD_ADJ(LT) \
+    do { \
+        if (rbgk) { \
+            if (forced) hipLaunchKernelGGL((adjoint_bulk<LT, 1, true>), gr, ablock, 0, L.stream, args); \
+            else hipLaunchKernelGGL((adjoint_bulk<LT, 1, false>), gr, ablock, 0, L.stream, args); \
+        } else { \
+            if (forced) hipLaunchKernelGGL((adjoint_bulk<LT, 0, true>), gr, ablock, 0, L.stream, args); \
+            else hipLaunchKernelGGL((adjoint_bulk<LT, 0, false>), gr, ablock, 0, L.stream, args); \
+        } \
+    } while (0)
     auto reverseKernel = [&](const AdjointArgs& args, const dim3& gr) {
-        if (rbgk) {
-            if (forced) hipLaunchKernelGGL((adjoint_bulk<1, true>), gr, ablock, 0, L.stream, args);
-            else hipLaunchKernelGGL((adjoint_bulk<1, false>), gr, ablock, 0, L.stream, args);
-        } else {
-            if (forced) hipLaunchKernelGGL((adjoint_bulk<0, true>), gr, ablock, 0, L.stream, args);
-            else hipLaunchKernelGGL((adjoint_bulk<0, false>), gr, ablock, 0, L.stream, args);
-        }
+        if (lam64) PLBAMD_ADJ(double);
+        else PLBAMD_ADJ(float);
     };
-    auto exchangeLam = [&](double* lam) {
-        if (L.multi && L.nSegs)
+#undef PLBAMD_ADJ
+    auto exchangeLam = [&](char* lam) {
+        if (!L.multi || !L.nSegs) return;
+        if (lam64)
             hipLaunchKernelGGL((exchange_envelopes<double>), dim3(L.nTasks), dim3(256), 0, L.stream, (void*)lam,
                                L.dGeom, L.dSegs, L.dTasks);
+        else
+            hipLaunchKernelGGL((exchange_envelopes<float>), dim3(L.nTasks), dim3(256), 0, L.stream, (void*)lam,
+                               L.dGeom, L.dSegs, L.dTasks);
     };
-    uint64_t rec = 0;
-    double* lamNext = lamA;
-    double* lamCur = lamB;
+    // the plane cells' reverse step runs on a second stream next to the bulk one
+    hipStream_t side = nullptr;
+    hipEvent_t evIn = nullptr, evOut = nullptr;
+    if (err == hipSuccess && L.nSpecial) {
+        err = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&evIn, hipEventDisableTiming);
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&evOut, hipEventDisableTiming);
+    }
+    char* lamNext = lamA;
+    char* lamCur = lamB;
     void* const arena = L.arena;
     const int64_t time0 = L.time;
     L.arena = work;   // the recompute runs the step kernels on the working state
@@ -947,14 +983,10 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
         if (err == hipSuccess && c == nCk - 1) {
             a.mom = mom + (uint64_t)len * momElems;
             a.explicitTerm = 1;
-            for (size_t b = 0; b < L.geom.size() && err == hipSuccess; ++b) {
-                a.g = L.geom[b];
-                a.lam = lamNext + L.geom[b].offset;
-                a.partials = part + rec * 8;
-                hipLaunchKernelGGL(adjoint_final, agrid[b], ablock, 0, L.stream, a);
-                rec += (uint64_t)agrid[b].x * agrid[b].y * agrid[b].z;
-                err = hipGetLastError();
-            }
+            a.lam = lamNext;
+            if (lam64) hipLaunchKernelGGL(adjoint_final<double>, agrid, ablock, 0, L.stream, a);
+            else hipLaunchKernelGGL(adjoint_final<float>, agrid, ablock, 0, L.stream, a);
+            err = hipGetLastError();
             exchangeLam(lamNext);
         }
         for (int m = len - 1; m >= 0 && err == hipSuccess; --m) {
@@ -962,30 +994,33 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
             a.mom = mom + (uint64_t)m * momElems;
             a.planeF = pf + (uint64_t)m * pfElems;
             a.explicitTerm = t > obj.first_step ? 1 : 0;
-            for (size_t b = 0; b < L.geom.size() && err == hipSuccess; ++b) {
-                a.g = L.geom[b];
-                a.lamNext = lamNext + L.geom[b].offset;
-                a.lam = lamCur + L.geom[b].offset;
-                a.partials = part + rec * 8;
-                reverseKernel(a, agrid[b]);
-                rec += (uint64_t)agrid[b].x * agrid[b].y * agrid[b].z;
-                err = hipGetLastError();
-            }
+            a.lamNext = lamNext;
+            a.lam = lamCur;
             if (err == hipSuccess && L.nSpecial) {
+                // plane cells (single block) on the side stream, after the previous step
                 AdjointSpecialArgs sa{};
-                a.g = L.geom[0];
-                a.lamNext = lamNext;
-                a.lam = lamCur;
-                sa.base = a;
+                AdjointArgs pa = a;
+                pa.g = L.geom[0];
+                sa.base = pa;
                 sa.cells = L.dSpecial;
                 sa.count = L.nSpecial;
                 sa.values = L.dValues;
-                sa.partials = part + rec * 8;
+                sa.partials = acc + adjWGs;
                 sa.gScale = gS;
-                hipLaunchKernelGGL(adjoint_planes, dim3(planeWGs), dim3(256), 0, L.stream, sa);
-                rec += planeWGs;
+                err = hipEventRecord(evIn, L.stream);
+                if (err == hipSuccess) err = hipStreamWaitEvent(side, evIn, 0);
+                if (err == hipSuccess) {
+                    if (lam64) hipLaunchKernelGGL(adjoint_planes<double>, dim3(planeWGs), dim3(256), 0, side, sa);
+                    else hipLaunchKernelGGL(adjoint_planes<float>, dim3(planeWGs), dim3(256), 0, side, sa);
+                    err = hipGetLastError();
+                }
+                if (err == hipSuccess) err = hipEventRecord(evOut, side);
+            }
+            if (err == hipSuccess) {
+                reverseKernel(a, agrid);
                 err = hipGetLastError();
             }
+            if (err == hipSuccess && L.nSpecial) err = hipStreamWaitEvent(L.stream, evOut, 0);
             exchangeLam(lamCur);
             std::swap(lamNext, lamCur);
         }
@@ -993,11 +1028,18 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     L.arena = arena;
     L.time = time0;
     if (err == hipSuccess) err = hipEventRecord(L.ev1, L.stream);
+    if (side) {
+        (void)hipStreamSynchronize(side);
+        (void)hipStreamDestroy(side);
+    }
+    if (evIn) (void)hipEventDestroy(evIn);
+    if (evOut) (void)hipEventDestroy(evOut);
     if (err == hipSuccess) err = hipEventSynchronize(L.ev1);
     if (err == hipSuccess) err = hipEventElapsedTime(&msB, L.ev0, L.ev1);
     L.statsOn = saveStats;
     if (err == hipSuccess) {
-        hipLaunchKernelGGL(sum_records, dim3(1), dim3(256), 0, L.stream, part, (uint32_t)rec, sum);
+        hipLaunchKernelGGL(sum_values, dim3(1), dim3(256), 0, L.stream, acc, (uint64_t)adjWGs, acc + adjWGs,
+                           (uint64_t)planeWGs, sum);
         err = hipGetLastError();
     }
     double hsum[8] = {0};

@@ -248,7 +248,7 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, const BlockGeom&
 }
 
 template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
-__global__ void __launch_bounds__(1024) bulk_step(StepArgs a) {
+__global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
     using T = typename Store<S>::T;
     const uint32_t wg = a.xcdRemap ? xcd_item(blockIdx.x, gridDim.x) : blockIdx.x;
     const int b = block_of(a, wg);
@@ -374,7 +374,7 @@ __host__ __device__ constexpr int zmove_index(int i) {
 constexpr int kZMoving = 10;
 
 template<typename S, int DYN, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
-__global__ void __launch_bounds__(1024) bulk_odd_staged(StepArgs a) {
+__global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
     using T = typename Store<S>::T;
     using R = typename Store<S>::R;
     // LDS words of at least 32 bits: 16-bit values would put two lanes on one bank word
@@ -608,7 +608,7 @@ __device__ __forceinline__ void odd_cell_general(const StepArgs& a, const BlockG
 }
 
 template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
-__global__ void __launch_bounds__(1024) bulk_pairs(StepArgs a) {
+__global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
     static_assert(sizeof(typename Store<S>::R) == 2, "pairs of 16-bit values");
     using T = typename Store<S>::T;
     extern __shared__ __align__(16) unsigned char ldsRaw[];
