@@ -128,7 +128,7 @@ def adjoint_ratio(repeats=3):
 
 def cpu_baseline():
     ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
-    n, steps = 128, 60
+    n, steps = 128, 200   # about 10-15 s of serial CPU work
     if os.path.exists(ref):
         out = subprocess.run([ref, "--bench", "double", str(n), str(steps)], capture_output=True,
                              text=True, timeout=600, check=True).stdout
