@@ -43,6 +43,7 @@ int fail(int code, const std::string& msg) {
 // Envelope width of multi-block lattices: the AA pattern needs two layers
 // refreshed every second step (kernels.h).
 constexpr int kEnvelope = 2;
+constexpr int kEvenThreads = 256;   // threads per workgroup of the linear even step
 
 struct Plane {
     int family, kind, dir, ori;
@@ -58,6 +59,10 @@ struct Lattice {
     std::vector<BlockGeom> geom;      // device geometry of each block (host copy)
     BlockGeom* dGeom = nullptr;
     uint16_t* dWgBlock = nullptr;   // block of every bulk work item (multi-block)
+    uint32_t* dWalls = nullptr;     // link-wall mask per allocated cell (cells next to link walls)
+    bool linearEven = false;        // even steps through bulk_even_linear
+    uint64_t nEven = 0;             // its workgroups
+    uint16_t* dEvenBlock = nullptr; // its block of every workgroup (multi-block)
     CopyTask* dTasks = nullptr;     // envelope exchange: one workgroup per task
     uint32_t nTasks = 0;
     uint64_t allocCells = 0;          // cells of all blocks (per-cell arrays)
@@ -103,7 +108,7 @@ struct Lattice {
     ~Lattice() {
         hipFree(arena); hipFree(dGeom); hipFree(dTags); hipFree(dSpecial); hipFree(dValues); hipFree(dAlpha);
         hipFree(dBulkPartials); hipFree(dSpecialPartials); hipFree(dStats); hipFree(dSegs);
-        hipFree(dWgBlock); hipFree(dTasks);
+        hipFree(dWgBlock); hipFree(dTasks); hipFree(dWalls); hipFree(dEvenBlock);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
         if (stream) hipStreamDestroy(stream);
@@ -245,8 +250,8 @@ int setup(Lattice& L) {
     L.zChunk = tx * (pairs ? 2 : cpt);
     L.staged = staged;
     L.pairs = pairs;
-    L.pairsLds = (size_t)kZMoving * (tx + 2) * 4 + 2 * (size_t)tx;
-    L.stagedLds = (size_t)kZMoving * (tx + 2) * std::max<size_t>(4, storageSize(d.storage)) + (size_t)tx;
+    L.pairsLds = (size_t)kZMoving * (tx + 2) * 4 + 2 * (size_t)tx * sizeof(uint32_t);
+    L.stagedLds = (size_t)kZMoving * (tx + 2) * std::max<size_t>(4, storageSize(d.storage)) + (size_t)tx * sizeof(uint32_t);
     if (L.stagedLds > 65536) L.staged = false;   // stay within the default LDS limit per workgroup
     uint64_t offset = 0, cellOffset = 0;
     uint64_t wg = 0;
@@ -266,6 +271,22 @@ int setup(Lattice& L) {
     }
     L.allocCells = cellOffset;
     L.nBulkPartials = wg;
+    // linear even-step decomposition: kEvenThreads threads, 1 or 2 cells each.
+    // Used for 16-bit storage (pair accesses) and for multi-block lattices (short
+    // rows); one-block FP64/FP32 lattices keep the row kernel, which measured
+    // faster there (TG 512^3 FP64: 17.9 vs 17.4 GLUPS).
+    L.linearEven = storageSize(d.storage) == 2 ? pairs : L.multi;
+    if (const char* e = getenv("PLBAMD_LINEAR_EVEN")) L.linearEven = L.linearEven && atoi(e) != 0;
+    {
+        const uint64_t perWG = (uint64_t)kEvenThreads * (storageSize(d.storage) == 2 ? 2 : 1);
+        uint64_t ew = 0;
+        for (BlockGeom& g : L.geom) {
+            g.evenBegin = (uint32_t)ew;
+            ew += ((uint64_t)g.nx * g.ny * g.nz + perWG - 1) / perWG;
+        }
+        if (ew >= (1ull << 31)) L.linearEven = false;
+        L.nEven = ew;
+    }
     L.elemSize = storageSize(d.storage);
     const size_t bytes = offset * L.elemSize;
     if (hipMalloc(&L.arena, bytes) != hipSuccess) {
@@ -284,7 +305,16 @@ int setup(Lattice& L) {
         PLBAMD_HIP(hipMalloc(&L.dWgBlock, wgBlock.size() * sizeof(uint16_t)));
         PLBAMD_HIP(hipMemcpy(L.dWgBlock, wgBlock.data(), wgBlock.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     }
-    PLBAMD_HIP(hipMalloc(&L.dBulkPartials, L.nBulkPartials * 8 * sizeof(double)));
+    PLBAMD_HIP(hipMalloc(&L.dBulkPartials, std::max(L.nBulkPartials, L.nEven) * 16 * sizeof(double)));
+    if (L.multi && L.linearEven) {
+        std::vector<uint16_t> eb(L.nEven);
+        for (size_t b = 0; b < L.geom.size(); ++b) {
+            const uint64_t end = b + 1 < L.geom.size() ? L.geom[b + 1].evenBegin : L.nEven;
+            std::fill(eb.begin() + L.geom[b].evenBegin, eb.begin() + end, (uint16_t)b);
+        }
+        PLBAMD_HIP(hipMalloc(&L.dEvenBlock, eb.size() * sizeof(uint16_t)));
+        PLBAMD_HIP(hipMemcpy(L.dEvenBlock, eb.data(), eb.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    }
     PLBAMD_HIP(hipMalloc(&L.dStats, 16 * sizeof(double)));
     PLBAMD_HIP(hipMemset(L.dStats, 0, 16 * sizeof(double)));
 
@@ -395,6 +425,7 @@ int rebuild(Lattice& L) {
         values.insert(values.end(), L.planes[p].values.begin(), L.planes[p].values.end());
     }
     std::vector<uint8_t> dev(L.allocCells, BT_BULK);
+    std::vector<uint32_t> walls;   // allocated on the first cell next to a link wall
     std::vector<double> alpha;
     if (!L.hostAlpha.empty()) alpha.assign(L.allocCells, 0.0);
     std::vector<SpecialCell> special;
@@ -455,9 +486,11 @@ int rebuild(Lattice& L) {
                         dev[slot] = (t == PLBAMD_TAG_BOUNCE_BACK) ? BT_BOUNCE_BACK : BT_NO_DYNAMICS;
                         anyNonBulk = true;
                     } else if (wall) {
-                        s.info = (uint16_t)SK_NEAR_WALL | bulkBit;
-                        dev[slot] = BT_SKIP;
-                        special.push_back(s);
+                        // background dynamics with the link mask, inside the bulk kernels
+                        if (walls.empty()) walls.assign(L.allocCells, 0u);
+                        walls[slot] = wall;
+                        dev[slot] = BT_WALL_NEAR;
+                        anyNonBulk = true;
                     }
                 }
     }
@@ -466,10 +499,15 @@ int rebuild(Lattice& L) {
     hipFree(L.dValues); L.dValues = nullptr;
     hipFree(L.dSpecialPartials); L.dSpecialPartials = nullptr;
     hipFree(L.dAlpha); L.dAlpha = nullptr;
+    hipFree(L.dWalls); L.dWalls = nullptr;
     L.nSpecial = (uint32_t)special.size();
     if (anyNonBulk) {
         PLBAMD_HIP(hipMalloc(&L.dTags, L.allocCells));
         PLBAMD_HIP(hipMemcpy(L.dTags, dev.data(), L.allocCells, hipMemcpyHostToDevice));
+    }
+    if (!walls.empty()) {
+        PLBAMD_HIP(hipMalloc(&L.dWalls, walls.size() * sizeof(uint32_t)));
+        PLBAMD_HIP(hipMemcpy(L.dWalls, walls.data(), walls.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     if (!special.empty()) {
         PLBAMD_HIP(hipMalloc(&L.dSpecial, special.size() * sizeof(SpecialCell)));
@@ -521,11 +559,36 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
     a.partials = L.dBulkPartials;
     a.xcdRemap = L.xcdRemap;
     a.wgBlock = L.dWgBlock;
+    a.walls = L.dWalls;
     const bool forced = L.dAlpha || L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0;
     const bool tags = L.dTags != nullptr;
     const dim3 grid((unsigned)L.nBulkPartials);
-    if (record) {
-        if (stats) return fail(PLBAMD_EINVAL, "recording step: statistics must be off");
+    if (record && stats) return fail(PLBAMD_EINVAL, "recording step: statistics must be off");
+    if (!odd && L.linearEven) {
+        // even steps in memory order (bulk_even_linear)
+        a.evenBlock = L.dEvenBlock;
+        const dim3 eg((unsigned)L.nEven), eb(kEvenThreads);
+        if (record) {
+            a.moments = record->moments;
+            a.momStride = record->momStride;
+            if constexpr (std::is_same<S, double>::value || sizeof(typename Store<S>::R) == 2) {
+#define PLBAMD_EVR(TG, FO) hipLaunchKernelGGL((bulk_even_linear<S, DYN, TG, false, FO, true>), eg, eb, 0, L.stream, a)
+                if (tags) { if (forced) PLBAMD_EVR(true, true); else PLBAMD_EVR(true, false); }
+                else { if (forced) PLBAMD_EVR(false, true); else PLBAMD_EVR(false, false); }
+#undef PLBAMD_EVR
+            } else {
+                return fail(PLBAMD_EINVAL, "recording step: FP64 or 16-bit storage");
+            }
+        } else {
+#define PLBAMD_EV(TG, ST, FO) hipLaunchKernelGGL((bulk_even_linear<S, DYN, TG, ST, FO>), eg, eb, 0, L.stream, a)
+#define PLBAMD_EV_F(TG, ST) if (forced) PLBAMD_EV(TG, ST, true); else PLBAMD_EV(TG, ST, false)
+#define PLBAMD_EV_S(TG) if (stats) { PLBAMD_EV_F(TG, true); } else { PLBAMD_EV_F(TG, false); }
+            if (tags) { PLBAMD_EV_S(true) } else { PLBAMD_EV_S(false) }
+#undef PLBAMD_EV_S
+#undef PLBAMD_EV_F
+#undef PLBAMD_EV
+        }
+    } else if (record) {
         a.moments = record->moments;
         a.momStride = record->momStride;
         if constexpr (std::is_same<S, double>::value) {
@@ -615,7 +678,8 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
         PLBAMD_HIP(hipGetLastError());
     }
     if (stats) {
-        hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(256), 0, L.stream, L.dBulkPartials, L.nBulkPartials,
+        hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(256), 0, L.stream, L.dBulkPartials,
+                           (!odd && L.linearEven) ? L.nEven : L.nBulkPartials,
                            L.dSpecialPartials, L.nSpecial ? L.nSpecialPartials : 0, L.dStats);
         PLBAMD_HIP(hipGetLastError());
     }
@@ -663,6 +727,7 @@ bool boxArgs(const Lattice& L, size_t bi, const plbamd_box& g, BoxArgs& a) {
     a.sly = g.y1 - g.y0 + 1; a.slz = g.z1 - g.z0 + 1;
     a.special = L.dSpecial;
     a.nSpecial = L.nSpecial;
+    a.walls = L.dWalls;
     return true;
 }
 

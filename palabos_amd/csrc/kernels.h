@@ -42,7 +42,9 @@
 
 namespace plbamd {
 
-enum BulkTag : uint8_t { BT_BULK = 0, BT_BOUNCE_BACK = 1, BT_NO_DYNAMICS = 2, BT_SKIP = 3 };
+// BT_WALL_NEAR: background dynamics next to a link-wise wall; the per-cell link
+// mask (bit i: neighbour x + c_i is a wall) selects the reflected populations.
+enum BulkTag : uint8_t { BT_BULK = 0, BT_BOUNCE_BACK = 1, BT_NO_DYNAMICS = 2, BT_SKIP = 3, BT_WALL_NEAR = 4 };
 
 // One block of a lattice. Coordinates are block-local (0..n-1 over the
 // allocated cells); the bulk (cells that own their values and take
@@ -58,7 +60,7 @@ struct BlockGeom {
     uint64_t slot;            // elements per slot
     uint64_t cellOffset;      // offset of the block's cells in per-cell arrays
     uint32_t wgBegin;         // first workgroup of the block in the bulk launch
-    uint32_t pad;
+    uint32_t evenBegin;       // first workgroup of the block in the linear even-step launch
 };
 
 __device__ __forceinline__ uint64_t lin(const BlockGeom& g, int x, int y, int z) {
@@ -175,6 +177,8 @@ struct StepArgs {
     uint64_t momStride;      //          [4][momStride] by cell (all blocks)
     int xcdRemap;            // 1: consecutive work items on the same XCD
     const uint16_t* wgBlock; // block of every work item (multi-block), or null (one block)
+    const uint32_t* walls;   // link-wall mask per cell (all blocks), read for BT_WALL_NEAR cells
+    const uint16_t* evenBlock; // linear even-step launch: block of every workgroup, or null
 };
 
 // Block of a work item: one table load (the binary search over the block table
@@ -216,12 +220,13 @@ __device__ __forceinline__ void store_moments(const T* fc, double* m, uint64_t s
 // recording and statistics), bounce-back swap, or nothing (NoDynamics).
 template<typename S, int DYN, bool STATS, bool FORCED, bool MOMENTS>
 __device__ __forceinline__ void collide_cell(const StepArgs& a, const BlockGeom& g, uint64_t cell, uint8_t tag,
-                                             bool inBulk, typename Store<S>::T fc[19], double st[8]) {
+                                             bool inBulk, typename Store<S>::T fc[19], double st[8],
+                                             uint32_t wall = 0, double* wf = nullptr) {
     using T = typename Store<S>::T;
     if constexpr (MOMENTS) {
-        if (tag == BT_BULK) store_moments(fc, a.moments, a.momStride, g.cellOffset + cell);
+        if (tag == BT_BULK || tag == BT_WALL_NEAR) store_moments(fc, a.moments, a.momStride, g.cellOffset + cell);
     }
-    if (tag == BT_BULK) {
+    if (tag == BT_BULK || tag == BT_WALL_NEAR) {
         const T omega = (T)a.omega;
         const T F[3] = {(T)a.force[0], (T)a.force[1], (T)a.force[2]};
         T rhoBar;
@@ -239,11 +244,33 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, const BlockGeom&
                     st[5] += (double)drag[1];
                     st[6] += (double)drag[2];
                 }
+                // link-wise wall: each outgoing population towards the wall carries
+                // 2 c_i f*_i (the stored, storage-rounded value is what is reflected)
+                if (wall) {
+#pragma unroll
+                    for (int i = 1; i < 19; ++i) {
+                        if (!((wall >> i) & 1u)) continue;
+                        const T fo = Store<S>::round(fc[i]);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) wf[c] += (double)((T)2 * ci(i, c) * fo);
+                    }
+                }
             }
         }
     } else if (tag == BT_BOUNCE_BACK) {
 #pragma unroll
         for (int i = 1; i <= 9; ++i) { T t = fc[i]; fc[i] = fc[i + 9]; fc[i + 9] = t; }
+    }
+}
+
+// Statistics of a bulk workgroup: two records, [rhoBar uSqr maxUSqr count Px Py Pz -]
+// and the link-wall force [Fx Fy - Fz - - - -].
+template<bool STATS>
+__device__ __forceinline__ void store_bulk_stats(double st[8], const double wf[3], double* partials, uint32_t wg) {
+    if constexpr (STATS) {
+        block_reduce_store(st, partials + 16 * (uint64_t)wg);
+        double w[8] = {wf[0], wf[1], 0, wf[2], 0, 0, 0, 0};
+        block_reduce_store(w, partials + 16 * (uint64_t)wg + 8);
     }
 }
 
@@ -263,6 +290,7 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
     const int zBegin = chunk * a.zChunk;
     const int zEnd = min(g.nz, zBegin + a.zChunk);
     double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double wf[3] = {0, 0, 0};
 
     if (y < g.ny) {
         int xm = 0, xp = 0, ym = 0, yp = 0;
@@ -285,9 +313,11 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
         for (int z = zBegin + (int)threadIdx.x; z < zEnd; z += blockDim.x) {
             const uint64_t cell = rowBase + z;
             uint8_t tag = BT_BULK;
+            uint32_t wall = 0;
             if constexpr (TAGS) {
                 tag = a.tags[g.cellOffset + cell];
                 if (tag == BT_SKIP) continue;
+                if (tag == BT_WALL_NEAR) wall = a.walls[g.cellOffset + cell];
             }
             int zm = 0, zp = 0;
             const bool vzm = shift(z, -1, g.nz, g.pz, zm), vzp = shift(z, 1, g.nz, g.pz, zp);
@@ -296,7 +326,7 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
             bool fast = true;
             if constexpr (ODD) {
                 const uint64_t active = __ballot(1);
-                fast = rowsValid && (__ballot(vzm && vzp) == active);
+                fast = rowsValid && (__ballot(vzm && vzp && wall == 0) == active);
             }
 
             T fc[19];
@@ -314,7 +344,7 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
 #pragma unroll
                 for (int i = 1; i < 19; ++i) {
                     const int ax = 1 - cx(i), ay = 1 - cy(i), az = 1 - cz(i);
-                    const bool ok = vx[ax] && vy[ay] && vz[az];
+                    const bool ok = vx[ax] && vy[ay] && vz[az] && !((wall >> opp(i)) & 1u);
                     const uint64_t src = ok ? (uint64_t)opp(i) * g.slot +
                                                   (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
                                             : (uint64_t)i * g.slot + cell;
@@ -323,7 +353,7 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
             }
 
             collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, rowInBulkXY && z >= g.bz0 && z <= g.bz1,
-                                                       fc, st);
+                                                       fc, st, wall, wf);
 
             if constexpr (!ODD) {
 #pragma unroll
@@ -339,7 +369,7 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
 #pragma unroll
                 for (int i = 1; i < 19; ++i) {
                     const int ax = 1 + cx(i), ay = 1 + cy(i), az = 1 + cz(i);
-                    const bool ok = vx[ax] && vy[ay] && vz[az];
+                    const bool ok = vx[ax] && vy[ay] && vz[az] && !((wall >> i) & 1u);
                     const uint64_t dst = ok ? (uint64_t)i * g.slot +
                                                   (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
                                             : (uint64_t)opp(i) * g.slot + cell;
@@ -348,7 +378,84 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
             }
         }
     }
-    if constexpr (STATS) block_reduce_store(st, a.partials + 8 * (uint64_t)wg);
+    store_bulk_stats<STATS>(st, wf, a.partials, wg);
+}
+
+// ---------------------------------------------------------------------------
+// Even step over a block's cells in memory order. The even step is local (each
+// cell reads and writes only its own slots), so a workgroup takes a run of
+// consecutive cells regardless of rows: every lane is busy even when the
+// block's rows are short. 16-bit storage moves two cells per thread with
+// 32-bit pair accesses (every block's cell count is even then).
+
+template<typename S, int DYN, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
+__global__ void __launch_bounds__(512) bulk_even_linear(StepArgs a) {
+    using T = typename Store<S>::T;
+    constexpr int CPT = sizeof(typename Store<S>::R) == 2 ? 2 : 1;
+    const uint32_t wg = blockIdx.x;
+    const int b = a.evenBlock ? (int)a.evenBlock[wg] : 0;
+    const BlockGeom g = a.blocks[b];
+    S* __restrict__ f = reinterpret_cast<S*>(a.arena) + g.offset;
+    const uint64_t cells = (uint64_t)g.nx * g.ny * g.nz;
+    const uint64_t c0 = ((uint64_t)(wg - g.evenBegin) * blockDim.x + threadIdx.x) * CPT;
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double wf[3] = {0, 0, 0};
+    if (c0 < cells) {
+        T fc[CPT][19];
+        if constexpr (CPT == 2) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                const uint32_t v = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(f + (uint64_t)i * g.slot + c0));
+                fc[0][i] = Store<S>::from_raw((uint16_t)(v & 0xffffu));
+                fc[1][i] = Store<S>::from_raw((uint16_t)(v >> 16));
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) fc[0][i] = Store<S>::load(f + (uint64_t)i * g.slot + c0);
+        }
+        bool comp[CPT];
+#pragma unroll
+        for (int h = 0; h < CPT; ++h) {
+            const uint64_t cell = c0 + h;
+            uint8_t tag = BT_BULK;
+            uint32_t wall = 0;
+            if constexpr (TAGS) {
+                tag = a.tags[g.cellOffset + cell];
+                if constexpr (STATS) {
+                    if (tag == BT_WALL_NEAR) wall = a.walls[g.cellOffset + cell];
+                }
+            }
+            comp[h] = tag != BT_SKIP;
+            bool inBulk = false;
+            if constexpr (STATS) {
+                const uint32_t plane = (uint32_t)g.ny * (uint32_t)g.nz;
+                const uint32_t cc = (uint32_t)cell;
+                const int x = (int)(cc / plane), rem = (int)(cc % plane);
+                const int y = rem / g.nz, z = rem % g.nz;
+                inBulk = x >= g.bx0 && x <= g.bx1 && y >= g.by0 && y <= g.by1 && z >= g.bz0 && z <= g.bz1;
+            }
+            if (comp[h]) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, inBulk, fc[h], st, wall, wf);
+        }
+        if constexpr (CPT == 2) {
+            if (comp[0] && comp[1]) {
+#pragma unroll
+                for (int i = 0; i < 19; ++i) {
+                    const uint32_t v = (uint32_t)Store<S>::to_raw(fc[0][i]) | ((uint32_t)Store<S>::to_raw(fc[1][i]) << 16);
+                    __builtin_nontemporal_store(v, reinterpret_cast<uint32_t*>(f + (uint64_t)opp(i) * g.slot + c0));
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 19; ++i) {
+                    if (comp[0]) Store<S>::store(f + (uint64_t)opp(i) * g.slot + c0, fc[0][i]);
+                    if (comp[1]) Store<S>::store(f + (uint64_t)opp(i) * g.slot + c0 + 1, fc[1][i]);
+                }
+            }
+        } else if (comp[0]) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) Store<S>::store(f + (uint64_t)opp(i) * g.slot + c0, fc[0][i]);
+        }
+    }
+    store_bulk_stats<STATS>(st, wf, a.partials, wg);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,6 +471,39 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
 // shifted outputs back through LDS so that the global stores are aligned too.
 // The 9 other directions are aligned already and go straight to registers.
 // Rows at a non-periodic x/y edge of the block take the per-lane general path.
+
+// One cell of an odd step by per-lane neighbour tests (rows at block edges).
+template<typename S, int DYN, bool STATS, bool FORCED, bool MOMENTS>
+__device__ __forceinline__ void odd_cell_general(const StepArgs& a, const BlockGeom& g, S* f, int z, uint64_t cell,
+                                                 uint8_t tag, bool inBulk, const int xs[3], const int ys[3],
+                                                 const bool vx[3], const bool vy[3], double st[8], uint32_t wall,
+                                                 double wf[3]) {
+    using T = typename Store<S>::T;
+    int zm = 0, zp = 0;
+    const bool vzm = shift(z, -1, g.nz, g.pz, zm), vzp = shift(z, 1, g.nz, g.pz, zp);
+    const int zs[3] = {zm, z, zp};
+    const bool vz[3] = {vzm, true, vzp};
+    T fc[19];
+    fc[0] = Store<S>::load(f + cell);
+#pragma unroll
+    for (int i = 1; i < 19; ++i) {
+        const int ax = 1 - cx(i), ay = 1 - cy(i), az = 1 - cz(i);
+        const bool ok = vx[ax] && vy[ay] && vz[az] && !((wall >> opp(i)) & 1u);
+        const uint64_t src = ok ? (uint64_t)opp(i) * g.slot + (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
+                                : (uint64_t)i * g.slot + cell;
+        fc[i] = Store<S>::load(f + src);
+    }
+    collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, inBulk, fc, st, wall, wf);
+    Store<S>::store(f + cell, fc[0]);
+#pragma unroll
+    for (int i = 1; i < 19; ++i) {
+        const int ax = 1 + cx(i), ay = 1 + cy(i), az = 1 + cz(i);
+        const bool ok = vx[ax] && vy[ay] && vz[az] && !((wall >> i) & 1u);
+        const uint64_t dst = ok ? (uint64_t)i * g.slot + (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
+                                : (uint64_t)opp(i) * g.slot + cell;
+        Store<S>::store(f + dst, fc[i]);
+    }
+}
 
 // Index of direction i among the z-moving directions, or -1.
 __host__ __device__ constexpr int zmove_index(int i) {
@@ -383,7 +523,9 @@ __global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
     const int W = (int)blockDim.x;            // segment length: one cell per thread
     const int P = W + 2;                      // pitch: low halo, segment, high halo
     L* buf = reinterpret_cast<L*>(ldsRaw);
-    uint8_t* skipped = ldsRaw + (size_t)kZMoving * P * sizeof(L);   // TAGS only: W flags
+    // TAGS only: per cell of the segment, the directions whose output it does not
+    // send along c_i (its link-wall mask; all bits when it is not collided here)
+    uint32_t* blocked = reinterpret_cast<uint32_t*>(ldsRaw + (size_t)kZMoving * P * sizeof(L));
 
     const uint32_t wg = a.xcdRemap ? xcd_item(blockIdx.x, gridDim.x) : blockIdx.x;
     const int b = block_of(a, wg);
@@ -402,6 +544,7 @@ __global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
     const uint64_t rowBase = ((uint64_t)x * g.ny + y) * g.nz;
     const uint64_t cell = rowBase + z;
     double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double wf[3] = {0, 0, 0};
 
     int xm = 0, xp = 0, ym = 0, yp = 0;
     const bool vxm = shift(x, -1, g.nx, g.px, xm), vxp = shift(x, 1, g.nx, g.px, xp);
@@ -414,8 +557,10 @@ __global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
     const bool inBulk = x >= g.bx0 && x <= g.bx1 && y >= g.by0 && y <= g.by1 && z >= g.bz0 && z <= g.bz1;
 
     uint8_t tag = BT_BULK;
+    uint32_t wall = 0;
     if constexpr (TAGS) {
         if (active) tag = a.tags[g.cellOffset + cell];
+        if (active && tag == BT_WALL_NEAR) wall = a.walls[g.cellOffset + cell];
     }
     const bool compute = active && tag != BT_SKIP;
     T fc[19];
@@ -424,31 +569,8 @@ __global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
         // general path: per-lane neighbour validity in x, y and z. Rows in the
         // outermost envelope layer of a block are skipped: nothing they would write
         // reaches the bulk, and the envelope exchange after this step replaces them.
-        if (compute && !outer_envelope_row(g, x, y)) {
-            int zm = 0, zp = 0;
-            const bool vzm = shift(z, -1, g.nz, g.pz, zm), vzp = shift(z, 1, g.nz, g.pz, zp);
-            const int zs[3] = {zm, z, zp};
-            const bool vz[3] = {vzm, true, vzp};
-            fc[0] = Store<S>::load(f + cell);
-#pragma unroll
-            for (int i = 1; i < 19; ++i) {
-                const int ax = 1 - cx(i), ay = 1 - cy(i), az = 1 - cz(i);
-                const bool ok = vx[ax] && vy[ay] && vz[az];
-                const uint64_t src = ok ? (uint64_t)opp(i) * g.slot + (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
-                                        : (uint64_t)i * g.slot + cell;
-                fc[i] = Store<S>::load(f + src);
-            }
-            collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, inBulk, fc, st);
-            Store<S>::store(f + cell, fc[0]);
-#pragma unroll
-            for (int i = 1; i < 19; ++i) {
-                const int ax = 1 + cx(i), ay = 1 + cy(i), az = 1 + cz(i);
-                const bool ok = vx[ax] && vy[ay] && vz[az];
-                const uint64_t dst = ok ? (uint64_t)i * g.slot + (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
-                                        : (uint64_t)opp(i) * g.slot + cell;
-                Store<S>::store(f + dst, fc[i]);
-            }
-        }
+        if (compute && !outer_envelope_row(g, x, y))
+            odd_cell_general<S, DYN, STATS, FORCED, MOMENTS>(a, g, f, z, cell, tag, inBulk, xs, ys, vx, vy, st, wall, wf);
     } else {
         S* rows[3][3];
 #pragma unroll
@@ -469,8 +591,13 @@ __global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
             const S* src = rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot;
-            if (cz(i) == 0) fc[i] = Store<S>::load(src + zc);
-            else raw[zmove_index(i)] = Store<S>::load_raw(src + zc);
+            if (cz(i) == 0) {
+                // a link wall at x - c_i: f_i comes back from the cell's own slot i
+                const S* p = ((wall >> opp(i)) & 1u) ? f + (uint64_t)i * g.slot + cell : src + zc;
+                fc[i] = Store<S>::load(p);
+            } else {
+                raw[zmove_index(i)] = Store<S>::load_raw(src + zc);
+            }
         }
         R halo[kZMoving];
         const bool loLane = t == 0 && vlo, hiLane = t == W - 1 && vhi;
@@ -491,7 +618,7 @@ __global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
             for (int k = 0; k < kZMoving; ++k) buf[k * P + hp] = halo[k];
         }
         if constexpr (TAGS) {
-            if (active) skipped[t] = compute ? 0 : 1;
+            if (active) blocked[t] = compute ? wall : 0xffffffffu;
         }
         __syncthreads();
         if (compute) {
@@ -500,9 +627,9 @@ __global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
                 if (cz(i) != 0) {
                     const int k = zmove_index(i);
                     int zs = 0;
-                    const bool ok = shift(z, -cz(i), g.nz, g.pz, zs);
+                    const bool ok = shift(z, -cz(i), g.nz, g.pz, zs) && !((wall >> opp(i)) & 1u);
                     fc[i] = ok ? Store<S>::from_raw((R)buf[k * P + t + 1 - cz(i)])
-                               : Store<S>::load(f + (uint64_t)i * g.slot + cell);   // edge reflection
+                               : Store<S>::load(f + (uint64_t)i * g.slot + cell);   // edge or wall reflection
                 }
             }
         }
@@ -510,16 +637,19 @@ __global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
 
         // 2. collision, aligned outputs to global, z-moving outputs to LDS
         if (compute) {
-            collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, inBulk, fc, st);
+            collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, inBulk, fc, st, wall, wf);
 #pragma unroll
             for (int i = 0; i < 19; ++i) {
+                const bool reflect = (wall >> i) & 1u;   // link wall at x + c_i
                 if (cz(i) == 0) {
-                    Store<S>::store(rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot + z, fc[i]);
+                    S* p = reflect ? f + (uint64_t)opp(i) * g.slot + cell
+                                   : rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot + z;
+                    Store<S>::store(p, fc[i]);
                 } else {
                     const int k = zmove_index(i);
                     int zd = 0;
-                    if (shift(z, cz(i), g.nz, g.pz, zd)) buf[k * P + t + 1 + cz(i)] = Store<S>::to_raw(fc[i]);
-                    else Store<S>::store(f + (uint64_t)opp(i) * g.slot + cell, fc[i]);   // edge reflection
+                    if (shift(z, cz(i), g.nz, g.pz, zd) && !reflect) buf[k * P + t + 1 + cz(i)] = Store<S>::to_raw(fc[i]);
+                    else Store<S>::store(f + (uint64_t)opp(i) * g.slot + cell, fc[i]);   // edge or wall reflection
                 }
             }
         }
@@ -541,17 +671,17 @@ __global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
                 S* dst = rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot;
                 const int p = t - cz(i);
                 bool ok = active && p >= 0 && p < len;
-                if constexpr (TAGS) ok = ok && !skipped[p < 0 ? 0 : (p >= len ? 0 : p)];
+                if constexpr (TAGS) ok = ok && !((blocked[p < 0 ? 0 : (p >= len ? 0 : p)] >> i) & 1u);
                 if (ok) Store<S>::store_raw(dst + z, raw[k]);
                 // halo position: the low one takes cell 0's output for c_z < 0,
                 // the high one cell len-1's output for c_z > 0
                 bool h = cz(i) < 0 ? loLane : hiLane;
-                if constexpr (TAGS) h = h && !skipped[cz(i) < 0 ? 0 : len - 1];
+                if constexpr (TAGS) h = h && !((blocked[cz(i) < 0 ? 0 : len - 1] >> i) & 1u);
                 if (h) Store<S>::store_raw(dst + (cz(i) < 0 ? zlo : zhi), halo[k]);
             }
         }
     }
-    if constexpr (STATS) block_reduce_store(st, a.partials + 8 * (uint64_t)wg);
+    store_bulk_stats<STATS>(st, wf, a.partials, wg);
 }
 
 // ---------------------------------------------------------------------------
@@ -575,38 +705,6 @@ __device__ __forceinline__ uint16_t lo16(uint32_t v) { return (uint16_t)(v & 0xf
 __device__ __forceinline__ uint16_t hi16(uint32_t v) { return (uint16_t)(v >> 16); }
 __device__ __forceinline__ uint32_t pack16(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
 
-// One cell of an odd step by per-lane neighbour tests (rows at block edges).
-template<typename S, int DYN, bool STATS, bool FORCED, bool MOMENTS>
-__device__ __forceinline__ void odd_cell_general(const StepArgs& a, const BlockGeom& g, S* f, int z, uint64_t cell,
-                                                 uint8_t tag, bool inBulk, const int xs[3], const int ys[3],
-                                                 const bool vx[3], const bool vy[3], double st[8]) {
-    using T = typename Store<S>::T;
-    int zm = 0, zp = 0;
-    const bool vzm = shift(z, -1, g.nz, g.pz, zm), vzp = shift(z, 1, g.nz, g.pz, zp);
-    const int zs[3] = {zm, z, zp};
-    const bool vz[3] = {vzm, true, vzp};
-    T fc[19];
-    fc[0] = Store<S>::load(f + cell);
-#pragma unroll
-    for (int i = 1; i < 19; ++i) {
-        const int ax = 1 - cx(i), ay = 1 - cy(i), az = 1 - cz(i);
-        const bool ok = vx[ax] && vy[ay] && vz[az];
-        const uint64_t src = ok ? (uint64_t)opp(i) * g.slot + (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
-                                : (uint64_t)i * g.slot + cell;
-        fc[i] = Store<S>::load(f + src);
-    }
-    collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, inBulk, fc, st);
-    Store<S>::store(f + cell, fc[0]);
-#pragma unroll
-    for (int i = 1; i < 19; ++i) {
-        const int ax = 1 + cx(i), ay = 1 + cy(i), az = 1 + cz(i);
-        const bool ok = vx[ax] && vy[ay] && vz[az];
-        const uint64_t dst = ok ? (uint64_t)i * g.slot + (((uint64_t)xs[ax] * g.ny + ys[ay]) * g.nz + zs[az])
-                                : (uint64_t)opp(i) * g.slot + cell;
-        Store<S>::store(f + dst, fc[i]);
-    }
-}
-
 template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
 __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
     static_assert(sizeof(typename Store<S>::R) == 2, "pairs of 16-bit values");
@@ -615,7 +713,9 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
     const int W = (int)blockDim.x;            // pairs per segment
     const int P = W + 2;                      // pitch in pair words: low halo, segment, high halo
     uint32_t* buf = reinterpret_cast<uint32_t*>(ldsRaw);
-    uint8_t* skipped = ldsRaw + (size_t)kZMoving * P * 4;   // ODD && TAGS: 2W flags
+    // ODD && TAGS: per cell, the directions whose output it does not send along c_i
+    // (its link-wall mask; all bits when it is not collided here), 2W words
+    uint32_t* blocked = reinterpret_cast<uint32_t*>(ldsRaw + (size_t)kZMoving * P * 4);
 
     const uint32_t wg = a.xcdRemap ? xcd_item(blockIdx.x, gridDim.x) : blockIdx.x;
     const int b = block_of(a, wg);
@@ -638,12 +738,16 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
     const bool inBulkA = rowInBulk && zA >= g.bz0 && zA <= g.bz1;
     const bool inBulkB = rowInBulk && zB >= g.bz0 && zB <= g.bz1;
     double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double wf[3] = {0, 0, 0};
 
     uint8_t tagA = BT_BULK, tagB = BT_BULK;
+    uint32_t wallA = 0, wallB = 0;
     if constexpr (TAGS) {
         if (active) {
             tagA = a.tags[g.cellOffset + cellA];
             tagB = a.tags[g.cellOffset + cellB];
+            if (tagA == BT_WALL_NEAR) wallA = a.walls[g.cellOffset + cellA];
+            if (tagB == BT_WALL_NEAR) wallB = a.walls[g.cellOffset + cellB];
         }
     }
     const bool compA = active && tagA != BT_SKIP, compB = active && tagB != BT_SKIP;
@@ -660,8 +764,8 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                 fb[i] = Store<S>::from_raw(hi16(v));
             }
         }
-        if (compA) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellA, tagA, inBulkA, fa, st);
-        if (compB) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellB, tagB, inBulkB, fb, st);
+        if (compA) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
+        if (compB) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
         if (compA && compB) {
 #pragma unroll
             for (int i = 0; i < 19; ++i)
@@ -685,8 +789,12 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             if (outer_envelope_row(g, x, y)) {
                 // outermost envelope layer: see bulk_odd_staged
             } else {
-            if (compA) odd_cell_general<S, DYN, STATS, FORCED, MOMENTS>(a, g, f, zA, cellA, tagA, inBulkA, xs, ys, vx, vy, st);
-            if (compB) odd_cell_general<S, DYN, STATS, FORCED, MOMENTS>(a, g, f, zB, cellB, tagB, inBulkB, xs, ys, vx, vy, st);
+            if (compA)
+                odd_cell_general<S, DYN, STATS, FORCED, MOMENTS>(a, g, f, zA, cellA, tagA, inBulkA, xs, ys, vx, vy, st,
+                                                                 wallA, wf);
+            if (compB)
+                odd_cell_general<S, DYN, STATS, FORCED, MOMENTS>(a, g, f, zB, cellB, tagB, inBulkB, xs, ys, vx, vy, st,
+                                                                 wallB, wf);
             }
         } else {
             S* rows[3][3];
@@ -708,6 +816,9 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                     const uint32_t v = ld_pair(src + zc);
                     fa[i] = Store<S>::from_raw(lo16(v));
                     fb[i] = Store<S>::from_raw(hi16(v));
+                    // link walls at x - c_i: f_i comes back from the cell's own slot i
+                    if ((wallA >> opp(i)) & 1u) fa[i] = Store<S>::load(f + (uint64_t)i * g.slot + cellA);
+                    if ((wallB >> opp(i)) & 1u) fb[i] = Store<S>::load(f + (uint64_t)i * g.slot + cellB);
                 } else {
                     raw[zmove_index(i)] = ld_pair(src + zc);
                 }
@@ -732,8 +843,8 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             }
             if constexpr (TAGS) {
                 if (active) {
-                    skipped[2 * t] = compA ? 0 : 1;
-                    skipped[2 * t + 1] = compB ? 0 : 1;
+                    blocked[2 * t] = compA ? wallA : 0xffffffffu;
+                    blocked[2 * t + 1] = compB ? wallB : 0xffffffffu;
                 }
             }
             __syncthreads();
@@ -744,42 +855,49 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                 for (int i = 0; i < 19; ++i) {
                     if (cz(i) == 0) continue;
                     const uint32_t* row = buf + zmove_index(i) * P;
+                    const bool wA = (wallA >> opp(i)) & 1u, wB = (wallB >> opp(i)) & 1u;
                     if (cz(i) > 0) {
-                        fa[i] = vzA ? Store<S>::from_raw(hi16(row[t])) : Store<S>::load(f + (uint64_t)i * g.slot + cellA);
-                        fb[i] = Store<S>::from_raw(lo16(row[t + 1]));
+                        fa[i] = (vzA && !wA) ? Store<S>::from_raw(hi16(row[t])) : Store<S>::load(f + (uint64_t)i * g.slot + cellA);
+                        fb[i] = !wB ? Store<S>::from_raw(lo16(row[t + 1])) : Store<S>::load(f + (uint64_t)i * g.slot + cellB);
                     } else {
-                        fa[i] = Store<S>::from_raw(hi16(row[t + 1]));
-                        fb[i] = vzB ? Store<S>::from_raw(lo16(row[t + 2])) : Store<S>::load(f + (uint64_t)i * g.slot + cellB);
+                        fa[i] = !wA ? Store<S>::from_raw(hi16(row[t + 1])) : Store<S>::load(f + (uint64_t)i * g.slot + cellA);
+                        fb[i] = (vzB && !wB) ? Store<S>::from_raw(lo16(row[t + 2])) : Store<S>::load(f + (uint64_t)i * g.slot + cellB);
                     }
                 }
             }
             __syncthreads();   // every LDS input is read before the outputs reuse the buffer
-            if (compA) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellA, tagA, inBulkA, fa, st);
-            if (compB) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellB, tagB, inBulkB, fb, st);
+            if (compA) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
+            if (compB) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
             if (active) {
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
+                    // link walls at x + c_i: the output goes back to the cell's own slot opp(i)
+                    const bool rA = compA && ((wallA >> i) & 1u), rB = compB && ((wallB >> i) & 1u);
+                    if (rA) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellA, fa[i]);
+                    if (rB) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellB, fb[i]);
                     if (cz(i) == 0) {
+                        const bool sA = compA && !rA, sB = compB && !rB;
                         S* dst = rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot + zA;
-                        if (compA && compB) st_pair(dst, pack16(Store<S>::to_raw(fa[i]), Store<S>::to_raw(fb[i])));
+                        if (sA && sB) st_pair(dst, pack16(Store<S>::to_raw(fa[i]), Store<S>::to_raw(fb[i])));
                         else {
-                            if (compA) Store<S>::store(dst, fa[i]);
-                            if (compB) Store<S>::store(dst + 1, fb[i]);
+                            if (sA) Store<S>::store(dst, fa[i]);
+                            if (sB) Store<S>::store(dst + 1, fb[i]);
                         }
                     } else {
                         buf[zmove_index(i) * P + t + 1] = pack16(Store<S>::to_raw(fa[i]), Store<S>::to_raw(fb[i]));
                         // edge reflection where the destination leaves a non-periodic z edge
-                        if (cz(i) > 0 && !vzB && compB) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellB, fb[i]);
-                        if (cz(i) < 0 && !vzA && compA) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellA, fa[i]);
+                        if (cz(i) > 0 && !vzB && compB && !rB) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellB, fb[i]);
+                        if (cz(i) < 0 && !vzA && compA && !rA) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellA, fa[i]);
                     }
                 }
             }
             __syncthreads();
             // destination pair words: position (zA, zB) takes, for c_z = +1, the
             // outputs of cells zA - 1 and zA; for c_z = -1, of cells zB and zB + 1
-            auto computed = [&](int c) {   // segment-local cell c was collided here
+            // segment-local cell c was collided here and sent its direction-i output along c_i
+            auto computed = [&](int c, int i) {
                 if (c < 0 || c >= len) return false;
-                if constexpr (TAGS) return skipped[c] == 0;
+                if constexpr (TAGS) return ((blocked[c] >> i) & 1u) == 0;
                 return true;
             };
 #pragma unroll
@@ -792,14 +910,14 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                     bool okLo, okHi;
                     if (cz(i) > 0) {
                         lo = hi16(row[t]);
-                        okLo = computed(2 * t - 1);
+                        okLo = computed(2 * t - 1, i);
                         hi = lo16(row[t + 1]);
-                        okHi = computed(2 * t);
+                        okHi = computed(2 * t, i);
                     } else {
                         lo = hi16(row[t + 1]);
-                        okLo = computed(2 * t + 1);
+                        okLo = computed(2 * t + 1, i);
                         hi = lo16(row[t + 2]);
-                        okHi = computed(2 * t + 2);
+                        okHi = computed(2 * t + 2, i);
                     }
                     if (okLo && okHi) st_pair(dst + zA, pack16(lo, hi));
                     else {
@@ -811,17 +929,17 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                 // output to z0 + len; c_z = -1 sends the first cell's to z0 - 1
                 if (cz(i) > 0) {
                     int zd = 0;
-                    if (t == W - 1 && shift(z0 + len - 1, 1, g.nz, g.pz, zd) && computed(len - 1))
+                    if (t == W - 1 && shift(z0 + len - 1, 1, g.nz, g.pz, zd) && computed(len - 1, i))
                         Store<S>::store_raw(dst + zd, hi16(row[nPairs]));
                 } else {
                     int zd = 0;
-                    if (t == 0 && shift(z0, -1, g.nz, g.pz, zd) && computed(0))
+                    if (t == 0 && shift(z0, -1, g.nz, g.pz, zd) && computed(0, i))
                         Store<S>::store_raw(dst + zd, lo16(row[1]));
                 }
             }
         }
     }
-    if constexpr (STATS) block_reduce_store(st, a.partials + 8 * (uint64_t)wg);
+    store_bulk_stats<STATS>(st, wf, a.partials, wg);
 }
 
 // ---------------------------------------------------------------------------
@@ -963,10 +1081,12 @@ __global__ void __launch_bounds__(256) reduce_partials(const double* bulk, uint6
                                                        double* out) {
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double wf[3] = {0, 0, 0};
     for (uint64_t b = threadIdx.x; b < nBulk; b += blockDim.x) {
-        const double* p = bulk + 8 * b;
+        const double* p = bulk + 16 * b;
         v[0] += p[0]; v[1] += p[1]; v[2] = fmax(v[2], p[2]); v[3] += p[3];
         v[4] += p[4]; v[5] += p[5]; v[6] += p[6];
+        wf[0] += p[8]; wf[1] += p[9]; wf[2] += p[11];
     }
     double sp[3] = {0, 0, 0};
     for (uint64_t b = threadIdx.x; b < nSpecial; b += blockDim.x) {
@@ -975,7 +1095,7 @@ __global__ void __launch_bounds__(256) reduce_partials(const double* bulk, uint6
         s[4] += p[4]; s[5] += p[5]; s[6] += p[6];
         sp[0] += p[8]; sp[1] += p[9]; sp[2] += p[11];
     }
-    double r[8] = {v[0] + s[0], v[1] + s[1], fmax(v[2], s[2]), v[3] + s[3], s[4], s[5], s[6], 0};
+    double r[8] = {v[0] + s[0], v[1] + s[1], fmax(v[2], s[2]), v[3] + s[3], s[4] + wf[0], s[5] + wf[1], s[6] + wf[2], 0};
     block_reduce_store(r, out);
     double q[8] = {v[4] + sp[0], v[5] + sp[1], 0, v[6] + sp[2], 0, 0, 0, 0};
     block_reduce_store(q, out + 8);
@@ -1039,6 +1159,7 @@ struct BoxArgs {
     int sly, slz;                       // staging box extents along y, z
     const SpecialCell* special;         // sorted by (block, cell)
     uint32_t nSpecial;
+    const uint32_t* walls;              // link-wall mask per cell (all blocks), or null
 };
 
 __device__ __forceinline__ int find_special(const BoxArgs& b, uint64_t cell) {
@@ -1053,7 +1174,10 @@ __device__ __forceinline__ int find_special(const BoxArgs& b, uint64_t cell) {
     return -1;
 }
 
+// Link-wall mask of a cell: bulk cells next to a wall keep it in the per-cell
+// array, special cells (planes, momentum-exchange nodes) in their record.
 __device__ __forceinline__ uint32_t wall_mask_of(const BoxArgs& b, uint64_t cell) {
+    if (b.walls && b.walls[b.g.cellOffset + cell]) return b.walls[b.g.cellOffset + cell];
     const int s = find_special(b, cell);
     return s >= 0 ? b.special[s].wall : 0u;
 }
@@ -1184,7 +1308,9 @@ __global__ void __launch_bounds__(256) moments_box(const void* arena, BoxArgs b,
         const BoxCell c = box_cell(b, k);
         const uint8_t tag = m.tags ? m.tags[b.g.cellOffset + c.cell] : (uint8_t)BT_BULK;
         const int sp = (tag == BT_SKIP) ? find_special(b, c.cell) : -1;
-        const uint32_t wall = (b.parity && sp >= 0) ? b.special[sp].wall : 0u;
+        const uint32_t wall = !b.parity ? 0u
+                              : (tag == BT_WALL_NEAR && b.walls) ? b.walls[b.g.cellOffset + c.cell]
+                              : (sp >= 0 ? b.special[sp].wall : 0u);
         T rhoBar = T(), u[3] = {T(), T(), T()};
         const bool solid = tag == BT_BOUNCE_BACK || tag == BT_NO_DYNAMICS ||
                            (tag == BT_SKIP && (sp < 0 || (b.special[sp].info & 3u) == SK_MOMENTUM_EXCHANGE));

@@ -111,7 +111,10 @@ LAUNCH_VARIANTS = {
     "segments64": {"PLBAMD_BULK_TX": "64"},
     "no_xcd_remap": {"PLBAMD_XCD_REMAP": "0"},
     "no_pairs": {"PLBAMD_PAIRS": "0"},
+    "row_even_steps": {"PLBAMD_LINEAR_EVEN": "0"},
 }
+# (PLBAMD_LINEAR_EVEN=0 only switches the linear even step off; the multi-block
+# tests run it on every storage, the 16-bit single-block cases here by default)
 
 
 @pytest.mark.parametrize("storage", ["f64", "f16"])
