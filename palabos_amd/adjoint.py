@@ -24,6 +24,8 @@ class AdjointConfig:
     outlet_x: int
     weight_pressure_drop: float = 1.0
     weight_drag: float = 1.0
+    weight_flux: float = 0.0
+    first_step: int = 0
 
 
 class BrinkmanObjective(torch.autograd.Function):
@@ -40,7 +42,8 @@ class BrinkmanObjective(torch.autograd.Function):
             lat.setPopulations(init_populations)
             J, ga, gs, _ = lat.adjointGradient(config.nsteps, config.checkpoint_every, config.inlet_x,
                                                config.outlet_x, config.weight_pressure_drop,
-                                               config.weight_drag)
+                                               config.weight_drag, config.weight_flux,
+                                               config.first_step)
         finally:
             lat.close()
         # dJ/ds_total = dJ/d(scale of the plane) / s, because the plane values were

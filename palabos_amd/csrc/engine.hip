@@ -91,6 +91,8 @@ struct Lattice {
     std::vector<uint32_t> planeValueBase;   // first value of each plane in dValues
     bool staged = false;           // odd steps through bulk_odd_staged
     bool pairs = false;            // 16-bit storage through bulk_pairs
+    bool packed = false;           // bulk_pairs with whole rows per workgroup
+    int ldsPitch = 0;              // packed: LDS pair words per staged direction
     size_t pairsLds = 0;           // its dynamic LDS bytes (odd steps)
     size_t stagedLds = 0;          // its dynamic LDS bytes
     double* dSpecialPartials = nullptr;
@@ -245,12 +247,48 @@ int setup(Lattice& L) {
         }
         tx = (int)L.bulkBlock.x;
     }
+    // Pair kernel with whole rows per workgroup: when every block's row fits one
+    // workgroup of up to 256 threads, take the width that leaves the fewest idle
+    // threads with each workgroup filled by as many whole rows as fit; used when
+    // it idles fewer threads than the row segments (short rows of multi-block
+    // lattices). PLBAMD_PACKED=0/1 overrides.
+    bool packed = false;
+    int packedW = 0;
+    if (pairs && !getenv("PLBAMD_BULK_TX")) {
+        uint64_t segThreads = 0;
+        for (const BlockGeom& g : L.geom) segThreads += (uint64_t)g.nx * g.ny * ((g.nz / 2 + tx - 1) / tx) * tx;
+        uint64_t best = UINT64_MAX;
+        for (int w = 64; w <= 256; w += 64) {
+            uint64_t threads = 0;
+            bool fits = true;
+            for (const BlockGeom& g : L.geom) {
+                const int units = g.nz / 2;
+                // at least two pairs per row: the low and the high halo come from two lanes
+                if (units > w || units < 2) { fits = false; break; }
+                const uint64_t rows = (uint64_t)g.nx * g.ny, r = (uint64_t)(w / units);
+                threads += (rows + r - 1) / r * w;
+            }
+            if (fits && threads <= best) { best = threads; packedW = w; }
+        }
+        packed = packedW > 0 && best < segThreads;
+        if (const char* e = getenv("PLBAMD_PACKED")) packed = packedW > 0 && atoi(e) != 0;
+        if (packed) tx = packedW;
+    }
     const int ty = (staged || pairs) ? 1 : std::max(1, txMax / tx);
     L.bulkBlock = dim3(tx, ty);
     L.zChunk = tx * (pairs ? 2 : cpt);
     L.staged = staged;
     L.pairs = pairs;
-    L.pairsLds = (size_t)kZMoving * (tx + 2) * 4 + 2 * (size_t)tx * sizeof(uint32_t);
+    L.packed = packed;
+    L.ldsPitch = 0;
+    if (packed) {
+        for (BlockGeom& g : L.geom) {
+            const int units = g.nz / 2;
+            g.rowsPerWG = tx / units;
+            L.ldsPitch = std::max(L.ldsPitch, g.rowsPerWG * (units + 2));
+        }
+    }
+    L.pairsLds = (size_t)kZMoving * (packed ? L.ldsPitch : tx + 2) * 4 + 2 * (size_t)tx * sizeof(uint32_t);
     L.stagedLds = (size_t)kZMoving * (tx + 2) * std::max<size_t>(4, storageSize(d.storage)) + (size_t)tx * sizeof(uint32_t);
     if (L.stagedLds > 65536) L.staged = false;   // stay within the default LDS limit per workgroup
     uint64_t offset = 0, cellOffset = 0;
@@ -264,7 +302,10 @@ int setup(Lattice& L) {
         g.nzChunks = (g.nz + L.zChunk - 1) / L.zChunk;
         g.nyGroups = (g.ny + ty - 1) / ty;
         g.wgBegin = (uint32_t)wg;
-        wg += (uint64_t)g.nx * g.nyGroups * g.nzChunks;
+        if (L.packed)
+            wg += ((uint64_t)g.nx * g.ny + g.rowsPerWG - 1) / g.rowsPerWG;
+        else
+            wg += (uint64_t)g.nx * g.nyGroups * g.nzChunks;
         if (wg >= (1ull << 31)) return fail(PLBAMD_EINVAL, "lattice_create: lattice too large for one launch");
         offset += g.slot * 19;
         cellOffset += cells;
@@ -560,6 +601,7 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
     a.xcdRemap = L.xcdRemap;
     a.wgBlock = L.dWgBlock;
     a.walls = L.dWalls;
+    a.ldsPitch = L.ldsPitch;
     const bool forced = L.dAlpha || L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0;
     const bool tags = L.dTags != nullptr;
     const dim3 grid((unsigned)L.nBulkPartials);
@@ -608,8 +650,15 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
 #undef PLBAMD_REC
         } else if constexpr (sizeof(typename Store<S>::R) == 2) {
             if (!L.pairs) return fail(PLBAMD_EINVAL, "recording step: 16-bit storage needs every block even along z");
-#define PLBAMD_RECP(O, TG, FO) \
-    hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, false, FO, true>), grid, L.bulkBlock, O ? L.pairsLds : 0, L.stream, a)
+#define PLBAMD_RECP(O, TG, FO)                                                                                  \
+    do {                                                                                                        \
+        if (L.packed)                                                                                           \
+            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, false, FO, true, true>), grid, L.bulkBlock,           \
+                               O ? L.pairsLds : 0, L.stream, a);                                                \
+        else                                                                                                    \
+            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, false, FO, true>), grid, L.bulkBlock,                 \
+                               O ? L.pairsLds : 0, L.stream, a);                                                \
+    } while (0)
 #define PLBAMD_RECP_F(O, TG) if (forced) PLBAMD_RECP(O, TG, true); else PLBAMD_RECP(O, TG, false)
 #define PLBAMD_RECP_T(O) if (tags) { PLBAMD_RECP_F(O, true); } else { PLBAMD_RECP_F(O, false); }
             if (odd) { PLBAMD_RECP_T(true) } else { PLBAMD_RECP_T(false) }
@@ -621,8 +670,15 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
         }
     } else if (L.pairs) {
         if constexpr (sizeof(typename Store<S>::R) == 2) {
-#define PLBAMD_PAIRS(O, TG, ST, FO) \
-    hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO>), grid, L.bulkBlock, O ? L.pairsLds : 0, L.stream, a)
+#define PLBAMD_PAIRS(O, TG, ST, FO)                                                                             \
+    do {                                                                                                        \
+        if (L.packed)                                                                                           \
+            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO, false, true>), grid, L.bulkBlock,             \
+                               O ? L.pairsLds : 0, L.stream, a);                                                \
+        else                                                                                                    \
+            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO>), grid, L.bulkBlock, O ? L.pairsLds : 0,      \
+                               L.stream, a);                                                                    \
+    } while (0)
 #define PLBAMD_PAIRS_F(O, TG, ST) if (forced) PLBAMD_PAIRS(O, TG, ST, true); else PLBAMD_PAIRS(O, TG, ST, false)
 #define PLBAMD_PAIRS_S(O, TG) if (stats) { PLBAMD_PAIRS_F(O, TG, true); } else { PLBAMD_PAIRS_F(O, TG, false); }
 #define PLBAMD_PAIRS_T(O) if (tags) { PLBAMD_PAIRS_S(O, true) } else { PLBAMD_PAIRS_S(O, false) }

@@ -56,6 +56,7 @@ struct BlockGeom {
     int bx1, by1, bz1;        // bulk end, local, inclusive
     int gx0, gy0, gz0;        // global coordinate of local (0,0,0), before periodic wrap
     int nzChunks, nyGroups;   // bulk-launch decomposition: row segments, y groups
+    int rowsPerWG;            // packed-row launches: whole rows per workgroup
     uint64_t offset;          // element offset of slot 0 in the arena
     uint64_t slot;            // elements per slot
     uint64_t cellOffset;      // offset of the block's cells in per-cell arrays
@@ -179,6 +180,7 @@ struct StepArgs {
     const uint16_t* wgBlock; // block of every work item (multi-block), or null (one block)
     const uint32_t* walls;   // link-wall mask per cell (all blocks), read for BT_WALL_NEAR cells
     const uint16_t* evenBlock; // linear even-step launch: block of every workgroup, or null
+    int ldsPitch;            // packed-row launches: LDS words per staged direction
 };
 
 // Block of a work item: one table load (the binary search over the block table
@@ -599,21 +601,33 @@ __global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
                 raw[zmove_index(i)] = Store<S>::load_raw(src + zc);
             }
         }
+        // Halo elements. Inside a periodic or segmented row: the source elements
+        // beyond the segment ends. At a non-periodic row end: the end cell's own
+        // slot-i value for the directions arriving from beyond that end (c_z = +1
+        // at the low end, -1 at the high end), so that the edge reflection is read
+        // from LDS like any other input.
         R halo[kZMoving];
         const bool loLane = t == 0 && vlo, hiLane = t == W - 1 && vhi;
-        if (loLane || hiLane) {
-            const int zh = loLane ? zlo : zhi;
+        const bool loEdge = t == 0 && !vlo, hiEdge = t == W - 1 && !vhi;
+        const bool edge = loEdge || hiEdge;
+        if (loLane || hiLane || edge) {
+            // edge lanes load every z-moving direction's end cell; the shifted read
+            // takes only the directions arriving from beyond the end
+            const int zh = edge ? (loEdge ? z0 : z0 + len - 1) : (loLane ? zlo : zhi);
 #pragma unroll
             for (int i = 0; i < 19; ++i)
-                if (cz(i) != 0)
-                    halo[zmove_index(i)] = Store<S>::load_raw(rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot + zh);
+                if (cz(i) != 0) {
+                    const S* src = edge ? f + rowBase + (uint64_t)i * g.slot
+                                        : rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot;
+                    halo[zmove_index(i)] = Store<S>::load_raw(src + zh);
+                }
         }
         if (active) {
 #pragma unroll
             for (int k = 0; k < kZMoving; ++k) buf[k * P + t + 1] = raw[k];
         }
-        if (loLane || hiLane) {
-            const int hp = loLane ? 0 : len + 1;
+        if (loLane || hiLane || edge) {
+            const int hp = (loLane || loEdge) ? 0 : len + 1;
 #pragma unroll
             for (int k = 0; k < kZMoving; ++k) buf[k * P + hp] = halo[k];
         }
@@ -626,10 +640,10 @@ __global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
             for (int i = 0; i < 19; ++i) {
                 if (cz(i) != 0) {
                     const int k = zmove_index(i);
-                    int zs = 0;
-                    const bool ok = shift(z, -cz(i), g.nz, g.pz, zs) && !((wall >> opp(i)) & 1u);
-                    fc[i] = ok ? Store<S>::from_raw((R)buf[k * P + t + 1 - cz(i)])
-                               : Store<S>::load(f + (uint64_t)i * g.slot + cell);   // edge or wall reflection
+                    // edge reflections are in the halo; a link wall at x - c_i
+                    // reflects from the cell's own slot i
+                    fc[i] = ((wall >> opp(i)) & 1u) ? Store<S>::load(f + (uint64_t)i * g.slot + cell)
+                                                    : Store<S>::from_raw((R)buf[k * P + t + 1 - cz(i)]);
                 }
             }
         }
@@ -705,13 +719,20 @@ __device__ __forceinline__ uint16_t lo16(uint32_t v) { return (uint16_t)(v & 0xf
 __device__ __forceinline__ uint16_t hi16(uint32_t v) { return (uint16_t)(v >> 16); }
 __device__ __forceinline__ uint32_t pack16(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
 
-template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
+// PACKED: a workgroup takes g.rowsPerWG whole rows of a block (rows shorter than
+// half the workgroup: the lanes a one-row segment would leave idle take the next
+// rows; every row holds at least two pairs, so that its low and high halo words
+// come from different lanes). Without it a workgroup takes one segment of up to
+// 2W cells of one row and the row is wavefront-uniform.
+template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false, bool PACKED = false>
 __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
     static_assert(sizeof(typename Store<S>::R) == 2, "pairs of 16-bit values");
     using T = typename Store<S>::T;
     extern __shared__ __align__(16) unsigned char ldsRaw[];
-    const int W = (int)blockDim.x;            // pairs per segment
-    const int P = W + 2;                      // pitch in pair words: low halo, segment, high halo
+    const int W = (int)blockDim.x;            // threads per workgroup
+    // pitch in pair words per staged direction: per segment a low halo word, the
+    // segment, a high halo word
+    const int P = PACKED ? a.ldsPitch : W + 2;
     uint32_t* buf = reinterpret_cast<uint32_t*>(ldsRaw);
     // ODD && TAGS: per cell, the directions whose output it does not send along c_i
     // (its link-wall mask; all bits when it is not collided here), 2W words
@@ -722,16 +743,38 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
     const BlockGeom g = a.blocks[b];
     S* __restrict__ f = reinterpret_cast<S*>(a.arena) + g.offset;
     const uint32_t local = wg - g.wgBegin;
-    const int chunk = (int)(local % (uint32_t)g.nzChunks);
-    const uint32_t rest = local / (uint32_t)g.nzChunks;
-    const int y = (int)(rest % (uint32_t)g.nyGroups);
-    const int x = (int)(rest / (uint32_t)g.nyGroups);
-    const int z0 = chunk * 2 * W;
-    const int len = min(g.nz - z0, 2 * W);    // even
-    const int nPairs = len / 2;
     const int t = (int)threadIdx.x;
-    const bool active = t < nPairs;
-    const int zA = z0 + 2 * t, zB = zA + 1;
+    int x, y, z0, len, p, lanes, sb;
+    bool rowOk;
+    if constexpr (PACKED) {
+        // segment = whole row; lanes t = r * units + p, r the row within the workgroup
+        const int units = g.nz / 2;
+        const int r = t / units;
+        p = t - r * units;
+        const uint32_t rowIdx = local * (uint32_t)g.rowsPerWG + (uint32_t)r;
+        rowOk = r < g.rowsPerWG && rowIdx < (uint32_t)g.nx * (uint32_t)g.ny;
+        x = rowOk ? (int)(rowIdx / (uint32_t)g.ny) : 0;
+        y = rowOk ? (int)(rowIdx % (uint32_t)g.ny) : 0;
+        z0 = 0;
+        len = g.nz;
+        lanes = units;
+        sb = r * (units + 2);
+    } else {
+        const int chunk = (int)(local % (uint32_t)g.nzChunks);
+        const uint32_t rest = local / (uint32_t)g.nzChunks;
+        y = (int)(rest % (uint32_t)g.nyGroups);
+        x = (int)(rest / (uint32_t)g.nyGroups);
+        z0 = chunk * 2 * W;
+        len = min(g.nz - z0, 2 * W);          // even
+        p = t;
+        lanes = W;
+        sb = 0;
+        rowOk = true;
+    }
+    const int nPairs = len / 2;
+    const int cb = 2 * (t - p);               // first entry of the segment in `blocked`
+    const bool active = rowOk && p < nPairs;
+    const int zA = z0 + 2 * p, zB = zA + 1;
     const uint64_t rowBase = ((uint64_t)x * g.ny + y) * g.nz;
     const uint64_t cellA = rowBase + zA, cellB = cellA + 1;
     const bool rowInBulk = x >= g.bx0 && x <= g.bx1 && y >= g.by0 && y <= g.by1;
@@ -785,18 +828,23 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
         const int ys[3] = {ym, y, yp};
         const bool vx[3] = {vxm, true, vxp};
         const bool vy[3] = {vym, true, vyp};
-        if (!(vxm && vxp && vym && vyp)) {
-            if (outer_envelope_row(g, x, y)) {
-                // outermost envelope layer: see bulk_odd_staged
-            } else {
+        const bool allValid = vxm && vxp && vym && vyp;
+        // rows on a block edge: per-lane edge selection, no LDS (the outermost
+        // envelope layer is not collided: see bulk_odd_staged)
+        if (rowOk && !allValid && !outer_envelope_row(g, x, y)) {
             if (compA)
                 odd_cell_general<S, DYN, STATS, FORCED, MOMENTS>(a, g, f, zA, cellA, tagA, inBulkA, xs, ys, vx, vy, st,
                                                                  wallA, wf);
             if (compB)
                 odd_cell_general<S, DYN, STATS, FORCED, MOMENTS>(a, g, f, zB, cellB, tagB, inBulkB, xs, ys, vx, vy, st,
                                                                  wallB, wf);
-            }
-        } else {
+        }
+        // staged rows. Unpacked, the row is workgroup-uniform; packed, every thread
+        // takes part in the barriers and the edge rows' lanes are masked off (their
+        // segments in LDS are used by no other row).
+        if (PACKED || allValid) {
+            const bool sAct = active && allValid;
+            const bool sCompA = compA && allValid, sCompB = compB && allValid;
             S* rows[3][3];
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax)
@@ -809,40 +857,69 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             const bool vhi = shift(z0 + len - 1, 1, g.nz, g.pz, zhi);
             const int zc = active ? zA : z0;
             uint32_t raw[kZMoving];
+            if (sAct) {
 #pragma unroll
-            for (int i = 0; i < 19; ++i) {
-                const S* src = rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot;
-                if (cz(i) == 0) {
-                    const uint32_t v = ld_pair(src + zc);
-                    fa[i] = Store<S>::from_raw(lo16(v));
-                    fb[i] = Store<S>::from_raw(hi16(v));
-                    // link walls at x - c_i: f_i comes back from the cell's own slot i
-                    if ((wallA >> opp(i)) & 1u) fa[i] = Store<S>::load(f + (uint64_t)i * g.slot + cellA);
-                    if ((wallB >> opp(i)) & 1u) fb[i] = Store<S>::load(f + (uint64_t)i * g.slot + cellB);
-                } else {
-                    raw[zmove_index(i)] = ld_pair(src + zc);
+                for (int i = 0; i < 19; ++i) {
+                    const S* src = rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot;
+                    if (cz(i) == 0) {
+                        const uint32_t v = ld_pair(src + zc);
+                        fa[i] = Store<S>::from_raw(lo16(v));
+                        fb[i] = Store<S>::from_raw(hi16(v));
+                        // link walls at x - c_i: f_i comes back from the cell's own slot i
+                        if ((wallA >> opp(i)) & 1u) fa[i] = Store<S>::load(f + (uint64_t)i * g.slot + cellA);
+                        if ((wallB >> opp(i)) & 1u) fb[i] = Store<S>::load(f + (uint64_t)i * g.slot + cellB);
+                    } else {
+                        raw[zmove_index(i)] = ld_pair(src + zc);
+                    }
                 }
             }
+            // TAGS: the own slot-i pair where a link wall reflects a z-moving
+            // direction, loaded with the other loads (not after the barrier)
+            uint32_t wref[kZMoving];
+            if constexpr (TAGS) {
+#pragma unroll
+                for (int i = 0; i < 19; ++i) {
+                    if (cz(i) == 0) continue;
+                    wref[zmove_index(i)] = 0;
+                    if (sAct && (((wallA | wallB) >> opp(i)) & 1u))
+                        wref[zmove_index(i)] = ld_pair(f + (uint64_t)i * g.slot + cellA);
+                }
+            }
+            // Halo words. Inside a periodic or segmented row: the source pairs beyond
+            // the segment ends. At a non-periodic row end: the end cell's own slot-i
+            // value in the half its shifted read takes (the high half of the low word
+            // for c_z = +1, the low half of the high word for c_z = -1), so that the
+            // edge reflection is read from LDS like any other input.
             uint32_t halo[kZMoving];
-            const bool loLane = t == 0 && vlo, hiLane = t == W - 1 && vhi;
-            if (loLane || hiLane) {
-                const int zh = loLane ? zlo : zhi;
+            const bool segOk = rowOk && allValid;
+            const bool loLane = segOk && p == 0 && vlo, hiLane = segOk && p == lanes - 1 && vhi;
+            const bool loEdge = segOk && p == 0 && !vlo, hiEdge = segOk && p == lanes - 1 && !vhi;
+            const bool edge = loEdge || hiEdge;
+            if (loLane || hiLane || edge) {
+                // edge lanes load every z-moving direction's end pair; the shifted
+                // read takes only the directions arriving from beyond the end
+                const int zh = edge ? (loEdge ? z0 : z0 + len - 2) : (loLane ? zlo : zhi);
 #pragma unroll
                 for (int i = 0; i < 19; ++i)
-                    if (cz(i) != 0)
-                        halo[zmove_index(i)] = ld_pair(rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot + zh);
+                    if (cz(i) != 0) {
+                        const S* src = edge ? f + rowBase + (uint64_t)i * g.slot
+                                            : rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot;
+                        const uint32_t v = ld_pair(src + zh);
+                        halo[zmove_index(i)] = !edge ? v : (loEdge ? (v << 16) : (v >> 16));
+                    }
             }
-            if (active) {
+            const int q = sb + p;                 // this lane's pair word in its segment, minus 1
+            if (sAct) {
 #pragma unroll
-                for (int k = 0; k < kZMoving; ++k) buf[k * P + t + 1] = raw[k];
+                for (int k = 0; k < kZMoving; ++k) buf[k * P + q + 1] = raw[k];
             }
-            if (loLane || hiLane) {
-                const int hp = loLane ? 0 : nPairs + 1;
+            if (loLane || hiLane || edge) {
+                const int hp = sb + ((loLane || loEdge) ? 0 : nPairs + 1);
 #pragma unroll
                 for (int k = 0; k < kZMoving; ++k) buf[k * P + hp] = halo[k];
             }
             if constexpr (TAGS) {
-                if (active) {
+                if (sAct) {
                     blocked[2 * t] = compA ? wallA : 0xffffffffu;
                     blocked[2 * t + 1] = compB ? wallB : 0xffffffffu;
                 }
@@ -850,25 +927,33 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             __syncthreads();
             // shifted inputs: c_z = +1 reads z - 1, c_z = -1 reads z + 1
             const bool vzA = zA > 0 || g.pz, vzB = zB < g.nz - 1 || g.pz;
-            if (active) {
+            if (sAct) {
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
                     if (cz(i) == 0) continue;
                     const uint32_t* row = buf + zmove_index(i) * P;
                     const bool wA = (wallA >> opp(i)) & 1u, wB = (wallB >> opp(i)) & 1u;
+                    uint16_t ra, rb;
                     if (cz(i) > 0) {
-                        fa[i] = (vzA && !wA) ? Store<S>::from_raw(hi16(row[t])) : Store<S>::load(f + (uint64_t)i * g.slot + cellA);
-                        fb[i] = !wB ? Store<S>::from_raw(lo16(row[t + 1])) : Store<S>::load(f + (uint64_t)i * g.slot + cellB);
+                        ra = hi16(row[q]);
+                        rb = lo16(row[q + 1]);
                     } else {
-                        fa[i] = !wA ? Store<S>::from_raw(hi16(row[t + 1])) : Store<S>::load(f + (uint64_t)i * g.slot + cellA);
-                        fb[i] = (vzB && !wB) ? Store<S>::from_raw(lo16(row[t + 2])) : Store<S>::load(f + (uint64_t)i * g.slot + cellB);
+                        ra = hi16(row[q + 1]);
+                        rb = lo16(row[q + 2]);
                     }
+                    if constexpr (TAGS) {
+                        // link wall at x - c_i: f_i comes back from the cell's own slot i
+                        if (wA) ra = lo16(wref[zmove_index(i)]);
+                        if (wB) rb = hi16(wref[zmove_index(i)]);
+                    }
+                    fa[i] = Store<S>::from_raw(ra);
+                    fb[i] = Store<S>::from_raw(rb);
                 }
             }
             __syncthreads();   // every LDS input is read before the outputs reuse the buffer
-            if (compA) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
-            if (compB) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
-            if (active) {
+            if (sCompA) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
+            if (sCompB) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
+            if (sAct) {
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
                     // link walls at x + c_i: the output goes back to the cell's own slot opp(i)
@@ -884,7 +969,7 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                             if (sB) Store<S>::store(dst + 1, fb[i]);
                         }
                     } else {
-                        buf[zmove_index(i) * P + t + 1] = pack16(Store<S>::to_raw(fa[i]), Store<S>::to_raw(fb[i]));
+                        buf[zmove_index(i) * P + q + 1] = pack16(Store<S>::to_raw(fa[i]), Store<S>::to_raw(fb[i]));
                         // edge reflection where the destination leaves a non-periodic z edge
                         if (cz(i) > 0 && !vzB && compB && !rB) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellB, fb[i]);
                         if (cz(i) < 0 && !vzA && compA && !rA) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellA, fa[i]);
@@ -897,7 +982,7 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             // segment-local cell c was collided here and sent its direction-i output along c_i
             auto computed = [&](int c, int i) {
                 if (c < 0 || c >= len) return false;
-                if constexpr (TAGS) return ((blocked[c] >> i) & 1u) == 0;
+                if constexpr (TAGS) return ((blocked[cb + c] >> i) & 1u) == 0;
                 return true;
             };
 #pragma unroll
@@ -905,19 +990,19 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                 if (cz(i) == 0) continue;
                 const uint32_t* row = buf + zmove_index(i) * P;
                 S* dst = rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot;
-                if (active) {
+                if (sAct) {
                     uint16_t lo, hi;
                     bool okLo, okHi;
                     if (cz(i) > 0) {
-                        lo = hi16(row[t]);
-                        okLo = computed(2 * t - 1, i);
-                        hi = lo16(row[t + 1]);
-                        okHi = computed(2 * t, i);
+                        lo = hi16(row[q]);
+                        okLo = computed(2 * p - 1, i);
+                        hi = lo16(row[q + 1]);
+                        okHi = computed(2 * p, i);
                     } else {
-                        lo = hi16(row[t + 1]);
-                        okLo = computed(2 * t + 1, i);
-                        hi = lo16(row[t + 2]);
-                        okHi = computed(2 * t + 2, i);
+                        lo = hi16(row[q + 1]);
+                        okLo = computed(2 * p + 1, i);
+                        hi = lo16(row[q + 2]);
+                        okHi = computed(2 * p + 2, i);
                     }
                     if (okLo && okHi) st_pair(dst + zA, pack16(lo, hi));
                     else {
@@ -929,12 +1014,12 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                 // output to z0 + len; c_z = -1 sends the first cell's to z0 - 1
                 if (cz(i) > 0) {
                     int zd = 0;
-                    if (t == W - 1 && shift(z0 + len - 1, 1, g.nz, g.pz, zd) && computed(len - 1, i))
-                        Store<S>::store_raw(dst + zd, hi16(row[nPairs]));
+                    if (segOk && p == lanes - 1 && shift(z0 + len - 1, 1, g.nz, g.pz, zd) && computed(len - 1, i))
+                        Store<S>::store_raw(dst + zd, hi16(row[sb + nPairs]));
                 } else {
                     int zd = 0;
-                    if (t == 0 && shift(z0, -1, g.nz, g.pz, zd) && computed(0, i))
-                        Store<S>::store_raw(dst + zd, lo16(row[1]));
+                    if (segOk && p == 0 && shift(z0, -1, g.nz, g.pz, zd) && computed(0, i))
+                        Store<S>::store_raw(dst + zd, lo16(row[sb + 1]));
                 }
             }
         }

@@ -111,6 +111,7 @@ LAUNCH_VARIANTS = {
     "segments64": {"PLBAMD_BULK_TX": "64"},
     "no_xcd_remap": {"PLBAMD_XCD_REMAP": "0"},
     "no_pairs": {"PLBAMD_PAIRS": "0"},
+    "row_segments": {"PLBAMD_PACKED": "0"},
     "row_even_steps": {"PLBAMD_LINEAR_EVEN": "0"},
 }
 # (PLBAMD_LINEAR_EVEN=0 only switches the linear even step off; the multi-block
@@ -124,8 +125,9 @@ LAUNCH_VARIANTS = {
 def test_gpu_launch_variants(gpu, monkeypatch, variant, name, storage):
     """The same results with the odd step unstaged (one kernel for both
     parities) or staged through LDS whatever the storage, with 64-cell row
-    segments (several segments per row: halo elements between workgroups), and
-    without the XCD-aware workgroup order."""
+    segments (several segments per row: halo elements between workgroups),
+    one row segment per workgroup instead of several whole rows (16-bit pair
+    kernel), and without the XCD-aware workgroup order."""
     for k, v in LAUNCH_VARIANTS[variant].items():
         monkeypatch.setenv(k, v)
     case, steps = CASES[name](storage)
