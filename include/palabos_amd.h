@@ -158,6 +158,23 @@ int plbamd_set_porosity(int handle, const double* alpha);
 int plbamd_upload(int handle, const plbamd_box* box, int elem, const void* data, size_t bytes);
 int plbamd_download(int handle, const plbamd_box* box, int elem, void* data, size_t bytes);
 
+/* Writes / reads the lattice as a Palabos checkpoint file, the format of
+ * saveBinaryBlock / loadBinaryBlock (io/serializerIO_3D.h:48,58;
+ * io/serializerIO_3D.cpp:37-66) for a MultiBlockLattice3D<T, D3Q19Descriptor>
+ * (force_externals = 0) or <T, ForcedD3Q19Descriptor> (force_externals = 1)
+ * with T = double (element PLBAMD_F64) or float (PLBAMD_F32): the base64 text
+ * of the data size (8 bytes, or 4 with enforce_uint, as the reference's
+ * enforceUint), then the base64 text of every cell of the bounding box in C
+ * order (x, y, z), each its 19 f-bar (+ the 3 force components) as T; cells in
+ * no block are zeros (IndexOrdering::forward, the reference's default). The
+ * device serializes and codes the cells. Loading checks the size header
+ * against this lattice, rounds to the storage type (nearest even), requires
+ * every cell's force externals to equal the lattice's force, and refreshes the
+ * multi-block envelopes; a multi-block lattice loads after an even number of
+ * steps. Errors: PLBAMD_EINVAL with a message (file, size or content). */
+int plbamd_save_binary_block(int handle, const char* file_name, int element, int force_externals, int enforce_uint);
+int plbamd_load_binary_block(int handle, const char* file_name, int element, int force_externals, int enforce_uint);
+
 /* Sets the populations of `box` to equilibrium for per-cell rho and u
  * (FP64, box cells and box cells x 3). Replaces initializeAtEquilibrium
  * (dataProcessors/dataInitializerFunctional3D.hh:925-944). */

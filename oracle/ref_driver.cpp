@@ -27,6 +27,15 @@
 //   step storedAvgRho storedAvgEnergy avgRho(computeAverageDensity) avgEnergy force_x force_y force_z
 //
 // Bench mode: ref_driver --bench T n steps  -> periodic BGK(1.8) n^3 lattice, prints MLUPS.
+//
+// Checkpoint mode (the reference's saveBinaryBlock / loadBinaryBlock):
+//   ref_driver --checkpoint save T plain|forced nx ny nz records.bin out.dat uint|size
+//       records.bin: per cell, C order, the 19 f-bar (+ the 3 force externals for
+//       "forced", ForcedD3Q19Descriptor) as T; they are set into a
+//       MultiBlockLattice3D, which saveBinaryBlock writes to out.dat
+//   ref_driver --checkpoint load T plain|forced nx ny nz in.dat records.bin uint|size
+//       loadBinaryBlock of in.dat into a fresh lattice; its cell records are written
+//   "uint" passes enforceUint = true (32-bit size header), "size" false.
 
 // The umbrella palabos3D.hh does not compile with g++ 11 (multiGrid), so the
 // lattice-core subset is included instead.
@@ -321,10 +330,61 @@ int runBench(plint n, plint steps) {
     return 0;
 }
 
+template<typename T, template<typename U> class Desc>
+int runCheckpoint(const std::string& mode, plint nx, plint ny, plint nz, const std::string& a, const std::string& b,
+                  bool enforceUint) {
+    typedef MultiBlockLattice3D<T,Desc> Lattice;
+    const plint numPop = Desc<T>::numPop, numExt = Desc<T>::ExternalField::numScalars;
+    const plint rec = numPop + numExt;
+    Lattice lat(nx, ny, nz, new BGKdynamics<T,Desc>((T)1));
+    std::vector<T> records((size_t)(nx * ny * nz * rec));
+    if (mode == "save") {
+        std::ifstream in(a.c_str(), std::ios::binary);
+        in.read(reinterpret_cast<char*>(records.data()), (std::streamsize)(records.size() * sizeof(T)));
+        if (!in) { std::fprintf(stderr, "cannot read %s\n", a.c_str()); return 1; }
+        size_t k = 0;
+        for (plint x = 0; x < nx; ++x)
+            for (plint y = 0; y < ny; ++y)
+                for (plint z = 0; z < nz; ++z, k += (size_t)rec) {
+                    Cell<T,Desc>& cell = lat.get(x, y, z);
+                    for (plint i = 0; i < numPop; ++i) cell[i] = records[k + i];
+                    for (plint e = 0; e < numExt; ++e) *cell.getExternal(e) = records[k + numPop + e];
+                }
+        saveBinaryBlock(lat, b, enforceUint);
+    } else {
+        loadBinaryBlock(lat, a, enforceUint);
+        size_t k = 0;
+        for (plint x = 0; x < nx; ++x)
+            for (plint y = 0; y < ny; ++y)
+                for (plint z = 0; z < nz; ++z, k += (size_t)rec) {
+                    Cell<T,Desc>& cell = lat.get(x, y, z);
+                    for (plint i = 0; i < numPop; ++i) records[k + i] = cell[i];
+                    for (plint e = 0; e < numExt; ++e) records[k + numPop + e] = *cell.getExternal(e);
+                }
+        std::ofstream out(b.c_str(), std::ios::binary);
+        out.write(reinterpret_cast<const char*>(records.data()), (std::streamsize)(records.size() * sizeof(T)));
+    }
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char* argv[]) {
     plbInit(&argc, &argv);
+    if (argc >= 2 && std::string(argv[1]) == "--checkpoint") {
+        if (argc < 11) {
+            std::fprintf(stderr, "usage: --checkpoint save|load double|float plain|forced nx ny nz in out uint|size\n");
+            return 2;
+        }
+        const std::string mode = argv[2], T = argv[3], desc = argv[4];
+        const plint nx = std::atol(argv[5]), ny = std::atol(argv[6]), nz = std::atol(argv[7]);
+        const bool enforceUint = std::string(argv[10]) == "uint";
+        if (desc == "forced")
+            return T == "float" ? runCheckpoint<float, descriptors::ForcedD3Q19Descriptor>(mode, nx, ny, nz, argv[8], argv[9], enforceUint)
+                                : runCheckpoint<double, descriptors::ForcedD3Q19Descriptor>(mode, nx, ny, nz, argv[8], argv[9], enforceUint);
+        return T == "float" ? runCheckpoint<float, descriptors::D3Q19Descriptor>(mode, nx, ny, nz, argv[8], argv[9], enforceUint)
+                            : runCheckpoint<double, descriptors::D3Q19Descriptor>(mode, nx, ny, nz, argv[8], argv[9], enforceUint);
+    }
     if (argc >= 2 && std::string(argv[1]) == "--bench") {
         if (argc < 5) { std::fprintf(stderr, "usage: --bench double|float n steps\n"); return 2; }
         plint n = std::atol(argv[3]), steps = std::atol(argv[4]);

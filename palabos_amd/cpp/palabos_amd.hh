@@ -413,6 +413,22 @@ public:
         detail::check(plbamd_initialize_at_equilibrium(handle, &cb, rho.data(), u.data()), "initialize_at_equilibrium");
         hostValid = false;
     }
+    // Checkpoint files of this lattice's type (io/serializerIO_3D.cpp:37-66).
+    void saveBinary(std::string const& fName, bool enforceUint) {
+        materialize();
+        pushHost();
+        detail::check(plbamd_save_binary_block(handle, fName.c_str(), StorageOf<T>::code, Descriptor<T>::forced ? 1 : 0,
+                                               enforceUint ? 1 : 0),
+                      "save_binary_block");
+    }
+    void loadBinary(std::string const& fName, bool enforceUint) {
+        materialize();
+        pushHost();
+        detail::check(plbamd_load_binary_block(handle, fName.c_str(), StorageOf<T>::code, Descriptor<T>::forced ? 1 : 0,
+                                               enforceUint ? 1 : 0),
+                      "load_binary_block");
+        hostValid = false;
+    }
     void moments(Box3D const& b, double* sums) {
         materialize();
         pushHost();
@@ -702,6 +718,17 @@ T computeAverageEnergy(MultiBlockLattice3D<T, Descriptor>& lattice, Box3D domain
 template<typename T, template<typename U> class Descriptor>
 T computeAverageEnergy(MultiBlockLattice3D<T, Descriptor>& lattice) {
     return computeAverageEnergy(lattice, lattice.getBoundingBox());
+}
+
+// io/serializerIO_3D.h:48,58: the file holds the raw f-bar (and, for
+// ForcedD3Q19Descriptor, the force externals) of every cell, base64-coded.
+template<typename T, template<typename U> class Descriptor>
+void saveBinaryBlock(MultiBlockLattice3D<T, Descriptor>& lattice, std::string fName, bool enforceUint = false) {
+    lattice.saveBinary(fName, enforceUint);
+}
+template<typename T, template<typename U> class Descriptor>
+void loadBinaryBlock(MultiBlockLattice3D<T, Descriptor>& lattice, std::string fName, bool enforceUint = false) {
+    lattice.loadBinary(fName, enforceUint);
 }
 
 // multiBlockLattice3D.hh:531-546

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -16,6 +17,7 @@
 
 #include "../../include/palabos_amd.h"
 #include "adjoint.h"
+#include "checkpoint.h"
 #include "kernels.h"
 
 namespace plbamd {
@@ -787,20 +789,11 @@ bool boxArgs(const Lattice& L, size_t bi, const plbamd_box& g, BoxArgs& a) {
     return true;
 }
 
+// Populations of box b between the lattice and a device buffer of boxCells(b)
+// * 19 elements of type elem, C order; launches only (no host copy, no wait).
 template<typename S>
-int transfer(Lattice& L, const plbamd_box& b, int elem, void* data, size_t bytes, bool up) {
-    const uint64_t n = boxCells(b);
-    const size_t hostElem = (elem == PLBAMD_F64) ? 8 : (elem == PLBAMD_F32 ? 4 : 2);
-    if (bytes != n * 19 * hostElem) return fail(PLBAMD_EINVAL, "transfer: byte count does not match the box");
-    if ((elem == PLBAMD_F16 || elem == PLBAMD_BF16) && elem != L.desc.storage)
-        return fail(PLBAMD_EINVAL, "transfer: 16-bit host data must match the storage type");
-    if (up && L.multi && (L.time & 1))
-        return fail(PLBAMD_ESTATE, "upload: a multi-block lattice accepts populations after an even number of steps");
-    PLBAMD_TRY(rebuild(L));
-    void* staging = nullptr;
-    PLBAMD_HIP(hipMalloc(&staging, bytes));
+hipError_t transferDevice(Lattice& L, const plbamd_box& b, int elem, void* staging, bool up) {
     hipError_t err = hipSuccess;
-    if (up) err = hipMemcpyAsync(staging, data, bytes, hipMemcpyHostToDevice, L.stream);
     for (size_t bi = 0; bi < L.geom.size() && err == hipSuccess; ++bi) {
         BoxArgs a;
         if (!boxArgs(L, bi, b, a)) continue;
@@ -826,12 +819,168 @@ int transfer(Lattice& L, const plbamd_box& b, int elem, void* data, size_t bytes
         }
         if (err == hipSuccess) err = hipGetLastError();
     }
+    return err;
+}
+
+template<typename S>
+int transfer(Lattice& L, const plbamd_box& b, int elem, void* data, size_t bytes, bool up) {
+    const uint64_t n = boxCells(b);
+    const size_t hostElem = (elem == PLBAMD_F64) ? 8 : (elem == PLBAMD_F32 ? 4 : 2);
+    if (bytes != n * 19 * hostElem) return fail(PLBAMD_EINVAL, "transfer: byte count does not match the box");
+    if ((elem == PLBAMD_F16 || elem == PLBAMD_BF16) && elem != L.desc.storage)
+        return fail(PLBAMD_EINVAL, "transfer: 16-bit host data must match the storage type");
+    if (up && L.multi && (L.time & 1))
+        return fail(PLBAMD_ESTATE, "upload: a multi-block lattice accepts populations after an even number of steps");
+    PLBAMD_TRY(rebuild(L));
+    void* staging = nullptr;
+    PLBAMD_HIP(hipMalloc(&staging, bytes));
+    hipError_t err = hipSuccess;
+    if (up) err = hipMemcpyAsync(staging, data, bytes, hipMemcpyHostToDevice, L.stream);
+    if (err == hipSuccess) err = transferDevice<S>(L, b, elem, staging, up);
     if (err == hipSuccess && up && exchange<S>(L) != PLBAMD_OK) err = hipErrorUnknown;
     if (err == hipSuccess && !up) err = hipMemcpyAsync(data, staging, bytes, hipMemcpyDeviceToHost, L.stream);
     if (err == hipSuccess) err = hipStreamSynchronize(L.stream);
     hipFree(staging);
     if (err != hipSuccess) return fail(PLBAMD_EHIP, std::string("transfer: ") + hipGetErrorString(err));
     return PLBAMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Palabos checkpoint files (saveBinaryBlock / loadBinaryBlock,
+// io/serializerIO_3D.cpp:37-66): the cells are serialized and base64-coded on
+// the device; the host writes or reads the text.
+
+std::string base64Host(const uint8_t* p, size_t n) {
+    std::string out;
+    for (size_t k = 0; k < n; k += 3) {
+        const uint32_t b0 = p[k], b1 = k + 1 < n ? p[k + 1] : 0u, b2 = k + 2 < n ? p[k + 2] : 0u;
+        const uint32_t g = (b0 << 16) | (b1 << 8) | b2;
+        out += b64_char(g >> 18);
+        out += b64_char((g >> 12) & 63);
+        out += k + 1 < n ? b64_char((g >> 6) & 63) : '=';
+        out += k + 2 < n ? b64_char(g & 63) : '=';
+    }
+    return out;
+}
+
+// Decodes the n bytes of 4 * ceil(n / 3) characters; false on a character
+// outside the alphabet where the bytes need one.
+bool unbase64Host(const char* s, size_t n, uint8_t* out) {
+    for (size_t k = 0; k < n; k += 3, s += 4) {
+        const size_t take = n - k < 3 ? n - k : 3;
+        uint32_t v[4];
+        for (int c = 0; c < 4; ++c) v[c] = b64_value((uint8_t)s[c]);
+        for (size_t c = 0; c <= take; ++c) if (v[c] > 63) return false;
+        const uint32_t x = ((v[0] & 63) << 18) | ((v[1] & 63) << 12) | ((v[2] & 63) << 6) | (v[3] & 63);
+        out[k] = (uint8_t)(x >> 16);
+        if (take > 1) out[k + 1] = (uint8_t)((x >> 8) & 255u);
+        if (take > 2) out[k + 2] = (uint8_t)(x & 255u);
+    }
+    return true;
+}
+
+// Device buffers of one call, freed on every return path.
+struct DeviceBuffers {
+    std::vector<void*> p;
+    void* get(size_t bytes) {
+        void* q = nullptr;
+        if (hipMalloc(&q, bytes) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+        p.push_back(q);
+        return q;
+    }
+    ~DeviceBuffers() { for (void* q : p) (void)hipFree(q); }
+};
+
+static dim3 itemsGrid(uint64_t n) { return dim3((unsigned)std::max<uint64_t>(1, (n + 255) / 256)); }
+
+template<typename S, typename T>
+int checkpointTyped(Lattice& L, const char* path, int elem, bool force, bool enforceUint, bool save) {
+    const plbamd_box whole{0, L.desc.nx - 1, 0, L.desc.ny - 1, 0, L.desc.nz - 1};
+    const uint64_t cells = boxCells(whole);
+    const int recLen = force ? 22 : 19;
+    const uint64_t dataBytes = cells * recLen * sizeof(T);
+    const uint64_t nChars = (dataBytes + 2) / 3 * 4;
+    if (enforceUint && dataBytes > 0xffffffffull)
+        return fail(PLBAMD_EINVAL, "binary block: the data does not fit a 32-bit size header (enforce_uint)");
+    if (dataBytes / 12 + 1 > (uint64_t)INT32_MAX * 256)
+        return fail(PLBAMD_EINVAL, "binary block: lattice too large for one conversion launch");
+    PLBAMD_TRY(rebuild(L));
+    DeviceBuffers buf;
+    T* pops = static_cast<T*>(buf.get(cells * 19 * sizeof(T)));
+    T* rec = force ? static_cast<T*>(buf.get(dataBytes)) : pops;
+    char* text = static_cast<char*>(buf.get((nChars + 15) / 16 * 16));
+    int* bad = static_cast<int*>(buf.get(2 * sizeof(int)));
+    if (!pops || !rec || !text || !bad) return fail(PLBAMD_ENOMEM, "binary block: cannot allocate device buffers");
+    const T fx = (T)L.desc.force[0], fy = (T)L.desc.force[1], fz = (T)L.desc.force[2];
+    if (save) {
+        // cells in no block are zeros, as the reference's forward ordering writes them
+        PLBAMD_HIP(hipMemsetAsync(pops, 0, cells * 19 * sizeof(T), L.stream));
+        PLBAMD_HIP(transferDevice<S>(L, whole, elem, pops, false));
+        if (force) hipLaunchKernelGGL((append_force<T>), itemsGrid(cells), dim3(256), 0, L.stream, pops, rec, cells, fx, fy, fz);
+        hipLaunchKernelGGL(base64_encode, itemsGrid(dataBytes / 12 + 1), dim3(256), 0, L.stream,
+                           reinterpret_cast<const uint8_t*>(rec), dataBytes, text);
+        PLBAMD_HIP(hipGetLastError());
+        std::vector<char> host(nChars);
+        PLBAMD_HIP(hipMemcpyAsync(host.data(), text, nChars, hipMemcpyDeviceToHost, L.stream));
+        PLBAMD_HIP(hipStreamSynchronize(L.stream));
+        uint8_t header[8];
+        const uint64_t size64 = dataBytes;
+        std::memcpy(header, &size64, 8);   // little-endian host
+        const std::string head = base64Host(header, enforceUint ? 4 : 8);
+        FILE* fp = std::fopen(path, "wb");
+        if (!fp) return fail(PLBAMD_EINVAL, std::string("save_binary_block: cannot open ") + path);
+        const bool ok = std::fwrite(head.data(), 1, head.size(), fp) == head.size() &&
+                        std::fwrite(host.data(), 1, host.size(), fp) == host.size();
+        if (std::fclose(fp) != 0 || !ok) return fail(PLBAMD_EINVAL, std::string("save_binary_block: cannot write ") + path);
+        return PLBAMD_OK;
+    }
+    if (L.multi && (L.time & 1))
+        return fail(PLBAMD_ESTATE, "load_binary_block: a multi-block lattice accepts populations after an even number of steps");
+    std::string file;
+    {
+        FILE* fp = std::fopen(path, "rb");
+        if (!fp) return fail(PLBAMD_EINVAL, std::string("load_binary_block: cannot open ") + path);
+        char chunk[1 << 16];
+        size_t got;
+        while ((got = std::fread(chunk, 1, sizeof(chunk), fp)) > 0) file.append(chunk, got);
+        std::fclose(fp);
+    }
+    // the reference reads with operator>>, which skips white space
+    if (file.find_first_of(" \t\r\n\v\f") != std::string::npos)
+        file.erase(std::remove_if(file.begin(), file.end(), [](char c) { return std::isspace((unsigned char)c) != 0; }),
+                   file.end());
+    const size_t headChars = enforceUint ? 8 : 12;
+    if (file.size() < headChars) return fail(PLBAMD_EINVAL, "load_binary_block: file shorter than its size header");
+    uint8_t header[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (!unbase64Host(file.data(), enforceUint ? 4 : 8, header))
+        return fail(PLBAMD_EINVAL, "load_binary_block: the size header is not base64");
+    uint64_t size64 = 0;
+    std::memcpy(&size64, header, 8);
+    if (size64 != dataBytes)
+        return fail(PLBAMD_EINVAL, "load_binary_block: the file holds " + std::to_string(size64) +
+                                       " bytes of cells, this lattice takes " + std::to_string(dataBytes));
+    if (file.size() - headChars < nChars) return fail(PLBAMD_EINVAL, "load_binary_block: the file is truncated");
+    PLBAMD_HIP(hipMemsetAsync(bad, 0, 2 * sizeof(int), L.stream));
+    PLBAMD_HIP(hipMemcpyAsync(text, file.data() + headChars, nChars, hipMemcpyHostToDevice, L.stream));
+    hipLaunchKernelGGL(base64_decode, itemsGrid(dataBytes / 12 + 1), dim3(256), 0, L.stream, text, dataBytes,
+                       reinterpret_cast<uint8_t*>(rec), bad);
+    if (force) hipLaunchKernelGGL((split_force<T>), itemsGrid(cells), dim3(256), 0, L.stream, rec, pops, cells, fx, fy, fz, bad + 1);
+    PLBAMD_HIP(hipGetLastError());
+    int flags[2] = {0, 0};
+    PLBAMD_HIP(hipMemcpyAsync(flags, bad, sizeof(flags), hipMemcpyDeviceToHost, L.stream));
+    PLBAMD_HIP(hipStreamSynchronize(L.stream));
+    if (flags[0]) return fail(PLBAMD_EINVAL, "load_binary_block: the cell data is not base64");
+    if (flags[1]) return fail(PLBAMD_EINVAL, "load_binary_block: the force externals differ from the lattice's force");
+    PLBAMD_HIP(transferDevice<S>(L, whole, elem, pops, true));
+    PLBAMD_TRY(exchange<S>(L));
+    PLBAMD_HIP(hipStreamSynchronize(L.stream));
+    return PLBAMD_OK;
+}
+
+template<typename S>
+int checkpoint(Lattice& L, const char* path, int elem, bool force, bool enforceUint, bool save) {
+    if (elem == PLBAMD_F64) return checkpointTyped<S, double>(L, path, elem, force, enforceUint, save);
+    return checkpointTyped<S, float>(L, path, elem, force, enforceUint, save);
 }
 
 template<typename S>
@@ -1354,6 +1503,29 @@ static int dispatchTransfer(int handle, const plbamd_box* box, int elem, void* d
     case PLBAMD_F16: return transfer<f16s>(*L, *box, elem, data, bytes, up);
     default: return transfer<bf16s>(*L, *box, elem, data, bytes, up);
     }
+}
+
+static int dispatchCheckpoint(int handle, const char* path, int elem, int force, int enforceUint, bool save) {
+    Lattice* L = find(handle);
+    const char* what = save ? "save_binary_block" : "load_binary_block";
+    if (!L) return fail(PLBAMD_EINVAL, std::string(what) + ": unknown handle");
+    if (!path) return fail(PLBAMD_EINVAL, std::string(what) + ": null file name");
+    if (elem != PLBAMD_F64 && elem != PLBAMD_F32)
+        return fail(PLBAMD_EINVAL, std::string(what) + ": the file element type is FP64 or FP32");
+    switch (L->desc.storage) {
+    case PLBAMD_F64: return checkpoint<double>(*L, path, elem, force != 0, enforceUint != 0, save);
+    case PLBAMD_F32: return checkpoint<float>(*L, path, elem, force != 0, enforceUint != 0, save);
+    case PLBAMD_F16: return checkpoint<f16s>(*L, path, elem, force != 0, enforceUint != 0, save);
+    default: return checkpoint<bf16s>(*L, path, elem, force != 0, enforceUint != 0, save);
+    }
+}
+
+int plbamd_save_binary_block(int handle, const char* file_name, int element, int force_externals, int enforce_uint) {
+    return dispatchCheckpoint(handle, file_name, element, force_externals, enforce_uint, true);
+}
+
+int plbamd_load_binary_block(int handle, const char* file_name, int element, int force_externals, int enforce_uint) {
+    return dispatchCheckpoint(handle, file_name, element, force_externals, enforce_uint, false);
 }
 
 int plbamd_upload(int handle, const plbamd_box* box, int elem, const void* data, size_t bytes) {

@@ -13,6 +13,7 @@ Populations are numpy arrays [x][y][z][19] of shifted f-bar in Palabos's
 D3Q19 order.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -21,6 +22,13 @@ from .native import Box, check, lib
 
 D3Q19 = "D3Q19"
 
+
+
+def _file_elem(T):
+    codes = {"f64": native.F64, "double": native.F64, "f32": native.F32, "float": native.F32}
+    if T not in codes:
+        raise ValueError("checkpoint element type must be f64 or f32")
+    return codes[T]
 
 class BGK:
     """BGKdynamics (basicDynamics/isoThermalDynamics.hh:318-330)."""
@@ -244,6 +252,21 @@ class Block3D:
         out = np.empty(b.shape() + (19,), dtype=dtype)
         check(lib().plbamd_download(self.handle, ctypes.byref(b), elem, out.ctypes.data, out.nbytes))
         return out
+
+    # -- checkpoint files -----------------------------------------------------
+    def saveBinaryBlock(self, fName, T="f64", forceExternals=False, enforceUint=False):
+        """Writes the lattice as a Palabos checkpoint file (saveBinaryBlock,
+        io/serializerIO_3D.cpp:37-49): the file a MultiBlockLattice3D<T, D3Q19>
+        (or <T, ForcedD3Q19> with forceExternals) of the same populations writes;
+        T is "f64" (double) or "f32" (float)."""
+        check(lib().plbamd_save_binary_block(self.handle, os.fsencode(fName), _file_elem(T), int(bool(forceExternals)),
+                                             int(bool(enforceUint))))
+
+    def loadBinaryBlock(self, fName, T="f64", forceExternals=False, enforceUint=False):
+        """Reads a Palabos checkpoint file (loadBinaryBlock,
+        io/serializerIO_3D.cpp:51-66) into the lattice."""
+        check(lib().plbamd_load_binary_block(self.handle, os.fsencode(fName), _file_elem(T), int(bool(forceExternals)),
+                                             int(bool(enforceUint))))
 
     # -- stepping -----------------------------------------------------------
     def collideAndStream(self, nsteps=1):

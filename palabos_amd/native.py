@@ -31,7 +31,7 @@ EXPORTS = (
     "plbamd_upload", "plbamd_download", "plbamd_initialize_at_equilibrium",
     "plbamd_collide_and_stream", "plbamd_collide_and_stream_timed", "plbamd_toggle_statistics",
     "plbamd_get_statistics", "plbamd_compute_macroscopic", "plbamd_reduce_moments",
-    "plbamd_adjoint_gradient", "plbamd_get_time",
+    "plbamd_adjoint_gradient", "plbamd_get_time", "plbamd_save_binary_block", "plbamd_load_binary_block",
 )
 
 
@@ -110,6 +110,8 @@ def lib():
         "plbamd_reduce_moments": [i32, pbox, vp],
         "plbamd_get_time": [i32, ctypes.POINTER(ctypes.c_int64)],
         "plbamd_adjoint_gradient": [i32, i32, i32, ctypes.POINTER(Objective), vp, vp, vp, vp],
+        "plbamd_save_binary_block": [i32, ctypes.c_char_p, i32, i32, i32],
+        "plbamd_load_binary_block": [i32, ctypes.c_char_p, i32, i32, i32],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

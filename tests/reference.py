@@ -61,3 +61,14 @@ def run(case, init_f64, dumps, bcstyle="dynamics", blocks=None, timeout=1800):
                                 "avg_rho": float(v[3]), "avg_energy": float(v[4]),
                                 "force": np.array([float(v[5]), float(v[6]), float(v[7])])}
         return pops, stats
+
+
+def checkpoint(mode, T, forced, shape, src, dst, enforce_uint=False, timeout=300):
+    """Runs the reference's saveBinaryBlock ("save": cell records -> file) or
+    loadBinaryBlock ("load": file -> cell records) through ref_driver."""
+    nx, ny, nz = shape
+    cmd = [DRIVER, "--checkpoint", mode, T, "forced" if forced else "plain", str(nx), str(ny), str(nz), src, dst,
+           "uint" if enforce_uint else "size"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"ref_driver --checkpoint failed ({r.returncode}): {r.stderr[-2000:]}")
