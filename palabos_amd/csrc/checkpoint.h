@@ -10,9 +10,8 @@
 // (core/cell.hh:103-111: the 3 force components for ForcedD3Q19Descriptor).
 // The base64 alphabet and padding are RFC 4648's (io/base64.hh:36-38,104-114).
 //
-// Both conversions are byte streams at HBM speed: a thread takes 12 bytes
-// (three 32-bit words) and 16 characters (one 128-bit word); the last partial
-// group is one thread's.
+// Both conversions are byte streams at HBM speed: a thread takes 48 bytes and
+// 64 characters in 128-bit accesses; the last partial group is one thread's.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -36,37 +35,52 @@ __host__ __device__ inline uint32_t b64_value(uint32_t c) {
 }
 
 // Four characters of three bytes b0 b1 b2, packed little-endian in a word.
+// The four 6-bit values sit one per byte and are mapped to the alphabet in
+// one pass over the word: character = v + 'A', + 6 from 26 on ('a'), - 75
+// from 52 on ('0'), - 15 from 62 on ('+'), + 3 from 63 on ('/'); every byte
+// stays within 0..137 (no carry or borrow between bytes).
 __host__ __device__ inline uint32_t b64_quad(uint32_t b0, uint32_t b1, uint32_t b2) {
     const uint32_t g = (b0 << 16) | (b1 << 8) | b2;
-    return (uint32_t)(uint8_t)b64_char(g >> 18) | ((uint32_t)(uint8_t)b64_char((g >> 12) & 63) << 8) |
-           ((uint32_t)(uint8_t)b64_char((g >> 6) & 63) << 16) | ((uint32_t)(uint8_t)b64_char(g & 63) << 24);
+    const uint32_t v = (g >> 18) | (((g >> 12) & 63u) << 8) | (((g >> 6) & 63u) << 16) | ((g & 63u) << 24);
+    constexpr uint32_t ones = 0x01010101u, high = 0x80808080u;
+    const uint32_t ge26 = ((v + (128u - 26u) * ones) & high) >> 7;
+    const uint32_t ge52 = ((v + (128u - 52u) * ones) & high) >> 7;
+    const uint32_t ge62 = ((v + (128u - 62u) * ones) & high) >> 7;
+    const uint32_t ge63 = ((v + (128u - 63u) * ones) & high) >> 7;
+    return (v + 65u * ones + 6u * ge26 + 3u * ge63) - (75u * ge52 + 15u * ge62);
 }
 
-// in: n bytes (4-byte aligned), out: 4 * ceil(n / 3) characters (16-byte aligned).
+// Bytes per thread: 48 (three 128-bit loads) to 64 characters (four 128-bit
+// stores); the last n % 48 bytes are one thread's.
+constexpr uint64_t kB64Bytes = 48, kB64Chars = 64;
+
+// in: n bytes (16-byte aligned), out: 4 * ceil(n / 3) characters (16-byte aligned).
 __global__ void __launch_bounds__(256) base64_encode(const uint8_t* __restrict__ in, uint64_t n, char* __restrict__ out) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t full = n / 12;
+    const uint64_t full = n / kB64Bytes;
     if (t < full) {
-        const uint32_t* w = reinterpret_cast<const uint32_t*>(in + 12 * t);
-        const uint32_t w0 = w[0], w1 = w[1], w2 = w[2];
-        uint32_t b[12];
+        const uint4* w = reinterpret_cast<const uint4*>(in + kB64Bytes * t);
+        const uint4 a = w[0], b = w[1], c = w[2];
+        const uint32_t words[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+        uint32_t q[16];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            b[k] = (w0 >> (8 * k)) & 255u;
-            b[4 + k] = (w1 >> (8 * k)) & 255u;
-            b[8 + k] = (w2 >> (8 * k)) & 255u;
+        for (int g = 0; g < 16; ++g) {
+            uint32_t by[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int pos = 3 * g + k;
+                by[k] = (words[pos / 4] >> (8 * (pos % 4))) & 255u;
+            }
+            q[g] = b64_quad(by[0], by[1], by[2]);
         }
-        uint4 q;
-        q.x = b64_quad(b[0], b[1], b[2]);
-        q.y = b64_quad(b[3], b[4], b[5]);
-        q.z = b64_quad(b[6], b[7], b[8]);
-        q.w = b64_quad(b[9], b[10], b[11]);
-        *reinterpret_cast<uint4*>(out + 16 * t) = q;
-    } else if (t == full && n % 12 != 0) {
-        // the last 1..11 bytes: whole groups, then a padded one
-        const uint64_t base = 12 * full;
+        uint4* o = reinterpret_cast<uint4*>(out + kB64Chars * t);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = make_uint4(q[4 * k], q[4 * k + 1], q[4 * k + 2], q[4 * k + 3]);
+    } else if (t == full && n % kB64Bytes != 0) {
+        // the last 1..47 bytes: whole groups, then a padded one
+        const uint64_t base = kB64Bytes * full;
         const uint32_t rem = (uint32_t)(n - base);
-        char* o = out + 16 * full;
+        char* o = out + kB64Chars * full;
         uint32_t k = 0;
         for (; k + 3 <= rem; k += 3) {
             const uint32_t q = b64_quad(in[base + k], in[base + k + 1], in[base + k + 2]);
@@ -84,40 +98,51 @@ __global__ void __launch_bounds__(256) base64_encode(const uint8_t* __restrict__
     }
 }
 
-// in: 4 * ceil(n / 3) characters (16-byte aligned), out: n bytes (4-byte
+// in: 4 * ceil(n / 3) characters (16-byte aligned), out: n bytes (16-byte
 // aligned). A character outside the alphabet where a value is expected sets
 // *bad (padding is accepted only where the byte count ends).
 __global__ void __launch_bounds__(256) base64_decode(const char* __restrict__ in, uint64_t n, uint8_t* __restrict__ out,
                                                      int* bad) {
+    // character -> value table (64: not in the alphabet)
+    __shared__ uint8_t lut[256];
+    for (uint32_t c = threadIdx.x; c < 256; c += blockDim.x) lut[c] = (uint8_t)b64_value(c);
+    __syncthreads();
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t full = n / 12;
+    const uint64_t full = n / kB64Bytes;
     if (t < full) {
-        const uint4 q = *reinterpret_cast<const uint4*>(in + 16 * t);
-        const uint32_t qs[4] = {q.x, q.y, q.z, q.w};
-        uint32_t bytes[12];
+        const uint4* w = reinterpret_cast<const uint4*>(in + kB64Chars * t);
+        uint32_t qs[16];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint4 v = w[k];
+            qs[4 * k] = v.x; qs[4 * k + 1] = v.y; qs[4 * k + 2] = v.z; qs[4 * k + 3] = v.w;
+        }
+        uint32_t words[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         uint32_t invalid = 0;
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
+        for (int g = 0; g < 16; ++g) {
             uint32_t v[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                v[c] = b64_value((qs[g] >> (8 * c)) & 255u);
+                v[c] = lut[(qs[g] >> (8 * c)) & 255u];
                 invalid |= v[c] >> 6;
             }
             const uint32_t x = ((v[0] & 63) << 18) | ((v[1] & 63) << 12) | ((v[2] & 63) << 6) | (v[3] & 63);
-            bytes[3 * g] = x >> 16;
-            bytes[3 * g + 1] = (x >> 8) & 255u;
-            bytes[3 * g + 2] = x & 255u;
-        }
-        uint32_t* w = reinterpret_cast<uint32_t*>(out + 12 * t);
+            const uint32_t by[3] = {x >> 16, (x >> 8) & 255u, x & 255u};
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            w[k] = bytes[4 * k] | (bytes[4 * k + 1] << 8) | (bytes[4 * k + 2] << 16) | (bytes[4 * k + 3] << 24);
+            for (int k = 0; k < 3; ++k) {
+                const int pos = 3 * g + k;
+                words[pos / 4] |= by[k] << (8 * (pos % 4));
+            }
+        }
+        uint4* o = reinterpret_cast<uint4*>(out + kB64Bytes * t);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) o[k] = make_uint4(words[4 * k], words[4 * k + 1], words[4 * k + 2], words[4 * k + 3]);
         if (invalid) atomicOr(bad, 1);
-    } else if (t == full && n % 12 != 0) {
-        const uint64_t base = 12 * full;
+    } else if (t == full && n % kB64Bytes != 0) {
+        const uint64_t base = kB64Bytes * full;
         const uint32_t rem = (uint32_t)(n - base);
-        const char* s = in + 16 * full;
+        const char* s = in + kB64Chars * full;
         uint32_t invalid = 0;
         for (uint32_t k = 0; k < rem; k += 3, s += 4) {
             const uint32_t take = rem - k < 3 ? rem - k : 3;   // bytes of this group

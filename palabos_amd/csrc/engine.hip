@@ -902,13 +902,13 @@ int checkpointTyped(Lattice& L, const char* path, int elem, bool force, bool enf
     const uint64_t nChars = (dataBytes + 2) / 3 * 4;
     if (enforceUint && dataBytes > 0xffffffffull)
         return fail(PLBAMD_EINVAL, "binary block: the data does not fit a 32-bit size header (enforce_uint)");
-    if (dataBytes / 12 + 1 > (uint64_t)INT32_MAX * 256)
+    if (dataBytes / kB64Bytes + 1 > (uint64_t)INT32_MAX * 256)
         return fail(PLBAMD_EINVAL, "binary block: lattice too large for one conversion launch");
     PLBAMD_TRY(rebuild(L));
     DeviceBuffers buf;
     T* pops = static_cast<T*>(buf.get(cells * 19 * sizeof(T)));
     T* rec = force ? static_cast<T*>(buf.get(dataBytes)) : pops;
-    char* text = static_cast<char*>(buf.get((nChars + 15) / 16 * 16));
+    char* text = static_cast<char*>(buf.get((nChars + 63) / 64 * 64));
     int* bad = static_cast<int*>(buf.get(2 * sizeof(int)));
     if (!pops || !rec || !text || !bad) return fail(PLBAMD_ENOMEM, "binary block: cannot allocate device buffers");
     const T fx = (T)L.desc.force[0], fy = (T)L.desc.force[1], fz = (T)L.desc.force[2];
@@ -917,7 +917,7 @@ int checkpointTyped(Lattice& L, const char* path, int elem, bool force, bool enf
         PLBAMD_HIP(hipMemsetAsync(pops, 0, cells * 19 * sizeof(T), L.stream));
         PLBAMD_HIP(transferDevice<S>(L, whole, elem, pops, false));
         if (force) hipLaunchKernelGGL((append_force<T>), itemsGrid(cells), dim3(256), 0, L.stream, pops, rec, cells, fx, fy, fz);
-        hipLaunchKernelGGL(base64_encode, itemsGrid(dataBytes / 12 + 1), dim3(256), 0, L.stream,
+        hipLaunchKernelGGL(base64_encode, itemsGrid(dataBytes / kB64Bytes + 1), dim3(256), 0, L.stream,
                            reinterpret_cast<const uint8_t*>(rec), dataBytes, text);
         PLBAMD_HIP(hipGetLastError());
         std::vector<char> host(nChars);
@@ -940,13 +940,19 @@ int checkpointTyped(Lattice& L, const char* path, int elem, bool force, bool enf
     {
         FILE* fp = std::fopen(path, "rb");
         if (!fp) return fail(PLBAMD_EINVAL, std::string("load_binary_block: cannot open ") + path);
+        if (std::fseek(fp, 0, SEEK_END) == 0) {
+            const long size = std::ftell(fp);
+            if (size > 0) file.reserve((size_t)size);
+            std::rewind(fp);
+        }
         char chunk[1 << 16];
         size_t got;
         while ((got = std::fread(chunk, 1, sizeof(chunk), fp)) > 0) file.append(chunk, got);
         std::fclose(fp);
     }
-    // the reference reads with operator>>, which skips white space
-    if (file.find_first_of(" \t\r\n\v\f") != std::string::npos)
+    // the reference reads with operator>>, which skips white space (every
+    // white-space character is at most ' '; every base64 character above it)
+    if (std::any_of(file.begin(), file.end(), [](char c) { return (unsigned char)c <= ' '; }))
         file.erase(std::remove_if(file.begin(), file.end(), [](char c) { return std::isspace((unsigned char)c) != 0; }),
                    file.end());
     const size_t headChars = enforceUint ? 8 : 12;
@@ -962,7 +968,7 @@ int checkpointTyped(Lattice& L, const char* path, int elem, bool force, bool enf
     if (file.size() - headChars < nChars) return fail(PLBAMD_EINVAL, "load_binary_block: the file is truncated");
     PLBAMD_HIP(hipMemsetAsync(bad, 0, 2 * sizeof(int), L.stream));
     PLBAMD_HIP(hipMemcpyAsync(text, file.data() + headChars, nChars, hipMemcpyHostToDevice, L.stream));
-    hipLaunchKernelGGL(base64_decode, itemsGrid(dataBytes / 12 + 1), dim3(256), 0, L.stream, text, dataBytes,
+    hipLaunchKernelGGL(base64_decode, itemsGrid(dataBytes / kB64Bytes + 1), dim3(256), 0, L.stream, text, dataBytes,
                        reinterpret_cast<uint8_t*>(rec), bad);
     if (force) hipLaunchKernelGGL((split_force<T>), itemsGrid(cells), dim3(256), 0, L.stream, rec, pops, cells, fx, fy, fz, bad + 1);
     PLBAMD_HIP(hipGetLastError());

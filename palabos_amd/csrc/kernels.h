@@ -1288,47 +1288,86 @@ __device__ __forceinline__ BoxCell box_cell(const BoxArgs& b, uint64_t k) {
     for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n_;                  \
          k += (uint64_t)gridDim.x * blockDim.x)
 
-template<typename S, typename H>
-__global__ void upload_box(void* arena, BoxArgs b, const H* host) {
+// Transfers between the arena (slot-major) and a staging buffer of cell
+// records ([cell][19], C order): 256 cells per workgroup pass through LDS, so
+// that the staging side is read or written as consecutive elements (a lane per
+// element) and the arena side as slots (a lane per cell). H is the staging
+// element (FP64 / FP32 values, or uint16 raw bits when RAW).
+template<typename S, typename H, bool RAW>
+__device__ __forceinline__ void upload_box_t(void* arena, BoxArgs b, const H* host) {
     using T = typename Store<S>::T;
+    __shared__ H tile[256 * 19];
+    __shared__ uint64_t stg[256];
     S* f = reinterpret_cast<S*>(arena) + b.g.offset;
-    PLBAMD_BOX_LOOP(b) {
-        const BoxCell c = box_cell(b, k);
-        const uint32_t wall = b.parity ? wall_mask_of(b, c.cell) : 0;
-        for (int i = 0; i < 19; ++i)
-            Store<S>::store(f + pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell), (T)host[c.staged * 19 + i]);
+    const uint64_t n = (uint64_t)b.ilx * b.ily * b.ilz;
+    for (uint64_t k0 = (uint64_t)blockIdx.x * 256; k0 < n; k0 += (uint64_t)gridDim.x * 256) {
+        const uint32_t cnt = (uint32_t)min((uint64_t)256, n - k0);
+        const uint64_t k = k0 + threadIdx.x;
+        BoxCell c{};
+        if (threadIdx.x < cnt) {
+            c = box_cell(b, k);
+            stg[threadIdx.x] = c.staged;
+        }
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < cnt * 19; e += 256) {
+            const uint32_t j = e / 19;
+            tile[e] = host[stg[j] * 19 + (e - j * 19)];
+        }
+        __syncthreads();
+        if (threadIdx.x < cnt) {
+            const uint32_t wall = b.parity ? wall_mask_of(b, c.cell) : 0;
+            for (int i = 0; i < 19; ++i) {
+                const uint64_t at = pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell);
+                if constexpr (RAW) reinterpret_cast<uint16_t*>(f)[at] = (uint16_t)tile[threadIdx.x * 19 + i];
+                else Store<S>::store(f + at, (T)tile[threadIdx.x * 19 + i]);
+            }
+        }
+        __syncthreads();
     }
 }
 
-template<typename S>
-__global__ void upload_box_raw(void* arena, BoxArgs b, const uint16_t* host) {
-    uint16_t* f = reinterpret_cast<uint16_t*>(reinterpret_cast<S*>(arena) + b.g.offset);
-    PLBAMD_BOX_LOOP(b) {
-        const BoxCell c = box_cell(b, k);
-        const uint32_t wall = b.parity ? wall_mask_of(b, c.cell) : 0;
-        for (int i = 0; i < 19; ++i) f[pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell)] = host[c.staged * 19 + i];
+template<typename S, typename H, bool RAW>
+__device__ __forceinline__ void download_box_t(const void* arena, BoxArgs b, H* host) {
+    __shared__ H tile[256 * 19];
+    __shared__ uint64_t stg[256];
+    const S* f = reinterpret_cast<const S*>(arena) + b.g.offset;
+    const uint64_t n = (uint64_t)b.ilx * b.ily * b.ilz;
+    for (uint64_t k0 = (uint64_t)blockIdx.x * 256; k0 < n; k0 += (uint64_t)gridDim.x * 256) {
+        const uint32_t cnt = (uint32_t)min((uint64_t)256, n - k0);
+        if (threadIdx.x < cnt) {
+            const BoxCell c = box_cell(b, k0 + threadIdx.x);
+            stg[threadIdx.x] = c.staged;
+            const uint32_t wall = b.parity ? wall_mask_of(b, c.cell) : 0;
+            for (int i = 0; i < 19; ++i) {
+                const uint64_t at = pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell);
+                if constexpr (RAW) tile[threadIdx.x * 19 + i] = (H)reinterpret_cast<const uint16_t*>(f)[at];
+                else tile[threadIdx.x * 19 + i] = (H)Store<S>::load(f + at);
+            }
+        }
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < cnt * 19; e += 256) {
+            const uint32_t j = e / 19;
+            host[stg[j] * 19 + (e - j * 19)] = tile[e];
+        }
+        __syncthreads();
     }
 }
 
 template<typename S, typename H>
-__global__ void download_box(const void* arena, BoxArgs b, H* host) {
-    const S* f = reinterpret_cast<const S*>(arena) + b.g.offset;
-    PLBAMD_BOX_LOOP(b) {
-        const BoxCell c = box_cell(b, k);
-        const uint32_t wall = b.parity ? wall_mask_of(b, c.cell) : 0;
-        for (int i = 0; i < 19; ++i)
-            host[c.staged * 19 + i] = (H)Store<S>::load(f + pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell));
-    }
+__global__ void __launch_bounds__(256) upload_box(void* arena, BoxArgs b, const H* host) {
+    upload_box_t<S, H, false>(arena, b, host);
 }
-
 template<typename S>
-__global__ void download_box_raw(const void* arena, BoxArgs b, uint16_t* host) {
-    const uint16_t* f = reinterpret_cast<const uint16_t*>(reinterpret_cast<const S*>(arena) + b.g.offset);
-    PLBAMD_BOX_LOOP(b) {
-        const BoxCell c = box_cell(b, k);
-        const uint32_t wall = b.parity ? wall_mask_of(b, c.cell) : 0;
-        for (int i = 0; i < 19; ++i) host[c.staged * 19 + i] = f[pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell)];
-    }
+__global__ void __launch_bounds__(256) upload_box_raw(void* arena, BoxArgs b, const uint16_t* host) {
+    upload_box_t<S, uint16_t, true>(arena, b, host);
+}
+template<typename S, typename H>
+__global__ void __launch_bounds__(256) download_box(const void* arena, BoxArgs b, H* host) {
+    download_box_t<S, H, false>(arena, b, host);
+}
+template<typename S>
+__global__ void __launch_bounds__(256) download_box_raw(const void* arena, BoxArgs b, uint16_t* host) {
+    download_box_t<S, uint16_t, true>(arena, b, host);
 }
 
 // Completed snapshot of wrapped-factory plane cells: the populations the

@@ -10,6 +10,7 @@ native library; the host only builds the geometry, reads back reductions and
 velocity slabs, and does the FP64 bookkeeping of the error norms.
 """
 import argparse
+import os
 import json
 import math
 import time
@@ -286,9 +287,42 @@ def porous_adjoint(shape=(384, 256, 256), count=400, r_range=(6.0, 14.0), seed=5
     return out
 
 
+def checkpoint_throughput(n=256, storage="f32", T="f32", directory=None):
+    """Saves an n^3 lattice as a Palabos checkpoint file and loads it back into
+    a second lattice (wall-clock times of both calls, file size, and that the
+    populations came back unchanged). The device kernels' own durations come
+    from a kernel trace of this run."""
+    import tempfile
+    rng = np.random.RandomState(2)
+    shape = (n, n, n)
+    a = _lat.Block3D(*shape, _lat.D3Q19, _lat.BGK(1.7), storage=storage, periodic=True)
+    b = _lat.Block3D(*shape, _lat.D3Q19, _lat.BGK(1.7), storage=storage, periodic=True)
+    try:
+        for x0 in range(0, n, 32):   # seeded state, uploaded in slabs
+            box = (x0, min(n, x0 + 32) - 1, 0, n - 1, 0, n - 1)
+            a.setPopulations((rng.standard_normal((box[1] - x0 + 1, n, n, 19)) * 0.02).astype(np.float32), box)
+        with tempfile.TemporaryDirectory(dir=directory) as tmp:
+            path = os.path.join(tmp, "lattice.dat")
+            t0 = time.perf_counter()
+            a.saveBinaryBlock(path, T)
+            t1 = time.perf_counter()
+            b.loadBinaryBlock(path, T)
+            t2 = time.perf_counter()
+            size = os.path.getsize(path)
+        same = all(np.array_equal(a.getPopulations((x0, min(n, x0 + 32) - 1, 0, n - 1, 0, n - 1)),
+                                  b.getPopulations((x0, min(n, x0 + 32) - 1, 0, n - 1, 0, n - 1)))
+                   for x0 in range(0, n, 32))
+    finally:
+        a.close()
+        b.close()
+    data = n ** 3 * 19 * (8 if T == "f64" else 4)
+    return {"shape": list(shape), "storage": storage, "file_element": T, "cell_bytes": data, "file_bytes": size,
+            "save_s": t1 - t0, "load_s": t2 - t1, "identical_after_load": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["taylor_green", "sphere_drag", "porous", "porous_adjoint"])
+    ap.add_argument("study", choices=["taylor_green", "sphere_drag", "porous", "porous_adjoint", "checkpoint"])
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--storage", default=None)
@@ -302,6 +336,8 @@ def main():
         r = sphere_drag(steps=args.steps or 5000, storage=args.storage or "f64")
     elif args.study == "porous":
         r = porous_permeability(steps=args.steps or 20000, storage=args.storage or "f16", scale=args.scale)
+    elif args.study == "checkpoint":
+        r = checkpoint_throughput(n=args.n if args.n != 512 else 256, storage=args.storage or "f32")
     else:
         r = porous_adjoint(steps=args.steps or 2000, storage=args.storage or "f16")
     print(json.dumps(r))
