@@ -212,6 +212,24 @@ int plbamd_compute_macroscopic(int handle, const plbamd_box* box, double* rho, d
  * (dataProcessors/dataAnalysisWrapper3D.hh:1226-1257). */
 int plbamd_reduce_moments(int handle, const plbamd_box* box, double* out);
 
+/* Density parameter of the solid dynamics of tag PLBAMD_TAG_BOUNCE_BACK,
+ * PLBAMD_TAG_NO_DYNAMICS or PLBAMD_TAG_MOMENTUM_EXCHANGE (the rho of
+ * BounceBack(rho), NoDynamics(rho), MomentumExchangeBounceBack(forceIds, rho);
+ * core/dynamics.h:574,687, bounceBackModels.h:62): what the analysis functions
+ * report for those cells (computeDensity = rho; computeRhoBar = rho - 1, 0 for
+ * NoDynamics). One value per kind; 1 until set. The collision does not use it. */
+int plbamd_set_solid_density(int handle, int tag, double rho);
+
+/* Analysis fields of `box` as Palabos's data-analysis functions compute them
+ * (dataProcessors/dataAnalysisWrapper3D.h:695-777; per cell the cell's
+ * dynamics decides, e.g. BounceBack reports rho = 1 and u = 0, a velocity
+ * plane its imposed velocity): out[cell][7] (box cells in C order) =
+ * computeDensity, computeRhoBar, computeVelocity (x, y, z),
+ * computeKineticEnergy (|u|^2 / 2), computeVelocityNorm, each computed in the
+ * lattice's arithmetic type (FP64, or FP32 for FP32 and 16-bit storage) and
+ * widened exactly to double. */
+int plbamd_compute_fields(int handle, const plbamd_box* box, double* out);
+
 /* Objective of plbamd_adjoint_gradient:
  *   J = 1/(T - t0) sum_{t=t0+1..T} [ weight_pressure_drop * cs2 * (<rho>_{x=inlet_x} - <rho>_{x=outlet_x})
  *                                  + weight_drag * omega * sum_cells alpha * j_x

@@ -22,8 +22,11 @@
 //   block x0 x1 y0 y1 z0 z1           optional; any number -> SparseBlockStructure3D multi-block
 //   dump s1 s2 ...                    steps after which populations are written (0 = after initialize)
 //   force fx fy fz                    optional; ForcedD3Q19 + GuoExternalForceBGKdynamics (bgk, no planes)
+//   fields 1                          optional; also write the analysis fields at every dump
 //   out prefix
-// Outputs: prefix_s<step>.bin (T, [nx][ny][nz][19]) and prefix_stats.txt lines
+// Outputs: prefix_s<step>.bin (T, [nx][ny][nz][19]) and prefix_stats.txt lines;
+// with "fields", prefix_fields_s<step>.bin (T, [nx][ny][nz][7]: computeDensity,
+// computeRhoBar, computeVelocity x y z, computeKineticEnergy, computeVelocityNorm)
 //   step storedAvgRho storedAvgEnergy avgRho(computeAverageDensity) avgEnergy force_x force_y force_z
 //
 // Bench mode: ref_driver --bench T n steps  -> periodic BGK(1.8) n^3 lattice, prints MLUPS.
@@ -96,6 +99,7 @@ struct CaseSpec {
     std::vector<plint> dumps;
     bool forced = false;
     double force[3] = {0, 0, 0};
+    bool fields = false;
 };
 
 std::vector<char> readFile(std::string const& name) {
@@ -133,6 +137,10 @@ CaseSpec parseCase(std::string const& name) {
             Box3D b;
             s >> b.x0 >> b.x1 >> b.y0 >> b.y1 >> b.z0 >> b.z1;
             c.blocks.push_back(b);
+        } else if (key == "fields") {
+            int v = 0;
+            s >> v;
+            c.fields = v != 0;
         } else if (key == "dump") {
             plint v;
             while (s >> v) c.dumps.push_back(v);
@@ -291,6 +299,26 @@ int runCase(CaseSpec const& c) {
         FILE* o = std::fopen(name, "wb");
         std::fwrite(&buf[0], sizeof(T), buf.size(), o);
         std::fclose(o);
+        if (c.fields) {
+            std::auto_ptr<MultiScalarField3D<T> > rho = computeDensity(lat);
+            std::auto_ptr<MultiScalarField3D<T> > rhoBar = computeRhoBar(lat);
+            std::auto_ptr<MultiTensorField3D<T,3> > u = computeVelocity(lat);
+            std::auto_ptr<MultiScalarField3D<T> > energy = computeKineticEnergy(lat);
+            std::auto_ptr<MultiScalarField3D<T> > norm = computeVelocityNorm(lat);
+            std::vector<T> fb((size_t)(c.nx*c.ny*c.nz*7));
+            size_t k = 0;
+            for (plint x = 0; x < c.nx; ++x) for (plint y = 0; y < c.ny; ++y) for (plint z = 0; z < c.nz; ++z) {
+                fb[k++] = rho->get(x, y, z);
+                fb[k++] = rhoBar->get(x, y, z);
+                for (int a = 0; a < 3; ++a) fb[k++] = u->get(x, y, z)[a];
+                fb[k++] = energy->get(x, y, z);
+                fb[k++] = norm->get(x, y, z);
+            }
+            std::snprintf(name, sizeof(name), "%s_fields_s%ld.bin", c.out.c_str(), (long)step);
+            FILE* fo = std::fopen(name, "wb");
+            std::fwrite(&fb[0], sizeof(T), fb.size(), fo);
+            std::fclose(fo);
+        }
         double fx = 0, fy = 0, fz = 0;
         if (haveMomentumExchange && step > 0) {
             fx = lat.getInternalStatistics().getSum(forceIds[0]);

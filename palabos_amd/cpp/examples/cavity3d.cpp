@@ -42,10 +42,18 @@ int run(plint n, plint steps, const char* out, bool wrapped) {
     std::fclose(fo);
     Array<T, 3> u;
     lattice.get(n / 2, n / 2, n / 2).computeVelocity(u);
+    // analysis fields (dataAnalysisWrapper3D.h): kinetic energy and velocity norm
+    // next to the lid's centre
+    std::unique_ptr<MultiScalarField3D<T>> energy = computeKineticEnergy(lattice);
+    std::unique_ptr<MultiScalarField3D<T>> norm = computeVelocityNorm(lattice);
+    std::unique_ptr<MultiTensorField3D<T, 3>> velocity = computeVelocity(lattice);
+    const plint c = n / 2, top = n - 2;
     std::printf("{\"avg_density\": %.17g, \"avg_energy\": %.17g, \"stored_density\": %.17g, "
-                "\"stored_energy\": %.17g, \"center_ux\": %.17g, \"steps\": %ld}\n",
+                "\"stored_energy\": %.17g, \"center_ux\": %.17g, \"steps\": %ld, "
+                "\"energy_below_lid\": %.17g, \"norm_below_lid\": %.17g, \"ux_below_lid\": %.17g}\n",
                 (double)computeAverageDensity(lattice), (double)computeAverageEnergy(lattice),
-                getStoredAverageDensity<T>(lattice), getStoredAverageEnergy<T>(lattice), (double)u[0], (long)steps);
+                getStoredAverageDensity<T>(lattice), getStoredAverageEnergy<T>(lattice), (double)u[0], (long)steps,
+                (double)energy->get(c, c, top), (double)norm->get(c, c, top), (double)velocity->get(c, c, top)[0]);
     delete bc;
     return 0;
 }

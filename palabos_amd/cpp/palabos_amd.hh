@@ -160,6 +160,8 @@ public:
     virtual T getOmega() const { return T(); }
     virtual void setOmega(T) {}
     virtual int getId() const { return (int)kind(); }
+    // density parameter of the solid dynamics (reported by the analysis functions)
+    virtual T getDensityParameter() const { return (T)1; }
     // bgk_ma2_equilibrium (dynamicsTemplates3D.h:1994-1997)
     virtual T computeEquilibrium(plint iPop, T rhoBar, Array<T, 3> const& j, T jSqr, T = T()) const {
         typedef Descriptor<T> D;
@@ -206,9 +208,11 @@ public:
 template<typename T, template<typename U> class Descriptor>
 class BounceBack : public Dynamics<T, Descriptor> {
 public:
-    explicit BounceBack(T rho_ = (T)1) : rho(rho_) {}
+    // core/dynamics.h:574: rho defaults to T()
+    explicit BounceBack(T rho_ = T()) : rho(rho_) {}
     BounceBack* clone() const override { return new BounceBack(*this); }
     DynKind kind() const override { return DynKind::BounceBack; }
+    T getDensityParameter() const override { return rho; }
 private:
     T rho;
 };
@@ -216,9 +220,11 @@ private:
 template<typename T, template<typename U> class Descriptor>
 class NoDynamics : public Dynamics<T, Descriptor> {
 public:
-    explicit NoDynamics(T rho_ = (T)1) : rho(rho_) {}
+    // core/dynamics.h:687
+    explicit NoDynamics(T rho_ = T()) : rho(rho_) {}
     NoDynamics* clone() const override { return new NoDynamics(*this); }
     DynKind kind() const override { return DynKind::NoDynamics; }
+    T getDensityParameter() const override { return rho; }
 private:
     T rho;
 };
@@ -226,9 +232,11 @@ private:
 template<typename T, template<typename U> class Descriptor>
 class MomentumExchangeBounceBack : public Dynamics<T, Descriptor> {
 public:
-    explicit MomentumExchangeBounceBack(Array<plint, 3> forceIds_, T rho_ = (T)1) : forceIds(forceIds_), rho(rho_) {}
+    // boundaryCondition/bounceBackModels.h:62
+    explicit MomentumExchangeBounceBack(Array<plint, 3> forceIds_, T rho_ = T()) : forceIds(forceIds_), rho(rho_) {}
     MomentumExchangeBounceBack* clone() const override { return new MomentumExchangeBounceBack(*this); }
     DynKind kind() const override { return DynKind::MomentumExchange; }
+    T getDensityParameter() const override { return rho; }
 private:
     Array<plint, 3> forceIds;
     T rho;
@@ -402,6 +410,16 @@ public:
             planesDirty = true;
         }
     }
+    // One density parameter per solid kind (the engine reports it for every
+    // cell of that kind); two different ones for one kind are refused.
+    void setSolidDensity(int tag, double rho) {
+        const int k = tag - PLBAMD_TAG_BOUNCE_BACK;
+        if (solidRhoSet[k] && solidRho[k] != rho)
+            throw PlbException("defineDynamics: one density parameter per solid dynamics kind");
+        solidRho[k] = rho;
+        solidRhoSet[k] = true;
+        tagsDirty = true;
+    }
     void setForce(Array<T, 3> const& F) {
         if (handle > 0) throw PlbException("the body force must be set before the lattice is first used");
         force[0] = F[0]; force[1] = F[1]; force[2] = F[2];
@@ -428,6 +446,14 @@ public:
                                                enforceUint ? 1 : 0),
                       "load_binary_block");
         hostValid = false;
+    }
+    // Analysis fields of a box: [cell][7] (density, rhoBar, u, energy, |u|).
+    void analysisFields(Box3D const& b, std::vector<double>& out) {
+        materialize();
+        pushHost();
+        out.resize((size_t)b.nCells() * 7);
+        plbamd_box cb = b.toC();
+        detail::check(plbamd_compute_fields(handle, &cb, out.data()), "compute_fields");
     }
     void moments(Box3D const& b, double* sums) {
         materialize();
@@ -472,6 +498,10 @@ private:
         }
         if (tagsDirty) {
             detail::check(plbamd_set_tags(handle, tags.data()), "set_tags");
+            for (int k = 0; k < 3; ++k)
+                if (solidRhoSet[k])
+                    detail::check(plbamd_set_solid_density(handle, PLBAMD_TAG_BOUNCE_BACK + k, solidRho[k]),
+                                  "set_solid_density");
             tagsDirty = false;
         }
         if (planesDirty) {
@@ -512,6 +542,8 @@ private:
     StatSubscriber subscriber;
     BlockStatistics stats;
     std::vector<uint8_t> tags;
+    double solidRho[3] = {1, 1, 1};
+    bool solidRhoSet[3] = {false, false, false};
     std::vector<Plane> planes;
     double force[3] = {0, 0, 0};
     int handle = 0;
@@ -620,6 +652,14 @@ void defineDynamics(MultiBlockLattice3D<T, Descriptor>& lattice, Box3D domain, D
         }
         tag = PLBAMD_TAG_BULK;
     }
+    if (tag >= PLBAMD_TAG_BOUNCE_BACK && tag <= PLBAMD_TAG_MOMENTUM_EXCHANGE) {
+        try {
+            lattice.setSolidDensity(tag, (double)dynamics->getDensityParameter());
+        } catch (...) {
+            delete dynamics;
+            throw;
+        }
+    }
     lattice.setTag(domain, tag);
     delete dynamics;  // defineDynamics takes ownership of the prototype
 }
@@ -718,6 +758,108 @@ T computeAverageEnergy(MultiBlockLattice3D<T, Descriptor>& lattice, Box3D domain
 template<typename T, template<typename U> class Descriptor>
 T computeAverageEnergy(MultiBlockLattice3D<T, Descriptor>& lattice) {
     return computeAverageEnergy(lattice, lattice.getBoundingBox());
+}
+
+// Scalar and vector fields on a box (the multiScalarField3D.h /
+// multiTensorField3D.h role for the analysis functions below): values in C
+// order over the box, get() in the box's own coordinates from 0.
+template<typename T>
+class MultiScalarField3D {
+public:
+    explicit MultiScalarField3D(Box3D const& b) : box(b), data((size_t)b.nCells()) {}
+    plint getNx() const { return box.getNx(); }
+    plint getNy() const { return box.getNy(); }
+    plint getNz() const { return box.getNz(); }
+    Box3D getBoundingBox() const { return Box3D(0, getNx() - 1, 0, getNy() - 1, 0, getNz() - 1); }
+    T& get(plint x, plint y, plint z) { return data[(size_t)((x * getNy() + y) * getNz() + z)]; }
+    T const& get(plint x, plint y, plint z) const { return data[(size_t)((x * getNy() + y) * getNz() + z)]; }
+private:
+    Box3D box;
+    std::vector<T> data;
+};
+template<typename T, int nDim>
+class MultiTensorField3D {
+public:
+    explicit MultiTensorField3D(Box3D const& b) : box(b), data((size_t)b.nCells()) {}
+    plint getNx() const { return box.getNx(); }
+    plint getNy() const { return box.getNy(); }
+    plint getNz() const { return box.getNz(); }
+    Array<T, nDim>& get(plint x, plint y, plint z) { return data[(size_t)((x * getNy() + y) * getNz() + z)]; }
+    Array<T, nDim> const& get(plint x, plint y, plint z) const { return data[(size_t)((x * getNy() + y) * getNz() + z)]; }
+private:
+    Box3D box;
+    std::vector<Array<T, nDim>> data;
+};
+
+namespace detail {
+template<typename T, template<typename U> class Descriptor>
+std::unique_ptr<MultiScalarField3D<T>> analysisScalar(MultiBlockLattice3D<T, Descriptor>& lattice, Box3D domain,
+                                                      int column) {
+    std::vector<double> f;
+    lattice.analysisFields(domain, f);
+    std::unique_ptr<MultiScalarField3D<T>> out(new MultiScalarField3D<T>(domain));
+    size_t k = 0;
+    for (plint x = 0; x < domain.getNx(); ++x)
+        for (plint y = 0; y < domain.getNy(); ++y)
+            for (plint z = 0; z < domain.getNz(); ++z, ++k) out->get(x, y, z) = (T)f[7 * k + (size_t)column];
+    return out;
+}
+}  // namespace detail
+
+// dataProcessors/dataAnalysisWrapper3D.h:695-777 (per cell the cell's dynamics
+// decides; the values are computed on the device in T).
+template<typename T, template<typename U> class Descriptor>
+std::unique_ptr<MultiScalarField3D<T>> computeDensity(MultiBlockLattice3D<T, Descriptor>& lattice, Box3D domain) {
+    return detail::analysisScalar(lattice, domain, 0);
+}
+template<typename T, template<typename U> class Descriptor>
+std::unique_ptr<MultiScalarField3D<T>> computeDensity(MultiBlockLattice3D<T, Descriptor>& lattice) {
+    return computeDensity(lattice, lattice.getBoundingBox());
+}
+template<typename T, template<typename U> class Descriptor>
+std::unique_ptr<MultiScalarField3D<T>> computeRhoBar(MultiBlockLattice3D<T, Descriptor>& lattice, Box3D domain) {
+    return detail::analysisScalar(lattice, domain, 1);
+}
+template<typename T, template<typename U> class Descriptor>
+std::unique_ptr<MultiScalarField3D<T>> computeRhoBar(MultiBlockLattice3D<T, Descriptor>& lattice) {
+    return computeRhoBar(lattice, lattice.getBoundingBox());
+}
+template<typename T, template<typename U> class Descriptor>
+std::unique_ptr<MultiScalarField3D<T>> computeKineticEnergy(MultiBlockLattice3D<T, Descriptor>& lattice, Box3D domain) {
+    return detail::analysisScalar(lattice, domain, 5);
+}
+template<typename T, template<typename U> class Descriptor>
+std::unique_ptr<MultiScalarField3D<T>> computeKineticEnergy(MultiBlockLattice3D<T, Descriptor>& lattice) {
+    return computeKineticEnergy(lattice, lattice.getBoundingBox());
+}
+template<typename T, template<typename U> class Descriptor>
+std::unique_ptr<MultiScalarField3D<T>> computeVelocityNorm(MultiBlockLattice3D<T, Descriptor>& lattice, Box3D domain) {
+    return detail::analysisScalar(lattice, domain, 6);
+}
+template<typename T, template<typename U> class Descriptor>
+std::unique_ptr<MultiScalarField3D<T>> computeVelocityNorm(MultiBlockLattice3D<T, Descriptor>& lattice) {
+    return computeVelocityNorm(lattice, lattice.getBoundingBox());
+}
+template<typename T, template<typename U> class Descriptor>
+std::unique_ptr<MultiScalarField3D<T>> computeVelocityComponent(MultiBlockLattice3D<T, Descriptor>& lattice,
+                                                                Box3D domain, plint iComponent) {
+    return detail::analysisScalar(lattice, domain, 2 + (int)iComponent);
+}
+template<typename T, template<typename U> class Descriptor>
+std::unique_ptr<MultiTensorField3D<T, 3>> computeVelocity(MultiBlockLattice3D<T, Descriptor>& lattice, Box3D domain) {
+    std::vector<double> f;
+    lattice.analysisFields(domain, f);
+    std::unique_ptr<MultiTensorField3D<T, 3>> out(new MultiTensorField3D<T, 3>(domain));
+    size_t k = 0;
+    for (plint x = 0; x < domain.getNx(); ++x)
+        for (plint y = 0; y < domain.getNy(); ++y)
+            for (plint z = 0; z < domain.getNz(); ++z, ++k)
+                for (int a = 0; a < 3; ++a) out->get(x, y, z)[a] = (T)f[7 * k + 2 + (size_t)a];
+    return out;
+}
+template<typename T, template<typename U> class Descriptor>
+std::unique_ptr<MultiTensorField3D<T, 3>> computeVelocity(MultiBlockLattice3D<T, Descriptor>& lattice) {
+    return computeVelocity(lattice, lattice.getBoundingBox());
 }
 
 // io/serializerIO_3D.h:48,58: the file holds the raw f-bar (and, for

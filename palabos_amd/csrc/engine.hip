@@ -79,6 +79,7 @@ struct Lattice {
     std::vector<uint8_t> hostTags;    // PLBAMD_TAG_* per domain cell, or empty
     std::vector<Plane> planes;
     std::vector<double> hostAlpha;    // per domain cell, or empty
+    double solidRho[3] = {1.0, 1.0, 1.0};  // density parameter of BounceBack, NoDynamics, momentum exchange
 
     bool dirty = true;                // device tags / specials / porosity need a rebuild
     bool anyWrapped = false;
@@ -1014,7 +1015,7 @@ int equilibrium(Lattice& L, const plbamd_box& b, const double* rho, const double
 }
 
 template<typename S>
-int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sums) {
+int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sums, double* fields = nullptr) {
     const uint64_t n = boxCells(b);
     PLBAMD_TRY(rebuild(L));
     std::vector<BoxArgs> parts;
@@ -1027,16 +1028,21 @@ int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sum
         }
     }
     const size_t fieldDoubles = (rho || u) ? n * 4 : 0;
+    const size_t analysisDoubles = fields ? n * 7 : 0;
     double* d = nullptr;
-    PLBAMD_HIP(hipMalloc(&d, (fieldDoubles + nRecords * 8 + 8) * sizeof(double)));
+    PLBAMD_HIP(hipMalloc(&d, (fieldDoubles + analysisDoubles + nRecords * 8 + 8) * sizeof(double)));
     double* dRho = rho ? d : nullptr;
     double* dU = u ? d + n : nullptr;
-    double* dPart = d + fieldDoubles;
+    double* dFields = fields ? d + fieldDoubles : nullptr;
+    double* dPart = d + fieldDoubles + analysisDoubles;
     double* dSum = dPart + nRecords * 8;
     MomentsArgs m{};
     m.tags = L.dTags;
     m.values = L.dValues;
     for (int c = 0; c < 3; ++c) m.force[c] = L.desc.force[c];
+    m.forced = L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0;
+    m.fields = dFields;
+    for (int k = 0; k < 3; ++k) m.solidRho[k] = L.solidRho[k];
     hipError_t err = hipSuccess;
     uint64_t rec = 0;
     for (const BoxArgs& a : parts) {
@@ -1053,6 +1059,8 @@ int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sum
     double host[8];
     if (err == hipSuccess && rho) err = hipMemcpyAsync(rho, dRho, n * sizeof(double), hipMemcpyDeviceToHost, L.stream);
     if (err == hipSuccess && u) err = hipMemcpyAsync(u, dU, 3 * n * sizeof(double), hipMemcpyDeviceToHost, L.stream);
+    if (err == hipSuccess && fields)
+        err = hipMemcpyAsync(fields, dFields, 7 * n * sizeof(double), hipMemcpyDeviceToHost, L.stream);
     if (err == hipSuccess) err = hipMemcpyAsync(host, dSum, sizeof(host), hipMemcpyDeviceToHost, L.stream);
     if (err == hipSuccess) err = hipStreamSynchronize(L.stream);
     hipFree(d);
@@ -1607,16 +1615,31 @@ int plbamd_get_statistics(int handle, double* out) {
     return PLBAMD_OK;
 }
 
-static int dispatchMoments(int handle, const plbamd_box* box, double* rho, double* u, double* sums) {
+static int dispatchMoments(int handle, const plbamd_box* box, double* rho, double* u, double* sums,
+                           double* fields = nullptr) {
     Lattice* L = find(handle);
     if (!L) return fail(PLBAMD_EINVAL, "moments: unknown handle");
     if (!box || !boxValid(*L, *box)) return fail(PLBAMD_EINVAL, "moments: box outside the lattice");
     switch (L->desc.storage) {
-    case PLBAMD_F64: return moments<double>(*L, *box, rho, u, sums);
-    case PLBAMD_F32: return moments<float>(*L, *box, rho, u, sums);
-    case PLBAMD_F16: return moments<f16s>(*L, *box, rho, u, sums);
-    default: return moments<bf16s>(*L, *box, rho, u, sums);
+    case PLBAMD_F64: return moments<double>(*L, *box, rho, u, sums, fields);
+    case PLBAMD_F32: return moments<float>(*L, *box, rho, u, sums, fields);
+    case PLBAMD_F16: return moments<f16s>(*L, *box, rho, u, sums, fields);
+    default: return moments<bf16s>(*L, *box, rho, u, sums, fields);
     }
+}
+
+int plbamd_set_solid_density(int handle, int tag, double rho) {
+    Lattice* L = find(handle);
+    if (!L) return fail(PLBAMD_EINVAL, "set_solid_density: unknown handle");
+    if (tag < PLBAMD_TAG_BOUNCE_BACK || tag > PLBAMD_TAG_MOMENTUM_EXCHANGE)
+        return fail(PLBAMD_EINVAL, "set_solid_density: tag must be BounceBack, NoDynamics or momentum exchange");
+    L->solidRho[tag - PLBAMD_TAG_BOUNCE_BACK] = rho;
+    return PLBAMD_OK;
+}
+
+int plbamd_compute_fields(int handle, const plbamd_box* box, double* out) {
+    if (!out) return fail(PLBAMD_EINVAL, "compute_fields: null output");
+    return dispatchMoments(handle, box, nullptr, nullptr, nullptr, out);
 }
 
 int plbamd_compute_macroscopic(int handle, const plbamd_box* box, double* rho, double* u) {

@@ -1418,10 +1418,27 @@ struct MomentsArgs {
     const uint8_t* tags;
     const double* values;
     double force[3];
+    int forced;              // Guo force: the velocity is j / rho + F / 2
+    double* fields;          // or null: per staged cell the 7 analysis values
+    double solidRho[3];      // density parameter of BounceBack, NoDynamics, momentum-exchange nodes
 };
 
 // Per-cell density and velocity as Palabos's analysis functions see them
 // (plane_macroscopic_any and the rules listed there), and their FP64 sums.
+// Bulk cells: rho = fullRho(get_rhoBar) with the sequential sum
+// (momentTemplates3D.h:375-380, BasicBulkDynamics::computeDensity,
+// core/dynamics.hh:278-281); u = get_j * invRho(get_rhoBar) (compute_uLb,
+// momentTemplates3D.h:138-144), or with a Guo force j * invRho + F / 2 where
+// rhoBar comes with j from get_rhoBar_j (ExternalForceDynamics::computeVelocity,
+// basicDynamics/externalForceDynamics.hh:59-74). BounceBack, NoDynamics and
+// momentum-exchange nodes (core/dynamics.hh:814-880, 994-1060;
+// bounceBackModels.hh:160-223): u = 0, rho = the dynamics' density parameter,
+// rhoBar = rho - 1, except NoDynamics, whose computeRhoBar is
+// 1 - SkordosFactor = 0 (core/dynamics.hh:1055-1058). Link-wall solids (no
+// Palabos class): rho = 1, rhoBar = 0, u = 0. With m.fields, the analysis fields
+// of dataAnalysisWrapper3D.h:695-777 in T: computeDensity, computeRhoBar,
+// computeVelocity (3), computeKineticEnergy = normSqr(u) / 2 and
+// computeVelocityNorm = sqrt(normSqr(u)) (dataAnalysisFunctional3D.hh:668-719).
 template<typename S>
 __global__ void __launch_bounds__(256) moments_box(const void* arena, BoxArgs b, MomentsArgs m, double* rhoOut,
                                                    double* uOut, double* partials) {
@@ -1438,7 +1455,18 @@ __global__ void __launch_bounds__(256) moments_box(const void* arena, BoxArgs b,
         T rhoBar = T(), u[3] = {T(), T(), T()};
         const bool solid = tag == BT_BOUNCE_BACK || tag == BT_NO_DYNAMICS ||
                            (tag == BT_SKIP && (sp < 0 || (b.special[sp].info & 3u) == SK_MOMENTUM_EXCHANGE));
-        if (!solid) {
+        T rho = T(1);
+        if (solid) {
+            if (tag == BT_BOUNCE_BACK) {
+                rho = (T)m.solidRho[0];
+                rhoBar = rho - (T)1;
+            } else if (tag == BT_NO_DYNAMICS) {
+                rho = (T)m.solidRho[1];
+            } else if (sp >= 0) {
+                rho = (T)m.solidRho[2];
+                rhoBar = rho - (T)1;
+            }
+        } else {
             T fc[19];
             for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell));
             if (sp >= 0 && (b.special[sp].info & 3u) == SK_PLANE) {
@@ -1447,17 +1475,28 @@ __global__ void __launch_bounds__(256) moments_box(const void* arena, BoxArgs b,
                 T rb, j[3];
                 rhobar_j(fc, rb, j);
                 rhoBar = rhobar_sum(fc);
-                const T ir = inv_rho(rhoBar);
-                for (int a2 = 0; a2 < 3; ++a2) u[a2] = j[a2] * ir + (T)m.force[a2] / (T)2;
+                const T ir = inv_rho(m.forced ? rb : rhoBar);
+                for (int a2 = 0; a2 < 3; ++a2) u[a2] = m.forced ? j[a2] * ir + (T)m.force[a2] / (T)2 : j[a2] * ir;
             }
+            rho = rhoBar + (T)1;
         }
-        if (rhoOut) rhoOut[c.staged] = (double)(rhoBar + (T)1);
+        if (rhoOut) rhoOut[c.staged] = (double)rho;
         if (uOut) {
             uOut[3 * c.staged] = (double)u[0];
             uOut[3 * c.staged + 1] = (double)u[1];
             uOut[3 * c.staged + 2] = (double)u[2];
         }
         const T uSqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        if (m.fields) {
+            double* o = m.fields + 7 * c.staged;
+            o[0] = (double)rho;
+            o[1] = (double)rhoBar;
+            o[2] = (double)u[0];
+            o[3] = (double)u[1];
+            o[4] = (double)u[2];
+            o[5] = (double)(uSqr / (T)2);
+            o[6] = (double)sqrt(uSqr);
+        }
         st[0] += (double)rhoBar;
         st[1] += (double)uSqr;
         st[3] += (double)u[0];

@@ -333,6 +333,41 @@ class Block3D:
     def density(self, box=None):
         return self.macroscopic(box)[0]
 
+    def setSolidDensity(self, dynamics, rho):
+        """Density parameter of a solid dynamics (BounceBack, NoDynamics or
+        MomentumExchangeBounceBack, class or tag): what the analysis functions
+        report for its cells (1 until set; Palabos's constructors default to 0)."""
+        tag = getattr(dynamics, "tag", dynamics)
+        check(lib().plbamd_set_solid_density(self.handle, int(tag), float(rho)))
+
+    # -- analysis fields (dataAnalysisWrapper3D.h:695-777) --------------------
+    def analysisFields(self, box=None):
+        """[nx, ny, nz, 7] of the box: computeDensity, computeRhoBar,
+        computeVelocity (x, y, z), computeKineticEnergy, computeVelocityNorm,
+        in the lattice's arithmetic type widened to float64."""
+        b = _box(box, self.shape)
+        out = np.empty(b.shape() + (7,), dtype=np.float64)
+        check(lib().plbamd_compute_fields(self.handle, ctypes.byref(b), out.ctypes.data))
+        return out
+
+    def computeDensity(self, box=None):
+        return self.analysisFields(box)[..., 0]
+
+    def computeRhoBar(self, box=None):
+        return self.analysisFields(box)[..., 1]
+
+    def computeVelocity(self, box=None):
+        return self.analysisFields(box)[..., 2:5]
+
+    def computeVelocityComponent(self, box=None, iComponent=0):
+        return self.analysisFields(box)[..., 2 + int(iComponent)]
+
+    def computeKineticEnergy(self, box=None):
+        return self.analysisFields(box)[..., 5]
+
+    def computeVelocityNorm(self, box=None):
+        return self.analysisFields(box)[..., 6]
+
     def velocity(self, box=None):
         return self.macroscopic(box)[1]
 

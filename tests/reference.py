@@ -17,8 +17,11 @@ def available():
     return os.path.exists(DRIVER)
 
 
-def run(case, init_f64, dumps, bcstyle="dynamics", blocks=None, timeout=1800):
-    """Returns ({step: populations in Palabos's T}, {step: stats row})."""
+def run(case, init_f64, dumps, bcstyle="dynamics", blocks=None, timeout=1800, fields=False):
+    """Returns ({step: populations in Palabos's T}, {step: stats row}); with
+    fields, also {step: [nx,ny,nz,7] analysis fields} (computeDensity,
+    computeRhoBar, computeVelocity x y z, computeKineticEnergy,
+    computeVelocityNorm) as a third item."""
     T = {"f64": "double", "f32": "float"}[case.storage]
     with tempfile.TemporaryDirectory() as tmp:
         lines = [f"T {T}", f"size {case.nx} {case.ny} {case.nz}", f"dyn {case.dynamics}",
@@ -45,6 +48,8 @@ def run(case, init_f64, dumps, bcstyle="dynamics", blocks=None, timeout=1800):
             lines.append("force " + " ".join(repr(float(v)) for v in case.force))
         for b in blocks or []:
             lines.append("block " + " ".join(map(str, b)))
+        if fields:
+            lines.append("fields 1")
         lines.append("dump " + " ".join(map(str, dumps)))
         out = os.path.join(tmp, "out")
         lines.append(f"out {out}")
@@ -60,6 +65,9 @@ def run(case, init_f64, dumps, bcstyle="dynamics", blocks=None, timeout=1800):
             stats[int(v[0])] = {"stored_avg_rho": float(v[1]), "stored_avg_energy": float(v[2]),
                                 "avg_rho": float(v[3]), "avg_energy": float(v[4]),
                                 "force": np.array([float(v[5]), float(v[6]), float(v[7])])}
+        if fields:
+            flds = {s: np.fromfile(f"{out}_fields_s{s}.bin", dtype=dt).reshape(case.shape + (7,)) for s in dumps}
+            return pops, stats, flds
         return pops, stats
 
 

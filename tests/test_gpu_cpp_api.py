@@ -37,10 +37,19 @@ def test_cpp_cavity_matches_palabos(gpu, tmp_path, name, precision, wrapped):
     case, _, data = load(name, GOLDEN_DIR)
     out = tmp_path / "f.bin"
     args = ["cavity3d", case.nx, 50, out, precision] + (["wrapped"] if wrapped else [])
-    _run(args)
+    r = _run(args)
     dt = np.float64 if precision == "double" else np.float32
     got = np.fromfile(out, dtype=dt).reshape(case.shape + (19,))
     assert np.array_equal(got, data["s50"])
+    fields_path = os.path.join(GOLDEN_DIR, f"fields_{name}.npz")
+    if os.path.exists(fields_path):
+        # computeKineticEnergy / computeVelocityNorm / computeVelocity of the facade
+        # against compiled Palabos's fields (tests/golden/make_fields_golden.py)
+        fl = np.load(fields_path, allow_pickle=False)["fields"]
+        c, top = case.nx // 2, case.nz - 2
+        assert r["energy_below_lid"] == float(fl[c, c, top, 5])
+        assert r["norm_below_lid"] == float(fl[c, c, top, 6])
+        assert r["ux_below_lid"] == float(fl[c, c, top, 2])
 
 
 def test_cpp_cavity64_1000_steps(gpu, tmp_path):

@@ -1,0 +1,99 @@
+"""Analysis fields (computeDensity, computeRhoBar, computeVelocity,
+computeKineticEnergy, computeVelocityNorm) through the native library, against
+the fields compiled Palabos computed on the same golden cases
+(tests/golden/fields_*.npz, made by tests/golden/make_fields_golden.py)."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import palabos_amd as plb
+from cases import gpu_lattice
+from golden_cases import build
+from oracle import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIXTURES = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "fields_*.npz")))
+NAMES = ["density", "rhoBar", "ux", "uy", "uz", "kinetic energy", "velocity norm"]
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fixtures_present():
+    assert len(FIXTURES) == 11
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=lambda p: os.path.basename(p)[7:-4])
+def test_fields_match_palabos(gpu, path):
+    z = np.load(path, allow_pickle=False)
+    name, step, want = str(z["case"]), int(z["step"]), z["fields"].astype(np.float64)
+    print(f"[fields] {name} step {step}", flush=True)
+    case, init, _ = build(name)
+    lat = gpu_lattice(case)
+    try:
+        # the reference driver builds BounceBack(1), NoDynamics(1) and
+        # MomentumExchangeBounceBack(forceIds) (density parameter T() = 0)
+        lat.setSolidDensity(plb.MomentumExchangeBounceBack, 0.0)
+        lat.setPopulations(oracle.to_storage(init, case.storage))
+        lat.collideAndStream(step)
+        got = lat.analysisFields()
+    finally:
+        lat.close()
+    for k, what in enumerate(NAMES):
+        bad = np.count_nonzero(got[..., k] != want[..., k])
+        assert bad == 0, f"{name}: {what} differs in {bad} cells (max |diff| {np.abs(got[..., k] - want[..., k]).max():.3e})"
+
+
+def test_fields_properties_large(gpu):
+    """Size-independent properties at 160^3 FP32 (4.1 M cells): energy and norm
+    are |u|^2 / 2 and |u| of the velocity columns in FP32, density is rhoBar + 1,
+    and the means agree with computeAverageDensity / Energy."""
+    n = 160
+    lat = plb.Block3D(n, n, n, plb.D3Q19, plb.RegularizedBGK(1.6), storage="f32", periodic=True)
+    try:
+        x = np.arange(n) * 2 * np.pi / n
+        u = np.zeros((n, n, n, 3))
+        u[..., 0] = 0.03 * np.sin(x)[:, None, None] * np.cos(x)[None, :, None]
+        u[..., 1] = -0.03 * np.cos(x)[:, None, None] * np.sin(x)[None, :, None]
+        rho = 1.0 + 0.01 * np.cos(x)[None, None, :] * np.ones((n, n, n))
+        lat.initializeAtEquilibrium(rho=rho, u=u)
+        lat.collideAndStream(10)
+        f = lat.analysisFields()
+        u32 = f[..., 2:5].astype(np.float32)
+        usq = (u32[..., 0] * u32[..., 0] + u32[..., 1] * u32[..., 1]) + u32[..., 2] * u32[..., 2]
+        np.testing.assert_array_equal(f[..., 5].astype(np.float32), usq / np.float32(2))
+        np.testing.assert_array_equal(f[..., 6].astype(np.float32), np.sqrt(usq))
+        np.testing.assert_array_equal(f[..., 0].astype(np.float32), f[..., 1].astype(np.float32) + np.float32(1))
+        assert f[..., 0].mean() == pytest.approx(lat.averageDensity(), rel=1e-6)
+        assert f[..., 5].mean() == pytest.approx(lat.averageEnergy(), rel=1e-6)
+    finally:
+        lat.close()
+
+
+def test_solid_density_parameters(gpu):
+    """BounceBack(rho) and MomentumExchangeBounceBack(..., rho) report rho and
+    rho - 1; NoDynamics(rho) reports rho and rhoBar 0 (core/dynamics.hh:814-880,
+    994-1060; bounceBackModels.hh:160-223); all report u = 0."""
+    shape = (6, 5, 4)
+    tags = np.zeros(shape, np.uint8)
+    tags[0] = plb.BounceBack.tag
+    tags[1] = plb.NoDynamics.tag
+    tags[2] = plb.MomentumExchangeBounceBack.tag
+    lat = plb.Block3D(*shape, plb.D3Q19, plb.BGK(1.2), storage="f64")
+    try:
+        lat.setTags(tags)
+        lat.setSolidDensity(plb.BounceBack, 0.5)
+        lat.setSolidDensity(plb.NoDynamics, 2.5)
+        lat.setSolidDensity(plb.MomentumExchangeBounceBack, 0.0)
+        f = lat.analysisFields()
+        np.testing.assert_array_equal(f[0, ..., 0], 0.5)
+        np.testing.assert_array_equal(f[0, ..., 1], -0.5)
+        np.testing.assert_array_equal(f[1, ..., 0], 2.5)
+        np.testing.assert_array_equal(f[1, ..., 1], 0.0)
+        np.testing.assert_array_equal(f[2, ..., 0], 0.0)
+        np.testing.assert_array_equal(f[2, ..., 1], -1.0)
+        np.testing.assert_array_equal(f[:3, ..., 2:], 0.0)
+        np.testing.assert_array_equal(f[3:, ..., 0], 1.0)   # rest state at equilibrium
+    finally:
+        lat.close()
