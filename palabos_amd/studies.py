@@ -320,9 +320,37 @@ def checkpoint_throughput(n=256, storage="f32", T="f32", directory=None):
             "save_s": t1 - t0, "load_s": t2 - t1, "identical_after_load": bool(same)}
 
 
+def fields_throughput(n=256, storage="f32", repeats=3):
+    """Analysis fields (density, rhoBar, velocity, kinetic energy, velocity
+    norm) of an n^3 lattice after a few steps: wall-clock time of each call
+    (including the 56 bytes per cell copied to the host) and a consistency
+    check; the device kernel's own duration comes from a kernel trace."""
+    shape = (n, n, n)
+    lat = _lat.Block3D(*shape, _lat.D3Q19, _lat.BGK(1.7), storage=storage, periodic=True)
+    try:
+        x = np.arange(n) * 2 * np.pi / n
+        u = np.zeros(shape + (3,))
+        u[..., 0] = 0.03 * np.sin(x)[:, None, None] * np.cos(x)[None, :, None]
+        u[..., 1] = -0.03 * np.cos(x)[:, None, None] * np.sin(x)[None, :, None]
+        lat.initializeAtEquilibrium(rho=1.0, u=u)
+        lat.collideAndStream(4)
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            f = lat.analysisFields()
+            times.append(time.perf_counter() - t0)
+        mean_energy = float(f[..., 5].mean())
+        avg_energy = lat.averageEnergy()
+    finally:
+        lat.close()
+    return {"shape": list(shape), "storage": storage, "call_s": times, "mean_kinetic_energy": mean_energy,
+            "average_energy": avg_energy}
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["taylor_green", "sphere_drag", "porous", "porous_adjoint", "checkpoint"])
+    ap.add_argument("study", choices=["taylor_green", "sphere_drag", "porous", "porous_adjoint", "checkpoint",
+                                      "fields"])
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--storage", default=None)
@@ -336,6 +364,8 @@ def main():
         r = sphere_drag(steps=args.steps or 5000, storage=args.storage or "f64")
     elif args.study == "porous":
         r = porous_permeability(steps=args.steps or 20000, storage=args.storage or "f16", scale=args.scale)
+    elif args.study == "fields":
+        r = fields_throughput(n=args.n if args.n != 512 else 256, storage=args.storage or "f32")
     elif args.study == "checkpoint":
         r = checkpoint_throughput(n=args.n if args.n != 512 else 256, storage=args.storage or "f32")
     else:
