@@ -89,6 +89,7 @@ struct Lattice {
     double* dValues = nullptr;
     double* dAlpha = nullptr;
     double* dBulkPartials = nullptr;
+    double* dBulkLevel1 = nullptr;  // kStatsLevel1 records: first level of the statistics reduction
     uint64_t nBulkPartials = 0;       // = workgroups of the bulk launch
     int xcdRemap = 0;
     std::vector<uint32_t> planeValueBase;   // first value of each plane in dValues
@@ -112,7 +113,7 @@ struct Lattice {
 
     ~Lattice() {
         hipFree(arena); hipFree(dGeom); hipFree(dTags); hipFree(dSpecial); hipFree(dValues); hipFree(dAlpha);
-        hipFree(dBulkPartials); hipFree(dSpecialPartials); hipFree(dStats); hipFree(dSegs);
+        hipFree(dBulkPartials); hipFree(dBulkLevel1); hipFree(dSpecialPartials); hipFree(dStats); hipFree(dSegs);
         hipFree(dWgBlock); hipFree(dTasks); hipFree(dWalls); hipFree(dEvenBlock);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
@@ -350,6 +351,7 @@ int setup(Lattice& L) {
         PLBAMD_HIP(hipMemcpy(L.dWgBlock, wgBlock.data(), wgBlock.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     }
     PLBAMD_HIP(hipMalloc(&L.dBulkPartials, std::max(L.nBulkPartials, L.nEven) * 16 * sizeof(double)));
+    PLBAMD_HIP(hipMalloc(&L.dBulkLevel1, kStatsLevel1 * 16 * sizeof(double)));
     if (L.multi && L.linearEven) {
         std::vector<uint16_t> eb(L.nEven);
         for (size_t b = 0; b < L.geom.size(); ++b) {
@@ -737,8 +739,19 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
         PLBAMD_HIP(hipGetLastError());
     }
     if (stats) {
-        hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(256), 0, L.stream, L.dBulkPartials,
-                           (!odd && L.linearEven) ? L.nEven : L.nBulkPartials,
+        // two levels when there are many records (a single workgroup reading
+        // 65536 records took 40 % of a 256^3 step): kStatsLevel1 workgroups in a
+        // fixed order, then one
+        const uint64_t nb = (!odd && L.linearEven) ? L.nEven : L.nBulkPartials;
+        const double* bulk = L.dBulkPartials;
+        uint64_t nr = nb;
+        if (nb > 4 * kStatsLevel1) {
+            hipLaunchKernelGGL(reduce_bulk_records, dim3(kStatsLevel1), dim3(256), 0, L.stream, L.dBulkPartials, nb,
+                               L.dBulkLevel1);
+            bulk = L.dBulkLevel1;
+            nr = kStatsLevel1;
+        }
+        hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(256), 0, L.stream, bulk, nr,
                            L.dSpecialPartials, L.nSpecial ? L.nSpecialPartials : 0, L.dStats);
         PLBAMD_HIP(hipGetLastError());
     }
@@ -1020,22 +1033,27 @@ int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sum
     PLBAMD_TRY(rebuild(L));
     std::vector<BoxArgs> parts;
     uint64_t nRecords = 0;
+    const bool planes = L.nSpecial > 0 && L.dValues != nullptr;
     for (size_t bi = 0; bi < L.geom.size(); ++bi) {
         BoxArgs a;
         if (boxArgs(L, bi, b, a)) {
             parts.push_back(a);
             nRecords += gridFor((uint64_t)a.ilx * a.ily * a.ilz).x;
+            if (planes) nRecords += gridFor(L.nSpecial).x;
         }
     }
     const size_t fieldDoubles = (rho || u) ? n * 4 : 0;
     const size_t analysisDoubles = fields ? n * 7 : 0;
+    // records reduced in two levels when there are many: 256 workgroups, then one
+    const uint32_t level1 = nRecords > 4096 ? 256u : 0u;
     double* d = nullptr;
-    PLBAMD_HIP(hipMalloc(&d, (fieldDoubles + analysisDoubles + nRecords * 8 + 8) * sizeof(double)));
+    PLBAMD_HIP(hipMalloc(&d, (fieldDoubles + analysisDoubles + nRecords * 8 + 8 * level1 + 8) * sizeof(double)));
     double* dRho = rho ? d : nullptr;
     double* dU = u ? d + n : nullptr;
     double* dFields = fields ? d + fieldDoubles : nullptr;
     double* dPart = d + fieldDoubles + analysisDoubles;
-    double* dSum = dPart + nRecords * 8;
+    double* dLevel1 = dPart + nRecords * 8;
+    double* dSum = dLevel1 + 8 * level1;
     MomentsArgs m{};
     m.tags = L.dTags;
     m.values = L.dValues;
@@ -1047,13 +1065,28 @@ int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sum
     uint64_t rec = 0;
     for (const BoxArgs& a : parts) {
         const dim3 grid = gridFor((uint64_t)a.ilx * a.ily * a.ilz);
-        hipLaunchKernelGGL((moments_box<S>), grid, dim3(256), 0, L.stream, L.arena, a, m, dRho, dU, dPart + rec * 8);
+        if (a.parity)
+            hipLaunchKernelGGL((moments_box<S, 1>), grid, dim3(256), 0, L.stream, L.arena, a, m, dRho, dU, dPart + rec * 8);
+        else
+            hipLaunchKernelGGL((moments_box<S, 0>), grid, dim3(256), 0, L.stream, L.arena, a, m, dRho, dU, dPart + rec * 8);
         rec += grid.x;
         err = hipGetLastError();
         if (err != hipSuccess) break;
+        if (planes) {
+            const dim3 pg = gridFor(L.nSpecial);
+            hipLaunchKernelGGL((moments_planes<S>), pg, dim3(256), 0, L.stream, L.arena, a, m, dRho, dU, dPart + rec * 8);
+            rec += pg.x;
+            err = hipGetLastError();
+            if (err != hipSuccess) break;
+        }
     }
     if (err == hipSuccess) {
-        hipLaunchKernelGGL(sum_records, dim3(1), dim3(256), 0, L.stream, dPart, (uint32_t)nRecords, dSum);
+        if (level1) {
+            hipLaunchKernelGGL(sum_records, dim3(level1), dim3(256), 0, L.stream, dPart, (uint32_t)nRecords, dLevel1);
+            hipLaunchKernelGGL(sum_records, dim3(1), dim3(256), 0, L.stream, dLevel1, level1, dSum);
+        } else {
+            hipLaunchKernelGGL(sum_records, dim3(1), dim3(256), 0, L.stream, dPart, (uint32_t)nRecords, dSum);
+        }
         err = hipGetLastError();
     }
     double host[8];

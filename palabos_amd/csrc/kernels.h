@@ -1154,6 +1154,24 @@ __global__ void __launch_bounds__(256) special_step(SpecialArgs a) {
 
 // Final statistics reduction: one workgroup, fixed order. out[0..7]:
 // sumRhoBar sumUSqr maxUSqr count Fx Fy Fz -, out[8..15]: Px Py - Pz (slot 2 is a max).
+// First level of the statistics reduction: workgroup w sums the bulk records
+// w*256 + t, strided by the grid, into one record of the same 16-double layout
+// (statistics in 0..7, the link-wall force in 8, 9, 11).
+constexpr uint32_t kStatsLevel1 = 256;
+__global__ void __launch_bounds__(256) reduce_bulk_records(const double* bulk, uint64_t nBulk, double* out) {
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double wf[3] = {0, 0, 0};
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nBulk; b += (uint64_t)gridDim.x * blockDim.x) {
+        const double* p = bulk + 16 * b;
+        v[0] += p[0]; v[1] += p[1]; v[2] = fmax(v[2], p[2]); v[3] += p[3];
+        v[4] += p[4]; v[5] += p[5]; v[6] += p[6];
+        wf[0] += p[8]; wf[1] += p[9]; wf[2] += p[11];
+    }
+    block_reduce_store(v, out + 16 * (uint64_t)blockIdx.x);
+    double q[8] = {wf[0], wf[1], 0, wf[2], 0, 0, 0, 0};
+    block_reduce_store(q, out + 16 * (uint64_t)blockIdx.x + 8);
+}
+
 __global__ void __launch_bounds__(256) reduce_partials(const double* bulk, uint64_t nBulk,
                                                        const double* special, uint64_t nSpecial,
                                                        double* out) {
@@ -1442,7 +1460,9 @@ struct MomentsArgs {
 // of dataAnalysisWrapper3D.h:695-777 in T: computeDensity, computeRhoBar,
 // computeVelocity (3), computeKineticEnergy = normSqr(u) / 2 and
 // computeVelocityNorm = sqrt(normSqr(u)) (dataAnalysisFunctional3D.hh:668-719).
-template<typename S>
+// PAR: the parity of the layout (b.parity), a template parameter so that the
+// natural layout's addressing is a plain slot offset.
+template<typename S, int PAR>
 __global__ void __launch_bounds__(256) moments_box(const void* arena, BoxArgs b, MomentsArgs m, double* rhoOut,
                                                    double* uOut, double* partials) {
     using T = typename Store<S>::T;
@@ -1456,7 +1476,7 @@ __global__ void __launch_bounds__(256) moments_box(const void* arena, BoxArgs b,
             const BoxCell c = box_cell(b, k);
             const uint8_t tag = m.tags ? m.tags[b.g.cellOffset + c.cell] : (uint8_t)BT_BULK;
             const int sp = (tag == BT_SKIP) ? find_special(b, c.cell) : -1;
-            const uint32_t wall = !b.parity ? 0u
+            const uint32_t wall = !PAR ? 0u
                                   : (tag == BT_WALL_NEAR && b.walls) ? b.walls[b.g.cellOffset + c.cell]
                                   : (sp >= 0 ? b.special[sp].wall : 0u);
             T rhoBar = T(), u[3] = {T(), T(), T()};
@@ -1473,18 +1493,16 @@ __global__ void __launch_bounds__(256) moments_box(const void* arena, BoxArgs b,
                     rho = (T)m.solidRho[2];
                     rhoBar = rho - (T)1;
                 }
+            } else if (sp >= 0 && (b.special[sp].info & 3u) == SK_PLANE) {
+                continue;   // plane cells: moments_planes (their closures would triple this kernel's registers)
             } else {
                 T fc[19];
-                for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + pop_index(b.g, b.parity, c.x, c.y, c.z, i, wall, c.cell));
-                if (sp >= 0 && (b.special[sp].info & 3u) == SK_PLANE) {
-                    plane_macroscopic_any(fc, b.special[sp].info, m.values + b.special[sp].value, rhoBar, u);
-                } else {
-                    T rb, j[3];
-                    rhobar_j(fc, rb, j);
-                    rhoBar = rhobar_sum(fc);
-                    const T ir = inv_rho(m.forced ? rb : rhoBar);
-                    for (int a2 = 0; a2 < 3; ++a2) u[a2] = m.forced ? j[a2] * ir + (T)m.force[a2] / (T)2 : j[a2] * ir;
-                }
+                for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + pop_index(b.g, PAR, c.x, c.y, c.z, i, wall, c.cell));
+                T rb, j[3];
+                rhobar_j(fc, rb, j);
+                rhoBar = rhobar_sum(fc);
+                const T ir = inv_rho(m.forced ? rb : rhoBar);
+                for (int a2 = 0; a2 < 3; ++a2) u[a2] = m.forced ? j[a2] * ir + (T)m.force[a2] / (T)2 : j[a2] * ir;
                 rho = rhoBar + (T)1;
             }
             if (rhoOut) rhoOut[c.staged] = (double)rho;
@@ -1516,11 +1534,63 @@ __global__ void __launch_bounds__(256) moments_box(const void* arena, BoxArgs b,
     if (partials) block_reduce_store(st, partials + 8 * (uint64_t)blockIdx.x);
 }
 
+// The plane cells of moments_box (velocity / pressure planes): their density and
+// velocity from the closure of their boundary dynamics (plane_macroscopic_any),
+// the same outputs and per-workgroup FP64 records.
+template<typename S>
+__global__ void __launch_bounds__(256) moments_planes(const void* arena, BoxArgs b, MomentsArgs m, double* rhoOut,
+                                                      double* uOut, double* partials) {
+    using T = typename Store<S>::T;
+    const S* f = reinterpret_cast<const S*>(arena) + b.g.offset;
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < b.nSpecial; k += gridDim.x * blockDim.x) {
+        const SpecialCell sc = b.special[k];
+        if (sc.block != b.block || (sc.info & 3u) != SK_PLANE) continue;
+        const int x = sc.x, y = sc.y, z = sc.z;
+        if (x < b.ix0 || x >= b.ix0 + b.ilx || y < b.iy0 || y >= b.iy0 + b.ily || z < b.iz0 || z >= b.iz0 + b.ilz)
+            continue;
+        T fc[19];
+        for (int i = 0; i < 19; ++i)
+            fc[i] = Store<S>::load(f + pop_index(b.g, b.parity, x, y, z, i, b.parity ? sc.wall : 0u, sc.cell));
+        T rhoBar, u[3];
+        plane_macroscopic_any(fc, sc.info, m.values + sc.value, rhoBar, u);
+        const T rho = rhoBar + (T)1;
+        const uint64_t staged = ((uint64_t)(b.ox + x - b.ix0) * b.sly + (b.oy + y - b.iy0)) * b.slz + (b.oz + z - b.iz0);
+        if (rhoOut) rhoOut[staged] = (double)rho;
+        if (uOut) {
+            uOut[3 * staged] = (double)u[0];
+            uOut[3 * staged + 1] = (double)u[1];
+            uOut[3 * staged + 2] = (double)u[2];
+        }
+        const T uSqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        if (m.fields) {
+            double* o = m.fields + 7 * staged;
+            o[0] = (double)rho;
+            o[1] = (double)rhoBar;
+            o[2] = (double)u[0];
+            o[3] = (double)u[1];
+            o[4] = (double)u[2];
+            o[5] = (double)(uSqr / (T)2);
+            o[6] = (double)sqrt(uSqr);
+        }
+        st[0] += (double)rhoBar;
+        st[1] += (double)uSqr;
+        st[3] += (double)u[0];
+        st[4] += (double)u[1];
+        st[5] += (double)u[2];
+    }
+    if (partials) block_reduce_store(st, partials + 8 * (uint64_t)blockIdx.x);
+}
+
+// Sum of n 8-double records (slot 2 unused) in a fixed order. One workgroup
+// per gridDim.x: workgroup w sums records w, w + gridDim.x, ... into out[w];
+// a launch of many workgroups followed by one of a single workgroup over
+// their outputs is the two-level form.
 __global__ void __launch_bounds__(256) sum_records(const double* p, uint32_t n, double* out) {
     double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t b = threadIdx.x; b < n; b += blockDim.x)
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x)
         for (int k = 0; k < 8; ++k) if (k != 2) r[k] += p[8 * b + k];
-    block_reduce_store(r, out);
+    block_reduce_store(r, out + 8 * (uint64_t)blockIdx.x);
 }
 
 }  // namespace plbamd
