@@ -71,7 +71,7 @@ __host__ __device__ inline Dual& operator-=(Dual& a, Dual b) { a = a - b; return
 __host__ __device__ inline Dual& operator*=(Dual& a, Dual b) { a = a * b; return a; }
 
 struct AdjointArgs {
-    const double* mom;        // moments of f^t: rhoBar, jx, jy, jz, [4][momStride] by cellOffset + cell
+    const void* mom;          // moments of f^t: rhoBar, jx, jy, jz, [4][momStride] by cellOffset + cell, type LT
     uint64_t momStride;
     const double* planeF;     // f^t of the special (plane) records, 19 per record
     const void* lamNext;      // lambda^{t+1} of this block (slot stride g.slot), FP64 or FP32
@@ -164,8 +164,10 @@ __device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGe
         gather_mu<LT>(g, lamNext, x, y, z, cell, mu);
         const double al = a.alpha ? a.alpha[idx] : 0.0;
         const double dm = 1.0 - al;
-        const double rhoBar = a.mom[idx];
-        const double j[3] = {a.mom[a.momStride + idx], a.mom[2 * a.momStride + idx], a.mom[3 * a.momStride + idx]};
+        const LT* mom = reinterpret_cast<const LT*>(a.mom);
+        const double rhoBar = (double)mom[idx];
+        const double j[3] = {(double)mom[a.momStride + idx], (double)mom[2 * a.momStride + idx],
+                             (double)mom[3 * a.momStride + idx]};
         const double rho = 1.0 + rhoBar;
         const double F[3] = {FORCED ? a.force[0] : 0.0, FORCED ? a.force[1] : 0.0, FORCED ? a.force[2] : 0.0};
         const double Jv[3] = {dm * (j[0] + rho * F[0] / 2), dm * (j[1] + rho * F[1] / 2), dm * (j[2] + rho * F[2] / 2)};
@@ -346,8 +348,9 @@ __global__ void __launch_bounds__(256) adjoint_final(AdjointArgs a) {
         const uint64_t idx = g.cellOffset + cell;
         const bool plane = a.tags && a.tags[idx] == BT_SKIP;
         const double al = (a.alpha && !plane) ? a.alpha[idx] : 0.0;
-        const double rhoBar = a.mom[idx];
-        const double jx = a.mom[a.momStride + idx];
+        const LT* mom = reinterpret_cast<const LT*>(a.mom);
+        const double rhoBar = (double)mom[idx];
+        const double jx = (double)mom[a.momStride + idx];
         const int gx = g.gx0 + x;
         double djdf[19];
         objective_gradient(a, gx, al, djdf);

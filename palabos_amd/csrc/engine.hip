@@ -586,7 +586,7 @@ int exchange(Lattice& L) {
 // statistics off), the kernels also write the pre-collision moments of every
 // cell and the pre-completion populations of every special record.
 struct Recording {
-    double* moments = nullptr;
+    void* moments = nullptr;       // MomentType<S> [4][momStride]
     uint64_t momStride = 0;
     double* specialF = nullptr;
 };
@@ -1178,8 +1178,9 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     // the adjoint state is FP64 for an FP64 forward, FP32 for a 16-bit one
     const bool lam64 = st == PLBAMD_F64;
     const size_t lamSize = lam64 ? sizeof(double) : sizeof(float);
-    char *ck = nullptr, *work = nullptr, *lamA = nullptr, *lamB = nullptr;
-    double *mom = nullptr, *pf = nullptr, *gA = nullptr, *gS = nullptr, *acc = nullptr, *sum = nullptr;
+    // the recorded moments have the adjoint state's type (MomentType of the storage)
+    char *ck = nullptr, *work = nullptr, *lamA = nullptr, *lamB = nullptr, *mom = nullptr;
+    double *pf = nullptr, *gA = nullptr, *gS = nullptr, *acc = nullptr, *sum = nullptr;
     uint16_t* dAdjBlock = nullptr;
     uint32_t* dAdjBegin = nullptr;
     auto release = [&]() {
@@ -1189,7 +1190,7 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     const uint64_t nAcc = (uint64_t)adjWGs + planeWGs;   // J accumulators: bulk workgroups, then plane ones
     hipError_t err = hipMalloc(&ck, stateBytes * nCk);
     if (err == hipSuccess) err = hipMalloc(&work, stateBytes);
-    if (err == hipSuccess) err = hipMalloc(&mom, momElems * (K + 1) * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc(&mom, momElems * (K + 1) * lamSize);
     if (err == hipSuccess) err = hipMalloc(&pf, pfElems * (K + 1) * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&lamA, arenaElems * lamSize);
     if (err == hipSuccess) err = hipMalloc(&lamB, arenaElems * lamSize);
@@ -1212,7 +1213,7 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     if (err == hipSuccess) err = hipMemsetAsync(gS, 0, std::max<uint32_t>(1, L.nSpecial) * sizeof(double), L.stream);
     if (err == hipSuccess) err = hipMemsetAsync(acc, 0, nAcc * sizeof(double), L.stream);
     // cells the odd steps skip (outermost envelope rows) keep zero moments
-    if (err == hipSuccess) err = hipMemsetAsync(mom, 0, momElems * (K + 1) * sizeof(double), L.stream);
+    if (err == hipSuccess) err = hipMemsetAsync(mom, 0, momElems * (K + 1) * lamSize, L.stream);
     const bool saveStats = L.statsOn;
     L.statsOn = false;
     float msF = 0, msB = 0;
@@ -1292,13 +1293,13 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
         const int nRec = (c == nCk - 1) ? len + 1 : len;
         for (int m = 0; m < nRec && err == hipSuccess; ++m) {
             Recording r;
-            r.moments = mom + (uint64_t)m * momElems;
+            r.moments = mom + (uint64_t)m * momElems * lamSize;
             r.momStride = cells;
             r.specialF = L.nSpecial ? pf + (uint64_t)m * pfElems : nullptr;
             if (recordStep(L, (m & 1) != 0, r) != PLBAMD_OK) err = hipErrorUnknown;
         }
         if (err == hipSuccess && c == nCk - 1) {
-            a.mom = mom + (uint64_t)len * momElems;
+            a.mom = mom + (uint64_t)len * momElems * lamSize;
             a.explicitTerm = 1;
             a.lam = lamNext;
             if (lam64) hipLaunchKernelGGL(adjoint_final<double>, agrid, ablock, 0, L.stream, a);
@@ -1308,7 +1309,7 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
         }
         for (int m = len - 1; m >= 0 && err == hipSuccess; --m) {
             const int t = t0 + m;
-            a.mom = mom + (uint64_t)m * momElems;
+            a.mom = mom + (uint64_t)m * momElems * lamSize;
             a.planeF = pf + (uint64_t)m * pfElems;
             a.explicitTerm = t > obj.first_step ? 1 : 0;
             a.lamNext = lamNext;

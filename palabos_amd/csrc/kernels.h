@@ -174,7 +174,7 @@ struct StepArgs {
     const uint8_t* tags;     // BulkTag per cell (all blocks) or null
     const double* alpha;     // Brinkman porosity per cell (all blocks) or null
     double* partials;        // 8 doubles per workgroup when statistics are on
-    double* moments;         // MOMENTS: rhoBar, jx, jy, jz of the pre-collision state,
+    void* moments;           // MOMENTS: rhoBar, jx, jy, jz of the pre-collision state (MomentType<S>),
     uint64_t momStride;      //          [4][momStride] by cell (all blocks)
     int xcdRemap;            // 1: consecutive work items on the same XCD
     const uint16_t* wgBlock; // block of every work item (multi-block), or null (one block)
@@ -199,10 +199,15 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
     return (b % 8) * q + b / 8;
 }
 
-// Pre-collision moments rhoBar = sum_i f_i, j = sum_i c_i f_i (plain order),
-// recorded for the adjoint's reverse sweep.
-template<typename T>
-__device__ __forceinline__ void store_moments(const T* fc, double* m, uint64_t stride, uint64_t idx) {
+// Pre-collision moments rhoBar = sum_i f_i, j = sum_i c_i f_i (plain order, FP64
+// sums), recorded for the adjoint's reverse sweep in the type of the adjoint
+// state: FP64 for an FP64 forward, FP32 for the others (whose populations are
+// FP32 values).
+template<typename S>
+using MomentType = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
+
+template<typename T, typename M>
+__device__ __forceinline__ void store_moments(const T* fc, M* m, uint64_t stride, uint64_t idx) {
     double rb = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
@@ -212,10 +217,10 @@ __device__ __forceinline__ void store_moments(const T* fc, double* m, uint64_t s
         jy += cy(i) * v;
         jz += cz(i) * v;
     }
-    m[idx] = rb;
-    m[stride + idx] = jx;
-    m[2 * stride + idx] = jy;
-    m[3 * stride + idx] = jz;
+    m[idx] = (M)rb;
+    m[stride + idx] = (M)jx;
+    m[2 * stride + idx] = (M)jy;
+    m[3 * stride + idx] = (M)jz;
 }
 
 // Collision of one cell by its bulk tag: background dynamics (with moments
@@ -226,7 +231,8 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, const BlockGeom&
                                              uint32_t wall = 0, double* wf = nullptr) {
     using T = typename Store<S>::T;
     if constexpr (MOMENTS) {
-        if (tag == BT_BULK || tag == BT_WALL_NEAR) store_moments(fc, a.moments, a.momStride, g.cellOffset + cell);
+        if (tag == BT_BULK || tag == BT_WALL_NEAR)
+            store_moments(fc, reinterpret_cast<MomentType<S>*>(a.moments), a.momStride, g.cellOffset + cell);
     }
     if (tag == BT_BULK || tag == BT_WALL_NEAR) {
         const T omega = (T)a.omega;
@@ -1052,7 +1058,7 @@ struct SpecialArgs {
     const double* values;
     double* partials;     // 16 doubles per workgroup when statistics are on:
                           // rhoBar uSqr maxUSqr count Fx Fy Fz -, then Px Py - Pz - - - -
-    double* moments;      // or null: pre-collision moments (layout of StepArgs::moments)
+    void* moments;        // or null: pre-collision moments (layout and type of StepArgs::moments)
     uint64_t momStride;
     double* dumpF;        // or null: pre-completion populations, 19 per record
 };
@@ -1081,7 +1087,7 @@ __global__ void __launch_bounds__(256) special_step(SpecialArgs a) {
 #pragma unroll
             for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + swapped_index(g, x, y, z, i, sc.wall, cell));
         }
-        if (a.moments) store_moments(fc, a.moments, a.momStride, g.cellOffset + cell);
+        if (a.moments) store_moments(fc, reinterpret_cast<MomentType<S>*>(a.moments), a.momStride, g.cellOffset + cell);
         if (a.dumpF) {
 #pragma unroll
             for (int i = 0; i < 19; ++i) a.dumpF[19 * (uint64_t)k + i] = (double)fc[i];
