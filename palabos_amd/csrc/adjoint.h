@@ -129,17 +129,17 @@ __device__ __forceinline__ void objective_gradient(const AdjointArgs& a, int gx,
 }
 
 // mu(y) gathered from lambda^{t+1} (transpose of streaming with edge reflection).
-template<typename LT>
+template<typename LT, typename C>
 __device__ __forceinline__ void gather_mu(const BlockGeom& g, const LT* lamNext, int x, int y, int z, uint64_t cell,
-                                          double mu[19]) {
-    mu[0] = (double)lamNext[cell];
+                                          C mu[19]) {
+    mu[0] = (C)lamNext[cell];
 #pragma unroll
     for (int i = 1; i < 19; ++i) {
         int dx, dy, dz;
         const bool ok = shift(x, cx(i), g.nx, g.px, dx) && shift(y, cy(i), g.ny, g.py, dy) &&
                         shift(z, cz(i), g.nz, g.pz, dz);
-        mu[i] = (double)(ok ? lamNext[(uint64_t)i * g.slot + lin(g, dx, dy, dz)]
-                            : lamNext[(uint64_t)opp(i) * g.slot + cell]);
+        mu[i] = (C)(ok ? lamNext[(uint64_t)i * g.slot + lin(g, dx, dy, dz)]
+                       : lamNext[(uint64_t)opp(i) * g.slot + cell]);
     }
 }
 
@@ -152,102 +152,111 @@ __device__ __forceinline__ int cx_of(uint32_t i) {
 }
 
 // Reverse step of one bulk (non-plane) cell; plane cells are left to adjoint_planes.
+// The arithmetic type C is the adjoint state's: FP64 for an FP64 forward, FP32
+// for a 16-bit one (whose forward arithmetic is FP32); the objective and the
+// porosity gradient accumulate in FP64.
 template<typename LT, int DYN, bool FORCED>
 __device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGeom& g, int x, int y, int z,
                                              double& J) {
+    using C = LT;
     const uint64_t cell = ((uint64_t)x * g.ny + y) * g.nz + z;
     const uint64_t idx = g.cellOffset + cell;
     const LT* lamNext = reinterpret_cast<const LT*>(a.lamNext) + g.offset;
     LT* lam = reinterpret_cast<LT*>(a.lam) + g.offset;
     if (!(a.tags && a.tags[idx] == BT_SKIP)) {
-        double mu[19];
-        gather_mu<LT>(g, lamNext, x, y, z, cell, mu);
-        const double al = a.alpha ? a.alpha[idx] : 0.0;
-        const double dm = 1.0 - al;
+        C mu[19];
+        gather_mu<LT, C>(g, lamNext, x, y, z, cell, mu);
+        const double al64 = a.alpha ? a.alpha[idx] : 0.0;
+        const C al = (C)al64;
+        const C dm = C(1) - al;
         const LT* mom = reinterpret_cast<const LT*>(a.mom);
-        const double rhoBar = (double)mom[idx];
-        const double j[3] = {(double)mom[a.momStride + idx], (double)mom[2 * a.momStride + idx],
-                             (double)mom[3 * a.momStride + idx]};
-        const double rho = 1.0 + rhoBar;
-        const double F[3] = {FORCED ? a.force[0] : 0.0, FORCED ? a.force[1] : 0.0, FORCED ? a.force[2] : 0.0};
-        const double Jv[3] = {dm * (j[0] + rho * F[0] / 2), dm * (j[1] + rho * F[1] / 2), dm * (j[2] + rho * F[2] / 2)};
-        const double J2 = Jv[0] * Jv[0] + Jv[1] * Jv[1] + Jv[2] * Jv[2];
-        const double w = a.omega;
-        double A = 0, B[3] = {0, 0, 0}, gG[3] = {0, 0, 0};
-        double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        const C rhoBar = (C)mom[idx];
+        const C j[3] = {(C)mom[a.momStride + idx], (C)mom[2 * a.momStride + idx], (C)mom[3 * a.momStride + idx]};
+        const C rho = C(1) + rhoBar;
+        const C F[3] = {FORCED ? (C)a.force[0] : C(0), FORCED ? (C)a.force[1] : C(0), FORCED ? (C)a.force[2] : C(0)};
+        const C Jv[3] = {dm * (j[0] + rho * F[0] / C(2)), dm * (j[1] + rho * F[1] / C(2)), dm * (j[2] + rho * F[2] / C(2))};
+        const C J2 = Jv[0] * Jv[0] + Jv[1] * Jv[1] + Jv[2] * Jv[2];
+        const C w = (C)a.omega;
+        C A = C(0), B[3] = {C(0), C(0), C(0)}, gG[3] = {C(0), C(0), C(0)};
+        C M[3][3] = {{C(0), C(0), C(0)}, {C(0), C(0), C(0)}, {C(0), C(0), C(0)}};
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
-            const double t = weight<double>(i);
-            const double cJ = cx(i) * Jv[0] + cy(i) * Jv[1] + cz(i) * Jv[2];
-            const double mt = mu[i] * t;
-            A += mt * (1.0 - (4.5 * cJ * cJ - 1.5 * J2) / (rho * rho));
+            const C t = weight<C>(i);
+            const C cJ = (C)cx(i) * Jv[0] + (C)cy(i) * Jv[1] + (C)cz(i) * Jv[2];
+            const C mt = mu[i] * t;
+            A += mt * (C(1) - (C(4.5) * cJ * cJ - C(1.5) * J2) / (rho * rho));
 #pragma unroll
-            for (int c = 0; c < 3; ++c) B[c] += mt * (3.0 * ci(i, c) + (9.0 * cJ * ci(i, c) - 3.0 * Jv[c]) / rho);
+            for (int c = 0; c < 3; ++c)
+                B[c] += mt * (C(3) * (C)ci(i, c) + (C(9) * cJ * (C)ci(i, c) - C(3) * Jv[c]) / rho);
             if constexpr (DYN == 1) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
 #pragma unroll
                     for (int e = 0; e < 3; ++e)
-                        M[c][e] += 4.5 * mt * (ci(i, c) * ci(i, e) - (c == e ? 1.0 / 3.0 : 0.0));
+                        M[c][e] += C(4.5) * mt * ((C)(ci(i, c) * ci(i, e)) - (c == e ? C(1) / C(3) : C(0)));
             }
             if constexpr (FORCED) {
-                const double cF = cx(i) * F[0] + cy(i) * F[1] + cz(i) * F[2];
+                const C cF = (C)cx(i) * F[0] + (C)cy(i) * F[1] + (C)cz(i) * F[2];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) gG[c] += mt * (-3.0 * F[c] + 9.0 * cF * ci(i, c));
+                for (int c = 0; c < 3; ++c) gG[c] += mt * (C(-3) * F[c] + C(9) * cF * (C)ci(i, c));
             }
         }
         if constexpr (FORCED) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) gG[c] *= (1.0 - w / 2.0);
+            for (int c = 0; c < 3; ++c) gG[c] *= (C(1) - w / C(2));
         }
-        const double BF = B[0] * F[0] + B[1] * F[1] + B[2] * F[2];
-        const double gj = gG[0] * j[0] + gG[1] * j[1] + gG[2] * j[2];
-        double s, v[3];
+        const C BF = B[0] * F[0] + B[1] * F[1] + B[2] * F[2];
+        const C gj = gG[0] * j[0] + gG[1] * j[1] + gG[2] * j[2];
+        C s, v[3];
         if constexpr (DYN == 0) {
-            s = w * (A + dm * BF / 2) - dm * gj / (rho * rho);
+            s = w * (A + dm * BF / C(2)) - dm * gj / (rho * rho);
 #pragma unroll
             for (int c = 0; c < 3; ++c) v[c] = w * dm * B[c] + dm * gG[c] / rho;
         } else {
-            double Mj[3];
+            C Mj[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) Mj[c] = M[c][0] * j[0] + M[c][1] * j[1] + M[c][2] * j[2];
-            const double jMj = j[0] * Mj[0] + j[1] * Mj[1] + j[2] * Mj[2];
-            const double trM = M[0][0] + M[1][1] + M[2][2];
-            s = A + dm * BF / 2 + (1 - w) * (jMj / (rho * rho) - trM / 3.0) - dm * gj / (rho * rho);
+            const C jMj = j[0] * Mj[0] + j[1] * Mj[1] + j[2] * Mj[2];
+            const C trM = M[0][0] + M[1][1] + M[2][2];
+            s = A + dm * BF / C(2) + (C(1) - w) * (jMj / (rho * rho) - trM / C(3)) - dm * gj / (rho * rho);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) v[c] = dm * B[c] - 2.0 * (1 - w) * Mj[c] / rho + dm * gG[c] / rho;
+            for (int c = 0; c < 3; ++c) v[c] = dm * B[c] - C(2) * (C(1) - w) * Mj[c] / rho + dm * gG[c] / rho;
         }
         const int gx = g.gx0 + x;
-        double djdf[19];
-        objective_gradient(a, gx, al, djdf);
+        // dj/df of the objective (objective_gradient), in C
+        const C p = (C)((gx == a.xin ? a.inW : 0.0) - (gx == a.xout ? a.outW : 0.0));
+        const C djBase = (C)a.wdp * p, djX = (C)(a.wdrag * a.omega * al64 + a.wflux);
+        const C invT = (C)a.invT;
 #pragma unroll
         for (int k = 0; k < 19; ++k) {
-            double val = s + v[0] * cx(k) + v[1] * cy(k) + v[2] * cz(k);
+            C val = s + v[0] * (C)cx(k) + v[1] * (C)cy(k) + v[2] * (C)cz(k);
             if constexpr (DYN == 0) {
-                val += (1 - w) * mu[k];
+                val += (C(1) - w) * mu[k];
             } else {
-                double cMc = 0;
+                C cMc = C(0);
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
 #pragma unroll
-                    for (int e = 0; e < 3; ++e) cMc += ci(k, c) * M[c][e] * ci(k, e);
-                val += (1 - w) * cMc;
+                    for (int e = 0; e < 3; ++e) cMc += (C)ci(k, c) * M[c][e] * (C)ci(k, e);
+                val += (C(1) - w) * cMc;
             }
-            if (a.explicitTerm) val += a.invT * djdf[k];
+            if (a.explicitTerm) val += invT * (djBase + djX * (C)cx(k));
             lam[(uint64_t)k * g.slot + cell] = (LT)val;
         }
         if (in_bulk(g, x, y, z)) {
-            const double wB = DYN == 0 ? w : 1.0;
-            double ga = -(wB * (B[0] * (j[0] + rho * F[0] / 2) + B[1] * (j[1] + rho * F[1] / 2) +
-                                B[2] * (j[2] + rho * F[2] / 2)) +
-                          gG[0] * (j[0] / rho + F[0] / 2) + gG[1] * (j[1] / rho + F[1] / 2) +
-                          gG[2] * (j[2] / rho + F[2] / 2));
+            const C wB = DYN == 0 ? w : C(1);
+            const C ga = -(wB * (B[0] * (j[0] + rho * F[0] / C(2)) + B[1] * (j[1] + rho * F[1] / C(2)) +
+                                 B[2] * (j[2] + rho * F[2] / C(2))) +
+                           gG[0] * (j[0] / rho + F[0] / C(2)) + gG[1] * (j[1] / rho + F[1] / C(2)) +
+                           gG[2] * (j[2] / rho + F[2] / C(2)));
+            double ga64 = (double)ga;
             if (a.explicitTerm) {
-                ga += a.invT * a.wdrag * w * j[0];
-                const double p = (gx == a.xin ? a.inW : 0.0) - (gx == a.xout ? a.outW : 0.0);
-                J += a.invT * (a.wdp * p * rho + a.wdrag * w * al * j[0] + a.wflux * j[0]);
+                ga64 += a.invT * a.wdrag * a.omega * (double)j[0];
+                const double p64 = (gx == a.xin ? a.inW : 0.0) - (gx == a.xout ? a.outW : 0.0);
+                J += a.invT * (a.wdp * p64 * (double)rho + a.wdrag * a.omega * al64 * (double)j[0] +
+                               a.wflux * (double)j[0]);
             }
-            if (a.gAlpha) a.gAlpha[idx] += ga;
+            if (a.gAlpha) a.gAlpha[idx] += ga64;
         }
     }
 }
