@@ -96,6 +96,7 @@ struct Lattice {
     bool staged = false;           // odd steps through bulk_odd_staged
     bool pairs = false;            // 16-bit storage through bulk_pairs
     bool packed = false;           // bulk_pairs with whole rows per workgroup
+    bool zWrap = false;            // every block periodic along z
     int ldsPitch = 0;              // packed: LDS pair words per staged direction
     size_t pairsLds = 0;           // its dynamic LDS bytes (odd steps)
     size_t stagedLds = 0;          // its dynamic LDS bytes
@@ -284,6 +285,8 @@ int setup(Lattice& L) {
     L.staged = staged;
     L.pairs = pairs;
     L.packed = packed;
+    L.zWrap = true;
+    for (const BlockGeom& g : L.geom) L.zWrap = L.zWrap && g.pz != 0;
     L.ldsPitch = 0;
     if (packed) {
         for (BlockGeom& g : L.geom) {
@@ -680,6 +683,9 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = 
         if (L.packed)                                                                                           \
             hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO, false, true>), grid, L.bulkBlock,             \
                                O ? L.pairsLds : 0, L.stream, a);                                                \
+        else if (O && L.zWrap)                                                                                  \
+            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO, false, false, true>), grid, L.bulkBlock,      \
+                               L.pairsLds, L.stream, a);                                                        \
         else                                                                                                    \
             hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO>), grid, L.bulkBlock, O ? L.pairsLds : 0,      \
                                L.stream, a);                                                                    \

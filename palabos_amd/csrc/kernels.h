@@ -722,8 +722,11 @@ __device__ __forceinline__ uint32_t pack16(uint16_t lo, uint16_t hi) { return (u
 // half the workgroup: the lanes a one-row segment would leave idle take the next
 // rows; every row holds at least two pairs, so that its low and high halo words
 // come from different lanes). Without it a workgroup takes one segment of up to
-// 2W cells of one row and the row is wavefront-uniform.
-template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false, bool PACKED = false>
+// 2W cells of one row and the row is wavefront-uniform. ZWRAP: every block is
+// periodic along z (no z edge: the edge handling is compiled out, which keeps
+// the kernel within 80 registers).
+template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false, bool PACKED = false,
+         bool ZWRAP = false>
 __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
     static_assert(sizeof(typename Store<S>::R) == 2, "pairs of 16-bit values");
     using T = typename Store<S>::T;
@@ -892,7 +895,7 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             uint32_t halo[kZMoving];
             const bool segOk = rowOk && allValid;
             const bool loLane = segOk && p == 0 && vlo, hiLane = segOk && p == lanes - 1 && vhi;
-            const bool loEdge = segOk && p == 0 && !vlo, hiEdge = segOk && p == lanes - 1 && !vhi;
+            const bool loEdge = !ZWRAP && segOk && p == 0 && !vlo, hiEdge = !ZWRAP && segOk && p == lanes - 1 && !vhi;
             const bool edge = loEdge || hiEdge;
             if (loLane || hiLane || edge) {
                 // edge lanes load every z-moving direction's end pair; the shifted
@@ -925,7 +928,7 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             }
             __syncthreads();
             // shifted inputs: c_z = +1 reads z - 1, c_z = -1 reads z + 1
-            const bool vzA = zA > 0 || g.pz, vzB = zB < g.nz - 1 || g.pz;
+            const bool vzA = ZWRAP || zA > 0 || g.pz, vzB = ZWRAP || zB < g.nz - 1 || g.pz;
             if (sAct) {
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
