@@ -260,7 +260,8 @@ typedef struct {
  * every block even along z); no per-cell tags (the walls are the domain
  * edges); planes only on single-block BGK lattices without a body force, with
  * zero porosity on them; started after an even number of steps. Multi-block
- * lattices refresh the envelopes of the adjoint state after every reverse step.
+ * lattices refresh the first envelope layer of the adjoint state after every
+ * reverse step.
  * No Palabos counterpart. */
 int plbamd_adjoint_gradient(int handle, int nsteps, int checkpoint_every, const plbamd_objective* objective,
                             double* J, double* grad_alpha, double* grad_plane_scale, double* timing_ms);

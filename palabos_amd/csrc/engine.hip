@@ -67,6 +67,11 @@ struct Lattice {
     uint16_t* dEvenBlock = nullptr; // its block of every workgroup (multi-block)
     CopyTask* dTasks = nullptr;     // envelope exchange: one workgroup per task
     uint32_t nTasks = 0;
+    // the adjoint state's exchange: the first envelope layer only (the reverse
+    // step reads lambda one cell beyond the bulk)
+    CopySegment* dSegs1 = nullptr;
+    CopyTask* dTasks1 = nullptr;
+    uint32_t nTasks1 = 0;
     uint64_t allocCells = 0;          // cells of all blocks (per-cell arrays)
     uint64_t cells = 0;               // cells of the domain
     size_t elemSize = 0;
@@ -114,7 +119,7 @@ struct Lattice {
 
     ~Lattice() {
         hipFree(arena); hipFree(dGeom); hipFree(dTags); hipFree(dSpecial); hipFree(dValues); hipFree(dAlpha);
-        hipFree(dBulkPartials); hipFree(dBulkLevel1); hipFree(dSpecialPartials); hipFree(dStats); hipFree(dSegs);
+        hipFree(dBulkPartials); hipFree(dBulkLevel1); hipFree(dSpecialPartials); hipFree(dStats); hipFree(dSegs); hipFree(dSegs1); hipFree(dTasks1);
         hipFree(dWgBlock); hipFree(dTasks); hipFree(dWalls); hipFree(dEvenBlock);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
@@ -367,15 +372,21 @@ int setup(Lattice& L) {
     PLBAMD_HIP(hipMalloc(&L.dStats, 16 * sizeof(double)));
     PLBAMD_HIP(hipMemset(L.dStats, 0, 16 * sizeof(double)));
 
-    // Envelope segments: the allocated box of block b intersected with the bulk
-    // of every block c shifted by every periodic image (not b itself unshifted).
-    if (L.multi) {
-        std::vector<CopySegment> segs;
-        uint64_t total = 0;
+    // Envelope segments: the allocated box of block b (or its bulk widened by
+    // `width` cells, clipped to the allocation) intersected with the bulk of
+    // every block c shifted by every periodic image (not b itself unshifted).
+    auto buildSegments = [&](int width, std::vector<CopySegment>& segs, uint64_t& total) {
+        segs.clear();
+        total = 0;
         for (size_t bi = 0; bi < L.geom.size(); ++bi) {
             const BlockGeom& gb = L.geom[bi];
-            const int alo[3] = {gb.gx0, gb.gy0, gb.gz0};
-            const int ahi[3] = {gb.gx0 + gb.nx - 1, gb.gy0 + gb.ny - 1, gb.gz0 + gb.nz - 1};
+            const int blo[3] = {gb.bx0, gb.by0, gb.bz0}, bhi[3] = {gb.bx1, gb.by1, gb.bz1};
+            const int ext[3] = {gb.nx, gb.ny, gb.nz}, org[3] = {gb.gx0, gb.gy0, gb.gz0};
+            int alo[3], ahi[3];
+            for (int a = 0; a < 3; ++a) {
+                alo[a] = org[a] + std::max(0, blo[a] - width);
+                ahi[a] = org[a] + std::min(ext[a] - 1, bhi[a] + width);
+            }
             for (size_t ci = 0; ci < L.geom.size(); ++ci) {
                 const plbamd_box& cb = L.bulk[ci];
                 const BlockGeom& gc = L.geom[ci];
@@ -415,20 +426,32 @@ int setup(Lattice& L) {
                         }
             }
         }
+    };
+    // device copy of a segment list and its tasks (256 cells each)
+    auto uploadSegments = [&](const std::vector<CopySegment>& segs, CopySegment** dSegs, CopyTask** dTasks,
+                              uint32_t* nTasks) -> int {
+        if (segs.empty()) return PLBAMD_OK;
+        PLBAMD_HIP(hipMalloc(dSegs, segs.size() * sizeof(CopySegment)));
+        PLBAMD_HIP(hipMemcpy(*dSegs, segs.data(), segs.size() * sizeof(CopySegment), hipMemcpyHostToDevice));
+        std::vector<CopyTask> tasks;
+        for (size_t k = 0; k < segs.size(); ++k) {
+            const uint64_t cnt = (uint64_t)segs[k].lx * segs[k].ly * segs[k].lz;
+            for (uint64_t f = 0; f < cnt; f += 256) tasks.push_back(CopyTask{(uint32_t)k, (uint32_t)f});
+        }
+        *nTasks = (uint32_t)tasks.size();
+        PLBAMD_HIP(hipMalloc(dTasks, tasks.size() * sizeof(CopyTask)));
+        PLBAMD_HIP(hipMemcpy(*dTasks, tasks.data(), tasks.size() * sizeof(CopyTask), hipMemcpyHostToDevice));
+        return PLBAMD_OK;
+    };
+    if (L.multi) {
+        std::vector<CopySegment> segs;
+        uint64_t total = 0;
+        buildSegments(1 << 30, segs, total);   // the whole allocation: both envelope layers
         L.nSegs = (int)segs.size();
         L.segCells = total;
-        if (!segs.empty()) {
-            PLBAMD_HIP(hipMalloc(&L.dSegs, segs.size() * sizeof(CopySegment)));
-            PLBAMD_HIP(hipMemcpy(L.dSegs, segs.data(), segs.size() * sizeof(CopySegment), hipMemcpyHostToDevice));
-            std::vector<CopyTask> tasks;
-            for (size_t k = 0; k < segs.size(); ++k) {
-                const uint64_t n = (uint64_t)segs[k].lx * segs[k].ly * segs[k].lz;
-                for (uint64_t f = 0; f < n; f += 256) tasks.push_back(CopyTask{(uint32_t)k, (uint32_t)f});
-            }
-            L.nTasks = (uint32_t)tasks.size();
-            PLBAMD_HIP(hipMalloc(&L.dTasks, tasks.size() * sizeof(CopyTask)));
-            PLBAMD_HIP(hipMemcpy(L.dTasks, tasks.data(), tasks.size() * sizeof(CopyTask), hipMemcpyHostToDevice));
-        }
+        PLBAMD_TRY(uploadSegments(segs, &L.dSegs, &L.dTasks, &L.nTasks));
+        buildSegments(1, segs, total);         // the first layer (the adjoint state)
+        PLBAMD_TRY(uploadSegments(segs, &L.dSegs1, &L.dTasks1, &L.nTasks1));
     }
     PLBAMD_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
     PLBAMD_HIP(hipEventCreate(&L.ev0));
@@ -1271,12 +1294,16 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
 #undef PLBAMD_ADJ
     auto exchangeLam = [&](char* lam) {
         if (!L.multi || !L.nSegs) return;
+        // the first envelope layer only: the reverse step gathers lambda one
+        // cell beyond the bulk, and every envelope cell it computes itself is
+        // replaced by this exchange
+        if (!L.nTasks1) return;
         if (lam64)
-            hipLaunchKernelGGL((exchange_envelopes<double>), dim3(L.nTasks), dim3(256), 0, L.stream, (void*)lam,
-                               L.dGeom, L.dSegs, L.dTasks);
+            hipLaunchKernelGGL((exchange_envelopes<double>), dim3(L.nTasks1), dim3(256), 0, L.stream, (void*)lam,
+                               L.dGeom, L.dSegs1, L.dTasks1);
         else
-            hipLaunchKernelGGL((exchange_envelopes<float>), dim3(L.nTasks), dim3(256), 0, L.stream, (void*)lam,
-                               L.dGeom, L.dSegs, L.dTasks);
+            hipLaunchKernelGGL((exchange_envelopes<float>), dim3(L.nTasks1), dim3(256), 0, L.stream, (void*)lam,
+                               L.dGeom, L.dSegs1, L.dTasks1);
     };
     // the plane cells' reverse step runs on a second stream next to the bulk one
     hipStream_t side = nullptr;
