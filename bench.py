@@ -190,7 +190,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the lattice would shard into x-slabs with per-face halo exchange over RCCL; see DESIGN.md)",
+              "(the lattice would shard into x-slabs with per-face halo exchange over RCCL)",
               file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
