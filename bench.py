@@ -189,7 +189,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true", help="run only the counter-calibration copy")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+        print("bench.py runs on one GPU only: the engine has no multi-GPU path "
               "(the lattice would shard into x-slabs with per-face halo exchange over RCCL)",
               file=sys.stderr)
         sys.exit(2)
