@@ -701,6 +701,52 @@ void runSteps(const OracleDesc& d, S* state, int nsteps, double* statsOut) {
     }
 }
 
+// Analysis fields of every cell (dataAnalysisWrapper3D.h:695-777), in T:
+// computeDensity, computeRhoBar, computeVelocity x y z, computeKineticEnergy,
+// computeVelocityNorm. Bulk cells: rho = fullRho(get_rhoBar) (sequential sum,
+// momentTemplates3D.h:375-382, core/dynamics.hh:278-281), u = get_j * invRho
+// (compute_uLb, momentTemplates3D.h:138-144), or with a Guo force
+// j * invRho(rhoBar of get_rhoBar_j) + F/2 (externalForceDynamics.hh:59-74);
+// BounceBack / NoDynamics / MomentumExchangeBounceBack: rho = their density
+// parameter, rhoBar = rho - 1 (NoDynamics: 0), u = 0 (core/dynamics.hh:814-880,
+// 994-1060; bounceBackModels.hh:160-223); link-wise wall solids rho 1, u 0;
+// energy normSqr(u) / 2 and norm sqrt(normSqr(u))
+// (dataAnalysisFunctional3D.hh:668-719). Returns false for a lattice with
+// planes (their closures are not restated here).
+template<typename S, typename T>
+bool fieldsAll(const OracleDesc& d, const S* state, const double* solidRho, double* out) {
+    if (d.nplanes > 0) return false;
+    Engine<T> eng(d);
+    const bool forced = d.force[0] != 0 || d.force[1] != 0 || d.force[2] != 0;
+    for (long k = 0; k < eng.N; ++k) {
+        T rho = (T)1, rhoBar = T(), u[3] = {T(), T(), T()};
+        const int tag = eng.tag[k];
+        if (tag == TAG_BB) { rho = (T)solidRho[0]; rhoBar = rho - (T)1; }
+        else if (tag == TAG_NODYN) { rho = (T)solidRho[1]; }
+        else if (tag == TAG_MEBB) { rho = (T)solidRho[2]; rhoBar = rho - (T)1; }
+        else if (tag == TAG_BULK) {
+            T f[19];
+            for (int i = 0; i < 19; ++i) f[i] = Store<S, T>::load(state[k*19 + i]);
+            rhoBar = get_rhoBar(f);
+            T rb, j[3];
+            get_rhoBar_j(f, rb, j);
+            const T ir = invRhoOf(forced ? rb : rhoBar);
+            for (int c = 0; c < 3; ++c) u[c] = forced ? j[c] * ir + (T)d.force[c] / (T)2 : j[c] * ir;
+            rho = rhoBar + (T)1;
+        }
+        const T uSqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        double* o = out + 7 * k;
+        o[0] = (double)rho;
+        o[1] = (double)rhoBar;
+        o[2] = (double)u[0];
+        o[3] = (double)u[1];
+        o[4] = (double)u[2];
+        o[5] = (double)(uSqr / (T)2);
+        o[6] = (double)std::sqrt(uSqr);
+    }
+    return true;
+}
+
 template<typename S, typename T>
 void completeAll(const OracleDesc& d, S* state) {
     Engine<T> eng(d);
@@ -816,6 +862,22 @@ int oracle_complete(const OracleDesc* d, void* state) {
     case 3: completeAll<BHalf, float>(*d, (BHalf*)state); return 0;
     }
     return -1;
+}
+
+// Analysis fields of `state` (see fieldsAll): out[cell][7]. solid_rho: density
+// parameters of BounceBack, NoDynamics, MomentumExchangeBounceBack. Returns 0,
+// -1 on a bad descriptor, -2 for a lattice with planes.
+int oracle_fields(const OracleDesc* d, const void* state, const double* solid_rho, double* out) {
+    if (!d || !state || !solid_rho || !out) return -1;
+    bool ok = false;
+    switch (d->storage) {
+    case 0: ok = fieldsAll<double, double>(*d, (const double*)state, solid_rho, out); break;
+    case 1: ok = fieldsAll<float, float>(*d, (const float*)state, solid_rho, out); break;
+    case 2: ok = fieldsAll<Half, float>(*d, (const Half*)state, solid_rho, out); break;
+    case 3: ok = fieldsAll<BHalf, float>(*d, (const BHalf*)state, solid_rho, out); break;
+    default: return -1;
+    }
+    return ok ? 0 : -2;
 }
 
 // Bulk equilibrium populations f-bar for (rho, u) per cell, FP64

@@ -70,6 +70,8 @@ def lib():
                                                ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         _lib.oracle_objective_dual.restype = ctypes.c_int
+        _lib.oracle_fields.argtypes = [ctypes.POINTER(_Desc), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _lib.oracle_fields.restype = ctypes.c_int
         for name in ("oracle_f32_to_f16", "oracle_f16_to_f32", "oracle_f32_to_bf16",
                      "oracle_bf16_to_f32"):
             getattr(_lib, name).argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
@@ -137,6 +139,25 @@ def run(case, f, nsteps):
     return state, {"avg_rhobar": stats[0], "avg_usqr": stats[1], "max_usqr": stats[2],
                    "force": stats[3:6].copy(), "count": int(stats[6]),
                    "porous": stats[7:10].copy()}
+
+
+def fields(case, f, solid_rho=(1.0, 1.0, 1.0)):
+    """Analysis fields [nx, ny, nz, 7] of the state `f` (computeDensity,
+    computeRhoBar, computeVelocity x y z, computeKineticEnergy,
+    computeVelocityNorm) in the storage's arithmetic type, widened to float64;
+    solid_rho: density parameters of BounceBack, NoDynamics,
+    MomentumExchangeBounceBack. Lattices with planes are not covered."""
+    dtype = STORAGE_DTYPES[case.storage]
+    state = np.ascontiguousarray(f, dtype=dtype)
+    assert state.shape == (case.nx, case.ny, case.nz, 19)
+    keep = _Keep()
+    d = _desc(case, keep)
+    sr = np.asarray(solid_rho, dtype=np.float64)
+    out = np.empty((case.nx, case.ny, case.nz, 7), dtype=np.float64)
+    rc = lib().oracle_fields(ctypes.byref(d), state.ctypes.data, sr.ctypes.data, out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle_fields rejected the case ({rc})")
+    return out
 
 
 def complete(case, f):

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import palabos_amd as plb
-from cases import gpu_lattice
+from cases import Case, gpu_lattice, random_state
 from golden_cases import build
 from oracle import oracle
 
@@ -97,3 +97,28 @@ def test_solid_density_parameters(gpu):
         np.testing.assert_array_equal(f[3:, ..., 0], 1.0)   # rest state at equilibrium
     finally:
         lat.close()
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("forced", [False, True], ids=["plain", "guo"])
+def test_sixteen_bit_fields_match_oracle(gpu, storage, forced):
+    """16-bit storage (FP32 arithmetic): the fields of the stored state equal the
+    oracle's on the same raw bits (the oracle's fields are pinned to Palabos's in
+    FP32 and FP64 by tests/test_oracle_fields.py)."""
+    tags = np.zeros((10, 9, 12), np.uint8)
+    tags[2, 3, 4] = plb.BounceBack.tag
+    tags[5, 5, 5] = plb.NoDynamics.tag
+    case = Case(10, 9, 12, storage=storage, dynamics="rbgk", omega=1.6, periodic=(True, True, True), tags=tags,
+                force=(1e-5, -2e-6, 3e-7) if forced else (0.0, 0.0, 0.0))
+    lat = gpu_lattice(case)
+    try:
+        lat.setPopulations(oracle.to_storage(random_state(case, 5), storage))
+        lat.collideAndStream(6)
+        raw = lat.getPopulations(dtype=np.uint16)
+        got = lat.analysisFields()
+    finally:
+        lat.close()
+    want = oracle.fields(case, raw)
+    for k, what in enumerate(NAMES):
+        bad = np.count_nonzero(got[..., k] != want[..., k])
+        assert bad == 0, f"{storage}: {what} differs in {bad} cells"
