@@ -3,6 +3,7 @@ declares (no device calls: these run without a GPU)."""
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -31,9 +32,12 @@ def test_library_exports_every_declared_symbol():
         getattr(lib, name)
 
 
-def test_last_error_is_empty_string_before_any_failure():
+def test_unknown_handle_is_rejected_with_a_message():
+    # Handle lookup happens before any device call, so this runs without a GPU.
     from palabos_amd import native
-    assert native.lib().plbamd_last_error() in (b"",) or isinstance(native.lib().plbamd_last_error(), bytes)
+    lib = native.lib()
+    assert lib.plbamd_lattice_destroy(123456) == native.EINVAL
+    assert b"unknown handle" in lib.plbamd_last_error()
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
@@ -42,3 +46,13 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(native, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(native.NativeLibraryError):
         native.lib()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # The benchmark has a single-GPU path only; asking for more must end before
+    # any device is touched, with a message that says why.
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
+    assert r.stdout == ""
