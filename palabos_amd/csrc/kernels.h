@@ -521,8 +521,13 @@ __host__ __device__ constexpr int zmove_index(int i) {
 }
 constexpr int kZMoving = 10;
 
+// FP32 statistics variants without forcing: asking for 6 waves per SIMD keeps
+// them at 80 VGPRs or fewer with no spills (99 otherwise, 4 waves), which the
+// short-row lattices need (one 128-lane workgroup per row). Every other
+// instantiation is unconstrained (1 = the default).
 template<typename S, int DYN, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
-__global__ void __launch_bounds__(512) bulk_odd_staged(StepArgs a) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
+    (sizeof(S) == 4 && STATS && !FORCED) ? 6 : 1))) bulk_odd_staged(StepArgs a) {
     using T = typename Store<S>::T;
     using R = typename Store<S>::R;
     // LDS words of at least 32 bits: 16-bit values would put two lanes on one bank word
