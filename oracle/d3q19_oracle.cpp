@@ -24,10 +24,17 @@
 //        -> pinned bit-exact against compiled Palabos (ForcedD3Q19 +
 //           GuoExternalForceBGKdynamics) golden vectors
 //   RBGK + Guo, Brinkman porosity term (no Palabos counterpart)
-//        -> parity unpinned against a reference; checked by invariants only
-//           (alpha = 0 and F = 0 reduce to the pinned paths, the Guo source
-//           adds exactly F of momentum per step, alpha = 1 cells relax to rest)
-//           [tests/test_oracle_invariants.py]
+//        -> pinned per cell by the closed forms of the post-collision mass,
+//           first and second moments and of the porous statistic, at random
+//           rho, 0 < alpha < 1 and F != 0, each shown to reject the plausible
+//           mistakes (regularizing around j instead of j_eq, damping j instead
+//           of u, the wrong velocity in the Guo source, PiNeq from j_eq)
+//           [tests/test_oracle_invariants.py:
+//            test_forced_porous_collision_moments_closed_form,
+//            test_rbgk_guo_without_porosity_closed_form,
+//            test_brinkman_intermediate_alpha_without_force];
+//           alpha = 0 and F = 0 reduce to the Palabos-pinned paths, alpha = 1
+//           cells relax to rest [test_brinkman_solid_relaxes_momentum_geometrically]
 
 #include <cstdint>
 #include <cstring>

@@ -259,3 +259,158 @@ def test_dual_number_derivatives_match_finite_differences():
     cp.planes[0].values = case.planes[0].values * (1 + eps)
     cm.planes[0].values = case.planes[0].values * (1 - eps)
     assert dJs == pytest.approx((J_of(cp) - J_of(cm)) / (2 * eps), rel=1e-7)
+
+
+# -- closed-form post-collision moments: RBGK + Guo, Brinkman at 0 < alpha < 1 ---
+#
+# Palabos defines BGK + Guo (externalForceDynamics.hh:215-232) but neither a
+# regularized BGK with a Guo force nor a Brinkman term, so these collisions are
+# pinned here by what the moment algebra of D3Q19 fixes per cell. With
+# rho = 1 + rhoBar, j and Pi = sum_i c_i c_i f-bar_i taken before the collision,
+#   u0 = j / rho + F / 2,  u = (1 - alpha) u0,  jeq = rho u,
+#   Pi_eq = cs2 rhoBar I + jeq jeq^T / rho     (second moment of the shifted feq),
+#   G1 = (1 - omega/2) F,  G2 = (1 - omega/2) (u F^T + F u^T)   (Guo source, moments 1, 2),
+# the post-collision moments are
+#   BGK:  rhoBar' = rhoBar,  j' = (1 - omega) j + omega jeq + G1,
+#         Pi' = (1 - omega) Pi + omega Pi_eq + G2;
+#   RBGK: rhoBar' = rhoBar,  j' = jeq + G1,
+#         Pi' = Pi_eq + (1 - omega) (Pi - cs2 rhoBar I - j j^T / rho) + G2,
+# (the regularized non-equilibrium part carries PiNeq of the pre-collision
+# moments, dynamicsTemplates3D.h:1902-1973 / momentTemplates3D.h:517-539, and no
+# first moment), and the porous statistic sums omega rho alpha u0 (BGK) or
+# rho alpha u0 (RBGK) over the cells. Each identity uses only the isotropy of
+# the D3Q19 weights up to fourth order, which the lattice has.
+
+
+def _cell_moments(f):
+    f = f.reshape(-1, 19).astype(np.float64)
+    rho_bar = f.sum(1)
+    j = f @ C
+    pi = np.einsum("ka,kb,nk->nab", C, C, f)
+    return rho_bar, j, pi
+
+
+def _unstream(f1):
+    """Post-collision populations of a fully periodic lattice after one step:
+    streaming moved f'_i(x) to x + c_i."""
+    return np.stack([np.roll(f1[..., i], shift=tuple(-C[i]), axis=(0, 1, 2)) for i in range(19)], -1)
+
+
+def _closed_form(dynamics, omega, F, alpha, rho_bar, j, pi, *, jeq_from_j=False, guo_u="damped",
+                 rlb_with_j=False, pineq_with_jeq=False):
+    """Post-collision (j', Pi') and the per-cell porous drag of the collisions
+    defined above; the keyword switches give plausible mistakes instead."""
+    rho = 1.0 + rho_bar
+    u0 = j / rho[:, None] + F / 2
+    u = (1 - alpha)[:, None] * u0
+    jeq = (1 - alpha)[:, None] * j + rho[:, None] * F / 2 if jeq_from_j else rho[:, None] * u
+    ug = {"damped": u, "undamped": u0, "no_half_force": (1 - alpha)[:, None] * j / rho[:, None]}[guo_u]
+    g1 = (1 - omega / 2) * F
+    g2 = (1 - omega / 2) * (np.einsum("na,b->nab", ug, F) + np.einsum("a,nb->nab", F, ug))
+    eye = np.eye(3)[None]
+    cs2 = 1.0 / 3.0
+    jj = np.einsum("na,nb->nab", jeq, jeq) / rho[:, None, None]
+    pi_eq = cs2 * rho_bar[:, None, None] * eye + jj
+    if dynamics == "bgk":
+        j1 = (1 - omega) * j + omega * jeq + g1
+        pi1 = (1 - omega) * pi + omega * pi_eq + g2
+        porous = omega * rho[:, None] * alpha[:, None] * u0
+    else:
+        jr = j if rlb_with_j else jeq
+        j1 = jr + g1
+        pi_eq_r = cs2 * rho_bar[:, None, None] * eye + np.einsum("na,nb->nab", jr, jr) / rho[:, None, None]
+        jn = jeq if pineq_with_jeq else j
+        pineq = pi - cs2 * rho_bar[:, None, None] * eye - np.einsum("na,nb->nab", jn, jn) / rho[:, None, None]
+        pi1 = pi_eq_r + (1 - omega) * pineq + g2
+        porous = rho[:, None] * alpha[:, None] * u0
+    return j1, pi1, porous.sum(0)
+
+
+# Mistakes each closed-form test must tell apart from the definition: forming
+# j_eq from damped j instead of damped u, regularizing around j instead of j_eq,
+# the undamped velocity or u without F/2 in the Guo source, PiNeq from j_eq.
+_MISTAKES = {
+    "bgk": [dict(jeq_from_j=True), dict(guo_u="undamped"), dict(guo_u="no_half_force")],
+    "rbgk": [dict(jeq_from_j=True), dict(rlb_with_j=True), dict(guo_u="undamped"),
+             dict(guo_u="no_half_force"), dict(pineq_with_jeq=True)],
+}
+
+
+@pytest.mark.parametrize("dynamics,omega", [("bgk", 1.3), ("bgk", 1.85), ("rbgk", 1.7), ("rbgk", 0.9)])
+def test_forced_porous_collision_moments_closed_form(dynamics, omega):
+    """RBGK + Guo and BGK + Guo with a Brinkman porosity 0 < alpha < 1 and random
+    rho: per cell, mass, first and second moments after the collision equal the
+    closed forms above to rounding, the porous statistic equals its sum, and
+    every plausible mistake in _MISTAKES misses by more than 1e3 times the
+    tolerance (e.g. regularizing around j instead of j_eq moves j' by
+    rho F / 2 - alpha rho u0, damping j instead of u moves j_eq by alpha rho F / 2)."""
+    shape = (6, 7, 8)
+    F = np.array([3e-4, -2e-4, 1.5e-4])
+    alpha = 0.05 + 0.9 * (uniform(shape, 40) * 0.5 + 0.5)
+    case = Case(*shape, dynamics=dynamics, omega=omega, periodic=(True, True, True),
+                force=tuple(F), alpha=alpha)
+    f0 = random_state(case, 41, rho_amp=0.2, u_amp=0.06, noise=2e-3)
+    rho_bar, j, pi = _cell_moments(f0)
+    f1, st = oracle.run(case, f0, 1)
+    rb1, j1, pi1 = _cell_moments(_unstream(f1))
+    a = alpha.reshape(-1)
+    tol = 2e-15
+    assert np.abs(rb1 - rho_bar).max() < tol
+    want_j, want_pi, want_porous = _closed_form(dynamics, omega, F, a, rho_bar, j, pi)
+    assert np.abs(j1 - want_j).max() < tol
+    assert np.abs(pi1 - want_pi).max() < tol
+    np.testing.assert_allclose(st["porous"], want_porous, rtol=1e-12, atol=1e-14)
+    for wrong in _MISTAKES[dynamics]:
+        wj, wpi, _ = _closed_form(dynamics, omega, F, a, rho_bar, j, pi, **wrong)
+        miss = max(np.abs(j1 - wj).max(), np.abs(pi1 - wpi).max())
+        assert miss > 1e3 * tol, f"closed form cannot tell {wrong} apart"
+    # the porous statistic of RBGK carries no omega, that of BGK does
+    other = _closed_form("rbgk" if dynamics == "bgk" else "bgk", omega, F, a, rho_bar, j, pi)[2]
+    assert np.abs(other - st["porous"]).max() > 1e-6
+
+
+@pytest.mark.parametrize("dynamics", ["bgk", "rbgk"])
+def test_rbgk_guo_without_porosity_closed_form(dynamics):
+    """alpha = 0 (no porosity field): j' = (1 - omega) j + omega rho u0 + (1 - omega/2) F for
+    BGK (Palabos's GuoExternalForceBGKdynamics, pinned by golden vectors) and
+    j' = j + rho F / 2 + (1 - omega/2) F for RBGK + Guo, with the second moments above."""
+    shape = (5, 6, 7)
+    F = np.array([-2.5e-4, 1e-4, 3e-4])
+    omega = 1.55
+    case = Case(*shape, dynamics=dynamics, omega=omega, periodic=(True, True, True), force=tuple(F))
+    f0 = random_state(case, 43, rho_amp=0.2, u_amp=0.06, noise=2e-3)
+    rho_bar, j, pi = _cell_moments(f0)
+    f1, _ = oracle.run(case, f0, 1)
+    rb1, j1, pi1 = _cell_moments(_unstream(f1))
+    zero = np.zeros(rho_bar.shape)
+    want_j, want_pi, _ = _closed_form(dynamics, omega, F, zero, rho_bar, j, pi)
+    assert np.abs(rb1 - rho_bar).max() < 2e-15
+    assert np.abs(j1 - want_j).max() < 2e-15
+    assert np.abs(pi1 - want_pi).max() < 2e-15
+    if dynamics == "rbgk":
+        rho = 1 + rho_bar
+        np.testing.assert_allclose(j1, j + rho[:, None] * F / 2 + (1 - omega / 2) * F, rtol=0, atol=2e-15)
+
+
+@pytest.mark.parametrize("dynamics", ["bgk", "rbgk"])
+def test_brinkman_intermediate_alpha_without_force(dynamics):
+    """No body force, 0 < alpha < 1: j' = (1 - omega) j + omega (1 - alpha) j (BGK) or
+    (1 - alpha) j (RBGK); damping u and damping j coincide here, so this pins the
+    damping factor itself (e.g. alpha in place of 1 - alpha fails by omega (1 - 2 alpha) j)."""
+    shape = (5, 6, 7)
+    omega = 1.45
+    alpha = 0.5 * (uniform(shape, 44) * 0.5 + 0.5) + 0.1
+    case = Case(*shape, dynamics=dynamics, omega=omega, periodic=(True, True, True), alpha=alpha)
+    f0 = random_state(case, 45, rho_amp=0.2, u_amp=0.06, noise=2e-3)
+    rho_bar, j, pi = _cell_moments(f0)
+    f1, st = oracle.run(case, f0, 1)
+    _, j1, pi1 = _cell_moments(_unstream(f1))
+    a = alpha.reshape(-1)[:, None]
+    keep = (1 - omega) + omega * (1 - a) if dynamics == "bgk" else (1 - a)
+    assert np.abs(j1 - keep * j).max() < 2e-15
+    wrong = (1 - omega) + omega * a if dynamics == "bgk" else a
+    assert np.abs(j1 - wrong * j).max() > 1e-4
+    removed = (omega if dynamics == "bgk" else 1.0) * (a * j).sum(0)
+    np.testing.assert_allclose(st["porous"], removed, rtol=1e-12, atol=1e-15)
+    want_j, want_pi, _ = _closed_form(dynamics, omega, np.zeros(3), a[:, 0], rho_bar, j, pi)
+    assert np.abs(pi1 - want_pi).max() < 2e-15
