@@ -3,31 +3,72 @@
 The library is compiled with floating-point contraction off so that FP64/FP32
 results keep the reference's rounding, and with packed FP32 instructions
 disabled (-packed-fp32-ops) for gfx950.
+
+The kernels are templates with many variants, so the library is built from
+several translation units compiled in parallel: the step launches once per
+storage type and dynamics (csrc/step.hip), the other typed operations once
+per storage type (csrc/storage_ops.hip), the adjoint (csrc/adjoint.hip) and the
+set-up and C ABI (csrc/engine.hip). Objects go to palabos_amd/_build/.
 """
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpalabos_amd.so")
-SOURCES = [os.path.join(HERE, "csrc", "engine.hip")]
-HEADERS = [os.path.join(HERE, "csrc", "d3q19_device.h"), os.path.join(ROOT, "include", "palabos_amd.h")]
+OBJ_DIR = os.path.join(HERE, "_build")
+HEADERS = [os.path.join(CSRC, "d3q19_device.h"), os.path.join(ROOT, "include", "palabos_amd.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result", "-Wno-unused-value",
 ]
 
+STORAGES = ["double", "float", "f16s", "bf16s"]
+
+
+def units():
+    """(object name, source, extra defines) of every translation unit."""
+    out = [("engine", "engine.hip", []), ("adjoint", "adjoint.hip", [])]
+    for s in STORAGES:
+        out.append((f"ops_{s}", "storage_ops.hip", [f"-DPLBAMD_S={s}"]))
+        for dyn in (0, 1):
+            out.append((f"step_{s}_{dyn}", "step.hip", [f"-DPLBAMD_S={s}", f"-DPLBAMD_DYN={dyn}"]))
+    return out
+
+
+def _jobs():
+    env = os.environ.get("MAX_JOBS")
+    n = int(env) if env and env.isdigit() else (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
 
 def build_library(verbose=False):
     """Compiles palabos_amd/libpalabos_amd.so (always recompiles)."""
-    cmd = [HIPCC] + FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", LIB + ".tmp"] + SOURCES
-    res = subprocess.run(cmd, capture_output=True, text=True)
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    include = ["-I", os.path.join(ROOT, "include")]
+
+    def compile_one(unit):
+        name, src, defs = unit
+        obj = os.path.join(OBJ_DIR, name + ".o")
+        cmd = [HIPCC] + FLAGS + include + defs + ["-c", os.path.join(CSRC, src), "-o", obj]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        return obj, res, cmd
+
+    with ThreadPoolExecutor(_jobs()) as pool:
+        results = list(pool.map(compile_one, units()))
+    for obj, res, cmd in results:
+        if res.returncode != 0:
+            raise RuntimeError("hipcc failed: " + " ".join(cmd) + "\n" + res.stdout + res.stderr)
+    link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + [o for o, _, _ in results]
+    res = subprocess.run(link, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
+        raise RuntimeError("link failed:\n" + res.stdout + res.stderr)
     os.replace(LIB + ".tmp", LIB)
     if verbose:
         print("built", LIB)

@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(256) adjoint_final(AdjointArgs a) {
 
 // Sum of the per-workgroup accumulators of the bulk and plane kernels (one
 // workgroup, fixed order).
-__global__ void __launch_bounds__(256) sum_values(const double* a, uint64_t na, const double* b, uint64_t nb,
+static __global__ void __launch_bounds__(256) sum_values(const double* a, uint64_t na, const double* b, uint64_t nb,
                                                   double* out) {
     double v = 0.0;
     for (uint64_t i = threadIdx.x; i < na; i += blockDim.x) v += a[i];

@@ -55,7 +55,7 @@ __host__ __device__ inline uint32_t b64_quad(uint32_t b0, uint32_t b1, uint32_t 
 constexpr uint64_t kB64Bytes = 48, kB64Chars = 64;
 
 // in: n bytes (16-byte aligned), out: 4 * ceil(n / 3) characters (16-byte aligned).
-__global__ void __launch_bounds__(256) base64_encode(const uint8_t* __restrict__ in, uint64_t n, char* __restrict__ out) {
+static __global__ void __launch_bounds__(256) base64_encode(const uint8_t* __restrict__ in, uint64_t n, char* __restrict__ out) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t full = n / kB64Bytes;
     if (t < full) {
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(256) base64_encode(const uint8_t* __restrict__
 // in: 4 * ceil(n / 3) characters (16-byte aligned), out: n bytes (16-byte
 // aligned). A character outside the alphabet where a value is expected sets
 // *bad (padding is accepted only where the byte count ends).
-__global__ void __launch_bounds__(256) base64_decode(const char* __restrict__ in, uint64_t n, uint8_t* __restrict__ out,
+static __global__ void __launch_bounds__(256) base64_decode(const char* __restrict__ in, uint64_t n, uint8_t* __restrict__ out,
                                                      int* bad) {
     // character -> value table (64: not in the alphabet)
     __shared__ uint8_t lut[256];

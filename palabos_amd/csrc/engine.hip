@@ -1,24 +1,7 @@
-// palabos_amd engine: D3Q19 collide-and-stream on gfx950 (MI355X) — host
-// runtime and C ABI (include/palabos_amd.h). The kernels and the data layout
-// are described in kernels.h.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cctype>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <map>
-#include <memory>
-#include <mutex>
-#include <string>
-#include <type_traits>
-#include <vector>
-
-#include "../../include/palabos_amd.h"
-#include "adjoint.h"
-#include "checkpoint.h"
-#include "kernels.h"
+// palabos_amd engine: D3Q19 collide-and-stream on gfx950 (MI355X) — lattice
+// set-up, the step loop and the C ABI (include/palabos_amd.h). The kernels and
+// the data layout are described in kernels.h.
+#include "engine.h"
 
 namespace plbamd {
 
@@ -28,104 +11,6 @@ int fail(int code, const std::string& msg) {
     g_lastError = msg;
     return code;
 }
-
-#define PLBAMD_HIP(expr)                                                                  \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail(PLBAMD_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-#define PLBAMD_TRY(expr)          \
-    do {                          \
-        int rc_ = (expr);         \
-        if (rc_) return rc_;      \
-    } while (0)
-
-// Envelope width of multi-block lattices: the AA pattern needs two layers
-// refreshed every second step (kernels.h).
-constexpr int kEnvelope = 2;
-constexpr int kEvenThreads = 256;   // threads per workgroup of the linear even step
-
-struct Plane {
-    int family, kind, dir, ori;
-    bool wrapped;
-    plbamd_box box;
-    std::vector<double> values;
-};
-
-struct Lattice {
-    plbamd_lattice_desc desc;
-    bool multi = false;
-    std::vector<plbamd_box> bulk;     // global bulk box of each block
-    std::vector<BlockGeom> geom;      // device geometry of each block (host copy)
-    BlockGeom* dGeom = nullptr;
-    uint16_t* dWgBlock = nullptr;   // block of every bulk work item (multi-block)
-    uint32_t* dWalls = nullptr;     // link-wall mask per allocated cell (cells next to link walls)
-    bool linearEven = false;        // even steps through bulk_even_linear
-    uint64_t nEven = 0;             // its workgroups
-    uint16_t* dEvenBlock = nullptr; // its block of every workgroup (multi-block)
-    CopyTask* dTasks = nullptr;     // envelope exchange: one workgroup per task
-    uint32_t nTasks = 0;
-    // the adjoint state's exchange: the first envelope layer only (the reverse
-    // step reads lambda one cell beyond the bulk)
-    CopySegment* dSegs1 = nullptr;
-    CopyTask* dTasks1 = nullptr;
-    uint32_t nTasks1 = 0;
-    uint64_t allocCells = 0;          // cells of all blocks (per-cell arrays)
-    uint64_t cells = 0;               // cells of the domain
-    size_t elemSize = 0;
-    void* arena = nullptr;
-    int64_t time = 0;
-    bool statsOn = true;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-
-    std::vector<uint8_t> hostTags;    // PLBAMD_TAG_* per domain cell, or empty
-    std::vector<Plane> planes;
-    std::vector<double> hostAlpha;    // per domain cell, or empty
-    double solidRho[3] = {1.0, 1.0, 1.0};  // density parameter of BounceBack, NoDynamics, momentum exchange
-
-    bool dirty = true;                // device tags / specials / porosity need a rebuild
-    bool anyWrapped = false;
-    uint8_t* dTags = nullptr;
-    SpecialCell* dSpecial = nullptr;
-    uint32_t nSpecial = 0;
-    double* dValues = nullptr;
-    double* dAlpha = nullptr;
-    double* dBulkPartials = nullptr;
-    double* dBulkLevel1 = nullptr;  // kStatsLevel1 records: first level of the statistics reduction
-    uint64_t nBulkPartials = 0;       // = workgroups of the bulk launch
-    int xcdRemap = 0;
-    std::vector<uint32_t> planeValueBase;   // first value of each plane in dValues
-    bool staged = false;           // odd steps through bulk_odd_staged
-    bool pairs = false;            // 16-bit storage through bulk_pairs
-    bool packed = false;           // bulk_pairs with whole rows per workgroup
-    bool zWrap = false;            // every block periodic along z
-    int ldsPitch = 0;              // packed: LDS pair words per staged direction
-    size_t pairsLds = 0;           // its dynamic LDS bytes (odd steps)
-    size_t stagedLds = 0;          // its dynamic LDS bytes
-    double* dSpecialPartials = nullptr;
-    uint64_t nSpecialPartials = 0;
-    double* dStats = nullptr;         // 16 doubles
-    bool statsValid = false;
-
-    CopySegment* dSegs = nullptr;
-    int nSegs = 0;
-    uint64_t segCells = 0;
-
-    dim3 bulkBlock;
-    int zChunk = 0;
-
-    ~Lattice() {
-        hipFree(arena); hipFree(dGeom); hipFree(dTags); hipFree(dSpecial); hipFree(dValues); hipFree(dAlpha);
-        hipFree(dBulkPartials); hipFree(dBulkLevel1); hipFree(dSpecialPartials); hipFree(dStats); hipFree(dSegs); hipFree(dSegs1); hipFree(dTasks1);
-        hipFree(dWgBlock); hipFree(dTasks); hipFree(dWalls); hipFree(dEvenBlock);
-        if (ev0) hipEventDestroy(ev0);
-        if (ev1) hipEventDestroy(ev1);
-        if (stream) hipStreamDestroy(stream);
-    }
-};
 
 std::mutex g_mutex;
 std::map<int, std::unique_ptr<Lattice>> g_lattices;
@@ -600,195 +485,6 @@ int rebuild(Lattice& L) {
 }
 
 template<typename S>
-int exchange(Lattice& L) {
-    if (!L.multi || L.nSegs == 0) return PLBAMD_OK;
-    hipLaunchKernelGGL((exchange_envelopes<S>), dim3(L.nTasks), dim3(256), 0, L.stream, L.arena, L.dGeom, L.dSegs,
-                       L.dTasks);
-    PLBAMD_HIP(hipGetLastError());
-    return PLBAMD_OK;
-}
-
-// One collide-and-stream step of every block. With `record` (FP64 BGK only,
-// statistics off), the kernels also write the pre-collision moments of every
-// cell and the pre-completion populations of every special record.
-struct Recording {
-    void* moments = nullptr;       // MomentType<S> [4][momStride]
-    uint64_t momStride = 0;
-    double* specialF = nullptr;
-};
-
-template<typename S, int DYN>
-int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record = nullptr) {
-    StepArgs a{};
-    a.arena = L.arena;
-    a.blocks = L.dGeom;
-    a.nBlocks = (int)L.geom.size();
-    a.zChunk = L.zChunk;
-    a.omega = L.desc.omega;
-    for (int c = 0; c < 3; ++c) a.force[c] = L.desc.force[c];
-    a.tags = L.dTags;
-    a.alpha = L.dAlpha;
-    a.partials = L.dBulkPartials;
-    a.xcdRemap = L.xcdRemap;
-    a.wgBlock = L.dWgBlock;
-    a.walls = L.dWalls;
-    a.ldsPitch = L.ldsPitch;
-    const bool forced = L.dAlpha || L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0;
-    const bool tags = L.dTags != nullptr;
-    const dim3 grid((unsigned)L.nBulkPartials);
-    if (record && stats) return fail(PLBAMD_EINVAL, "recording step: statistics must be off");
-    if (!odd && L.linearEven) {
-        // even steps in memory order (bulk_even_linear)
-        a.evenBlock = L.dEvenBlock;
-        const dim3 eg((unsigned)L.nEven), eb(kEvenThreads);
-        if (record) {
-            a.moments = record->moments;
-            a.momStride = record->momStride;
-            if constexpr (std::is_same<S, double>::value || sizeof(typename Store<S>::R) == 2) {
-#define PLBAMD_EVR(TG, FO) hipLaunchKernelGGL((bulk_even_linear<S, DYN, TG, false, FO, true>), eg, eb, 0, L.stream, a)
-                if (tags) { if (forced) PLBAMD_EVR(true, true); else PLBAMD_EVR(true, false); }
-                else { if (forced) PLBAMD_EVR(false, true); else PLBAMD_EVR(false, false); }
-#undef PLBAMD_EVR
-            } else {
-                return fail(PLBAMD_EINVAL, "recording step: FP64 or 16-bit storage");
-            }
-        } else {
-#define PLBAMD_EV(TG, ST, FO) hipLaunchKernelGGL((bulk_even_linear<S, DYN, TG, ST, FO>), eg, eb, 0, L.stream, a)
-#define PLBAMD_EV_F(TG, ST) if (forced) PLBAMD_EV(TG, ST, true); else PLBAMD_EV(TG, ST, false)
-#define PLBAMD_EV_S(TG) if (stats) { PLBAMD_EV_F(TG, true); } else { PLBAMD_EV_F(TG, false); }
-            if (tags) { PLBAMD_EV_S(true) } else { PLBAMD_EV_S(false) }
-#undef PLBAMD_EV_S
-#undef PLBAMD_EV_F
-#undef PLBAMD_EV
-        }
-    } else if (record) {
-        a.moments = record->moments;
-        a.momStride = record->momStride;
-        if constexpr (std::is_same<S, double>::value) {
-#define PLBAMD_REC(O, TG, FO)                                                                                   \
-    do {                                                                                                        \
-        if (O && L.staged)                                                                                      \
-            hipLaunchKernelGGL((bulk_odd_staged<double, DYN, TG, false, FO, true>), grid, L.bulkBlock,           \
-                               L.stagedLds, L.stream, a);                                                       \
-        else                                                                                                    \
-            hipLaunchKernelGGL((bulk_step<double, DYN, O, TG, false, FO, true>), grid, L.bulkBlock, 0, L.stream, a); \
-    } while (0)
-#define PLBAMD_REC_F(O, TG) if (forced) PLBAMD_REC(O, TG, true); else PLBAMD_REC(O, TG, false)
-#define PLBAMD_REC_T(O) if (tags) { PLBAMD_REC_F(O, true); } else { PLBAMD_REC_F(O, false); }
-            if (odd) { PLBAMD_REC_T(true) } else { PLBAMD_REC_T(false) }
-#undef PLBAMD_REC_T
-#undef PLBAMD_REC_F
-#undef PLBAMD_REC
-        } else if constexpr (sizeof(typename Store<S>::R) == 2) {
-            if (!L.pairs) return fail(PLBAMD_EINVAL, "recording step: 16-bit storage needs every block even along z");
-#define PLBAMD_RECP(O, TG, FO)                                                                                  \
-    do {                                                                                                        \
-        if (L.packed)                                                                                           \
-            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, false, FO, true, true>), grid, L.bulkBlock,           \
-                               O ? L.pairsLds : 0, L.stream, a);                                                \
-        else                                                                                                    \
-            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, false, FO, true>), grid, L.bulkBlock,                 \
-                               O ? L.pairsLds : 0, L.stream, a);                                                \
-    } while (0)
-#define PLBAMD_RECP_F(O, TG) if (forced) PLBAMD_RECP(O, TG, true); else PLBAMD_RECP(O, TG, false)
-#define PLBAMD_RECP_T(O) if (tags) { PLBAMD_RECP_F(O, true); } else { PLBAMD_RECP_F(O, false); }
-            if (odd) { PLBAMD_RECP_T(true) } else { PLBAMD_RECP_T(false) }
-#undef PLBAMD_RECP_T
-#undef PLBAMD_RECP_F
-#undef PLBAMD_RECP
-        } else {
-            return fail(PLBAMD_EINVAL, "recording step: FP64 or 16-bit storage");
-        }
-    } else if (L.pairs) {
-        if constexpr (sizeof(typename Store<S>::R) == 2) {
-#define PLBAMD_PAIRS(O, TG, ST, FO)                                                                             \
-    do {                                                                                                        \
-        if (L.packed)                                                                                           \
-            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO, false, true>), grid, L.bulkBlock,             \
-                               O ? L.pairsLds : 0, L.stream, a);                                                \
-        else if (O && L.zWrap)                                                                                  \
-            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO, false, false, true>), grid, L.bulkBlock,      \
-                               L.pairsLds, L.stream, a);                                                        \
-        else                                                                                                    \
-            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO>), grid, L.bulkBlock, O ? L.pairsLds : 0,      \
-                               L.stream, a);                                                                    \
-    } while (0)
-#define PLBAMD_PAIRS_F(O, TG, ST) if (forced) PLBAMD_PAIRS(O, TG, ST, true); else PLBAMD_PAIRS(O, TG, ST, false)
-#define PLBAMD_PAIRS_S(O, TG) if (stats) { PLBAMD_PAIRS_F(O, TG, true); } else { PLBAMD_PAIRS_F(O, TG, false); }
-#define PLBAMD_PAIRS_T(O) if (tags) { PLBAMD_PAIRS_S(O, true) } else { PLBAMD_PAIRS_S(O, false) }
-            if (odd) { PLBAMD_PAIRS_T(true) } else { PLBAMD_PAIRS_T(false) }
-#undef PLBAMD_PAIRS_T
-#undef PLBAMD_PAIRS_S
-#undef PLBAMD_PAIRS_F
-#undef PLBAMD_PAIRS
-        } else {
-            return fail(PLBAMD_EINVAL, "pair kernel: 16-bit storage only");
-        }
-    } else {
-#define PLBAMD_BULK(O, TG, ST, FO)                                                                              \
-    do {                                                                                                        \
-        if (O && L.staged)                                                                                      \
-            hipLaunchKernelGGL((bulk_odd_staged<S, DYN, TG, ST, FO>), grid, L.bulkBlock, L.stagedLds, L.stream,  \
-                               a);                                                                              \
-        else                                                                                                    \
-            hipLaunchKernelGGL((bulk_step<S, DYN, O, TG, ST, FO>), grid, L.bulkBlock, 0, L.stream, a);         \
-    } while (0)
-#define PLBAMD_BULK_F(O, TG, ST) if (forced) PLBAMD_BULK(O, TG, ST, true); else PLBAMD_BULK(O, TG, ST, false)
-#define PLBAMD_BULK_S(O, TG) if (stats) { PLBAMD_BULK_F(O, TG, true); } else { PLBAMD_BULK_F(O, TG, false); }
-#define PLBAMD_BULK_T(O) if (tags) { PLBAMD_BULK_S(O, true) } else { PLBAMD_BULK_S(O, false) }
-        if (odd) { PLBAMD_BULK_T(true) } else { PLBAMD_BULK_T(false) }
-#undef PLBAMD_BULK_T
-#undef PLBAMD_BULK_S
-#undef PLBAMD_BULK_F
-#undef PLBAMD_BULK
-    }
-    PLBAMD_HIP(hipGetLastError());
-    if (L.nSpecial) {
-        SpecialArgs s{};
-        s.arena = L.arena; s.blocks = L.dGeom; s.omega = L.desc.omega;
-        for (int c = 0; c < 3; ++c) s.force[c] = L.desc.force[c];
-        s.alpha = L.dAlpha;
-        s.cells = L.dSpecial; s.count = L.nSpecial; s.values = L.dValues;
-        s.partials = L.dSpecialPartials;
-        if (record) {
-            s.moments = record->moments;
-            s.momStride = record->momStride;
-            s.dumpF = record->specialF;
-        }
-        dim3 sg((L.nSpecial + 255) / 256), sb(256);
-#define PLBAMD_SPECIAL(O, ST, FO) hipLaunchKernelGGL((special_step<S, DYN, O, ST, FO>), sg, sb, 0, L.stream, s)
-#define PLBAMD_SPECIAL_F(O, ST) if (forced) PLBAMD_SPECIAL(O, ST, true); else PLBAMD_SPECIAL(O, ST, false)
-        if (odd) {
-            if (stats) { PLBAMD_SPECIAL_F(true, true); } else { PLBAMD_SPECIAL_F(true, false); }
-        } else {
-            if (stats) { PLBAMD_SPECIAL_F(false, true); } else { PLBAMD_SPECIAL_F(false, false); }
-        }
-#undef PLBAMD_SPECIAL_F
-#undef PLBAMD_SPECIAL
-        PLBAMD_HIP(hipGetLastError());
-    }
-    if (stats) {
-        // two levels when there are many records (a single workgroup reading
-        // 65536 records took 40 % of a 256^3 step): kStatsLevel1 workgroups in a
-        // fixed order, then one
-        const uint64_t nb = (!odd && L.linearEven) ? L.nEven : L.nBulkPartials;
-        const double* bulk = L.dBulkPartials;
-        uint64_t nr = nb;
-        if (nb > 4 * kStatsLevel1) {
-            hipLaunchKernelGGL(reduce_bulk_records, dim3(kStatsLevel1), dim3(256), 0, L.stream, L.dBulkPartials, nb,
-                               L.dBulkLevel1);
-            bulk = L.dBulkLevel1;
-            nr = kStatsLevel1;
-        }
-        hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(256), 0, L.stream, bulk, nr,
-                           L.dSpecialPartials, L.nSpecial ? L.nSpecialPartials : 0, L.dStats);
-        PLBAMD_HIP(hipGetLastError());
-    }
-    if (odd) PLBAMD_TRY(exchange<S>(L));
-    return PLBAMD_OK;
-}
-
-template<typename S>
 int launchStep(Lattice& L, bool odd, bool stats) {
     return L.desc.dynamics == PLBAMD_BGK ? launchStepTyped<S, 0>(L, odd, stats)
                                          : launchStepTyped<S, 1>(L, odd, stats);
@@ -832,67 +528,6 @@ bool boxArgs(const Lattice& L, size_t bi, const plbamd_box& g, BoxArgs& a) {
     return true;
 }
 
-// Populations of box b between the lattice and a device buffer of boxCells(b)
-// * 19 elements of type elem, C order; launches only (no host copy, no wait).
-template<typename S>
-hipError_t transferDevice(Lattice& L, const plbamd_box& b, int elem, void* staging, bool up) {
-    hipError_t err = hipSuccess;
-    for (size_t bi = 0; bi < L.geom.size() && err == hipSuccess; ++bi) {
-        BoxArgs a;
-        if (!boxArgs(L, bi, b, a)) continue;
-        const dim3 grid = gridFor((uint64_t)a.ilx * a.ily * a.ilz), block(256);
-        if (up) {
-            if (elem == PLBAMD_F64) hipLaunchKernelGGL((upload_box<S, double>), grid, block, 0, L.stream, L.arena, a, (const double*)staging);
-            else if (elem == PLBAMD_F32) hipLaunchKernelGGL((upload_box<S, float>), grid, block, 0, L.stream, L.arena, a, (const float*)staging);
-            else hipLaunchKernelGGL((upload_box_raw<S>), grid, block, 0, L.stream, L.arena, a, (const uint16_t*)staging);
-        } else {
-            if (elem == PLBAMD_F64) hipLaunchKernelGGL((download_box<S, double>), grid, block, 0, L.stream, L.arena, a, (double*)staging);
-            else if (elem == PLBAMD_F32) hipLaunchKernelGGL((download_box<S, float>), grid, block, 0, L.stream, L.arena, a, (float*)staging);
-            else hipLaunchKernelGGL((download_box_raw<S>), grid, block, 0, L.stream, L.arena, a, (uint16_t*)staging);
-            err = hipGetLastError();
-            if (err == hipSuccess && L.anyWrapped && L.nSpecial) {
-                const dim3 g2 = gridFor(L.nSpecial);
-                if (elem == PLBAMD_F64)
-                    hipLaunchKernelGGL((complete_download<S, double>), g2, block, 0, L.stream, L.arena, a, L.dValues, (double*)staging, 0);
-                else if (elem == PLBAMD_F32)
-                    hipLaunchKernelGGL((complete_download<S, float>), g2, block, 0, L.stream, L.arena, a, L.dValues, (float*)staging, 0);
-                else
-                    hipLaunchKernelGGL((complete_download<S, float>), g2, block, 0, L.stream, L.arena, a, L.dValues, (float*)staging, 1);
-            }
-        }
-        if (err == hipSuccess) err = hipGetLastError();
-    }
-    return err;
-}
-
-template<typename S>
-int transfer(Lattice& L, const plbamd_box& b, int elem, void* data, size_t bytes, bool up) {
-    const uint64_t n = boxCells(b);
-    const size_t hostElem = (elem == PLBAMD_F64) ? 8 : (elem == PLBAMD_F32 ? 4 : 2);
-    if (bytes != n * 19 * hostElem) return fail(PLBAMD_EINVAL, "transfer: byte count does not match the box");
-    if ((elem == PLBAMD_F16 || elem == PLBAMD_BF16) && elem != L.desc.storage)
-        return fail(PLBAMD_EINVAL, "transfer: 16-bit host data must match the storage type");
-    if (up && L.multi && (L.time & 1))
-        return fail(PLBAMD_ESTATE, "upload: a multi-block lattice accepts populations after an even number of steps");
-    PLBAMD_TRY(rebuild(L));
-    void* staging = nullptr;
-    PLBAMD_HIP(hipMalloc(&staging, bytes));
-    hipError_t err = hipSuccess;
-    if (up) err = hipMemcpyAsync(staging, data, bytes, hipMemcpyHostToDevice, L.stream);
-    if (err == hipSuccess) err = transferDevice<S>(L, b, elem, staging, up);
-    if (err == hipSuccess && up && exchange<S>(L) != PLBAMD_OK) err = hipErrorUnknown;
-    if (err == hipSuccess && !up) err = hipMemcpyAsync(data, staging, bytes, hipMemcpyDeviceToHost, L.stream);
-    if (err == hipSuccess) err = hipStreamSynchronize(L.stream);
-    hipFree(staging);
-    if (err != hipSuccess) return fail(PLBAMD_EHIP, std::string("transfer: ") + hipGetErrorString(err));
-    return PLBAMD_OK;
-}
-
-// ---------------------------------------------------------------------------
-// Palabos checkpoint files (saveBinaryBlock / loadBinaryBlock,
-// io/serializerIO_3D.cpp:37-66): the cells are serialized and base64-coded on
-// the device; the host writes or reads the text.
-
 std::string base64Host(const uint8_t* p, size_t n) {
     std::string out;
     for (size_t k = 0; k < n; k += 3) {
@@ -922,229 +557,7 @@ bool unbase64Host(const char* s, size_t n, uint8_t* out) {
     return true;
 }
 
-// Device buffers of one call, freed on every return path.
-struct DeviceBuffers {
-    std::vector<void*> p;
-    void* get(size_t bytes) {
-        void* q = nullptr;
-        if (hipMalloc(&q, bytes) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
-        p.push_back(q);
-        return q;
-    }
-    ~DeviceBuffers() { for (void* q : p) (void)hipFree(q); }
-};
-
-static dim3 itemsGrid(uint64_t n) { return dim3((unsigned)std::max<uint64_t>(1, (n + 255) / 256)); }
-
-template<typename S, typename T>
-int checkpointTyped(Lattice& L, const char* path, int elem, bool force, bool enforceUint, bool save) {
-    const plbamd_box whole{0, L.desc.nx - 1, 0, L.desc.ny - 1, 0, L.desc.nz - 1};
-    const uint64_t cells = boxCells(whole);
-    const int recLen = force ? 22 : 19;
-    const uint64_t dataBytes = cells * recLen * sizeof(T);
-    const uint64_t nChars = (dataBytes + 2) / 3 * 4;
-    if (enforceUint && dataBytes > 0xffffffffull)
-        return fail(PLBAMD_EINVAL, "binary block: the data does not fit a 32-bit size header (enforce_uint)");
-    if (dataBytes / kB64Bytes + 1 > (uint64_t)INT32_MAX * 256)
-        return fail(PLBAMD_EINVAL, "binary block: lattice too large for one conversion launch");
-    PLBAMD_TRY(rebuild(L));
-    DeviceBuffers buf;
-    T* pops = static_cast<T*>(buf.get(cells * 19 * sizeof(T)));
-    T* rec = force ? static_cast<T*>(buf.get(dataBytes)) : pops;
-    char* text = static_cast<char*>(buf.get((nChars + 63) / 64 * 64));
-    int* bad = static_cast<int*>(buf.get(2 * sizeof(int)));
-    if (!pops || !rec || !text || !bad) return fail(PLBAMD_ENOMEM, "binary block: cannot allocate device buffers");
-    const T fx = (T)L.desc.force[0], fy = (T)L.desc.force[1], fz = (T)L.desc.force[2];
-    if (save) {
-        // cells in no block are zeros, as the reference's forward ordering writes them
-        PLBAMD_HIP(hipMemsetAsync(pops, 0, cells * 19 * sizeof(T), L.stream));
-        PLBAMD_HIP(transferDevice<S>(L, whole, elem, pops, false));
-        if (force) hipLaunchKernelGGL((append_force<T>), itemsGrid(cells), dim3(256), 0, L.stream, pops, rec, cells, fx, fy, fz);
-        hipLaunchKernelGGL(base64_encode, itemsGrid(dataBytes / kB64Bytes + 1), dim3(256), 0, L.stream,
-                           reinterpret_cast<const uint8_t*>(rec), dataBytes, text);
-        PLBAMD_HIP(hipGetLastError());
-        std::vector<char> host(nChars);
-        PLBAMD_HIP(hipMemcpyAsync(host.data(), text, nChars, hipMemcpyDeviceToHost, L.stream));
-        PLBAMD_HIP(hipStreamSynchronize(L.stream));
-        uint8_t header[8];
-        const uint64_t size64 = dataBytes;
-        std::memcpy(header, &size64, 8);   // little-endian host
-        const std::string head = base64Host(header, enforceUint ? 4 : 8);
-        FILE* fp = std::fopen(path, "wb");
-        if (!fp) return fail(PLBAMD_EINVAL, std::string("save_binary_block: cannot open ") + path);
-        const bool ok = std::fwrite(head.data(), 1, head.size(), fp) == head.size() &&
-                        std::fwrite(host.data(), 1, host.size(), fp) == host.size();
-        if (std::fclose(fp) != 0 || !ok) return fail(PLBAMD_EINVAL, std::string("save_binary_block: cannot write ") + path);
-        return PLBAMD_OK;
-    }
-    if (L.multi && (L.time & 1))
-        return fail(PLBAMD_ESTATE, "load_binary_block: a multi-block lattice accepts populations after an even number of steps");
-    std::string file;
-    {
-        FILE* fp = std::fopen(path, "rb");
-        if (!fp) return fail(PLBAMD_EINVAL, std::string("load_binary_block: cannot open ") + path);
-        if (std::fseek(fp, 0, SEEK_END) == 0) {
-            const long size = std::ftell(fp);
-            if (size > 0) file.reserve((size_t)size);
-            std::rewind(fp);
-        }
-        char chunk[1 << 16];
-        size_t got;
-        while ((got = std::fread(chunk, 1, sizeof(chunk), fp)) > 0) file.append(chunk, got);
-        std::fclose(fp);
-    }
-    // the reference reads with operator>>, which skips white space (every
-    // white-space character is at most ' '; every base64 character above it)
-    if (std::any_of(file.begin(), file.end(), [](char c) { return (unsigned char)c <= ' '; }))
-        file.erase(std::remove_if(file.begin(), file.end(), [](char c) { return std::isspace((unsigned char)c) != 0; }),
-                   file.end());
-    const size_t headChars = enforceUint ? 8 : 12;
-    if (file.size() < headChars) return fail(PLBAMD_EINVAL, "load_binary_block: file shorter than its size header");
-    uint8_t header[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (!unbase64Host(file.data(), enforceUint ? 4 : 8, header))
-        return fail(PLBAMD_EINVAL, "load_binary_block: the size header is not base64");
-    uint64_t size64 = 0;
-    std::memcpy(&size64, header, 8);
-    if (size64 != dataBytes)
-        return fail(PLBAMD_EINVAL, "load_binary_block: the file holds " + std::to_string(size64) +
-                                       " bytes of cells, this lattice takes " + std::to_string(dataBytes));
-    if (file.size() - headChars < nChars) return fail(PLBAMD_EINVAL, "load_binary_block: the file is truncated");
-    PLBAMD_HIP(hipMemsetAsync(bad, 0, 2 * sizeof(int), L.stream));
-    PLBAMD_HIP(hipMemcpyAsync(text, file.data() + headChars, nChars, hipMemcpyHostToDevice, L.stream));
-    hipLaunchKernelGGL(base64_decode, itemsGrid(dataBytes / kB64Bytes + 1), dim3(256), 0, L.stream, text, dataBytes,
-                       reinterpret_cast<uint8_t*>(rec), bad);
-    if (force) hipLaunchKernelGGL((split_force<T>), itemsGrid(cells), dim3(256), 0, L.stream, rec, pops, cells, fx, fy, fz, bad + 1);
-    PLBAMD_HIP(hipGetLastError());
-    int flags[2] = {0, 0};
-    PLBAMD_HIP(hipMemcpyAsync(flags, bad, sizeof(flags), hipMemcpyDeviceToHost, L.stream));
-    PLBAMD_HIP(hipStreamSynchronize(L.stream));
-    if (flags[0]) return fail(PLBAMD_EINVAL, "load_binary_block: the cell data is not base64");
-    if (flags[1]) return fail(PLBAMD_EINVAL, "load_binary_block: the force externals differ from the lattice's force");
-    PLBAMD_HIP(transferDevice<S>(L, whole, elem, pops, true));
-    PLBAMD_TRY(exchange<S>(L));
-    PLBAMD_HIP(hipStreamSynchronize(L.stream));
-    return PLBAMD_OK;
-}
-
-template<typename S>
-int checkpoint(Lattice& L, const char* path, int elem, bool force, bool enforceUint, bool save) {
-    if (elem == PLBAMD_F64) return checkpointTyped<S, double>(L, path, elem, force, enforceUint, save);
-    return checkpointTyped<S, float>(L, path, elem, force, enforceUint, save);
-}
-
-template<typename S>
-int equilibrium(Lattice& L, const plbamd_box& b, const double* rho, const double* u) {
-    const uint64_t n = boxCells(b);
-    if (L.multi && (L.time & 1))
-        return fail(PLBAMD_ESTATE, "initialize_at_equilibrium: a multi-block lattice accepts populations after an even number of steps");
-    PLBAMD_TRY(rebuild(L));
-    double* d = nullptr;
-    PLBAMD_HIP(hipMalloc(&d, n * 4 * sizeof(double)));
-    hipError_t err = hipMemcpyAsync(d, rho, n * sizeof(double), hipMemcpyHostToDevice, L.stream);
-    if (err == hipSuccess) err = hipMemcpyAsync(d + n, u, 3 * n * sizeof(double), hipMemcpyHostToDevice, L.stream);
-    for (size_t bi = 0; bi < L.geom.size() && err == hipSuccess; ++bi) {
-        BoxArgs a;
-        if (!boxArgs(L, bi, b, a)) continue;
-        hipLaunchKernelGGL((equilibrium_box<S>), gridFor((uint64_t)a.ilx * a.ily * a.ilz), dim3(256), 0, L.stream,
-                           L.arena, a, d, d + n);
-        err = hipGetLastError();
-    }
-    if (err == hipSuccess && exchange<S>(L) != PLBAMD_OK) err = hipErrorUnknown;
-    if (err == hipSuccess) err = hipStreamSynchronize(L.stream);
-    hipFree(d);
-    if (err != hipSuccess) return fail(PLBAMD_EHIP, std::string("equilibrium: ") + hipGetErrorString(err));
-    return PLBAMD_OK;
-}
-
-template<typename S>
-int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sums, double* fields = nullptr) {
-    const uint64_t n = boxCells(b);
-    PLBAMD_TRY(rebuild(L));
-    std::vector<BoxArgs> parts;
-    uint64_t nRecords = 0;
-    const bool planes = L.nSpecial > 0 && L.dValues != nullptr;
-    for (size_t bi = 0; bi < L.geom.size(); ++bi) {
-        BoxArgs a;
-        if (boxArgs(L, bi, b, a)) {
-            parts.push_back(a);
-            nRecords += gridFor((uint64_t)a.ilx * a.ily * a.ilz).x;
-            if (planes) nRecords += gridFor(L.nSpecial).x;
-        }
-    }
-    const size_t fieldDoubles = (rho || u) ? n * 4 : 0;
-    const size_t analysisDoubles = fields ? n * 7 : 0;
-    // records reduced in two levels when there are many: 256 workgroups, then one
-    const uint32_t level1 = nRecords > 4096 ? 256u : 0u;
-    double* d = nullptr;
-    PLBAMD_HIP(hipMalloc(&d, (fieldDoubles + analysisDoubles + nRecords * 8 + 8 * level1 + 8) * sizeof(double)));
-    double* dRho = rho ? d : nullptr;
-    double* dU = u ? d + n : nullptr;
-    double* dFields = fields ? d + fieldDoubles : nullptr;
-    double* dPart = d + fieldDoubles + analysisDoubles;
-    double* dLevel1 = dPart + nRecords * 8;
-    double* dSum = dLevel1 + 8 * level1;
-    MomentsArgs m{};
-    m.tags = L.dTags;
-    m.values = L.dValues;
-    for (int c = 0; c < 3; ++c) m.force[c] = L.desc.force[c];
-    m.forced = L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0;
-    m.fields = dFields;
-    for (int k = 0; k < 3; ++k) m.solidRho[k] = L.solidRho[k];
-    hipError_t err = hipSuccess;
-    uint64_t rec = 0;
-    for (const BoxArgs& a : parts) {
-        const dim3 grid = gridFor((uint64_t)a.ilx * a.ily * a.ilz);
-        if (a.parity)
-            hipLaunchKernelGGL((moments_box<S, 1>), grid, dim3(256), 0, L.stream, L.arena, a, m, dRho, dU, dPart + rec * 8);
-        else
-            hipLaunchKernelGGL((moments_box<S, 0>), grid, dim3(256), 0, L.stream, L.arena, a, m, dRho, dU, dPart + rec * 8);
-        rec += grid.x;
-        err = hipGetLastError();
-        if (err != hipSuccess) break;
-        if (planes) {
-            const dim3 pg = gridFor(L.nSpecial);
-            hipLaunchKernelGGL((moments_planes<S>), pg, dim3(256), 0, L.stream, L.arena, a, m, dRho, dU, dPart + rec * 8);
-            rec += pg.x;
-            err = hipGetLastError();
-            if (err != hipSuccess) break;
-        }
-    }
-    if (err == hipSuccess) {
-        if (level1) {
-            hipLaunchKernelGGL(sum_records, dim3(level1), dim3(256), 0, L.stream, dPart, (uint32_t)nRecords, dLevel1);
-            hipLaunchKernelGGL(sum_records, dim3(1), dim3(256), 0, L.stream, dLevel1, level1, dSum);
-        } else {
-            hipLaunchKernelGGL(sum_records, dim3(1), dim3(256), 0, L.stream, dPart, (uint32_t)nRecords, dSum);
-        }
-        err = hipGetLastError();
-    }
-    double host[8];
-    if (err == hipSuccess && rho) err = hipMemcpyAsync(rho, dRho, n * sizeof(double), hipMemcpyDeviceToHost, L.stream);
-    if (err == hipSuccess && u) err = hipMemcpyAsync(u, dU, 3 * n * sizeof(double), hipMemcpyDeviceToHost, L.stream);
-    if (err == hipSuccess && fields)
-        err = hipMemcpyAsync(fields, dFields, 7 * n * sizeof(double), hipMemcpyDeviceToHost, L.stream);
-    if (err == hipSuccess) err = hipMemcpyAsync(host, dSum, sizeof(host), hipMemcpyDeviceToHost, L.stream);
-    if (err == hipSuccess) err = hipStreamSynchronize(L.stream);
-    hipFree(d);
-    if (err != hipSuccess) return fail(PLBAMD_EHIP, std::string("moments: ") + hipGetErrorString(err));
-    if (sums) {
-        sums[0] = host[0];  // sum of rhoBar
-        sums[1] = host[1];  // sum of |u|^2
-        sums[2] = host[3];  // sum of u
-        sums[3] = host[4];
-        sums[4] = host[5];
-    }
-    return PLBAMD_OK;
-}
-
-
-// ---------------------------------------------------------------------------
-// Adjoint gradient (adjoint.h): forward T steps with a checkpoint every K
-// steps, then segments backwards: recompute the segment from its checkpoint
-// with the forward kernels, recording per step the pre-collision moments
-// (rhoBar, j) of every cell and the populations of the plane cells, then K
-// reverse steps.
-
+// The recording step of the adjoint (FP64 and 16-bit storage).
 int recordStep(Lattice& L, bool odd, const Recording& r) {
     const bool bgk = L.desc.dynamics == PLBAMD_BGK;
     switch (L.desc.storage) {
@@ -1153,287 +566,6 @@ int recordStep(Lattice& L, bool odd, const Recording& r) {
     case PLBAMD_BF16: return bgk ? launchStepTyped<bf16s, 0>(L, odd, false, &r) : launchStepTyped<bf16s, 1>(L, odd, false, &r);
     default: return fail(PLBAMD_EINVAL, "adjoint: FP64 or 16-bit storage");
     }
-}
-
-int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, double* Jout, double* gradAlpha,
-                    double* gradScale, double* timing) {
-    const int st = L.desc.storage;
-    if (st != PLBAMD_F64 && st != PLBAMD_F16 && st != PLBAMD_BF16)
-        return fail(PLBAMD_EINVAL, "adjoint: FP64 or 16-bit storage");
-    const bool forced = L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0;
-    if (!L.hostTags.empty()) return fail(PLBAMD_EINVAL, "adjoint: no per-cell tags (walls are the domain edges)");
-    if (T <= 0 || K <= 0 || (K & 1)) return fail(PLBAMD_EINVAL, "adjoint: steps > 0 and an even checkpoint interval");
-    if (obj.first_step < 0 || obj.first_step >= T) return fail(PLBAMD_EINVAL, "adjoint: first_step in [0, nsteps)");
-    if (L.time & 1) return fail(PLBAMD_ESTATE, "adjoint: start after an even number of steps");
-    if (obj.inlet_x < 0 || obj.inlet_x >= L.desc.nx || obj.outlet_x < 0 || obj.outlet_x >= L.desc.nx)
-        return fail(PLBAMD_EINVAL, "adjoint: measuring planes outside the lattice");
-    if (!L.planes.empty()) {
-        if (L.multi) return fail(PLBAMD_EINVAL, "adjoint: boundary planes on single-block lattices only");
-        if (L.desc.dynamics != PLBAMD_BGK) return fail(PLBAMD_EINVAL, "adjoint: boundary planes with BGK only");
-        if (!L.hostAlpha.empty())
-            for (const Plane& p : L.planes) {
-                const plbamd_box& b = p.box;
-                for (int x = b.x0; x <= b.x1; ++x)
-                    for (int y = b.y0; y <= b.y1; ++y)
-                        for (int z = b.z0; z <= b.z1; ++z)
-                            if (L.hostAlpha[((uint64_t)x * L.desc.ny + y) * L.desc.nz + z] != 0.0)
-                                return fail(PLBAMD_EINVAL, "adjoint: porosity must vanish on boundary planes");
-            }
-    }
-    PLBAMD_TRY(rebuild(L));
-    if (st != PLBAMD_F64 && !L.pairs) return fail(PLBAMD_EINVAL, "adjoint: 16-bit storage needs every block even along z");
-    const BlockGeom& last = L.geom.back();
-    const uint64_t arenaElems = last.offset + last.slot * 19;
-    const size_t stateBytes = arenaElems * L.elemSize;
-    const int nCk = (T + K - 1) / K;
-    const uint64_t cells = L.allocCells;
-    const uint64_t momElems = 4 * cells;                       // per recorded step
-    const uint64_t pfElems = 19 * (uint64_t)std::max<uint32_t>(1, L.nSpecial);
-    // reverse kernels: one launch over every block, 64 (z) x 4 (y) cells per workgroup
-    const dim3 ablock(64, 4);
-    std::vector<uint32_t> adjBegin(L.geom.size() + 1, 0);
-    for (size_t b = 0; b < L.geom.size(); ++b) {
-        const BlockGeom& g = L.geom[b];
-        adjBegin[b + 1] = adjBegin[b] + (uint32_t)(((g.nz + 63) / 64) * ((g.ny + 3) / 4) * g.nx);
-    }
-    const uint32_t adjWGs = adjBegin.back();
-    std::vector<uint16_t> adjBlock(adjWGs);
-    for (size_t b = 0; b < L.geom.size(); ++b)
-        std::fill(adjBlock.begin() + adjBegin[b], adjBlock.begin() + adjBegin[b + 1], (uint16_t)b);
-    const dim3 agrid(adjWGs);
-    const uint32_t planeWGs = (uint32_t)((20 * (uint64_t)L.nSpecial + 255) / 256);
-    // device memory: checkpoints, a working state, the recorded moments and plane
-    // populations of one segment, two adjoint states, gradients, partial sums
-    // the adjoint state is FP64 for an FP64 forward, FP32 for a 16-bit one
-    const bool lam64 = st == PLBAMD_F64;
-    const size_t lamSize = lam64 ? sizeof(double) : sizeof(float);
-    // the recorded moments have the adjoint state's type (MomentType of the storage)
-    char *ck = nullptr, *work = nullptr, *lamA = nullptr, *lamB = nullptr, *mom = nullptr;
-    double *pf = nullptr, *gA = nullptr, *gS = nullptr, *acc = nullptr, *sum = nullptr;
-    uint16_t* dAdjBlock = nullptr;
-    uint32_t* dAdjBegin = nullptr;
-    auto release = [&]() {
-        hipFree(ck); hipFree(work); hipFree(mom); hipFree(pf); hipFree(lamA); hipFree(lamB); hipFree(gA);
-        hipFree(gS); hipFree(acc); hipFree(sum); hipFree(dAdjBlock); hipFree(dAdjBegin);
-    };
-    const uint64_t nAcc = (uint64_t)adjWGs + planeWGs;   // J accumulators: bulk workgroups, then plane ones
-    hipError_t err = hipMalloc(&ck, stateBytes * nCk);
-    if (err == hipSuccess) err = hipMalloc(&work, stateBytes);
-    if (err == hipSuccess) err = hipMalloc(&mom, momElems * (K + 1) * lamSize);
-    if (err == hipSuccess) err = hipMalloc(&pf, pfElems * (K + 1) * sizeof(double));
-    if (err == hipSuccess) err = hipMalloc(&lamA, arenaElems * lamSize);
-    if (err == hipSuccess) err = hipMalloc(&lamB, arenaElems * lamSize);
-    if (err == hipSuccess) err = hipMalloc(&gA, cells * sizeof(double));
-    if (err == hipSuccess) err = hipMalloc(&gS, std::max<uint32_t>(1, L.nSpecial) * sizeof(double));
-    if (err == hipSuccess) err = hipMalloc(&acc, nAcc * sizeof(double));
-    if (err == hipSuccess) err = hipMalloc(&sum, 8 * sizeof(double));
-    if (err == hipSuccess) err = hipMalloc(&dAdjBlock, adjBlock.size() * sizeof(uint16_t));
-    if (err == hipSuccess) err = hipMalloc(&dAdjBegin, adjBegin.size() * sizeof(uint32_t));
-    if (err == hipSuccess)
-        err = hipMemcpy(dAdjBlock, adjBlock.data(), adjBlock.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
-    if (err == hipSuccess)
-        err = hipMemcpy(dAdjBegin, adjBegin.data(), adjBegin.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
-    if (err != hipSuccess) {
-        (void)hipGetLastError();
-        release();
-        return fail(PLBAMD_ENOMEM, "adjoint: cannot allocate the checkpoint and adjoint buffers");
-    }
-    if (err == hipSuccess) err = hipMemsetAsync(gA, 0, cells * sizeof(double), L.stream);
-    if (err == hipSuccess) err = hipMemsetAsync(gS, 0, std::max<uint32_t>(1, L.nSpecial) * sizeof(double), L.stream);
-    if (err == hipSuccess) err = hipMemsetAsync(acc, 0, nAcc * sizeof(double), L.stream);
-    // cells the odd steps skip (outermost envelope rows) keep zero moments
-    if (err == hipSuccess) err = hipMemsetAsync(mom, 0, momElems * (K + 1) * lamSize, L.stream);
-    const bool saveStats = L.statsOn;
-    L.statsOn = false;
-    float msF = 0, msB = 0;
-    // forward, with checkpoints (even time: the AA array is in natural layout)
-    if (err == hipSuccess) err = hipEventRecord(L.ev0, L.stream);
-    for (int c = 0; c < nCk && err == hipSuccess; ++c) {
-        err = hipMemcpyAsync(ck + (uint64_t)c * stateBytes, L.arena, stateBytes, hipMemcpyDeviceToDevice, L.stream);
-        if (err != hipSuccess) break;
-        if (step(L, std::min(K, T - c * K)) != PLBAMD_OK) err = hipErrorUnknown;
-    }
-    if (err == hipSuccess) err = hipEventRecord(L.ev1, L.stream);
-    if (err == hipSuccess) err = hipEventSynchronize(L.ev1);
-    if (err == hipSuccess) err = hipEventElapsedTime(&msF, L.ev0, L.ev1);
-    // backward
-    AdjointArgs a{};
-    a.momStride = cells;
-    a.alpha = L.dAlpha;
-    a.gAlpha = gA;
-    a.tags = L.dTags;
-    a.omega = L.desc.omega;
-    for (int c = 0; c < 3; ++c) a.force[c] = L.desc.force[c];
-    a.invT = 1.0 / (T - obj.first_step);
-    a.wdp = obj.weight_pressure_drop;
-    a.wdrag = obj.weight_drag;
-    a.wflux = obj.weight_flux / ((double)L.desc.nx * L.desc.ny * L.desc.nz);
-    a.inW = (1.0 / 3.0) / ((double)L.desc.ny * L.desc.nz);
-    a.outW = a.inW;
-    a.xin = obj.inlet_x;
-    a.xout = obj.outlet_x;
-    a.blocks = L.dGeom;
-    a.wgBlock = dAdjBlock;
-    a.wgBegin = dAdjBegin;
-    a.acc = acc;
-    const bool rbgk = L.desc.dynamics == PLBAMD_RBGK;
-#define PLBAMD_ADJ(LT) \
-    do { \
-        if (rbgk) { \
-            if (forced) hipLaunchKernelGGL((adjoint_bulk<LT, 1, true>), gr, ablock, 0, L.stream, args); \
-            else hipLaunchKernelGGL((adjoint_bulk<LT, 1, false>), gr, ablock, 0, L.stream, args); \
-        } else { \
-            if (forced) hipLaunchKernelGGL((adjoint_bulk<LT, 0, true>), gr, ablock, 0, L.stream, args); \
-            else hipLaunchKernelGGL((adjoint_bulk<LT, 0, false>), gr, ablock, 0, L.stream, args); \
-        } \
-    } while (0)
-    auto reverseKernel = [&](const AdjointArgs& args, const dim3& gr) {
-        if (lam64) PLBAMD_ADJ(double);
-        else PLBAMD_ADJ(float);
-    };
-#undef PLBAMD_ADJ
-    auto exchangeLam = [&](char* lam) {
-        if (!L.multi || !L.nSegs) return;
-        // the first envelope layer only: the reverse step gathers lambda one
-        // cell beyond the bulk, and every envelope cell it computes itself is
-        // replaced by this exchange
-        if (!L.nTasks1) return;
-        if (lam64)
-            hipLaunchKernelGGL((exchange_envelopes<double>), dim3(L.nTasks1), dim3(256), 0, L.stream, (void*)lam,
-                               L.dGeom, L.dSegs1, L.dTasks1);
-        else
-            hipLaunchKernelGGL((exchange_envelopes<float>), dim3(L.nTasks1), dim3(256), 0, L.stream, (void*)lam,
-                               L.dGeom, L.dSegs1, L.dTasks1);
-    };
-    // the plane cells' reverse step runs on a second stream next to the bulk one
-    hipStream_t side = nullptr;
-    hipEvent_t evIn = nullptr, evOut = nullptr;
-    if (err == hipSuccess && L.nSpecial) {
-        err = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
-        if (err == hipSuccess) err = hipEventCreateWithFlags(&evIn, hipEventDisableTiming);
-        if (err == hipSuccess) err = hipEventCreateWithFlags(&evOut, hipEventDisableTiming);
-    }
-    char* lamNext = lamA;
-    char* lamCur = lamB;
-    void* const arena = L.arena;
-    const int64_t time0 = L.time;
-    L.arena = work;   // the recompute runs the step kernels on the working state
-    if (err == hipSuccess) err = hipEventRecord(L.ev0, L.stream);
-    for (int c = nCk - 1; c >= 0 && err == hipSuccess; --c) {
-        const int t0 = c * K, len = std::min(K, T - t0);
-        err = hipMemcpyAsync(work, ck + (uint64_t)c * stateBytes, stateBytes, hipMemcpyDeviceToDevice, L.stream);
-        // steps t0 .. t0 + len - 1 (and, in the last segment, one more for f^T)
-        const int nRec = (c == nCk - 1) ? len + 1 : len;
-        for (int m = 0; m < nRec && err == hipSuccess; ++m) {
-            Recording r;
-            r.moments = mom + (uint64_t)m * momElems * lamSize;
-            r.momStride = cells;
-            r.specialF = L.nSpecial ? pf + (uint64_t)m * pfElems : nullptr;
-            if (recordStep(L, (m & 1) != 0, r) != PLBAMD_OK) err = hipErrorUnknown;
-        }
-        if (err == hipSuccess && c == nCk - 1) {
-            a.mom = mom + (uint64_t)len * momElems * lamSize;
-            a.explicitTerm = 1;
-            a.lam = lamNext;
-            if (lam64) hipLaunchKernelGGL(adjoint_final<double>, agrid, ablock, 0, L.stream, a);
-            else hipLaunchKernelGGL(adjoint_final<float>, agrid, ablock, 0, L.stream, a);
-            err = hipGetLastError();
-            exchangeLam(lamNext);
-        }
-        for (int m = len - 1; m >= 0 && err == hipSuccess; --m) {
-            const int t = t0 + m;
-            a.mom = mom + (uint64_t)m * momElems * lamSize;
-            a.planeF = pf + (uint64_t)m * pfElems;
-            a.explicitTerm = t > obj.first_step ? 1 : 0;
-            a.lamNext = lamNext;
-            a.lam = lamCur;
-            if (err == hipSuccess && L.nSpecial) {
-                // plane cells (single block) on the side stream, after the previous step
-                AdjointSpecialArgs sa{};
-                AdjointArgs pa = a;
-                pa.g = L.geom[0];
-                sa.base = pa;
-                sa.cells = L.dSpecial;
-                sa.count = L.nSpecial;
-                sa.values = L.dValues;
-                sa.partials = acc + adjWGs;
-                sa.gScale = gS;
-                err = hipEventRecord(evIn, L.stream);
-                if (err == hipSuccess) err = hipStreamWaitEvent(side, evIn, 0);
-                if (err == hipSuccess) {
-                    if (lam64) hipLaunchKernelGGL(adjoint_planes<double>, dim3(planeWGs), dim3(256), 0, side, sa);
-                    else hipLaunchKernelGGL(adjoint_planes<float>, dim3(planeWGs), dim3(256), 0, side, sa);
-                    err = hipGetLastError();
-                }
-                if (err == hipSuccess) err = hipEventRecord(evOut, side);
-            }
-            if (err == hipSuccess) {
-                reverseKernel(a, agrid);
-                err = hipGetLastError();
-            }
-            if (err == hipSuccess && L.nSpecial) err = hipStreamWaitEvent(L.stream, evOut, 0);
-            exchangeLam(lamCur);
-            std::swap(lamNext, lamCur);
-        }
-    }
-    L.arena = arena;
-    L.time = time0;
-    if (err == hipSuccess) err = hipEventRecord(L.ev1, L.stream);
-    if (side) {
-        (void)hipStreamSynchronize(side);
-        (void)hipStreamDestroy(side);
-    }
-    if (evIn) (void)hipEventDestroy(evIn);
-    if (evOut) (void)hipEventDestroy(evOut);
-    if (err == hipSuccess) err = hipEventSynchronize(L.ev1);
-    if (err == hipSuccess) err = hipEventElapsedTime(&msB, L.ev0, L.ev1);
-    L.statsOn = saveStats;
-    if (err == hipSuccess) {
-        hipLaunchKernelGGL(sum_values, dim3(1), dim3(256), 0, L.stream, acc, (uint64_t)adjWGs, acc + adjWGs,
-                           (uint64_t)planeWGs, sum);
-        err = hipGetLastError();
-    }
-    double hsum[8] = {0};
-    if (err == hipSuccess) err = hipMemcpyAsync(hsum, sum, sizeof(hsum), hipMemcpyDeviceToHost, L.stream);
-    std::vector<double> hgA(gradAlpha ? cells : 0), hgS(std::max<uint32_t>(1, L.nSpecial));
-    if (err == hipSuccess && gradAlpha)
-        err = hipMemcpyAsync(hgA.data(), gA, cells * sizeof(double), hipMemcpyDeviceToHost, L.stream);
-    if (err == hipSuccess) err = hipMemcpyAsync(hgS.data(), gS, hgS.size() * sizeof(double), hipMemcpyDeviceToHost, L.stream);
-    std::vector<SpecialCell> hs(L.nSpecial);
-    if (err == hipSuccess && L.nSpecial)
-        err = hipMemcpyAsync(hs.data(), L.dSpecial, L.nSpecial * sizeof(SpecialCell), hipMemcpyDeviceToHost, L.stream);
-    if (err == hipSuccess) err = hipStreamSynchronize(L.stream);
-    release();
-    if (err != hipSuccess) return fail(PLBAMD_EHIP, std::string("adjoint: ") + hipGetErrorString(err));
-    if (Jout) *Jout = hsum[0];
-    if (gradAlpha) {
-        // bulk cells of every block to the global [nx][ny][nz] array
-        const int ny = L.desc.ny, nz = L.desc.nz;
-        for (const BlockGeom& g : L.geom)
-            for (int x = g.bx0; x <= g.bx1; ++x)
-                for (int y = g.by0; y <= g.by1; ++y)
-                    for (int z = g.bz0; z <= g.bz1; ++z) {
-                        const uint64_t local = ((uint64_t)x * g.ny + y) * g.nz + z;
-                        const uint64_t glob = ((uint64_t)(g.gx0 + x) * ny + (g.gy0 + y)) * nz + (g.gz0 + z);
-                        gradAlpha[glob] = hgA[g.cellOffset + local];
-                    }
-    }
-    if (gradScale) {
-        // plane of each special record from its value index
-        std::vector<uint32_t> base(L.planes.size() + 1, 0);
-        for (size_t p = 0; p < L.planes.size(); ++p) base[p + 1] = base[p] + (uint32_t)L.planes[p].values.size();
-        for (size_t p = 0; p < L.planes.size(); ++p) gradScale[p] = 0.0;
-        for (uint32_t k = 0; k < L.nSpecial; ++k) {
-            if ((hs[k].info & 3u) != SK_PLANE) continue;
-            const size_t p = std::upper_bound(base.begin(), base.end(), hs[k].value) - base.begin() - 1;
-            gradScale[p] += hgS[k];
-        }
-    }
-    if (timing) {
-        timing[0] = msF;
-        timing[1] = msB;
-    }
-    return PLBAMD_OK;
 }
 
 int registerLattice(std::unique_ptr<Lattice> L, int* handle) {

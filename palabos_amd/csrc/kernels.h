@@ -1172,7 +1172,7 @@ __global__ void __launch_bounds__(256) special_step(SpecialArgs a) {
 // w*256 + t, strided by the grid, into one record of the same 16-double layout
 // (statistics in 0..7, the link-wall force in 8, 9, 11).
 constexpr uint32_t kStatsLevel1 = 256;
-__global__ void __launch_bounds__(256) reduce_bulk_records(const double* bulk, uint64_t nBulk, double* out) {
+static __global__ void __launch_bounds__(256) reduce_bulk_records(const double* bulk, uint64_t nBulk, double* out) {
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double wf[3] = {0, 0, 0};
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nBulk; b += (uint64_t)gridDim.x * blockDim.x) {
@@ -1186,7 +1186,7 @@ __global__ void __launch_bounds__(256) reduce_bulk_records(const double* bulk, u
     block_reduce_store(q, out + 16 * (uint64_t)blockIdx.x + 8);
 }
 
-__global__ void __launch_bounds__(256) reduce_partials(const double* bulk, uint64_t nBulk,
+static __global__ void __launch_bounds__(256) reduce_partials(const double* bulk, uint64_t nBulk,
                                                        const double* special, uint64_t nSpecial,
                                                        double* out) {
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1600,7 +1600,7 @@ __global__ void __launch_bounds__(256) moments_planes(const void* arena, BoxArgs
 // per gridDim.x: workgroup w sums records w, w + gridDim.x, ... into out[w];
 // a launch of many workgroups followed by one of a single workgroup over
 // their outputs is the two-level form.
-__global__ void __launch_bounds__(256) sum_records(const double* p, uint32_t n, double* out) {
+static __global__ void __launch_bounds__(256) sum_records(const double* p, uint32_t n, double* out) {
     double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x)
         for (int k = 0; k < 8; ++k) if (k != 2) r[k] += p[8 * b + k];
