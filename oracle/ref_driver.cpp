@@ -39,6 +39,14 @@
 //   ref_driver --checkpoint load T plain|forced nx ny nz in.dat records.bin uint|size
 //       loadBinaryBlock of in.dat into a fresh lattice; its cell records are written
 //   "uint" passes enforceUint = true (32-bit size header), "size" false.
+//
+// Units mode (the reference's IncomprFlowParam, core/units.h:39-95):
+//   ref_driver --units physU latU Re physLen N lx ly lz t
+//       IncomprFlowParam<double>(physU, latU, Re, physLen, N, lx, ly, lz), or the
+//       six-argument constructor (latU, Re, N, lx, ly, lz) when physU is 0; prints
+//       one line: getNx getNy getNz getNx(true) getNy(true) getNz(true)
+//       getLatticeNu getTau getOmega getDeltaX getDeltaT nStep(t) getLx getLy getLz
+//       getPhysicalU getPhysicalLength (reals as %.17g).
 
 // The umbrella palabos3D.hh does not compile with g++ 11 (multiGrid), so the
 // lattice-core subset is included instead.
@@ -397,8 +405,26 @@ int runCheckpoint(const std::string& mode, plint nx, plint ny, plint nz, const s
 
 }  // namespace
 
+int runUnits(char* argv[]) {
+    const double physU = std::atof(argv[2]), latU = std::atof(argv[3]), Re = std::atof(argv[4]);
+    const double physLen = std::atof(argv[5]);
+    const plint N = std::atol(argv[6]);
+    const double lx = std::atof(argv[7]), ly = std::atof(argv[8]), lz = std::atof(argv[9]), t = std::atof(argv[10]);
+    IncomprFlowParam<double> p = physU != 0 ? IncomprFlowParam<double>(physU, latU, Re, physLen, N, lx, ly, lz)
+                                            : IncomprFlowParam<double>(latU, Re, N, lx, ly, lz);
+    std::printf("%ld %ld %ld %ld %ld %ld %.17g %.17g %.17g %.17g %.17g %ld %.17g %.17g %.17g %.17g %.17g\n",
+                (long)p.getNx(), (long)p.getNy(), (long)p.getNz(), (long)p.getNx(true), (long)p.getNy(true),
+                (long)p.getNz(true), p.getLatticeNu(), p.getTau(), p.getOmega(), p.getDeltaX(), p.getDeltaT(),
+                (long)p.nStep(t), p.getLx(), p.getLy(), p.getLz(), p.getPhysicalU(), p.getPhysicalLength());
+    return 0;
+}
+
 int main(int argc, char* argv[]) {
     plbInit(&argc, &argv);
+    if (argc >= 2 && std::string(argv[1]) == "--units") {
+        if (argc < 11) { std::fprintf(stderr, "usage: --units physU latU Re physLen N lx ly lz t\n"); return 2; }
+        return runUnits(argv);
+    }
     if (argc >= 2 && std::string(argv[1]) == "--checkpoint") {
         if (argc < 11) {
             std::fprintf(stderr, "usage: --checkpoint save|load double|float plain|forced nx ny nz in out uint|size\n");

@@ -891,21 +891,35 @@ double getStoredMaxVelocity(MultiBlockLattice3D<T, Descriptor> const& lattice) {
 template<typename T>
 class IncomprFlowParam {
 public:
+    /// Physical velocity and length given (core/units.h:49-52).
     IncomprFlowParam(T physicalU_, T latticeU_, T Re_, T physicalLength_, plint resolution_, T lx_, T ly_, T lz_ = T())
         : physicalU(physicalU_), latticeU(latticeU_), Re(Re_), physicalLength(physicalLength_),
           resolution(resolution_), lx(lx_), ly(ly_), lz(lz_) {}
-    T getRe() const { return Re; }
+    /// Dimensionless form: physical velocity and length 1 (core/units.h:54-59).
+    IncomprFlowParam(T latticeU_, T Re_, plint resolution_, T lx_, T ly_, T lz_ = T())
+        : physicalU((T)1), latticeU(latticeU_), Re(Re_), physicalLength((T)1),
+          resolution(resolution_), lx(lx_), ly(ly_), lz(lz_) {}
     T getLatticeU() const { return latticeU; }
+    T getPhysicalU() const { return physicalU; }
+    T getRe() const { return Re; }
+    T getPhysicalLength() const { return physicalLength; }
+    plint getResolution() const { return resolution; }
+    /// extents in dimensionless units: the physical length times lx, ly, lz
+    T getLx() const { return physicalLength * lx; }
+    T getLy() const { return physicalLength * ly; }
+    T getLz() const { return physicalLength * lz; }
     T getDeltaX() const { return physicalLength / (T)resolution; }
     T getDeltaT() const { return getDeltaX() * latticeU / physicalU; }
+    /// dimensionless length or time to cells or steps, rounded to nearest (int, as the reference)
+    plint nCell(T l) const { return (int)(l / getDeltaX() + (T)0.5); }
+    plint nStep(T t) const { return (int)(t / getDeltaT() + (T)0.5); }
+    /// cells along x, y, z; one more with offLattice
+    plint getNx(bool offLattice = false) const { return nCell(getLx()) + 1 + (int)offLattice; }
+    plint getNy(bool offLattice = false) const { return nCell(getLy()) + 1 + (int)offLattice; }
+    plint getNz(bool offLattice = false) const { return nCell(getLz()) + 1 + (int)offLattice; }
     T getLatticeNu() const { return latticeU * (T)resolution / Re; }
     T getTau() const { return (T)3 * getLatticeNu() + (T)0.5; }
     T getOmega() const { return (T)1 / getTau(); }
-    plint getResolution() const { return resolution; }
-    plint nCell(T l) const { return (plint)(l / getDeltaX() + (T)0.5); }
-    plint getNx() const { return nCell(lx) + 1; }
-    plint getNy() const { return nCell(ly) + 1; }
-    plint getNz() const { return nCell(lz) + 1; }
 private:
     T physicalU, latticeU, Re, physicalLength;
     plint resolution;
