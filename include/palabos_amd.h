@@ -212,6 +212,21 @@ int plbamd_compute_macroscopic(int handle, const plbamd_box* box, double* rho, d
  * (dataProcessors/dataAnalysisWrapper3D.hh:1226-1257). */
 int plbamd_reduce_moments(int handle, const plbamd_box* box, double* out);
 
+/* plbamd_reduce_moments without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * captured into a HIP graph, tight serving loops, harnesses that forbid
+ * hipMalloc). The per-workgroup partial records go to a fixed device array of
+ * the library (8 MiB per GPU, held until the process ends) instead of memory
+ * allocated per call; the call makes no hipMalloc / hipFree. Same kernels and
+ * the same fixed-order reduction, so `out` equals plbamd_reduce_moments's bit
+ * for bit. Limits: at most 131072 partial records (every single-block lattice
+ * fits; a multi-block box needing more is refused with PLBAMD_EINVAL), and
+ * PLBAMD_ESTATE when the lattice's tags, planes or porosity changed since its
+ * last step or plbamd_reduce_moments (their device set-up allocates). Every
+ * call on a GPU shares that GPU's array; the library serialises the calls of
+ * one process, and each process has its own array. */
+int plbamd_reduce_moments_noalloc(int handle, const plbamd_box* box, double* out);
+
 /* Density parameter of the solid dynamics of tag PLBAMD_TAG_BOUNCE_BACK,
  * PLBAMD_TAG_NO_DYNAMICS or PLBAMD_TAG_MOMENTUM_EXCHANGE (the rho of
  * BounceBack(rho), NoDynamics(rho), MomentumExchangeBounceBack(forceIds, rho);

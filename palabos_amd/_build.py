@@ -7,8 +7,9 @@ disabled (-packed-fp32-ops) for gfx950.
 The kernels are templates with many variants, so the library is built from
 several translation units compiled in parallel: the step launches once per
 storage type and dynamics (csrc/step.hip), the other typed operations once
-per storage type (csrc/storage_ops.hip), the adjoint (csrc/adjoint.hip) and the
-set-up and C ABI (csrc/engine.hip). Objects go to palabos_amd/_build/.
+per storage type (csrc/storage_ops.hip), the adjoint (csrc/adjoint.hip), the
+allocation-free moments entry point (csrc/noalloc.hip) and the set-up and C ABI
+(csrc/engine.hip). Objects go to palabos_amd/_build/.
 """
 import os
 import subprocess
@@ -34,7 +35,7 @@ STORAGES = ["double", "float", "f16s", "bf16s"]
 
 def units():
     """(object name, source, extra defines) of every translation unit."""
-    out = [("engine", "engine.hip", []), ("adjoint", "adjoint.hip", [])]
+    out = [("engine", "engine.hip", []), ("adjoint", "adjoint.hip", []), ("noalloc", "noalloc.hip", [])]
     for s in STORAGES:
         out.append((f"ops_{s}", "storage_ops.hip", [f"-DPLBAMD_S={s}"]))
         for dyn in (0, 1):
@@ -77,7 +78,7 @@ def build_library(verbose=False):
 
 CPP_DIR = os.path.join(HERE, "cpp")
 CPP_BIN = os.path.join(CPP_DIR, "bin")
-CPP_EXAMPLES = ["cavity3d", "coprocessor"]
+CPP_EXAMPLES = ["cavity3d", "coprocessor", "units"]
 
 
 def build_cpp_examples(verbose=False):

@@ -59,6 +59,7 @@ struct Plane {
 
 struct Lattice {
     plbamd_lattice_desc desc;
+    int device = -1;                  // HIP device the lattice lives on
     bool multi = false;
     std::vector<plbamd_box> bulk;     // global bulk box of each block
     std::vector<BlockGeom> geom;      // device geometry of each block (host copy)
@@ -182,8 +183,22 @@ template<typename S>
 int checkpoint(Lattice& L, const char* path, int elem, bool force, bool enforceUint, bool save);
 template<typename S>
 int equilibrium(Lattice& L, const plbamd_box& b, const double* rho, const double* u);
+// Caller-provided room for the per-workgroup records of moments(): `base`
+// holds maxRecords records, then kMomentsLevel1 first-level records, then the
+// final record (8 doubles each). With it, moments() allocates nothing and
+// refuses lattices whose device set-up is out of date (rebuild allocates).
+constexpr uint32_t kMomentsLevel1 = 256;
+struct MomentsScratch {
+    double* base;
+    uint64_t maxRecords;
+};
 template<typename S>
-int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sums, double* fields = nullptr);
+int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sums, double* fields = nullptr,
+            const MomentsScratch* scratch = nullptr);
+
+// Finds the device address of the fixed scratch array of
+// plbamd_reduce_moments_noalloc on `device`, which must be current (noalloc.hip).
+int scratchSetup(int device);
 
 }  // namespace plbamd
 

@@ -64,6 +64,7 @@ int validateDesc(const plbamd_lattice_desc* desc) {
 // envelope exchange segments, and allocates the device memory.
 int setup(Lattice& L) {
     const plbamd_lattice_desc& d = L.desc;
+    PLBAMD_HIP(hipGetDevice(&L.device));
     const int n[3] = {d.nx, d.ny, d.nz};
     const int per[3] = {d.periodic[0] != 0, d.periodic[1] != 0, d.periodic[2] != 0};
     // Launch shape: TX threads along z (a multiple of the 64-lane wavefront),
@@ -586,7 +587,8 @@ int plbamd_init(int device) {
     PLBAMD_HIP(hipGetDeviceCount(&count));
     if (device < 0 || device >= count) return fail(PLBAMD_EINVAL, "plbamd_init: no such device");
     PLBAMD_HIP(hipSetDevice(device));
-    return PLBAMD_OK;
+    // the device address of the fixed scratch of plbamd_reduce_moments_noalloc
+    return scratchSetup(device);
 }
 
 const char* plbamd_last_error(void) { return g_lastError.c_str(); }

@@ -201,32 +201,47 @@ int equilibrium(Lattice& L, const plbamd_box& b, const double* rho, const double
 }
 
 template<typename S>
-int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sums, double* fields) {
+int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sums, double* fields,
+            const MomentsScratch* scratch) {
     const uint64_t n = boxCells(b);
-    PLBAMD_TRY(rebuild(L));
-    std::vector<BoxArgs> parts;
+    if (scratch) {
+        if (rho || u || fields) return fail(PLBAMD_EINVAL, "moments: a fixed scratch takes the sums only");
+        if (L.dirty)
+            return fail(PLBAMD_ESTATE, "reduce_moments_noalloc: the lattice's tags, planes or porosity changed since "
+                                       "its last step; run a step or plbamd_reduce_moments first");
+    } else {
+        PLBAMD_TRY(rebuild(L));
+    }
     uint64_t nRecords = 0;
     const bool planes = L.nSpecial > 0 && L.dValues != nullptr;
     for (size_t bi = 0; bi < L.geom.size(); ++bi) {
         BoxArgs a;
-        if (boxArgs(L, bi, b, a)) {
-            parts.push_back(a);
-            nRecords += gridFor((uint64_t)a.ilx * a.ily * a.ilz).x;
-            if (planes) nRecords += gridFor(L.nSpecial).x;
-        }
+        if (!boxArgs(L, bi, b, a)) continue;
+        nRecords += gridFor((uint64_t)a.ilx * a.ily * a.ilz).x;
+        if (planes) nRecords += gridFor(L.nSpecial).x;
     }
     const size_t fieldDoubles = (rho || u) ? n * 4 : 0;
     const size_t analysisDoubles = fields ? n * 7 : 0;
-    // records reduced in two levels when there are many: 256 workgroups, then one
-    const uint32_t level1 = nRecords > 4096 ? 256u : 0u;
+    // records reduced in two levels when there are many: kMomentsLevel1 workgroups, then one
+    const uint32_t level1 = nRecords > 4096 ? kMomentsLevel1 : 0u;
     double* d = nullptr;
-    PLBAMD_HIP(hipMalloc(&d, (fieldDoubles + analysisDoubles + nRecords * 8 + 8 * level1 + 8) * sizeof(double)));
+    double *dPart, *dLevel1;
+    if (scratch) {
+        if (nRecords > scratch->maxRecords)
+            return fail(PLBAMD_EINVAL, "reduce_moments_noalloc: the box needs " + std::to_string(nRecords) +
+                                           " partial records, the fixed scratch holds " +
+                                           std::to_string(scratch->maxRecords));
+        dPart = scratch->base;
+        dLevel1 = dPart + 8 * scratch->maxRecords;
+    } else {
+        PLBAMD_HIP(hipMalloc(&d, (fieldDoubles + analysisDoubles + nRecords * 8 + 8 * level1 + 8) * sizeof(double)));
+        dPart = d + fieldDoubles + analysisDoubles;
+        dLevel1 = dPart + nRecords * 8;
+    }
     double* dRho = rho ? d : nullptr;
     double* dU = u ? d + n : nullptr;
     double* dFields = fields ? d + fieldDoubles : nullptr;
-    double* dPart = d + fieldDoubles + analysisDoubles;
-    double* dLevel1 = dPart + nRecords * 8;
-    double* dSum = dLevel1 + 8 * level1;
+    double* dSum = dLevel1 + 8 * (scratch ? kMomentsLevel1 : level1);
     MomentsArgs m{};
     m.tags = L.dTags;
     m.values = L.dValues;
@@ -236,7 +251,9 @@ int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sum
     for (int k = 0; k < 3; ++k) m.solidRho[k] = L.solidRho[k];
     hipError_t err = hipSuccess;
     uint64_t rec = 0;
-    for (const BoxArgs& a : parts) {
+    for (size_t bi = 0; bi < L.geom.size(); ++bi) {
+        BoxArgs a;
+        if (!boxArgs(L, bi, b, a)) continue;
         const dim3 grid = gridFor((uint64_t)a.ilx * a.ily * a.ilz);
         if (a.parity)
             hipLaunchKernelGGL((moments_box<S, 1>), grid, dim3(256), 0, L.stream, L.arena, a, m, dRho, dU, dPart + rec * 8);
@@ -269,7 +286,7 @@ int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sum
         err = hipMemcpyAsync(fields, dFields, 7 * n * sizeof(double), hipMemcpyDeviceToHost, L.stream);
     if (err == hipSuccess) err = hipMemcpyAsync(host, dSum, sizeof(host), hipMemcpyDeviceToHost, L.stream);
     if (err == hipSuccess) err = hipStreamSynchronize(L.stream);
-    hipFree(d);
+    if (d) hipFree(d);
     if (err != hipSuccess) return fail(PLBAMD_EHIP, std::string("moments: ") + hipGetErrorString(err));
     if (sums) {
         sums[0] = host[0];  // sum of rhoBar
@@ -287,6 +304,7 @@ template hipError_t transferDevice<PLBAMD_S>(Lattice&, const plbamd_box&, int, v
 template int transfer<PLBAMD_S>(Lattice&, const plbamd_box&, int, void*, size_t, bool);
 template int checkpoint<PLBAMD_S>(Lattice&, const char*, int, bool, bool, bool);
 template int equilibrium<PLBAMD_S>(Lattice&, const plbamd_box&, const double*, const double*);
-template int moments<PLBAMD_S>(Lattice&, const plbamd_box&, double*, double*, double*, double*);
+template int moments<PLBAMD_S>(Lattice&, const plbamd_box&, double*, double*, double*, double*,
+                                const MomentsScratch*);
 
 }  // namespace plbamd

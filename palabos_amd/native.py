@@ -32,7 +32,7 @@ EXPORTS = (
     "plbamd_collide_and_stream", "plbamd_collide_and_stream_timed", "plbamd_toggle_statistics",
     "plbamd_get_statistics", "plbamd_compute_macroscopic", "plbamd_reduce_moments",
     "plbamd_adjoint_gradient", "plbamd_get_time", "plbamd_save_binary_block", "plbamd_load_binary_block",
-    "plbamd_compute_fields", "plbamd_set_solid_density",
+    "plbamd_compute_fields", "plbamd_set_solid_density", "plbamd_reduce_moments_noalloc",
 )
 
 
@@ -109,6 +109,7 @@ def lib():
         "plbamd_get_statistics": [i32, vp],
         "plbamd_compute_macroscopic": [i32, pbox, vp, vp],
         "plbamd_reduce_moments": [i32, pbox, vp],
+        "plbamd_reduce_moments_noalloc": [i32, pbox, vp],
         "plbamd_compute_fields": [i32, pbox, vp],
         "plbamd_set_solid_density": [i32, i32, ctypes.c_double],
         "plbamd_get_time": [i32, ctypes.POINTER(ctypes.c_int64)],
