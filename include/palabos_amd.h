@@ -63,6 +63,21 @@ extern "C" {
 #define PLBAMD_VELOCITY 0
 #define PLBAMD_PRESSURE 1
 
+/* Boundary types (boundary::BcType, boundaryCondition/boundaryCondition.h:40), set
+ * with plbamd_set_boundary_type. Velocity planes, edges and corners: NEUMANN and
+ * OUTFLOW copy the velocity of the cell one step inside into the boundary after
+ * every step (CopyVelocityFunctional3D, neumannCondition3D.hh:116-128), FREESLIP
+ * its tangential components (the normal ones zeroed, :151-172), NORMAL_OUTFLOW
+ * its normal components (:196-217); pressure planes: NEUMANN copies the density
+ * (CopyDensityFunctional3D, :241-252). DIRICHLET and DENSITY keep the value set
+ * (boundaryInstantiator3D.h:1042-1063, :1097-1102, :1139-1157, :1216-1232). */
+#define PLBAMD_BC_DIRICHLET 0
+#define PLBAMD_BC_NEUMANN 1
+#define PLBAMD_BC_FREESLIP 2
+#define PLBAMD_BC_DENSITY 3
+#define PLBAMD_BC_OUTFLOW 4
+#define PLBAMD_BC_NORMAL_OUTFLOW 5
+
 /* Statistics layout returned by plbamd_get_statistics (FP64). */
 #define PLBAMD_STAT_AVG_RHOBAR 0 /* average pre-collision rhoBar of cells taking statistics   */
 #define PLBAMD_STAT_AVG_USQR 1   /* average pre-collision |u|^2                               */
@@ -138,8 +153,44 @@ int plbamd_set_tags(int handle, const uint8_t* tags);
 int plbamd_add_plane(int handle, int family, int kind, int dir, int ori,
                      const plbamd_box* box, int* plane);
 
-/* Sets the imposed values of a plane, x,y,z C order over its box: velocity planes
- * take 3 doubles (u) per cell, pressure planes 1 double (rho) per cell.
+/* Adds an external velocity edge: the cells of `box` (one cell thick along the two
+ * axes after `plane`, (plane+1)%3 and (plane+2)%3) take StoreVelocityDynamics over
+ * the background dynamics and, after every stream, the finite-difference edge
+ * closure OuterVelocityEdgeFunctional3D<plane, normal1, normal2>
+ * (finiteDifferenceBoundaryProcessor3D.hh:130-217); normal1 / normal2 (-1 or +1)
+ * are the outward normal along (plane+1)%3 / (plane+2)%3. Replaces
+ * OnLatticeBoundaryCondition3D::addExternalVelocityEdge{0,1,2}{N,P}{N,P}
+ * (boundaryCondition3D.h; boundaryInstantiator3D.h:1113-1153) of the Zou-He and
+ * regularized factories, which install the same edge. Velocities (3 doubles per
+ * cell, zero at first) through plbamd_set_plane_values with the returned id. Single-
+ * block lattices without a body force; the stencil (the cells one step inside
+ * along both normals, the neighbours along the edge) must lie in the lattice and be
+ * background, plane, edge or corner cells (PLBAMD_EINVAL at the next step
+ * otherwise). */
+int plbamd_add_velocity_edge(int handle, int plane, int normal1, int normal2, const plbamd_box* box, int* id);
+
+/* Adds an external velocity corner at (x, y, z) with outward normal (x_normal,
+ * y_normal, z_normal), each -1 or +1: StoreVelocityDynamics and, after every
+ * stream, OuterVelocityCornerFunctional3D (finiteDifferenceBoundaryProcessor3D.hh:
+ * 221-313). Replaces OnLatticeBoundaryCondition3D::addExternalVelocityCorner{N,P}^3
+ * (boundaryInstantiator3D.h:1197-1232). Velocity (3 doubles) through
+ * plbamd_set_plane_values. Same limits as plbamd_add_velocity_edge. */
+int plbamd_add_velocity_corner(int handle, int x_normal, int y_normal, int z_normal, int x, int y, int z, int* id);
+
+/* Sets the boundary type of a plane, edge or corner (PLBAMD_BC_*; the bcType
+ * argument of the add*Boundary / addExternalVelocity* methods,
+ * boundaryCondition3D.h:40-200). Non-Dirichlet types: single-block lattices. */
+int plbamd_set_boundary_type(int handle, int id, int bc_type);
+
+/* One pass of the lattice's data processors (Neumann-type copies, edge and corner
+ * closures) on the current state, as MultiBlock3D::initialize does before the
+ * first step (multiBlock/multiBlock3D.cpp:330-332); every step runs them after its
+ * stream. No effect on a lattice without edges, corners or non-Dirichlet types. */
+int plbamd_execute_processors(int handle);
+
+/* Sets the imposed values of a plane, edge or corner, x,y,z C order over its box:
+ * velocity planes, edges and corners take 3 doubles (u) per cell, pressure planes
+ * 1 double (rho) per cell.
  * Replaces setBoundaryVelocity / setBoundaryDensity
  * (dataProcessors/dataInitializerFunctional3D.hh:830-845, :878-888). */
 int plbamd_set_plane_values(int handle, int plane, const double* values, size_t count);

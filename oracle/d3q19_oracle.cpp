@@ -148,6 +148,10 @@ typedef struct {
     int32_t dir, ori;   // plane normal axis, orientation (-1 or +1)
     int32_t box[6];     // x0 x1 y0 y1 z0 z1, inclusive
     const double* values;  // velocity: 3 per box cell; pressure: 1 (rho) per box cell; x,y,z C order
+    int32_t shape;      // 0 plane, 1 external velocity edge (dir = edge axis), 2 external velocity corner
+    int32_t normal[3];  // edges / corners: outward normal (0 along an edge's axis)
+    int32_t bc_type;    // boundary::BcType (boundaryCondition.h:40): 0 dirichlet, 1 neumann,
+                        // 2 freeslip, 3 density, 4 outflow, 5 normalOutflow
 } OraclePlane;
 
 typedef struct {
@@ -161,14 +165,14 @@ typedef struct {
     const OraclePlane* planes;
     double force[3];     // Guo body force per unit volume (all zero: plain dynamics)
     const double* alpha; // nullable; Brinkman porosity per cell in [0,1]
-    int32_t bc_wrapped;  // unused by the step; documents the snapshot convention of the caller
+    int32_t bc_wrapped;  // wrapped boundary factories: processors read plane cells completed
 } OracleDesc;
 
 }  // extern "C"
 
 namespace {
 
-enum { TAG_BULK = 0, TAG_BB = 1, TAG_NODYN = 2, TAG_MEBB = 3, TAG_WALL = 4, TAG_PLANE = 5 };
+enum { TAG_BULK = 0, TAG_BB = 1, TAG_NODYN = 2, TAG_MEBB = 3, TAG_WALL = 4, TAG_PLANE = 5, TAG_STORE = 6 };
 
 // Moments ------------------------------------------------------------------
 
@@ -364,6 +368,9 @@ struct Stats {
     double porous[3] = {0, 0, 0};
 };
 
+// Storage conversion of one population (defined below, per storage type).
+template<typename S, typename T> struct Store;
+
 template<typename T>
 struct Engine {
     const OracleDesc& d;
@@ -374,6 +381,9 @@ struct Engine {
     std::vector<uint32_t> fluidDirs;   // per cell, bit i set: direction i is a fluid direction (MEBB)
     const T* alphaT = nullptr;         // porosity in the compute type (derivative-carrying runs)
     T planeScale = T(1.0);             // scale of every velocity plane's imposed velocity
+    // Stored boundary values of every plane, edge and corner (the velocity or
+    // density their dynamics hold; the Neumann-type processors rewrite them).
+    mutable std::vector<std::vector<double>> vals;
 
     explicit Engine(const OracleDesc& d_) : d(d_) {
         N = (long)d.nx * d.ny * d.nz;
@@ -382,14 +392,18 @@ struct Engine {
         boxIndexOf.assign(N, 0);
         fluidDirs.assign(N, 0);
         if (d.tags) for (long k = 0; k < N; ++k) tag[k] = d.tags[k];
+        vals.resize(d.nplanes);
         for (int p = 0; p < d.nplanes; ++p) {
             const int* b = d.planes[p].box;
+            const long cells = (long)(b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1);
+            const long per = (d.planes[p].shape == 0 && d.planes[p].kind == 1) ? 1 : 3;
+            vals[p].assign(d.planes[p].values, d.planes[p].values + cells * per);
             long idx = 0;
             for (int x = b[0]; x <= b[1]; ++x)
                 for (int y = b[2]; y <= b[3]; ++y)
                     for (int z = b[4]; z <= b[5]; ++z, ++idx) {
                         long k = lin(x, y, z);
-                        tag[k] = TAG_PLANE;
+                        tag[k] = d.planes[p].shape == 0 ? TAG_PLANE : TAG_STORE;
                         planeOf[k] = p;
                         boxIndexOf[k] = idx;
                     }
@@ -507,8 +521,8 @@ struct Engine {
         T rhoBar;
         T j[3];
         if (P.kind == 0) {
-            T u[3] = {planeScale * (T)P.values[3*bi], planeScale * (T)P.values[3*bi+1],
-                      planeScale * (T)P.values[3*bi+2]};
+            const std::vector<double>& v = vals[planeOf[k]];
+            T u[3] = {planeScale * (T)v[3*bi], planeScale * (T)v[3*bi+1], planeScale * (T)v[3*bi+2]};
             T rhoOnWall = T();
             for (int n = 0; n < nOnWall; ++n) rhoOnWall += f[onWall[n]];
             T rhoNormal = T();
@@ -518,7 +532,7 @@ struct Engine {
             T rho = rhoBar + (T)1;
             for (int c = 0; c < 3; ++c) j[c] = rho * u[c];
         } else {
-            rhoBar = (T)P.values[bi] - (T)1;  // StoreDensityDynamics::defineDensity, boundaryDynamics.hh:251-255
+            rhoBar = (T)vals[planeOf[k]][bi] - (T)1;  // StoreDensityDynamics::defineDensity, boundaryDynamics.hh:251-255
             j[0] = j[1] = j[2] = T();
             T rhoOnWall = T();
             for (int n = 0; n < nOnWall; ++n) rhoOnWall += f[onWall[n]];
@@ -632,11 +646,269 @@ struct Engine {
             completePlane(f, k);
             baseCollide(f, st, k);
             break;
+        case TAG_STORE:  // StoreVelocityDynamics: CompositeDynamics::collide, base collision
+                         // (core/dynamics.hh:470-475; boundaryDynamics.hh:171-209)
+            baseCollide(f, st, k);
+            break;
         }
+    }
+
+    // ----- data processors after the stream (MultiBlock3D::executeInternalProcessors):
+    // the Neumann-type copies (boundaryCondition/neumannCondition3D.hh:116-252) and
+    // the finite-difference edge and corner closures
+    // (finiteDifferenceBoundaryProcessor3D.hh:130-329), in the order the boundary
+    // objects were added, each object's copy before its closure
+    // (boundaryInstantiator3D.h:1017-1247).
+
+    bool hasProcessors() const {
+        for (int p = 0; p < d.nplanes; ++p)
+            if (d.planes[p].shape != 0 || copyKind(p) != 0) return true;
+        return false;
+    }
+
+    // 0 none, 1 velocity, 2 tangential velocity, 3 normal velocity, 4 density
+    int copyKind(int p) const {
+        const OraclePlane& P = d.planes[p];
+        const int t = P.bc_type;
+        if (P.shape == 0 && P.kind == 1) return t == 1 ? 4 : 0;       // addPressureBoundary
+        if (t == 1 || t == 4) return 1;                                // neumann, outflow
+        if (t == 2) return 2;                                          // freeslip
+        if (t == 5) return 3;                                          // normalOutflow
+        return 0;
+    }
+
+    void outwardNormal(int p, int n[3]) const {
+        const OraclePlane& P = d.planes[p];
+        for (int a = 0; a < 3; ++a) n[a] = P.shape == 0 ? (a == P.dir ? P.ori : 0) : P.normal[a];
+    }
+
+    // The 19 populations of cell k as its dynamics sees them in a processor: a
+    // plane cell of a wrapped factory has been completed by its own processor
+    // (wrappedLocalBoundaryProcessor3D.hh:41-62) before any later processor runs.
+    template<typename S>
+    void cellPops(const S* state, long k, T f[19]) const {
+        for (int i = 0; i < 19; ++i) f[i] = Store<S, T>::load(state[k*19 + i]);
+        if (tag[k] == TAG_PLANE && d.bc_wrapped) completePlane(f, k);
+    }
+
+    // Cell::computeDensity of cell k: bulk and StoreVelocity cells fullRho of
+    // get_rhoBar (core/dynamics.hh:278-281, boundaryDynamics.hh:82-87); plane
+    // cells fullRho of their Dirichlet rhoBar (boundaryDynamics.hh:640-679,
+    // StoreDensityDynamics :229-233).
+    template<typename S>
+    T densityOf(const S* state, long k) const {
+        T f[19];
+        cellPops(state, k, f);
+        if (tag[k] == TAG_PLANE) {
+            T rhoBar, u[3];
+            planeMacroscopic(f, k, rhoBar, u);
+            return rhoBar + (T)1;
+        }
+        return get_rhoBar(f) + (T)1;
+    }
+
+    // Cell::computeVelocity of cell k: bulk j / rho (compute_uLb with the
+    // sequential rhoBar, momentTemplates3D.h:138-144); StoreVelocity and velocity
+    // plane cells their stored velocity (boundaryDynamics.hh:324-330); pressure
+    // planes j / rho of their closure (:776-835).
+    template<typename S>
+    void velocityOf(const S* state, long k, T u[3]) const {
+        if (tag[k] == TAG_STORE || (tag[k] == TAG_PLANE && d.planes[planeOf[k]].kind == 0)) {
+            const double* v = &vals[planeOf[k]][3 * boxIndexOf[k]];
+            for (int c = 0; c < 3; ++c) u[c] = (T)v[c];
+            return;
+        }
+        T f[19];
+        cellPops(state, k, f);
+        if (tag[k] == TAG_PLANE) {
+            T rhoBar;
+            planeMacroscopic(f, k, rhoBar, u);
+            return;
+        }
+        const T rhoBar = get_rhoBar(f);
+        T rb, j[3];
+        get_rhoBar_j(f, rb, j);
+        const T ir = invRhoOf(rhoBar);
+        for (int c = 0; c < 3; ++c) u[c] = j[c] * ir;
+    }
+
+    // rhoBar and velocity a plane cell reports (Dirichlet closure from its known populations).
+    void planeMacroscopic(const T* f, long k, T& rhoBar, T u[3]) const {
+        const OraclePlane& P = d.planes[planeOf[k]];
+        const long bi = boxIndexOf[k];
+        T onWall = T(), normal = T();
+        for (int i = 0; i < 19; ++i) if (C[i][P.dir] == 0) onWall += f[i];
+        for (int i = 0; i < 19; ++i) if (C[i][P.dir] == P.ori) normal += f[i];
+        if (P.kind == 0) {
+            const double* v = &vals[planeOf[k]][3 * bi];
+            for (int c = 0; c < 3; ++c) u[c] = (T)v[c];
+            const T velNormal = (T)P.ori * u[P.dir];
+            rhoBar = ((T)2*normal + onWall - velNormal) / ((T)1 + velNormal);
+        } else {
+            rhoBar = (T)vals[planeOf[k]][bi] - (T)1;
+            T j[3] = {T(), T(), T()};
+            j[P.dir] = (T)P.ori * ((T)2*normal + onWall - rhoBar);
+            const T ir = invRhoOf(rhoBar);
+            for (int c = 0; c < 3; ++c) u[c] = j[c] * ir;
+        }
+    }
+
+    long neighbour(int x, int y, int z, int dx, int dy, int dz) const {
+        int a = x + dx, b = y + dy, c = z + dz;
+        if (!wrap(a, b, c)) return -1;
+        return lin(a, b, c);
+    }
+
+    // IsoThermalBulkDynamics::regularize (isoThermalDynamics.hh:55-69).
+    static void regularize(T* f, T rhoBar, const T j[3], T jSqr, const T pi[6]) {
+        const T invRho = invRhoOf(rhoBar);
+        f[0] = equilibrium(0, rhoBar, invRho, j, jSqr) + fromPiToFneq(0, pi);
+        for (int i = 1; i <= 9; ++i) {
+            f[i] = equilibrium(i, rhoBar, invRho, j, jSqr);
+            f[i+9] = equilibrium(i+9, rhoBar, invRho, j, jSqr);
+            T fNeq = fromPiToFneq(i, pi);
+            f[i] += fNeq;
+            f[i+9] += fNeq;
+        }
+    }
+
+    // pi from the velocity gradient dA_uB[a][b] = d_a u_b (finiteDifferenceBoundaryProcessor3D.hh:162-172).
+    void gradientToPi(const T g[3][3], T rho, T pi[6]) const {
+        const T omega = (T)d.omega;
+        const T sToPi = -rho / (T)3 / omega;
+        pi[0] = (T)2 * g[0][0] * sToPi;
+        pi[3] = (T)2 * g[1][1] * sToPi;
+        pi[5] = (T)2 * g[2][2] * sToPi;
+        pi[1] = (g[0][1] + g[1][0]) * sToPi;
+        pi[2] = (g[0][2] + g[2][0]) * sToPi;
+        pi[4] = (g[1][2] + g[2][1]) * sToPi;
+    }
+
+    template<typename S>
+    void storePops(S* state, long k, const T f[19]) const {
+        for (int i = 0; i < 19; ++i) state[k*19 + i] = Store<S, T>::store(f[i]);
+    }
+
+    // OuterVelocityEdgeFunctional3D::processCell (finiteDifferenceBoundaryProcessor3D.hh:144-195).
+    template<typename S>
+    bool edgeCell(S* state, int p, int x, int y, int z) const {
+        const OraclePlane& P = d.planes[p];
+        const int plane = P.dir, d1 = (plane + 1) % 3, d2 = (plane + 2) % 3;
+        const int n1 = P.normal[d1], n2 = P.normal[d2];
+        int s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0}, sp[3] = {0, 0, 0};
+        s1[d1] = -n1; s2[d2] = -n2; sp[plane] = 1;
+        const long k = lin(x, y, z);
+        const long k10 = neighbour(x, y, z, s1[0], s1[1], s1[2]);
+        const long k01 = neighbour(x, y, z, s2[0], s2[1], s2[2]);
+        const long kp = neighbour(x, y, z, sp[0], sp[1], sp[2]);
+        const long km = neighbour(x, y, z, -sp[0], -sp[1], -sp[2]);
+        if (k10 < 0 || k01 < 0 || kp < 0 || km < 0) return false;
+        const T rho10 = densityOf(state, k10);
+        const T rho01 = densityOf(state, k01);
+        const T rho = (T)0.5 * (rho01 + rho10);
+        T u0[3], u10[3], u01[3], up[3], um[3];
+        velocityOf(state, k, u0);
+        velocityOf(state, k10, u10);
+        velocityOf(state, k01, u01);
+        velocityOf(state, kp, up);
+        velocityOf(state, km, um);
+        T g[3][3];
+        for (int b = 0; b < 3; ++b) {
+            g[plane][b] = (up[b] - um[b]) / (T)2;           // fd::ctl_diff
+            g[d1][b] = (T)(-n1) * (-u0[b] + u10[b]);        // -orientation * fd::o1_fwd_diff
+            g[d2][b] = (T)(-n2) * (-u0[b] + u01[b]);
+        }
+        T pi[6];
+        gradientToPi(g, rho, pi);
+        T j[3];
+        for (int c = 0; c < 3; ++c) j[c] = rho * u0[c];
+        const T jSqr = j[0]*j[0] + j[1]*j[1] + j[2]*j[2];
+        T f[19];
+        regularize(f, rho - (T)1, j, jSqr, pi);
+        storePops(state, k, f);
+        return true;
+    }
+
+    // OuterVelocityCornerFunctional3D::processCell (finiteDifferenceBoundaryProcessor3D.hh:262-313).
+    template<typename S>
+    bool cornerCell(S* state, int p, int x, int y, int z) const {
+        const int* n = d.planes[p].normal;
+        const long k = lin(x, y, z);
+        const long k100 = neighbour(x, y, z, -n[0], 0, 0);
+        const long k010 = neighbour(x, y, z, 0, -n[1], 0);
+        const long k001 = neighbour(x, y, z, 0, 0, -n[2]);
+        if (k100 < 0 || k010 < 0 || k001 < 0) return false;
+        const T rho100 = densityOf(state, k100);
+        const T rho010 = densityOf(state, k010);
+        const T rho001 = densityOf(state, k001);
+        const T rho = (T)1 / (T)3 * (rho001 + rho010 + rho100);
+        T u0[3], ux[3], uy[3], uz[3];
+        velocityOf(state, k, u0);
+        velocityOf(state, k100, ux);
+        velocityOf(state, k010, uy);
+        velocityOf(state, k001, uz);
+        T g[3][3];
+        for (int b = 0; b < 3; ++b) {
+            g[0][b] = (T)(-n[0]) * (-u0[b] + ux[b]);
+            g[1][b] = (T)(-n[1]) * (-u0[b] + uy[b]);
+            g[2][b] = (T)(-n[2]) * (-u0[b] + uz[b]);
+        }
+        T pi[6];
+        gradientToPi(g, rho, pi);
+        T j[3];
+        for (int c = 0; c < 3; ++c) j[c] = rho * u0[c];
+        const T jSqr = j[0]*j[0] + j[1]*j[1] + j[2]*j[2];
+        T f[19];
+        regularize(f, rho - (T)1, j, jSqr, pi);
+        storePops(state, k, f);
+        return true;
+    }
+
+    // CopyVelocity / CopyTangentialVelocity / CopyNormalVelocity / CopyDensity of one cell.
+    template<typename S>
+    bool copyCell(const S* state, int p, int kind, int x, int y, int z, long bi) const {
+        int n[3];
+        outwardNormal(p, n);
+        const long src = neighbour(x, y, z, -n[0], -n[1], -n[2]);
+        if (src < 0) return false;
+        if (kind == 4) {
+            vals[p][bi] = (double)densityOf(state, src);
+            return true;
+        }
+        T u[3];
+        velocityOf(state, src, u);
+        for (int a = 0; a < 3; ++a) {
+            if (kind == 2 && n[a] != 0) u[a] = T();
+            if (kind == 3 && n[a] == 0) u[a] = T();
+        }
+        for (int a = 0; a < 3; ++a) vals[p][3 * bi + a] = (double)u[a];
+        return true;
+    }
+
+    // One pass of every processor; false if a processor's stencil leaves the lattice.
+    template<typename S>
+    bool runProcessors(S* state) const {
+        for (int p = 0; p < d.nplanes; ++p) {
+            const OraclePlane& P = d.planes[p];
+            const int* b = P.box;
+            const int ck = copyKind(p);
+            if (ck) {
+                long bi = 0;
+                for (int x = b[0]; x <= b[1]; ++x) for (int y = b[2]; y <= b[3]; ++y)
+                    for (int z = b[4]; z <= b[5]; ++z, ++bi)
+                        if (!copyCell(state, p, ck, x, y, z, bi)) return false;
+            }
+            if (P.shape == 0) continue;
+            for (int x = b[0]; x <= b[1]; ++x) for (int y = b[2]; y <= b[3]; ++y)
+                for (int z = b[4]; z <= b[5]; ++z) {
+                    const bool ok = P.shape == 1 ? edgeCell(state, p, x, y, z) : cornerCell(state, p, x, y, z);
+                    if (!ok) return false;
+                }
+        }
+        return true;
     }
 };
 
-template<typename S, typename T> struct Store;
 template<> struct Store<double, double> {
     static double load(double s) { return s; }
     static double store(double v) { return v; }
@@ -657,8 +929,11 @@ template<> struct Store<BHalf, float> {
 };
 
 template<typename S, typename T>
-void runSteps(const OracleDesc& d, S* state, int nsteps, double* statsOut) {
+int runSteps(const OracleDesc& d, S* state, int nsteps, double* statsOut, bool initialize, bool snapshot = false) {
     Engine<T> eng(d);
+    const bool processors = eng.hasProcessors();
+    // MultiBlock3D::initialize (multiBlock3D.cpp:330-332): one pass of the processors
+    if (initialize && processors && !eng.runProcessors(state)) return -3;
     const long N = eng.N;
     std::vector<S> post((size_t)N * 19);
     Stats st;
@@ -693,6 +968,7 @@ void runSteps(const OracleDesc& d, S* state, int nsteps, double* statsOut) {
                 }
             }
         }
+        if (processors && !eng.runProcessors(state)) return -3;
     }
     if (statsOut) {
         statsOut[0] = st.count ? st.sumRhoBar / (double)st.count : 0.0;
@@ -706,6 +982,18 @@ void runSteps(const OracleDesc& d, S* state, int nsteps, double* statsOut) {
         statsOut[8] = st.porous[1];
         statsOut[9] = st.porous[2];
     }
+    if (snapshot) {
+        // what a wrapped factory's processor leaves in the plane cells: the
+        // completion with the boundary values as they are now
+        for (long k = 0; k < N; ++k) {
+            if (eng.tag[k] != TAG_PLANE) continue;
+            T f[19];
+            for (int i = 0; i < 19; ++i) f[i] = Store<S, T>::load(state[k*19 + i]);
+            eng.completePlane(f, k);
+            for (int i = 0; i < 19; ++i) state[k*19 + i] = Store<S, T>::store(f[i]);
+        }
+    }
+    return 0;
 }
 
 // Analysis fields of every cell (dataAnalysisWrapper3D.h:695-777), in T:
@@ -849,10 +1137,27 @@ extern "C" {
 int oracle_run(const OracleDesc* d, void* state, int nsteps, double* statsOut) {
     if (!d || !state || d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || nsteps < 0) return -1;
     switch (d->storage) {
-    case 0: runSteps<double, double>(*d, (double*)state, nsteps, statsOut); return 0;
-    case 1: runSteps<float, float>(*d, (float*)state, nsteps, statsOut); return 0;
-    case 2: runSteps<Half, float>(*d, (Half*)state, nsteps, statsOut); return 0;
-    case 3: runSteps<BHalf, float>(*d, (BHalf*)state, nsteps, statsOut); return 0;
+    case 0: return runSteps<double, double>(*d, (double*)state, nsteps, statsOut, false);
+    case 1: return runSteps<float, float>(*d, (float*)state, nsteps, statsOut, false);
+    case 2: return runSteps<Half, float>(*d, (Half*)state, nsteps, statsOut, false);
+    case 3: return runSteps<BHalf, float>(*d, (BHalf*)state, nsteps, statsOut, false);
+    }
+    return -1;
+}
+
+// As oracle_run, with flags: bit 0, one pass of the data processors on the
+// initial state first (MultiBlock3D::initialize, multiBlock3D.cpp:330-332);
+// bit 1, the plane cells completed at the end with the boundary values of that
+// moment (the snapshot a wrapped factory leaves). Returns -3 if a processor's
+// stencil leaves the lattice.
+int oracle_run_ex(const OracleDesc* d, void* state, int nsteps, double* statsOut, int flags) {
+    if (!d || !state || d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || nsteps < 0) return -1;
+    const bool ini = (flags & 1) != 0, snap = (flags & 2) != 0;
+    switch (d->storage) {
+    case 0: return runSteps<double, double>(*d, (double*)state, nsteps, statsOut, ini, snap);
+    case 1: return runSteps<float, float>(*d, (float*)state, nsteps, statsOut, ini, snap);
+    case 2: return runSteps<Half, float>(*d, (Half*)state, nsteps, statsOut, ini, snap);
+    case 3: return runSteps<BHalf, float>(*d, (BHalf*)state, nsteps, statsOut, ini, snap);
     }
     return -1;
 }

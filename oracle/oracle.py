@@ -27,12 +27,16 @@ STORAGE_DTYPES = {"f64": np.float64, "f32": np.float32, "f16": np.uint16, "bf16"
 DYNAMICS_CODES = {"bgk": 0, "rbgk": 1}
 FAMILY_CODES = {"zouhe": 0, "regularized": 1}
 KIND_CODES = {"velocity": 0, "pressure": 1}
+SHAPE_CODES = {"plane": 0, "edge": 1, "corner": 2}
+# boundary::BcType (boundaryCondition/boundaryCondition.h:40)
+BC_TYPE_CODES = {"dirichlet": 0, "neumann": 1, "freeslip": 2, "density": 3, "outflow": 4, "normalOutflow": 5}
 
 
 class _Plane(ctypes.Structure):
     _fields_ = [("family", ctypes.c_int32), ("kind", ctypes.c_int32),
                 ("dir", ctypes.c_int32), ("ori", ctypes.c_int32),
-                ("box", ctypes.c_int32 * 6), ("values", ctypes.c_void_p)]
+                ("box", ctypes.c_int32 * 6), ("values", ctypes.c_void_p),
+                ("shape", ctypes.c_int32), ("normal", ctypes.c_int32 * 3), ("bc_type", ctypes.c_int32)]
 
 
 class _Desc(ctypes.Structure):
@@ -62,6 +66,9 @@ def lib():
         _lib.oracle_run.argtypes = [ctypes.POINTER(_Desc), ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_void_p]
         _lib.oracle_run.restype = ctypes.c_int
+        _lib.oracle_run_ex.argtypes = [ctypes.POINTER(_Desc), ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_int]
+        _lib.oracle_run_ex.restype = ctypes.c_int
         _lib.oracle_complete.argtypes = [ctypes.POINTER(_Desc), ctypes.c_void_p]
         _lib.oracle_complete.restype = ctypes.c_int
         _lib.oracle_equilibrium.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p,
@@ -114,6 +121,10 @@ def _desc(case, keep):
                 arr[i].box[k] = p.box[k]
             vals = np.ascontiguousarray(p.values, dtype=np.float64)
             arr[i].values = keep.ptr(vals)
+            arr[i].shape = SHAPE_CODES[getattr(p, "shape", "plane")]
+            for a in range(3):
+                arr[i].normal[a] = int(getattr(p, "normal", (0, 0, 0))[a])
+            arr[i].bc_type = BC_TYPE_CODES[getattr(p, "bc_type", "dirichlet")]
         keep.refs.append(arr)
         d.planes = ctypes.addressof(arr)
     for a in range(3):
@@ -125,17 +136,25 @@ def _desc(case, keep):
     return d
 
 
-def run(case, f, nsteps):
-    """Returns (populations after nsteps, stats dict). `f` is not modified."""
+def run(case, f, nsteps, initialize=False, wrapped=False, snapshot=False):
+    """Returns (populations after nsteps, stats dict). `f` is not modified.
+    initialize: one pass of the data processors first (Palabos's
+    MultiBlock3D::initialize). wrapped: the boundary objects come from a
+    wrapped factory (processors read plane cells completed). snapshot: the plane
+    cells are returned completed with the boundary values of the last step, as a
+    wrapped factory leaves them (the values Neumann-type boundaries copied are
+    lost between calls, so a snapshot is taken in the same call)."""
     dtype = STORAGE_DTYPES[case.storage]
     state = np.array(f, dtype=dtype, order="C", copy=True)
     assert state.shape == (case.nx, case.ny, case.nz, 19)
     keep = _Keep()
     d = _desc(case, keep)
+    d.bc_wrapped = int(bool(wrapped))
     stats = np.zeros(10, dtype=np.float64)
-    rc = lib().oracle_run(ctypes.byref(d), state.ctypes.data, int(nsteps), stats.ctypes.data)
+    flags = (1 if initialize else 0) | (2 if snapshot else 0)
+    rc = lib().oracle_run_ex(ctypes.byref(d), state.ctypes.data, int(nsteps), stats.ctypes.data, flags)
     if rc != 0:
-        raise ValueError("oracle_run rejected the case")
+        raise ValueError(f"oracle_run rejected the case ({rc})")
     return state, {"avg_rhobar": stats[0], "avg_usqr": stats[1], "max_usqr": stats[2],
                    "force": stats[3:6].copy(), "count": int(stats[6]),
                    "porous": stats[7:10].copy()}

@@ -16,9 +16,17 @@
 //   bcfamily zouhe|regularized        createZouHe... or createLocal...
 //   init file                         double f-bar, C order [nx][ny][nz][19]
 //   tags file                         uint8 [nx][ny][nz]: 0 bulk 1 BounceBack 2 NoDynamics 3 MomentumExchangeBounceBack
-//   bc vel|press dir ori x0 x1 y0 y1 z0 z1 file
+//   bc vel|press dir ori x0 x1 y0 y1 z0 z1 file [bctype]
 //                                     vel: 3 doubles per box cell, press: 1 double rho per box cell,
-//                                     box cells in x,y,z C order
+//                                     box cells in x,y,z C order; bctype: boundary::BcType as an
+//                                     integer (boundaryCondition.h:40; default 0 = dirichlet)
+//   edge plane n1 n2 x0 x1 y0 y1 z0 z1 file [bctype]
+//                                     addExternalVelocityEdge<plane><n1><n2> (n = -1 | 1), velocities
+//   corner xn yn zn x y z file [bctype]
+//                                     addExternalVelocityCorner<xn><yn><zn>, one velocity
+//   blockbc vel|press bctype file     set{Velocity,Pressure}ConditionOnBlockBoundaries on the whole
+//                                     lattice; file: per lattice cell, C order, 3 velocity doubles
+//                                     (vel) or 1 density (press), defined on every boundary cell
 //   block x0 x1 y0 y1 z0 z1           optional; any number -> SparseBlockStructure3D multi-block
 //   dump s1 s2 ...                    steps after which populations are written (0 = after initialize)
 //   force fx fy fz                    optional; ForcedD3Q19 + GuoExternalForceBGKdynamics (bgk, no planes)
@@ -87,9 +95,17 @@ using namespace plb;
 namespace {
 
 struct BcSpec {
-    std::string kind;
-    int dir, ori;
+    std::string kind;        // vel | press | edge | corner
+    int dir = 0, ori = 0;    // planes; edges: dir = plane, ori unused
+    int normal[3] = {0, 0, 0};  // edges: n1, n2 in normal[1], normal[2]; corners: x, y, z normals
     Box3D box;
+    std::string file;
+    int bcType = 0;
+};
+
+struct BlockBcSpec {
+    std::string kind;        // vel | press
+    int bcType = 0;
     std::string file;
 };
 
@@ -103,6 +119,7 @@ struct CaseSpec {
     std::string bcfamily = "zouhe";
     std::string init, tags, out;
     std::vector<BcSpec> bcs;
+    std::vector<BlockBcSpec> blockBcs;
     std::vector<Box3D> blocks;
     std::vector<plint> dumps;
     bool forced = false;
@@ -140,7 +157,27 @@ CaseSpec parseCase(std::string const& name) {
             BcSpec b;
             s >> b.kind >> b.dir >> b.ori >> b.box.x0 >> b.box.x1 >> b.box.y0 >> b.box.y1
               >> b.box.z0 >> b.box.z1 >> b.file;
+            if (!(s >> b.bcType)) b.bcType = 0;
             c.bcs.push_back(b);
+        } else if (key == "edge") {
+            BcSpec b;
+            b.kind = "edge";
+            s >> b.dir >> b.normal[1] >> b.normal[2] >> b.box.x0 >> b.box.x1 >> b.box.y0 >> b.box.y1
+              >> b.box.z0 >> b.box.z1 >> b.file;
+            if (!(s >> b.bcType)) b.bcType = 0;
+            c.bcs.push_back(b);
+        } else if (key == "corner") {
+            BcSpec b;
+            b.kind = "corner";
+            plint x, y, z;
+            s >> b.normal[0] >> b.normal[1] >> b.normal[2] >> x >> y >> z >> b.file;
+            b.box = Box3D(x, x, y, y, z, z);
+            if (!(s >> b.bcType)) b.bcType = 0;
+            c.bcs.push_back(b);
+        } else if (key == "blockbc") {
+            BlockBcSpec b;
+            s >> b.kind >> b.bcType >> b.file;
+            c.blockBcs.push_back(b);
         } else if (key == "block") {
             Box3D b;
             s >> b.x0 >> b.x1 >> b.y0 >> b.y1 >> b.z0 >> b.z1;
@@ -163,25 +200,63 @@ CaseSpec parseCase(std::string const& name) {
 template<typename T, template<typename U> class D>
 void addVelocityPlane(OnLatticeBoundaryCondition3D<T,D>& bc,
                       MultiBlockLattice3D<T,D>& lat,
-                      int dir, int ori, Box3D box) {
-    if (dir == 0 && ori < 0) bc.addVelocityBoundary0N(box, lat);
-    else if (dir == 0) bc.addVelocityBoundary0P(box, lat);
-    else if (dir == 1 && ori < 0) bc.addVelocityBoundary1N(box, lat);
-    else if (dir == 1) bc.addVelocityBoundary1P(box, lat);
-    else if (dir == 2 && ori < 0) bc.addVelocityBoundary2N(box, lat);
-    else bc.addVelocityBoundary2P(box, lat);
+                      int dir, int ori, Box3D box, boundary::BcType t) {
+    if (dir == 0 && ori < 0) bc.addVelocityBoundary0N(box, lat, t);
+    else if (dir == 0) bc.addVelocityBoundary0P(box, lat, t);
+    else if (dir == 1 && ori < 0) bc.addVelocityBoundary1N(box, lat, t);
+    else if (dir == 1) bc.addVelocityBoundary1P(box, lat, t);
+    else if (dir == 2 && ori < 0) bc.addVelocityBoundary2N(box, lat, t);
+    else bc.addVelocityBoundary2P(box, lat, t);
 }
 
 template<typename T, template<typename U> class D>
 void addPressurePlane(OnLatticeBoundaryCondition3D<T,D>& bc,
                       MultiBlockLattice3D<T,D>& lat,
-                      int dir, int ori, Box3D box) {
-    if (dir == 0 && ori < 0) bc.addPressureBoundary0N(box, lat);
-    else if (dir == 0) bc.addPressureBoundary0P(box, lat);
-    else if (dir == 1 && ori < 0) bc.addPressureBoundary1N(box, lat);
-    else if (dir == 1) bc.addPressureBoundary1P(box, lat);
-    else if (dir == 2 && ori < 0) bc.addPressureBoundary2N(box, lat);
-    else bc.addPressureBoundary2P(box, lat);
+                      int dir, int ori, Box3D box, boundary::BcType t) {
+    if (dir == 0 && ori < 0) bc.addPressureBoundary0N(box, lat, t);
+    else if (dir == 0) bc.addPressureBoundary0P(box, lat, t);
+    else if (dir == 1 && ori < 0) bc.addPressureBoundary1N(box, lat, t);
+    else if (dir == 1) bc.addPressureBoundary1P(box, lat, t);
+    else if (dir == 2 && ori < 0) bc.addPressureBoundary2N(box, lat, t);
+    else bc.addPressureBoundary2P(box, lat, t);
+}
+
+// addExternalVelocityEdge<plane><n1><n2> (boundaryCondition3D.h), n = -1 | +1
+template<typename T, template<typename U> class D>
+void addVelocityEdge(OnLatticeBoundaryCondition3D<T,D>& bc, MultiBlockLattice3D<T,D>& lat,
+                     int plane, int n1, int n2, Box3D box, boundary::BcType t) {
+    const int k = plane * 4 + (n1 > 0 ? 2 : 0) + (n2 > 0 ? 1 : 0);
+    switch (k) {
+    case 0: bc.addExternalVelocityEdge0NN(box, lat, t); break;
+    case 1: bc.addExternalVelocityEdge0NP(box, lat, t); break;
+    case 2: bc.addExternalVelocityEdge0PN(box, lat, t); break;
+    case 3: bc.addExternalVelocityEdge0PP(box, lat, t); break;
+    case 4: bc.addExternalVelocityEdge1NN(box, lat, t); break;
+    case 5: bc.addExternalVelocityEdge1NP(box, lat, t); break;
+    case 6: bc.addExternalVelocityEdge1PN(box, lat, t); break;
+    case 7: bc.addExternalVelocityEdge1PP(box, lat, t); break;
+    case 8: bc.addExternalVelocityEdge2NN(box, lat, t); break;
+    case 9: bc.addExternalVelocityEdge2NP(box, lat, t); break;
+    case 10: bc.addExternalVelocityEdge2PN(box, lat, t); break;
+    default: bc.addExternalVelocityEdge2PP(box, lat, t); break;
+    }
+}
+
+// addExternalVelocityCorner<xn><yn><zn>
+template<typename T, template<typename U> class D>
+void addVelocityCorner(OnLatticeBoundaryCondition3D<T,D>& bc, MultiBlockLattice3D<T,D>& lat,
+                       const int n[3], plint x, plint y, plint z, boundary::BcType t) {
+    const int k = (n[0] > 0 ? 4 : 0) + (n[1] > 0 ? 2 : 0) + (n[2] > 0 ? 1 : 0);
+    switch (k) {
+    case 0: bc.addExternalVelocityCornerNNN(x, y, z, lat, t); break;
+    case 1: bc.addExternalVelocityCornerNNP(x, y, z, lat, t); break;
+    case 2: bc.addExternalVelocityCornerNPN(x, y, z, lat, t); break;
+    case 3: bc.addExternalVelocityCornerNPP(x, y, z, lat, t); break;
+    case 4: bc.addExternalVelocityCornerPNN(x, y, z, lat, t); break;
+    case 5: bc.addExternalVelocityCornerPNP(x, y, z, lat, t); break;
+    case 6: bc.addExternalVelocityCornerPPN(x, y, z, lat, t); break;
+    default: bc.addExternalVelocityCornerPPP(x, y, z, lat, t); break;
+    }
 }
 
 template<typename T, template<typename U> class D>
@@ -245,7 +320,7 @@ int runCase(CaseSpec const& c) {
 
     // Boundary planes.
     OnLatticeBoundaryCondition3D<T,D>* bc = 0;
-    if (!c.bcs.empty()) {
+    if (!c.bcs.empty() || !c.blockBcs.empty()) {
         if (c.bcfamily == "zouhe")
             bc = (c.bcstyle == "dynamics") ? createDynamicsBasedZouHeBoundaryCondition3D<T,D>()
                                            : createZouHeBoundaryCondition3D<T,D>();
@@ -254,8 +329,16 @@ int runCase(CaseSpec const& c) {
                                            : createLocalBoundaryCondition3D<T,D>();
         for (pluint i = 0; i < c.bcs.size(); ++i) {
             BcSpec const& b = c.bcs[i];
-            if (b.kind == "vel") addVelocityPlane<T,D>(*bc, lat, b.dir, b.ori, b.box);
-            else addPressurePlane<T,D>(*bc, lat, b.dir, b.ori, b.box);
+            const boundary::BcType t = (boundary::BcType)b.bcType;
+            if (b.kind == "vel") addVelocityPlane<T,D>(*bc, lat, b.dir, b.ori, b.box, t);
+            else if (b.kind == "press") addPressurePlane<T,D>(*bc, lat, b.dir, b.ori, b.box, t);
+            else if (b.kind == "edge") addVelocityEdge<T,D>(*bc, lat, b.dir, b.normal[1], b.normal[2], b.box, t);
+            else addVelocityCorner<T,D>(*bc, lat, b.normal, b.box.x0, b.box.y0, b.box.z0, t);
+        }
+        for (pluint i = 0; i < c.blockBcs.size(); ++i) {
+            const boundary::BcType t = (boundary::BcType)c.blockBcs[i].bcType;
+            if (c.blockBcs[i].kind == "vel") bc->setVelocityConditionOnBlockBoundaries(lat, t);
+            else bc->setPressureConditionOnBlockBoundaries(lat, t);
         }
     }
 
@@ -278,13 +361,26 @@ int runCase(CaseSpec const& c) {
             for (plint y = b.box.y0; y <= b.box.y1; ++y)
                 for (plint z = b.box.z0; z <= b.box.z1; ++z, ++k) {
                     Cell<T,D>& cell = lat.get(x, y, z);
-                    if (b.kind == "vel") {
+                    if (b.kind != "press") {
                         Array<T,3> u((T)v[3*k], (T)v[3*k+1], (T)v[3*k+2]);
                         cell.defineVelocity(u);
                     } else {
                         cell.defineDensity((T)v[k]);
                     }
                 }
+    }
+    // Boundary values of the block-boundary conditions: every cell of the outer
+    // surface (defineVelocity / defineDensity leave other dynamics unchanged).
+    for (pluint i = 0; i < c.blockBcs.size(); ++i) {
+        std::vector<char> raw = readFile(c.blockBcs[i].file);
+        const double* v = reinterpret_cast<const double*>(&raw[0]);
+        for (plint x = 0; x < c.nx; ++x) for (plint y = 0; y < c.ny; ++y) for (plint z = 0; z < c.nz; ++z) {
+            if (x != 0 && x != c.nx - 1 && y != 0 && y != c.ny - 1 && z != 0 && z != c.nz - 1) continue;
+            const plint k = (x * c.ny + y) * c.nz + z;
+            Cell<T,D>& cell = lat.get(x, y, z);
+            if (c.blockBcs[i].kind == "vel") cell.defineVelocity(Array<T,3>((T)v[3*k], (T)v[3*k+1], (T)v[3*k+2]));
+            else cell.defineDensity((T)v[k]);
+        }
     }
     lat.initialize();
 

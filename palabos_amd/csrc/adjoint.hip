@@ -19,6 +19,9 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     if (L.time & 1) return fail(PLBAMD_ESTATE, "adjoint: start after an even number of steps");
     if (obj.inlet_x < 0 || obj.inlet_x >= L.desc.nx || obj.outlet_x < 0 || obj.outlet_x >= L.desc.nx)
         return fail(PLBAMD_EINVAL, "adjoint: measuring planes outside the lattice");
+    for (const Plane& p : L.planes)
+        if (p.shape != SHAPE_PLANE || p.bcType != PLBAMD_BC_DIRICHLET)
+            return fail(PLBAMD_EINVAL, "adjoint: boundary edges, corners and non-Dirichlet boundary types are not differentiated");
     if (!L.planes.empty()) {
         if (L.multi) return fail(PLBAMD_EINVAL, "adjoint: boundary planes on single-block lattices only");
         if (L.desc.dynamics != PLBAMD_BGK) return fail(PLBAMD_EINVAL, "adjoint: boundary planes with BGK only");

@@ -24,6 +24,7 @@
 #include "../../include/palabos_amd.h"
 #include "adjoint.h"
 #include "checkpoint.h"
+#include "fd_boundary.h"
 #include "kernels.h"
 
 namespace plbamd {
@@ -50,11 +51,18 @@ int fail(int code, const std::string& msg);
 constexpr int kEnvelope = 2;
 constexpr int kEvenThreads = 256;   // threads per workgroup of the linear even step
 
+// A boundary object: a plane (velocity or pressure), an external velocity
+// edge or an external velocity corner, with its boundary type.
+enum BoundaryShape { SHAPE_PLANE = 0, SHAPE_EDGE = 1, SHAPE_CORNER = 2 };
+
 struct Plane {
     int family, kind, dir, ori;
     bool wrapped;
     plbamd_box box;
     std::vector<double> values;
+    int shape = SHAPE_PLANE;
+    int normal[3] = {0, 0, 0};   // outward normal (planes: ori along dir)
+    int bcType = PLBAMD_BC_DIRICHLET;
 };
 
 struct Lattice {
@@ -118,13 +126,19 @@ struct Lattice {
     int nSegs = 0;
     uint64_t segCells = 0;
 
+    // data processors after the stream (fd_boundary.h): records of the four
+    // phases [procPhase[p], procPhase[p + 1])
+    ProcCell* dProc = nullptr;
+    uint32_t procPhase[5] = {0, 0, 0, 0, 0};
+    bool anyCopy = false;             // some boundary copies its value (the device values change)
+
     dim3 bulkBlock;
     int zChunk = 0;
 
     ~Lattice() {
         hipFree(arena); hipFree(dGeom); hipFree(dTags); hipFree(dSpecial); hipFree(dValues); hipFree(dAlpha);
         hipFree(dBulkPartials); hipFree(dBulkLevel1); hipFree(dSpecialPartials); hipFree(dStats); hipFree(dSegs); hipFree(dSegs1); hipFree(dTasks1);
-        hipFree(dWgBlock); hipFree(dTasks); hipFree(dWalls); hipFree(dEvenBlock);
+        hipFree(dWgBlock); hipFree(dTasks); hipFree(dWalls); hipFree(dEvenBlock); hipFree(dProc);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
         if (stream) hipStreamDestroy(stream);
@@ -195,6 +209,10 @@ struct MomentsScratch {
 template<typename S>
 int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sums, double* fields = nullptr,
             const MomentsScratch* scratch = nullptr);
+// One pass of the data processors on the state of layout parity `parity`
+// (launches only).
+template<typename S>
+int launchProcessors(Lattice& L, int parity);
 
 // Finds the device address of the fixed scratch array of
 // plbamd_reduce_moments_noalloc on `device`, which must be current (noalloc.hip).

@@ -3,6 +3,8 @@
 // the data layout are described in kernels.h.
 #include "engine.h"
 
+#include <unordered_map>
+
 namespace plbamd {
 
 thread_local std::string g_lastError;
@@ -345,10 +347,165 @@ int setup(Lattice& L) {
     return PLBAMD_OK;
 }
 
+// Copy processor of a boundary object (0 none; ProcOp otherwise): the
+// boundary types that install one (boundaryInstantiator3D.h:1042-1063,
+// :1097-1102, :1139-1157, :1216-1232).
+static int copyOp(const Plane& P) {
+    if (P.shape == SHAPE_PLANE && P.kind == PLBAMD_PRESSURE) return P.bcType == PLBAMD_BC_NEUMANN ? PROC_COPY_DENSITY : 0;
+    if (P.bcType == PLBAMD_BC_NEUMANN || P.bcType == PLBAMD_BC_OUTFLOW) return PROC_COPY_VELOCITY;
+    if (P.bcType == PLBAMD_BC_FREESLIP) return PROC_COPY_TANGENTIAL;
+    if (P.bcType == PLBAMD_BC_NORMAL_OUTFLOW) return PROC_COPY_NORMAL;
+    return 0;
+}
+
+// The data-processor records (fd_boundary.h) of a single-block lattice, in four
+// phases: copies of planes and edges, edge closures, copies of corners, corner
+// closures. Palabos runs each object's copy, then its closure, object by object
+// in the order they were added; the phases give the same results when no cell is
+// read in the phase that writes it and every pair of touching objects was added
+// in phase order (faces, edges, corners), which this checks.
+int buildProcessors(Lattice& L, const std::vector<int>& planeOf, const std::vector<uint32_t>& boxIndex,
+                    const std::vector<SpecialCell>& special, std::vector<ProcCell>& out) {
+    out.clear();
+    L.anyCopy = false;
+    bool any = false;
+    for (const Plane& P : L.planes) any = any || P.shape != SHAPE_PLANE || copyOp(P) != 0;
+    for (int k = 0; k < 5; ++k) L.procPhase[k] = 0;
+    if (!any) return PLBAMD_OK;
+    if (L.multi) return fail(PLBAMD_EINVAL, "edges, corners and non-Dirichlet boundary types need a single-block lattice");
+    const int nx = L.desc.nx, ny = L.desc.ny, nz = L.desc.nz;
+    const int per[3] = {L.desc.periodic[0] != 0, L.desc.periodic[1] != 0, L.desc.periodic[2] != 0};
+    auto at = [&](int x, int y, int z) { return ((uint64_t)x * ny + y) * nz + z; };
+    std::unordered_map<uint64_t, uint32_t> specialAt;
+    for (uint32_t k = 0; k < special.size(); ++k) specialAt[at(special[k].x, special[k].y, special[k].z)] = k;
+    const bool haveTags = !L.hostTags.empty();
+    std::vector<ProcCell> phase[4];
+    // write phases of every object: its stored values (copy), its populations
+    // (closure of an edge or corner; a wrapped plane's completion, which the
+    // device performs when the plane is read)
+    auto valuesPhase = [&](int p) { return copyOp(L.planes[p]) ? (L.planes[p].shape == SHAPE_CORNER ? 2 : 0) : -1; };
+    auto popsPhase = [&](int p) {
+        const Plane& P = L.planes[p];
+        if (P.shape == SHAPE_EDGE) return 1;
+        if (P.shape == SHAPE_CORNER) return 3;
+        return P.wrapped ? 0 : -1;
+    };
+    std::string err;
+    // what a processor reads of a neighbour: its velocity, or its density and velocity
+    enum { READ_U = 1, READ_RHO = 2 };
+    // a neighbour of object `owner` in phase `ph`; returns its reference, or fails
+    auto neighbour = [&](int owner, int ph, int x, int y, int z, int dx, int dy, int dz, int reads,
+                         uint32_t& ref) -> bool {
+        int v[3] = {x + dx, y + dy, z + dz};
+        const int n[3] = {nx, ny, nz};
+        for (int a = 0; a < 3; ++a) {
+            if (v[a] >= 0 && v[a] < n[a]) continue;
+            if (!per[a]) { err = "a boundary processor's stencil leaves the lattice"; return false; }
+            v[a] = (v[a] % n[a] + n[a]) % n[a];
+        }
+        const uint64_t c = at(v[0], v[1], v[2]);
+        auto it = specialAt.find(c);
+        if (it == specialAt.end()) {
+            if (haveTags && L.hostTags[c] != PLBAMD_TAG_BULK) {
+                err = "a boundary processor reads a solid cell";
+                return false;
+            }
+            ref = (uint32_t)c;
+            return true;
+        }
+        const uint32_t kind = special[it->second].info & 3u;
+        if (kind != SK_PLANE && kind != SK_STORE_VELOCITY) {
+            err = "a boundary processor reads a momentum-exchange cell";
+            return false;
+        }
+        const int w = planeOf[c];
+        // a velocity read of an edge, corner or velocity plane takes its stored
+        // value only; every other read takes the populations (and a plane's value)
+        const bool storedU = kind == SK_STORE_VELOCITY || L.planes[w].kind == PLBAMD_VELOCITY;
+        const bool readsPops = (reads & READ_RHO) || !storedU;
+        // the sequential order: w's writes are seen iff w was added before the reader
+        // (or is the reader itself, whose copy precedes its closure)
+        for (int wp : {valuesPhase(w), readsPops ? popsPhase(w) : -1}) {
+            if (wp < 0) continue;
+            if (wp == ph) {
+                err = "a boundary processor reads a cell written in the same pass";
+                return false;
+            }
+            const bool seen = wp < ph;
+            const bool before = w < owner || (w == owner && wp == valuesPhase(w));
+            if (seen != before) {
+                err = "boundary objects that touch must be added faces first, then edges, then corners";
+                return false;
+            }
+        }
+        ref = it->second | kNbSpecial;
+        return true;
+    };
+    for (int p = 0; p < (int)L.planes.size(); ++p) {
+        const Plane& P = L.planes[p];
+        const plbamd_box& b = P.box;
+        const int op = copyOp(P);
+        if (op) L.anyCopy = true;
+        int n[3];
+        for (int a = 0; a < 3; ++a) n[a] = P.shape == SHAPE_PLANE ? (a == P.dir ? P.ori : 0) : P.normal[a];
+        for (int x = b.x0; x <= b.x1; ++x)
+            for (int y = b.y0; y <= b.y1; ++y)
+                for (int z = b.z0; z <= b.z1; ++z) {
+                    const uint64_t c = at(x, y, z);
+                    if (planeOf[c] != p) continue;
+                    ProcCell r{};
+                    r.special = specialAt.at(c);
+                    for (int a = 0; a < 3; ++a) r.n[a] = (int8_t)n[a];
+                    if (op) {
+                        const int ph = P.shape == SHAPE_CORNER ? 2 : 0;
+                        r.op = (uint8_t)op;
+                        if (!neighbour(p, ph, x, y, z, -n[0], -n[1], -n[2], op == PROC_COPY_DENSITY ? READ_RHO : READ_U,
+                                       r.nb[0]))
+                            return fail(PLBAMD_EINVAL, err);
+                        phase[ph].push_back(r);
+                    }
+                    if (P.shape == SHAPE_EDGE) {
+                        const int pl = P.dir, d1 = (pl + 1) % 3, d2 = (pl + 2) % 3;
+                        int s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0}, sp[3] = {0, 0, 0};
+                        s1[d1] = -n[d1]; s2[d2] = -n[d2]; sp[pl] = 1;
+                        r.op = PROC_EDGE;
+                        if (!neighbour(p, 1, x, y, z, s1[0], s1[1], s1[2], READ_RHO | READ_U, r.nb[0]) ||
+                            !neighbour(p, 1, x, y, z, s2[0], s2[1], s2[2], READ_RHO | READ_U, r.nb[1]) ||
+                            !neighbour(p, 1, x, y, z, sp[0], sp[1], sp[2], READ_U, r.nb[2]) ||
+                            !neighbour(p, 1, x, y, z, -sp[0], -sp[1], -sp[2], READ_U, r.nb[3]))
+                            return fail(PLBAMD_EINVAL, err);
+                        phase[1].push_back(r);
+                    } else if (P.shape == SHAPE_CORNER) {
+                        r.op = PROC_CORNER;
+                        if (!neighbour(p, 3, x, y, z, -n[0], 0, 0, READ_RHO | READ_U, r.nb[0]) ||
+                            !neighbour(p, 3, x, y, z, 0, -n[1], 0, READ_RHO | READ_U, r.nb[1]) ||
+                            !neighbour(p, 3, x, y, z, 0, 0, -n[2], READ_RHO | READ_U, r.nb[2]))
+                            return fail(PLBAMD_EINVAL, err);
+                        phase[3].push_back(r);
+                    }
+                }
+    }
+    (void)boxIndex;
+    uint32_t k = 0;
+    for (int ph = 0; ph < 4; ++ph) {
+        L.procPhase[ph] = k;
+        out.insert(out.end(), phase[ph].begin(), phase[ph].end());
+        k += (uint32_t)phase[ph].size();
+    }
+    L.procPhase[4] = k;
+    return PLBAMD_OK;
+}
+
 // Builds the device tags, the special-cell list and the porosity array of every
 // block from the domain-wide description (tags, planes, porosity).
 int rebuild(Lattice& L) {
     if (!L.dirty) return PLBAMD_OK;
+    if (L.anyCopy && L.dValues) {
+        // the copy processors rewrote the stored values on the device: keep them
+        for (size_t p = 0; p < L.planeValueBase.size() && p < L.planes.size(); ++p)
+            PLBAMD_HIP(hipMemcpy(L.planes[p].values.data(), L.dValues + L.planeValueBase[p],
+                                 L.planes[p].values.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
     const int nx = L.desc.nx, ny = L.desc.ny, nz = L.desc.nz;
     const int per[3] = {L.desc.periodic[0] != 0, L.desc.periodic[1] != 0, L.desc.periodic[2] != 0};
     auto at = [&](int x, int y, int z) { return ((uint64_t)x * ny + y) * nz + z; };
@@ -368,6 +525,9 @@ int rebuild(Lattice& L) {
             for (int x = b.x0; x <= b.x1; ++x)
                 for (int y = b.y0; y <= b.y1; ++y)
                     for (int z = b.z0; z <= b.z1; ++z, ++k) {
+                        const int prev = planeOf[at(x, y, z)];
+                        if (prev >= 0 && (L.planes[prev].shape != SHAPE_PLANE || L.planes[p].shape != SHAPE_PLANE))
+                            return fail(PLBAMD_EINVAL, "boundary objects overlap at an edge or corner cell");
                         planeOf[at(x, y, z)] = (int)p;
                         boxIndex[at(x, y, z)] = k;
                     }
@@ -425,7 +585,13 @@ int rebuild(Lattice& L) {
                     s.block = (uint16_t)bi;
                     s.wall = wall;
                     const uint16_t bulkBit = (uint16_t)((bulk ? 1 : 0) << 8);
-                    if (p >= 0) {
+                    if (p >= 0 && L.planes[p].shape != SHAPE_PLANE) {
+                        // edge or corner: StoreVelocityDynamics over the background dynamics
+                        s.info = (uint16_t)SK_STORE_VELOCITY | bulkBit;
+                        s.value = planeValueBase[p] + boxIndex[c] * 3u;
+                        dev[slot] = BT_SKIP;
+                        special.push_back(s);
+                    } else if (p >= 0) {
                         const Plane& P = L.planes[p];
                         s.info = (uint16_t)(SK_PLANE | (P.family << 2) | (P.kind << 3) | (P.dir << 4) |
                                             ((P.ori > 0 ? 1 : 0) << 6) | ((P.wrapped ? 1 : 0) << 7)) | bulkBit;
@@ -451,6 +617,13 @@ int rebuild(Lattice& L) {
                         anyNonBulk = true;
                     }
                 }
+    }
+    std::vector<ProcCell> procs;
+    PLBAMD_TRY(buildProcessors(L, planeOf, boxIndex, special, procs));
+    hipFree(L.dProc); L.dProc = nullptr;
+    if (!procs.empty()) {
+        PLBAMD_HIP(hipMalloc(&L.dProc, procs.size() * sizeof(ProcCell)));
+        PLBAMD_HIP(hipMemcpy(L.dProc, procs.data(), procs.size() * sizeof(ProcCell), hipMemcpyHostToDevice));
     }
     hipFree(L.dTags); L.dTags = nullptr;
     hipFree(L.dSpecial); L.dSpecial = nullptr;
@@ -676,6 +849,77 @@ int plbamd_add_plane(int handle, int family, int kind, int dir, int ori, const p
     L->planes.push_back(p);
     if (plane) *plane = (int)L->planes.size() - 1;
     L->dirty = true;
+    return PLBAMD_OK;
+}
+
+static int addBoundaryObject(Lattice& L, const Plane& p, int* id) {
+    if (L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0)
+        return fail(PLBAMD_EINVAL, "boundaries are not combined with a body force");
+    L.planes.push_back(p);
+    if (id) *id = (int)L.planes.size() - 1;
+    L.dirty = true;
+    return PLBAMD_OK;
+}
+
+int plbamd_add_velocity_edge(int handle, int plane, int normal1, int normal2, const plbamd_box* box, int* id) {
+    Lattice* L = find(handle);
+    if (!L) return fail(PLBAMD_EINVAL, "add_velocity_edge: unknown handle");
+    if (!box || !boxValid(*L, *box)) return fail(PLBAMD_EINVAL, "add_velocity_edge: box outside the lattice");
+    if (plane < 0 || plane > 2 || (normal1 != -1 && normal1 != 1) || (normal2 != -1 && normal2 != 1))
+        return fail(PLBAMD_EINVAL, "add_velocity_edge: plane in 0..2, normals -1 or +1");
+    const int lo[3] = {box->x0, box->y0, box->z0}, hi[3] = {box->x1, box->y1, box->z1};
+    const int d1 = (plane + 1) % 3, d2 = (plane + 2) % 3;
+    if (lo[d1] != hi[d1] || lo[d2] != hi[d2])
+        return fail(PLBAMD_EINVAL, "add_velocity_edge: the box must be one cell thick across the edge");
+    Plane p;
+    p.family = PLBAMD_REGULARIZED; p.kind = PLBAMD_VELOCITY; p.dir = plane; p.ori = 0; p.box = *box; p.wrapped = false;
+    p.shape = SHAPE_EDGE;
+    p.normal[d1] = normal1;
+    p.normal[d2] = normal2;
+    p.values.assign(boxCells(*box) * 3, 0.0);
+    return addBoundaryObject(*L, p, id);
+}
+
+int plbamd_add_velocity_corner(int handle, int x_normal, int y_normal, int z_normal, int x, int y, int z, int* id) {
+    Lattice* L = find(handle);
+    if (!L) return fail(PLBAMD_EINVAL, "add_velocity_corner: unknown handle");
+    const plbamd_box b{x, x, y, y, z, z};
+    if (!boxValid(*L, b)) return fail(PLBAMD_EINVAL, "add_velocity_corner: cell outside the lattice");
+    const int n[3] = {x_normal, y_normal, z_normal};
+    for (int a = 0; a < 3; ++a)
+        if (n[a] != -1 && n[a] != 1) return fail(PLBAMD_EINVAL, "add_velocity_corner: normals -1 or +1");
+    Plane p;
+    p.family = PLBAMD_REGULARIZED; p.kind = PLBAMD_VELOCITY; p.dir = 0; p.ori = 0; p.box = b; p.wrapped = false;
+    p.shape = SHAPE_CORNER;
+    for (int a = 0; a < 3; ++a) p.normal[a] = n[a];
+    p.values.assign(3, 0.0);
+    return addBoundaryObject(*L, p, id);
+}
+
+int plbamd_set_boundary_type(int handle, int id, int bc_type) {
+    Lattice* L = find(handle);
+    if (!L) return fail(PLBAMD_EINVAL, "set_boundary_type: unknown handle");
+    if (id < 0 || id >= (int)L->planes.size()) return fail(PLBAMD_EINVAL, "set_boundary_type: unknown boundary");
+    if (bc_type < PLBAMD_BC_DIRICHLET || bc_type > PLBAMD_BC_NORMAL_OUTFLOW)
+        return fail(PLBAMD_EINVAL, "set_boundary_type: unknown boundary type");
+    L->planes[id].bcType = bc_type;
+    L->dirty = true;
+    return PLBAMD_OK;
+}
+
+int plbamd_execute_processors(int handle) {
+    Lattice* L = find(handle);
+    if (!L) return fail(PLBAMD_EINVAL, "execute_processors: unknown handle");
+    PLBAMD_TRY(rebuild(*L));
+    if (!L->dProc) return PLBAMD_OK;
+    const int parity = (int)(L->time & 1);
+    switch (L->desc.storage) {
+    case PLBAMD_F64: PLBAMD_TRY(launchProcessors<double>(*L, parity)); break;
+    case PLBAMD_F32: PLBAMD_TRY(launchProcessors<float>(*L, parity)); break;
+    case PLBAMD_F16: PLBAMD_TRY(launchProcessors<f16s>(*L, parity)); break;
+    default: PLBAMD_TRY(launchProcessors<bf16s>(*L, parity)); break;
+    }
+    PLBAMD_HIP(hipStreamSynchronize(L->stream));
     return PLBAMD_OK;
 }
 

@@ -1035,11 +1035,17 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Special-cell kernel: boundary planes, momentum-exchange nodes, cells next to
-// link-wise walls. Same AA addressing as the bulk kernel, with the wall mask
-// added to the edge test.
+// Special-cell kernel: boundary planes, momentum-exchange nodes, the cells of
+// external velocity edges and corners. Same AA addressing as the bulk kernel,
+// with the wall mask added to the edge test.
+//
+// SK_STORE_VELOCITY: StoreVelocityDynamics over the background dynamics
+// (boundaryCondition/boundaryDynamics.hh:171-209), the dynamics of the edge and
+// corner cells of an external velocity condition: collides as the background
+// dynamics, reports its stored velocity; the finite-difference edge and corner
+// processors (fd_boundary.h) rewrite its populations after every stream.
 
-enum SpecialKind : uint32_t { SK_PLANE = 0, SK_MOMENTUM_EXCHANGE = 1, SK_NEAR_WALL = 2 };
+enum SpecialKind : uint32_t { SK_PLANE = 0, SK_MOMENTUM_EXCHANGE = 1, SK_STORE_VELOCITY = 2 };
 
 struct SpecialCell {
     uint32_t cell;        // block-local linear cell index
@@ -1507,8 +1513,10 @@ __global__ void __launch_bounds__(256) moments_box(const void* arena, BoxArgs b,
                     rho = (T)m.solidRho[2];
                     rhoBar = rho - (T)1;
                 }
-            } else if (sp >= 0 && (b.special[sp].info & 3u) == SK_PLANE) {
-                continue;   // plane cells: moments_planes (their closures would triple this kernel's registers)
+            } else if (sp >= 0 && ((b.special[sp].info & 3u) == SK_PLANE ||
+                                   (b.special[sp].info & 3u) == SK_STORE_VELOCITY)) {
+                continue;   // plane, edge and corner cells: moments_planes (their closures would
+                            // triple this kernel's registers)
             } else {
                 T fc[19];
                 for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(f + pop_index(b.g, PAR, c.x, c.y, c.z, i, wall, c.cell));
@@ -1550,7 +1558,10 @@ __global__ void __launch_bounds__(256) moments_box(const void* arena, BoxArgs b,
 
 // The plane cells of moments_box (velocity / pressure planes): their density and
 // velocity from the closure of their boundary dynamics (plane_macroscopic_any),
-// the same outputs and per-workgroup FP64 records.
+// the same outputs and per-workgroup FP64 records. Edge and corner cells
+// (StoreVelocityDynamics): computeDensity = fullRho(get_rhoBar) of the
+// populations, computeRhoBar = rhoBar of that density, computeVelocity = the
+// stored velocity (boundaryDynamics.hh:82-87, :324-344).
 template<typename S>
 __global__ void __launch_bounds__(256) moments_planes(const void* arena, BoxArgs b, MomentsArgs m, double* rhoOut,
                                                       double* uOut, double* partials) {
@@ -1559,16 +1570,23 @@ __global__ void __launch_bounds__(256) moments_planes(const void* arena, BoxArgs
     double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < b.nSpecial; k += gridDim.x * blockDim.x) {
         const SpecialCell sc = b.special[k];
-        if (sc.block != b.block || (sc.info & 3u) != SK_PLANE) continue;
+        const uint32_t kind = sc.info & 3u;
+        if (sc.block != b.block || (kind != SK_PLANE && kind != SK_STORE_VELOCITY)) continue;
         const int x = sc.x, y = sc.y, z = sc.z;
         if (x < b.ix0 || x >= b.ix0 + b.ilx || y < b.iy0 || y >= b.iy0 + b.ily || z < b.iz0 || z >= b.iz0 + b.ilz)
             continue;
         T fc[19];
         for (int i = 0; i < 19; ++i)
             fc[i] = Store<S>::load(f + pop_index(b.g, b.parity, x, y, z, i, b.parity ? sc.wall : 0u, sc.cell));
-        T rhoBar, u[3];
-        plane_macroscopic_any(fc, sc.info, m.values + sc.value, rhoBar, u);
-        const T rho = rhoBar + (T)1;
+        T rhoBar, u[3], rho;
+        if (kind == SK_PLANE) {
+            plane_macroscopic_any(fc, sc.info, m.values + sc.value, rhoBar, u);
+            rho = rhoBar + (T)1;
+        } else {
+            rho = rhobar_sum(fc) + (T)1;
+            rhoBar = rho - (T)1;
+            for (int c = 0; c < 3; ++c) u[c] = (T)m.values[sc.value + c];
+        }
         const uint64_t staged = ((uint64_t)(b.ox + x - b.ix0) * b.sly + (b.oy + y - b.iy0)) * b.slz + (b.oz + z - b.iz0);
         if (rhoOut) rhoOut[staged] = (double)rho;
         if (uOut) {

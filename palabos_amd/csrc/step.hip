@@ -159,6 +159,8 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record) {
 #undef PLBAMD_SPECIAL
         PLBAMD_HIP(hipGetLastError());
     }
+    // data processors after the stream: Neumann-type copies, edge and corner closures
+    if (L.dProc && !record) PLBAMD_TRY(launchProcessors<S>(L, odd ? 0 : 1));
     if (stats) {
         // two levels when there are many records (a single workgroup reading
         // 65536 records took 40 % of a 256^3 step): kStatsLevel1 workgroups in a

@@ -299,6 +299,30 @@ int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sum
 }
 
 
+template<typename S>
+int launchProcessors(Lattice& L, int parity) {
+    if (!L.dProc) return PLBAMD_OK;
+    ProcArgs a{};
+    a.arena = L.arena;
+    a.blocks = L.dGeom;
+    a.special = L.dSpecial;
+    a.values = L.dValues;
+    a.tags = L.dTags;
+    a.walls = L.dWalls;
+    a.cells = L.dProc;
+    a.parity = parity;
+    a.omega = L.desc.omega;
+    for (int ph = 0; ph < 4; ++ph) {
+        a.begin = L.procPhase[ph];
+        a.end = L.procPhase[ph + 1];
+        if (a.end == a.begin) continue;
+        hipLaunchKernelGGL((boundary_processors<S>), dim3((a.end - a.begin + 255) / 256), dim3(256), 0, L.stream, a);
+        PLBAMD_HIP(hipGetLastError());
+    }
+    return PLBAMD_OK;
+}
+
+template int launchProcessors<PLBAMD_S>(Lattice&, int);
 template int exchange<PLBAMD_S>(Lattice&);
 template hipError_t transferDevice<PLBAMD_S>(Lattice&, const plbamd_box&, int, void*, bool);
 template int transfer<PLBAMD_S>(Lattice&, const plbamd_box&, int, void*, size_t, bool);

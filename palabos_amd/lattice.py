@@ -115,6 +115,7 @@ class Block3D:
         self.periodic = tuple(bool(p) for p in periodic)
         self._tags = None
         self._planes = []
+        self._plane_values = {}   # velocity boundaries: the values last set from here, by id
 
     # -- lifetime -----------------------------------------------------------
     def close(self):
@@ -168,32 +169,127 @@ class Block3D:
             self._tags[x0:x1 + 1, y0:y1 + 1, z0:z1 + 1] = dynamics.tag
         check(lib().plbamd_set_tags(self.handle, self._tags.ctypes.data))
 
-    def _add_plane(self, family, kind, direction, orientation, box):
+    def _add_plane(self, family, kind, direction, orientation, box, wrapped=False, bc_type="dirichlet"):
         fam = {"zouhe": native.ZOUHE, "regularized": native.REGULARIZED}[family]
+        if wrapped:
+            fam |= native.WRAPPED
         pid = ctypes.c_int(-1)
         b = Box.of(box)
         check(lib().plbamd_add_plane(self.handle, fam, kind, int(direction), int(orientation),
                                      ctypes.byref(b), ctypes.byref(pid)))
         self._planes.append((kind, b))
+        self.setBoundaryType(pid.value, bc_type)
         return pid.value
 
-    def addVelocityBoundary(self, direction, orientation, box, family="zouhe"):
-        """add{Velocity}Boundary{dir}{N|P} of the dynamics-based Zou-He or regularized
-        boundary condition (boundaryCondition3D.h:40-200). Returns a plane id."""
-        return self._add_plane(family, native.VELOCITY, direction, orientation, box)
+    def addVelocityBoundary(self, direction, orientation, box, family="zouhe", wrapped=False, bc_type="dirichlet"):
+        """add{Velocity}Boundary{dir}{N|P}(box, lattice, bcType) of the Zou-He or
+        regularized boundary condition (boundaryCondition3D.h:40-200): the
+        dynamics-based factory, or with wrapped=True the wrapped one
+        (createZouHeBoundaryCondition3D / createLocalBoundaryCondition3D). Returns
+        a boundary id."""
+        return self._add_plane(family, native.VELOCITY, direction, orientation, box, wrapped, bc_type)
 
-    def addPressureBoundary(self, direction, orientation, box, family="zouhe"):
-        return self._add_plane(family, native.PRESSURE, direction, orientation, box)
+    def addPressureBoundary(self, direction, orientation, box, family="zouhe", wrapped=False, bc_type="dirichlet"):
+        return self._add_plane(family, native.PRESSURE, direction, orientation, box, wrapped, bc_type)
+
+    def addExternalVelocityEdge(self, plane, normal1, normal2, box, bc_type="dirichlet"):
+        """addExternalVelocityEdge{plane}{normal1}{normal2}(box, lattice, bcType)
+        (boundaryInstantiator3D.h:1113-1153): StoreVelocityDynamics and the
+        finite-difference edge closure; normal1 / normal2 (-1 | +1) are the outward
+        normal along (plane+1)%3 / (plane+2)%3. Returns a boundary id."""
+        pid = ctypes.c_int(-1)
+        b = Box.of(box)
+        check(lib().plbamd_add_velocity_edge(self.handle, int(plane), int(normal1), int(normal2), ctypes.byref(b),
+                                             ctypes.byref(pid)))
+        self._planes.append((native.VELOCITY, b))
+        self.setBoundaryType(pid.value, bc_type)
+        return pid.value
+
+    def addExternalVelocityCorner(self, x_normal, y_normal, z_normal, x, y, z, bc_type="dirichlet"):
+        """addExternalVelocityCorner{xN}{yN}{zN}(x, y, z, lattice, bcType)
+        (boundaryInstantiator3D.h:1197-1232). Returns a boundary id."""
+        pid = ctypes.c_int(-1)
+        check(lib().plbamd_add_velocity_corner(self.handle, int(x_normal), int(y_normal), int(z_normal),
+                                               int(x), int(y), int(z), ctypes.byref(pid)))
+        self._planes.append((native.VELOCITY, Box(int(x), int(x), int(y), int(y), int(z), int(z))))
+        self.setBoundaryType(pid.value, bc_type)
+        return pid.value
+
+    def setVelocityConditionOnBlockBoundaries(self, family="regularized", wrapped=True, bc_type="dirichlet"):
+        """OnLatticeBoundaryCondition3D::setVelocityConditionOnBlockBoundaries(lattice,
+        bcType) (boundaryCondition3D.hh:300-400): velocity conditions on the six face
+        interiors, the twelve edges and the eight corners of the whole lattice, in
+        that order. Returns the 26 boundary ids."""
+        n = self.shape
+        ids = []
+        for d in range(3):
+            for ori in (-1, 1):
+                lo = [1, 1, 1]
+                hi = [n[0] - 2, n[1] - 2, n[2] - 2]
+                lo[d] = hi[d] = 0 if ori < 0 else n[d] - 1
+                ids.append(self.addVelocityBoundary(d, ori, (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]), family,
+                                                    wrapped, bc_type))
+        for plane in range(3):
+            d1, d2 = (plane + 1) % 3, (plane + 2) % 3
+            for n1 in (-1, 1):
+                for n2 in (-1, 1):
+                    lo = [0, 0, 0]
+                    hi = [0, 0, 0]
+                    lo[plane], hi[plane] = 1, n[plane] - 2
+                    lo[d1] = hi[d1] = 0 if n1 < 0 else n[d1] - 1
+                    lo[d2] = hi[d2] = 0 if n2 < 0 else n[d2] - 1
+                    ids.append(self.addExternalVelocityEdge(plane, n1, n2, (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]),
+                                                            bc_type))
+        for xn in (-1, 1):
+            for yn in (-1, 1):
+                for zn in (-1, 1):
+                    ids.append(self.addExternalVelocityCorner(xn, yn, zn, 0 if xn < 0 else n[0] - 1,
+                                                              0 if yn < 0 else n[1] - 1, 0 if zn < 0 else n[2] - 1,
+                                                              bc_type))
+        return ids
+
+    def setBoundaryType(self, boundary, bc_type):
+        """bcType of a plane, edge or corner: 'dirichlet', 'neumann', 'freeslip',
+        'density', 'outflow' or 'normalOutflow' (boundary::BcType)."""
+        check(lib().plbamd_set_boundary_type(self.handle, int(boundary), native.BC_TYPES[bc_type]))
+
+    def initialize(self):
+        """MultiBlock3D::initialize: one pass of the data processors (copies of
+        Neumann-type boundaries, edge and corner closures)."""
+        check(lib().plbamd_execute_processors(self.handle))
+
+    def setBoundaryVelocityIn(self, box, u):
+        """setBoundaryVelocity(lattice, box, u) (dataInitializerWrapper3D.h): u on every
+        velocity plane, edge and corner cell inside the box."""
+        x0, x1, y0, y1, z0, z1 = box
+        u = np.asarray(u, dtype=np.float64)
+        for pid, (kind, b) in enumerate(self._planes):
+            if kind != native.VELOCITY:
+                continue
+            ix0, ix1 = max(x0, b.x0), min(x1, b.x1)
+            iy0, iy1 = max(y0, b.y0), min(y1, b.y1)
+            iz0, iz1 = max(z0, b.z0), min(z1, b.z1)
+            if ix0 > ix1 or iy0 > iy1 or iz0 > iz1:
+                continue
+            vals = self._plane_values.get(pid)
+            if vals is None:
+                vals = np.zeros(b.shape() + (3,))
+            vals[ix0 - b.x0:ix1 - b.x0 + 1, iy0 - b.y0:iy1 - b.y0 + 1, iz0 - b.z0:iz1 - b.z0 + 1] = u
+            self._plane_values[pid] = vals
+            flat = np.ascontiguousarray(vals.reshape(-1, 3))
+            check(lib().plbamd_set_plane_values(self.handle, pid, flat.ctypes.data, flat.size))
 
     def setBoundaryVelocity(self, plane, u):
-        """setBoundaryVelocity: a 3-vector or an array [box cells, 3] in box C order."""
+        """setBoundaryVelocity of one velocity plane, edge or corner: a 3-vector or an
+        array [box cells, 3] in box C order."""
         kind, b = self._planes[plane]
         if kind != native.VELOCITY:
-            raise ValueError("not a velocity plane")
+            raise ValueError("not a velocity boundary")
         n = b.cells()
         u = np.asarray(u, dtype=np.float64)
         vals = np.broadcast_to(u, (n, 3)) if u.shape == (3,) else u.reshape(n, 3)
         vals = np.ascontiguousarray(vals)
+        self._plane_values[plane] = vals.reshape(b.shape() + (3,)).copy()
         check(lib().plbamd_set_plane_values(self.handle, plane, vals.ctypes.data, vals.size))
 
     def setBoundaryDensity(self, plane, rho):

@@ -17,6 +17,8 @@ F64, F32, F16, BF16 = 0, 1, 2, 3
 BGK, RBGK = 0, 1
 TAG_BULK, TAG_BOUNCE_BACK, TAG_NO_DYNAMICS, TAG_MOMENTUM_EXCHANGE, TAG_WALL_LINK = 0, 1, 2, 3, 4
 ZOUHE, REGULARIZED = 0, 1
+WRAPPED = 0x100
+BC_TYPES = {"dirichlet": 0, "neumann": 1, "freeslip": 2, "density": 3, "outflow": 4, "normalOutflow": 5}
 VELOCITY, PRESSURE = 0, 1
 NUM_STATS = 10
 
@@ -33,6 +35,8 @@ EXPORTS = (
     "plbamd_get_statistics", "plbamd_compute_macroscopic", "plbamd_reduce_moments",
     "plbamd_adjoint_gradient", "plbamd_get_time", "plbamd_save_binary_block", "plbamd_load_binary_block",
     "plbamd_compute_fields", "plbamd_set_solid_density", "plbamd_reduce_moments_noalloc",
+    "plbamd_add_velocity_edge", "plbamd_add_velocity_corner", "plbamd_set_boundary_type",
+    "plbamd_execute_processors",
 )
 
 
@@ -110,6 +114,10 @@ def lib():
         "plbamd_compute_macroscopic": [i32, pbox, vp, vp],
         "plbamd_reduce_moments": [i32, pbox, vp],
         "plbamd_reduce_moments_noalloc": [i32, pbox, vp],
+        "plbamd_add_velocity_edge": [i32, i32, i32, i32, pbox, ctypes.POINTER(ctypes.c_int)],
+        "plbamd_add_velocity_corner": [i32, i32, i32, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_int)],
+        "plbamd_set_boundary_type": [i32, i32, i32],
+        "plbamd_execute_processors": [i32],
         "plbamd_compute_fields": [i32, pbox, vp],
         "plbamd_set_solid_density": [i32, i32, ctypes.c_double],
         "plbamd_get_time": [i32, ctypes.POINTER(ctypes.c_int64)],

@@ -21,6 +21,9 @@ class Plane:
     ori: int
     box: Tuple[int, int, int, int, int, int]
     values: np.ndarray     # velocity: [cells, 3]; pressure: [cells] (rho)
+    shape: str = "plane"   # 'plane' | 'edge' (dir = the edge's axis) | 'corner'
+    normal: Tuple[int, int, int] = (0, 0, 0)   # edges / corners: outward normal
+    bc_type: str = "dirichlet"                 # boundary::BcType name
 
     def cells(self):
         x0, x1, y0, y1, z0, z1 = self.box
@@ -145,10 +148,94 @@ def cavity_case(n, omega, u_lid, storage="f64", family="zouhe", ny=None, nz=None
                 planes=[Plane(family, "velocity", 2, 1, box, vel)])
 
 
+def seeded_velocity(seed, amp=0.02):
+    """A deterministic velocity per cell (x, y, z) -> (ux, uy, uz) in [-amp, amp)."""
+    def f(x, y, z):
+        return tuple(amp * uniform((3,), seed * 100003 + (x * 1000 + y) * 1000 + z))
+    return f
+
+
+def doctest_cavity_case(nx, ny, nz, storage="f64"):
+    """The 3D regression case of the reference's jlabos doctest
+    (jlabos/src/testing/test_cylinder.py:78-130): BGK with omega = 1/(3 nu + 1/2),
+    nu = 0.02 ny / 100; regularized velocity conditions of the wrapped factory
+    (createLocalBoundaryCondition3D) on every face, edge and corner, zero velocity
+    except the lid Box3D(1, nx-2, 1, ny-2, nz-1, nz-1) with u = (0.02, 0, 0); the
+    lattice starts at rest. The doctest runs 30 steps."""
+    u_max, re = 0.02, 100.0
+    nu = u_max * ny / re
+    tau = 3 * nu + 0.5
+
+    def vel(x, y, z):
+        return (u_max, 0.0, 0.0) if (z == nz - 1 and 1 <= x <= nx - 2 and 1 <= y <= ny - 2) else (0.0, 0.0, 0.0)
+
+    planes = block_boundary_objects((nx, ny, nz), "regularized", velocity=vel)
+    return Case(nx, ny, nz, storage=storage, omega=1.0 / tau, planes=planes)
+
+
+def block_boundary_objects(shape, family, bc_type="dirichlet", velocity=None, kind="velocity"):
+    """The boundary objects OnLatticeBoundaryCondition3D::set{Velocity,Pressure}
+    ConditionOnBlockBoundaries installs on a whole nx x ny x nz lattice
+    (boundaryCondition3D.hh:300-400, BlockSurface3D boxes of core/blockSurface3D.cpp:44-198),
+    in its order: the six face interiors (0N 0P 1N 1P 2N 2P), then, for velocity
+    conditions, the twelve edges (plane 0 NN NP PN PP, plane 1 ..., plane 2 ...;
+    normal1 along (plane+1)%3, normal2 along (plane+2)%3) and the eight corners
+    (NNN NNP NPN NPP PNN PNP PPN PPP). velocity(x, y, z) -> (ux, uy, uz) gives the
+    stored velocity of each cell (default 0); pressure faces store rho = 1."""
+    n = shape
+    lo_hi = [(0, n[a] - 1) for a in range(3)]
+    vel = velocity or (lambda x, y, z: (0.0, 0.0, 0.0))
+
+    def cells(box):
+        x0, x1, y0, y1, z0, z1 = box
+        return [(x, y, z) for x in range(x0, x1 + 1) for y in range(y0, y1 + 1) for z in range(z0, z1 + 1)]
+
+    def values(box):
+        return np.array([vel(*c) for c in cells(box)], dtype=np.float64).reshape(-1, 3)
+
+    out = []
+    for d in range(3):
+        for ori in (-1, 1):
+            lo = [1, 1, 1]
+            hi = [n[0] - 2, n[1] - 2, n[2] - 2]
+            lo[d] = hi[d] = lo_hi[d][0] if ori < 0 else lo_hi[d][1]
+            box = (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
+            if kind == "velocity":
+                out.append(Plane(family, "velocity", d, ori, box, values(box), bc_type=bc_type))
+            else:
+                out.append(Plane(family, "pressure", d, ori, box, np.ones(len(cells(box))), bc_type=bc_type))
+    if kind != "velocity":
+        return out
+    for plane in range(3):
+        d1, d2 = (plane + 1) % 3, (plane + 2) % 3
+        for n1 in (-1, 1):
+            for n2 in (-1, 1):
+                lo = [0, 0, 0]
+                hi = [0, 0, 0]
+                lo[plane], hi[plane] = 1, n[plane] - 2
+                lo[d1] = hi[d1] = 0 if n1 < 0 else n[d1] - 1
+                lo[d2] = hi[d2] = 0 if n2 < 0 else n[d2] - 1
+                normal = [0, 0, 0]
+                normal[d1], normal[d2] = n1, n2
+                box = (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
+                out.append(Plane(family, "velocity", plane, 0, box, values(box), shape="edge",
+                                 normal=tuple(normal), bc_type=bc_type))
+    for xn in (-1, 1):
+        for yn in (-1, 1):
+            for zn in (-1, 1):
+                x = 0 if xn < 0 else n[0] - 1
+                y = 0 if yn < 0 else n[1] - 1
+                z = 0 if zn < 0 else n[2] - 1
+                box = (x, x, y, y, z, z)
+                out.append(Plane(family, "velocity", 0, 0, box, values(box), shape="corner",
+                                 normal=(xn, yn, zn), bc_type=bc_type))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Running a case on the HIP engine
 
-def gpu_lattice(case):
+def gpu_lattice(case, wrapped=False):
     import palabos_amd as plb
     dyn = plb.BGK(case.omega) if case.dynamics == "bgk" else plb.RegularizedBGK(case.omega)
     force = case.force if any(case.force) else None
@@ -157,22 +244,35 @@ def gpu_lattice(case):
     if case.tags is not None:
         lat.setTags(case.tags)
     for p in case.planes:
-        if p.kind == "velocity":
-            pid = lat.addVelocityBoundary(p.dir, p.ori, p.box, family=p.family)
+        bt = getattr(p, "bc_type", "dirichlet")
+        shape = getattr(p, "shape", "plane")
+        if shape == "edge":
+            d1, d2 = (p.dir + 1) % 3, (p.dir + 2) % 3
+            pid = lat.addExternalVelocityEdge(p.dir, p.normal[d1], p.normal[d2], p.box, bc_type=bt)
+            lat.setBoundaryVelocity(pid, p.values)
+        elif shape == "corner":
+            pid = lat.addExternalVelocityCorner(*p.normal, p.box[0], p.box[2], p.box[4], bc_type=bt)
+            lat.setBoundaryVelocity(pid, p.values)
+        elif p.kind == "velocity":
+            pid = lat.addVelocityBoundary(p.dir, p.ori, p.box, family=p.family, wrapped=wrapped, bc_type=bt)
             lat.setBoundaryVelocity(pid, p.values)
         else:
-            pid = lat.addPressureBoundary(p.dir, p.ori, p.box, family=p.family)
+            pid = lat.addPressureBoundary(p.dir, p.ori, p.box, family=p.family, wrapped=wrapped, bc_type=bt)
             lat.setBoundaryDensity(pid, p.values)
     if case.alpha is not None:
         lat.setPorosity(case.alpha)
     return lat
 
 
-def gpu_run(case, f_storage, nsteps, stats=True):
-    """Uploads a storage-typed state, runs nsteps on the GPU, returns (state, stats)."""
-    lat = gpu_lattice(case)
+def gpu_run(case, f_storage, nsteps, stats=True, initialize=False, wrapped=False):
+    """Uploads a storage-typed state, runs nsteps on the GPU, returns (state, stats).
+    initialize: one pass of the data processors first (MultiBlock3D::initialize);
+    wrapped: planes of the wrapped factories (downloads return them completed)."""
+    lat = gpu_lattice(case, wrapped=wrapped)
     try:
         lat.setPopulations(f_storage)
+        if initialize:
+            lat.initialize()
         lat.toggleInternalStatistics(stats)
         lat.collideAndStream(nsteps)
         dtype = np.uint16 if case.storage in ("f16", "bf16") else (

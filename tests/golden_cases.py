@@ -6,8 +6,8 @@ from it, so a test reads everything it needs from the fixture itself.
 """
 import numpy as np
 
-from cases import (Case, Plane, TAG_BB, TAG_MEBB, TAG_NODYN, cavity_case, channel_case, random_state,
-                   rest_state)
+from cases import (Case, Plane, TAG_BB, TAG_MEBB, TAG_NODYN, block_boundary_objects, cavity_case, channel_case,
+                   doctest_cavity_case, random_state, rest_state, seeded_velocity)
 
 # Uneven block decomposition of the 12 x 10 x 9 multi-block case (inclusive boxes).
 MULTIBLOCK_BLOCKS = [
@@ -36,6 +36,22 @@ GOLDEN = {
     "guo_f32": {"dumps": [11]},
     "multiblock_mebb_f64": {"dumps": [10], "blocks": MULTIBLOCK_BLOCKS},
     "cavity64_f64": {"dumps": [1000], "summary": True},
+    # velocity conditions on every face, edge and corner (setVelocityConditionOnBlockBoundaries)
+    "blockbc_regularized_wrapped_f64": {"dumps": [1, 7], "bcstyle": "wrapped"},
+    "blockbc_regularized_rbgk_f64": {"dumps": [1, 7]},
+    "blockbc_zouhe_f64": {"dumps": [7]},
+    "blockbc_zouhe_wrapped_f32": {"dumps": [1, 7], "bcstyle": "wrapped"},
+    "blockbc_regularized_f32": {"dumps": [7]},
+    "blockbc_outflow_f64": {"dumps": [1, 6]},
+    "blockbc_freeslip_f64": {"dumps": [6]},
+    "blockbc_normaloutflow_wrapped_f64": {"dumps": [6], "bcstyle": "wrapped"},
+    "pressure_neumann_f64": {"dumps": [5]},
+    # the jlabos doctest cavities (jlabos/src/testing/test_cylinder.py:85-98), 30 steps
+    "doctest_cavity_20x20x20_f64": {"dumps": [30], "summary": True, "bcstyle": "wrapped"},
+    "doctest_cavity_39x39x39_f64": {"dumps": [30], "summary": True, "bcstyle": "wrapped"},
+    "doctest_cavity_58x58x58_f64": {"dumps": [30], "summary": True, "bcstyle": "wrapped"},
+    "doctest_cavity_20x31x42_f64": {"dumps": [30], "summary": True, "bcstyle": "wrapped"},
+    "doctest_cavity_20x31x42_f32": {"dumps": [30], "summary": True, "bcstyle": "wrapped"},
 }
 
 
@@ -102,6 +118,22 @@ def _make(name):
         case, init_seed = c, 17
     elif base == "cavity64":
         case = cavity_case(64, 1 / 0.6, 0.05, storage=storage)
+    elif base.startswith("blockbc_"):
+        family = "zouhe" if "zouhe" in base else "regularized"
+        bc_type = {"outflow": "outflow", "freeslip": "freeslip", "normaloutflow": "normalOutflow"}.get(
+            base.split("_")[1], "dirichlet")
+        shape = (8, 9, 10)
+        case = Case(*shape, storage=storage, dynamics="rbgk" if "rbgk" in base else "bgk", omega=1.6,
+                    planes=block_boundary_objects(shape, family, bc_type=bc_type, velocity=seeded_velocity(5)))
+        init_seed = 3
+    elif base == "pressure_neumann":
+        shape = (8, 9, 10)
+        case = Case(*shape, storage=storage, omega=1.7,
+                    planes=block_boundary_objects(shape, "zouhe", bc_type="neumann", kind="pressure"))
+        init_seed = 3
+    elif base.startswith("doctest_cavity_"):
+        nx, ny, nz = (int(v) for v in base.split("_")[2].split("x"))
+        case = doctest_cavity_case(nx, ny, nz, storage=storage)
     else:
         raise KeyError(name)
     return case, init_seed

@@ -17,18 +17,24 @@ def available():
     return os.path.exists(DRIVER)
 
 
-def run(case, init_f64, dumps, bcstyle="dynamics", blocks=None, timeout=1800, fields=False):
+def run(case, init_f64, dumps, bcstyle="dynamics", blocks=None, timeout=1800, fields=False, block_bcs=None,
+        family=None):
     """Returns ({step: populations in Palabos's T}, {step: stats row}); with
     fields, also {step: [nx,ny,nz,7] analysis fields} (computeDensity,
     computeRhoBar, computeVelocity x y z, computeKineticEnergy,
-    computeVelocityNorm) as a third item."""
+    computeVelocityNorm) as a third item. block_bcs: [(kind 'vel' | 'press',
+    bc type name, per-cell values of the whole lattice)] installed with
+    set{Velocity,Pressure}ConditionOnBlockBoundaries; family: the boundary
+    factory when the case has no planes of its own."""
     T = {"f64": "double", "f32": "float"}[case.storage]
     with tempfile.TemporaryDirectory() as tmp:
         lines = [f"T {T}", f"size {case.nx} {case.ny} {case.nz}", f"dyn {case.dynamics}",
                  f"omega {case.omega!r}",
                  "periodic " + " ".join(str(int(bool(p))) for p in case.periodic),
                  f"bcstyle {bcstyle}"]
-        fams = {p.family for p in case.planes}
+        fams = {p.family for p in case.planes if getattr(p, "shape", "plane") == "plane"}
+        if family:
+            fams |= {family}
         assert len(fams) <= 1, "one boundary family per case"
         if fams:
             lines.append(f"bcfamily {fams.pop()}")
@@ -39,11 +45,26 @@ def run(case, init_f64, dumps, bcstyle="dynamics", blocks=None, timeout=1800, fi
             tp = os.path.join(tmp, "tags.bin")
             np.ascontiguousarray(case.tags, dtype=np.uint8).tofile(tp)
             lines.append(f"tags {tp}")
+        bct = {"dirichlet": 0, "neumann": 1, "freeslip": 2, "density": 3, "outflow": 4, "normalOutflow": 5}
         for k, p in enumerate(case.planes):
             vp = os.path.join(tmp, f"bc{k}.bin")
             np.ascontiguousarray(p.values, dtype=np.float64).tofile(vp)
-            kind = "vel" if p.kind == "velocity" else "press"
-            lines.append(f"bc {kind} {p.dir} {p.ori} " + " ".join(map(str, p.box)) + f" {vp}")
+            t = bct[getattr(p, "bc_type", "dirichlet")]
+            shape = getattr(p, "shape", "plane")
+            if shape == "edge":
+                d1, d2 = (p.dir + 1) % 3, (p.dir + 2) % 3
+                lines.append(f"edge {p.dir} {p.normal[d1]} {p.normal[d2]} " + " ".join(map(str, p.box)) +
+                             f" {vp} {t}")
+            elif shape == "corner":
+                x0, _, y0, _, z0, _ = p.box
+                lines.append("corner " + " ".join(map(str, p.normal)) + f" {x0} {y0} {z0} {vp} {t}")
+            else:
+                kind = "vel" if p.kind == "velocity" else "press"
+                lines.append(f"bc {kind} {p.dir} {p.ori} " + " ".join(map(str, p.box)) + f" {vp} {t}")
+        for k, (kind, t, vals) in enumerate(block_bcs or []):
+            vp = os.path.join(tmp, f"blockbc{k}.bin")
+            np.ascontiguousarray(vals, dtype=np.float64).tofile(vp)
+            lines.append(f"blockbc {kind} {bct[t]} {vp}")
         if any(case.force):
             lines.append("force " + " ".join(repr(float(v)) for v in case.force))
         for b in blocks or []:
