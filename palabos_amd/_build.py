@@ -78,7 +78,7 @@ def build_library(verbose=False):
 
 CPP_DIR = os.path.join(HERE, "cpp")
 CPP_BIN = os.path.join(CPP_DIR, "bin")
-CPP_EXAMPLES = ["cavity3d", "coprocessor", "units"]
+CPP_EXAMPLES = ["cavity3d", "coprocessor", "units", "cylinder3d"]
 
 
 def build_cpp_examples(verbose=False):

@@ -90,6 +90,12 @@ struct Box3D {
     }
 };
 
+// boundary::BcType (boundaryCondition/boundaryCondition.h:40); the integer
+// values are the C ABI's PLBAMD_BC_* constants.
+namespace boundary {
+typedef enum { dirichlet, neumann, freeslip, density, outflow, normalOutflow } BcType;
+}
+
 inline bool intersect(Box3D const& a, Box3D const& b, Box3D& out) {
     out = Box3D(std::max(a.x0, b.x0), std::min(a.x1, b.x1), std::max(a.y0, b.y0), std::min(a.y1, b.y1),
                 std::max(a.z0, b.z0), std::min(a.z1, b.z1));
@@ -356,8 +362,15 @@ public:
         return Cell<T, Descriptor>(&mirror[index(x, y, z) * 19], this, x, y, z);
     }
 
-    // MultiBlock3D::initialize (multiBlock3D.cpp:330-332): no envelope to fill on one block.
-    void initialize() { materialize(); }
+    // MultiBlock3D::initialize (multiBlock3D.cpp:330-332): one pass of the data
+    // processors (Neumann-type copies, edge and corner closures); no envelope to
+    // fill on one block.
+    void initialize() {
+        materialize();
+        pushHost();
+        detail::check(plbamd_execute_processors(handle), "execute_processors");
+        hostValid = false;
+    }
 
     // MultiBlockLattice3D::collideAndStream (multiBlockLattice3D.hh:318-360).
     void collideAndStream() { collideAndStream(1); }
@@ -391,11 +404,42 @@ public:
                 for (plint z = b.z0; z <= b.z1; ++z) tags[index(x, y, z)] = tag;
         tagsDirty = true;
     }
-    int addPlane(int family, int kind, int dir, int ori, Box3D const& box, bool wrapped) {
+    int addPlane(int family, int kind, int dir, int ori, Box3D const& box, bool wrapped,
+                 boundary::BcType bcType = boundary::dirichlet) {
         Plane p;
         p.family = family; p.kind = kind; p.dir = dir; p.ori = ori; p.box = box; p.wrapped = wrapped;
         p.values.assign((size_t)box.nCells() * (kind == PLBAMD_VELOCITY ? 3 : 1), kind == PLBAMD_VELOCITY ? 0.0 : 1.0);
         p.id = -1;
+        p.bcType = (int)bcType;
+        planes.push_back(p);
+        planesDirty = true;
+        return (int)planes.size() - 1;
+    }
+    // An external velocity edge along `plane` (normals along (plane+1)%3 and
+    // (plane+2)%3) or corner (normals along x, y, z): StoreVelocityDynamics and the
+    // finite-difference processor (boundaryInstantiator3D.h:1113-1232).
+    int addEdge(int plane, int normal1, int normal2, Box3D const& box, boundary::BcType bcType) {
+        Plane p;
+        p.family = PLBAMD_REGULARIZED; p.kind = PLBAMD_VELOCITY; p.dir = plane; p.ori = 0; p.box = box;
+        p.wrapped = false;
+        p.shape = 1;
+        p.normal[0] = normal1; p.normal[1] = normal2;
+        p.values.assign((size_t)box.nCells() * 3, 0.0);
+        p.id = -1;
+        p.bcType = (int)bcType;
+        planes.push_back(p);
+        planesDirty = true;
+        return (int)planes.size() - 1;
+    }
+    int addCorner(int xNormal, int yNormal, int zNormal, plint x, plint y, plint z, boundary::BcType bcType) {
+        Plane p;
+        p.family = PLBAMD_REGULARIZED; p.kind = PLBAMD_VELOCITY; p.dir = 0; p.ori = 0; p.box = Box3D(x, x, y, y, z, z);
+        p.wrapped = false;
+        p.shape = 2;
+        p.normal[0] = xNormal; p.normal[1] = yNormal; p.normal[2] = zNormal;
+        p.values.assign(3, 0.0);
+        p.id = -1;
+        p.bcType = (int)bcType;
         planes.push_back(p);
         planesDirty = true;
         return (int)planes.size() - 1;
@@ -476,6 +520,9 @@ private:
         Box3D box;
         std::vector<double> values;
         bool valuesDirty = true;
+        int shape = 0;                 // 0 plane, 1 edge, 2 corner
+        int normal[3] = {0, 0, 0};     // edge: normal1, normal2; corner: x, y, z normals
+        int bcType = 0;
     };
     friend class Cell<T, Descriptor>;
 
@@ -508,8 +555,18 @@ private:
             for (auto& p : planes) {
                 if (p.id < 0) {
                     plbamd_box cb = p.box.toC();
-                    const int fam = p.family | (p.wrapped ? PLBAMD_WRAPPED : 0);
-                    detail::check(plbamd_add_plane(handle, fam, p.kind, p.dir, p.ori, &cb, &p.id), "add_plane");
+                    if (p.shape == 1) {
+                        detail::check(plbamd_add_velocity_edge(handle, p.dir, p.normal[0], p.normal[1], &cb, &p.id),
+                                      "add_velocity_edge");
+                    } else if (p.shape == 2) {
+                        detail::check(plbamd_add_velocity_corner(handle, p.normal[0], p.normal[1], p.normal[2],
+                                                                 cb.x0, cb.y0, cb.z0, &p.id),
+                                      "add_velocity_corner");
+                    } else {
+                        const int fam = p.family | (p.wrapped ? PLBAMD_WRAPPED : 0);
+                        detail::check(plbamd_add_plane(handle, fam, p.kind, p.dir, p.ori, &cb, &p.id), "add_plane");
+                    }
+                    if (p.bcType != 0) detail::check(plbamd_set_boundary_type(handle, p.id, p.bcType), "set_boundary_type");
                 }
                 if (p.valuesDirty) {
                     detail::check(plbamd_set_plane_values(handle, p.id, p.values.data(), p.values.size()),
@@ -585,30 +642,103 @@ public:
     OnLatticeBoundaryCondition3D(int family_, bool wrapped_) : family(family_), wrapped(wrapped_) {}
     virtual ~OnLatticeBoundaryCondition3D() {}
     typedef MultiBlockLattice3D<T, Descriptor> L;
-    void addVelocityBoundary0N(Box3D b, L& l) { l.addPlane(family, PLBAMD_VELOCITY, 0, -1, b, wrapped); }
-    void addVelocityBoundary0P(Box3D b, L& l) { l.addPlane(family, PLBAMD_VELOCITY, 0, 1, b, wrapped); }
-    void addVelocityBoundary1N(Box3D b, L& l) { l.addPlane(family, PLBAMD_VELOCITY, 1, -1, b, wrapped); }
-    void addVelocityBoundary1P(Box3D b, L& l) { l.addPlane(family, PLBAMD_VELOCITY, 1, 1, b, wrapped); }
-    void addVelocityBoundary2N(Box3D b, L& l) { l.addPlane(family, PLBAMD_VELOCITY, 2, -1, b, wrapped); }
-    void addVelocityBoundary2P(Box3D b, L& l) { l.addPlane(family, PLBAMD_VELOCITY, 2, 1, b, wrapped); }
-    void addPressureBoundary0N(Box3D b, L& l) { l.addPlane(family, PLBAMD_PRESSURE, 0, -1, b, wrapped); }
-    void addPressureBoundary0P(Box3D b, L& l) { l.addPlane(family, PLBAMD_PRESSURE, 0, 1, b, wrapped); }
-    void addPressureBoundary1N(Box3D b, L& l) { l.addPlane(family, PLBAMD_PRESSURE, 1, -1, b, wrapped); }
-    void addPressureBoundary1P(Box3D b, L& l) { l.addPlane(family, PLBAMD_PRESSURE, 1, 1, b, wrapped); }
-    void addPressureBoundary2N(Box3D b, L& l) { l.addPlane(family, PLBAMD_PRESSURE, 2, -1, b, wrapped); }
-    void addPressureBoundary2P(Box3D b, L& l) { l.addPlane(family, PLBAMD_PRESSURE, 2, 1, b, wrapped); }
-    // Velocity condition on the six face interiors of `block`
-    // (boundaryCondition3D.hh:300-400 without the edge and corner processors).
-    void setVelocityConditionOnBlockBoundaries(L& l, Box3D block) {
-        plint x0 = block.x0, x1 = block.x1, y0 = block.y0, y1 = block.y1, z0 = block.z0, z1 = block.z1;
-        addVelocityBoundary0N(Box3D(x0, x0, y0 + 1, y1 - 1, z0 + 1, z1 - 1), l);
-        addVelocityBoundary0P(Box3D(x1, x1, y0 + 1, y1 - 1, z0 + 1, z1 - 1), l);
-        addVelocityBoundary1N(Box3D(x0 + 1, x1 - 1, y0, y0, z0 + 1, z1 - 1), l);
-        addVelocityBoundary1P(Box3D(x0 + 1, x1 - 1, y1, y1, z0 + 1, z1 - 1), l);
-        addVelocityBoundary2N(Box3D(x0 + 1, x1 - 1, y0 + 1, y1 - 1, z0, z0), l);
-        addVelocityBoundary2P(Box3D(x0 + 1, x1 - 1, y0 + 1, y1 - 1, z1, z1), l);
+    typedef boundary::BcType B;
+    static constexpr B D = boundary::dirichlet;
+    void addVelocityBoundary0N(Box3D b, L& l, B t = D) { l.addPlane(family, PLBAMD_VELOCITY, 0, -1, b, wrapped, t); }
+    void addVelocityBoundary0P(Box3D b, L& l, B t = D) { l.addPlane(family, PLBAMD_VELOCITY, 0, 1, b, wrapped, t); }
+    void addVelocityBoundary1N(Box3D b, L& l, B t = D) { l.addPlane(family, PLBAMD_VELOCITY, 1, -1, b, wrapped, t); }
+    void addVelocityBoundary1P(Box3D b, L& l, B t = D) { l.addPlane(family, PLBAMD_VELOCITY, 1, 1, b, wrapped, t); }
+    void addVelocityBoundary2N(Box3D b, L& l, B t = D) { l.addPlane(family, PLBAMD_VELOCITY, 2, -1, b, wrapped, t); }
+    void addVelocityBoundary2P(Box3D b, L& l, B t = D) { l.addPlane(family, PLBAMD_VELOCITY, 2, 1, b, wrapped, t); }
+    void addPressureBoundary0N(Box3D b, L& l, B t = D) { l.addPlane(family, PLBAMD_PRESSURE, 0, -1, b, wrapped, t); }
+    void addPressureBoundary0P(Box3D b, L& l, B t = D) { l.addPlane(family, PLBAMD_PRESSURE, 0, 1, b, wrapped, t); }
+    void addPressureBoundary1N(Box3D b, L& l, B t = D) { l.addPlane(family, PLBAMD_PRESSURE, 1, -1, b, wrapped, t); }
+    void addPressureBoundary1P(Box3D b, L& l, B t = D) { l.addPlane(family, PLBAMD_PRESSURE, 1, 1, b, wrapped, t); }
+    void addPressureBoundary2N(Box3D b, L& l, B t = D) { l.addPlane(family, PLBAMD_PRESSURE, 2, -1, b, wrapped, t); }
+    void addPressureBoundary2P(Box3D b, L& l, B t = D) { l.addPlane(family, PLBAMD_PRESSURE, 2, 1, b, wrapped, t); }
+
+    // External velocity edges (boundaryCondition3D.h): the Zou-He and regularized
+    // factories install the same StoreVelocityDynamics + OuterVelocityEdgeFunctional3D.
+    void addExternalVelocityEdge0NN(Box3D b, L& l, B t = D) { l.addEdge(0, -1, -1, b, t); }
+    void addExternalVelocityEdge0NP(Box3D b, L& l, B t = D) { l.addEdge(0, -1, 1, b, t); }
+    void addExternalVelocityEdge0PN(Box3D b, L& l, B t = D) { l.addEdge(0, 1, -1, b, t); }
+    void addExternalVelocityEdge0PP(Box3D b, L& l, B t = D) { l.addEdge(0, 1, 1, b, t); }
+    void addExternalVelocityEdge1NN(Box3D b, L& l, B t = D) { l.addEdge(1, -1, -1, b, t); }
+    void addExternalVelocityEdge1NP(Box3D b, L& l, B t = D) { l.addEdge(1, -1, 1, b, t); }
+    void addExternalVelocityEdge1PN(Box3D b, L& l, B t = D) { l.addEdge(1, 1, -1, b, t); }
+    void addExternalVelocityEdge1PP(Box3D b, L& l, B t = D) { l.addEdge(1, 1, 1, b, t); }
+    void addExternalVelocityEdge2NN(Box3D b, L& l, B t = D) { l.addEdge(2, -1, -1, b, t); }
+    void addExternalVelocityEdge2NP(Box3D b, L& l, B t = D) { l.addEdge(2, -1, 1, b, t); }
+    void addExternalVelocityEdge2PN(Box3D b, L& l, B t = D) { l.addEdge(2, 1, -1, b, t); }
+    void addExternalVelocityEdge2PP(Box3D b, L& l, B t = D) { l.addEdge(2, 1, 1, b, t); }
+    void addExternalVelocityCornerNNN(plint x, plint y, plint z, L& l, B t = D) { l.addCorner(-1, -1, -1, x, y, z, t); }
+    void addExternalVelocityCornerNNP(plint x, plint y, plint z, L& l, B t = D) { l.addCorner(-1, -1, 1, x, y, z, t); }
+    void addExternalVelocityCornerNPN(plint x, plint y, plint z, L& l, B t = D) { l.addCorner(-1, 1, -1, x, y, z, t); }
+    void addExternalVelocityCornerNPP(plint x, plint y, plint z, L& l, B t = D) { l.addCorner(-1, 1, 1, x, y, z, t); }
+    void addExternalVelocityCornerPNN(plint x, plint y, plint z, L& l, B t = D) { l.addCorner(1, -1, -1, x, y, z, t); }
+    void addExternalVelocityCornerPNP(plint x, plint y, plint z, L& l, B t = D) { l.addCorner(1, -1, 1, x, y, z, t); }
+    void addExternalVelocityCornerPPN(plint x, plint y, plint z, L& l, B t = D) { l.addCorner(1, 1, -1, x, y, z, t); }
+    void addExternalVelocityCornerPPP(plint x, plint y, plint z, L& l, B t = D) { l.addCorner(1, 1, 1, x, y, z, t); }
+
+    // Velocity conditions on the six face interiors, the twelve edges and the
+    // eight corners of `block` that intersect `applicationDomain`
+    // (boundaryCondition3D.hh:300-400, BlockSurface3D boxes of
+    // core/blockSurface3D.cpp:44-198, boundary width 1).
+    void setVelocityConditionOnBlockBoundaries(L& l, Box3D block, Box3D app, B t = D) {
+        const plint x0 = block.x0, x1 = block.x1, y0 = block.y0, y1 = block.y1, z0 = block.z0, z1 = block.z1;
+        Box3D i;
+        if (intersect(Box3D(x0, x0, y0 + 1, y1 - 1, z0 + 1, z1 - 1), app, i)) addVelocityBoundary0N(i, l, t);
+        if (intersect(Box3D(x1, x1, y0 + 1, y1 - 1, z0 + 1, z1 - 1), app, i)) addVelocityBoundary0P(i, l, t);
+        if (intersect(Box3D(x0 + 1, x1 - 1, y0, y0, z0 + 1, z1 - 1), app, i)) addVelocityBoundary1N(i, l, t);
+        if (intersect(Box3D(x0 + 1, x1 - 1, y1, y1, z0 + 1, z1 - 1), app, i)) addVelocityBoundary1P(i, l, t);
+        if (intersect(Box3D(x0 + 1, x1 - 1, y0 + 1, y1 - 1, z0, z0), app, i)) addVelocityBoundary2N(i, l, t);
+        if (intersect(Box3D(x0 + 1, x1 - 1, y0 + 1, y1 - 1, z1, z1), app, i)) addVelocityBoundary2P(i, l, t);
+        // edges: plane x (y, z normals), plane y (z, x normals), plane z (x, y normals)
+        if (intersect(Box3D(x0 + 1, x1 - 1, y0, y0, z0, z0), app, i)) addExternalVelocityEdge0NN(i, l, t);
+        if (intersect(Box3D(x0 + 1, x1 - 1, y0, y0, z1, z1), app, i)) addExternalVelocityEdge0NP(i, l, t);
+        if (intersect(Box3D(x0 + 1, x1 - 1, y1, y1, z0, z0), app, i)) addExternalVelocityEdge0PN(i, l, t);
+        if (intersect(Box3D(x0 + 1, x1 - 1, y1, y1, z1, z1), app, i)) addExternalVelocityEdge0PP(i, l, t);
+        if (intersect(Box3D(x0, x0, y0 + 1, y1 - 1, z0, z0), app, i)) addExternalVelocityEdge1NN(i, l, t);
+        if (intersect(Box3D(x1, x1, y0 + 1, y1 - 1, z0, z0), app, i)) addExternalVelocityEdge1NP(i, l, t);
+        if (intersect(Box3D(x0, x0, y0 + 1, y1 - 1, z1, z1), app, i)) addExternalVelocityEdge1PN(i, l, t);
+        if (intersect(Box3D(x1, x1, y0 + 1, y1 - 1, z1, z1), app, i)) addExternalVelocityEdge1PP(i, l, t);
+        if (intersect(Box3D(x0, x0, y0, y0, z0 + 1, z1 - 1), app, i)) addExternalVelocityEdge2NN(i, l, t);
+        if (intersect(Box3D(x0, x0, y1, y1, z0 + 1, z1 - 1), app, i)) addExternalVelocityEdge2NP(i, l, t);
+        if (intersect(Box3D(x1, x1, y0, y0, z0 + 1, z1 - 1), app, i)) addExternalVelocityEdge2PN(i, l, t);
+        if (intersect(Box3D(x1, x1, y1, y1, z0 + 1, z1 - 1), app, i)) addExternalVelocityEdge2PP(i, l, t);
+        if (intersect(Box3D(x0, x0, y0, y0, z0, z0), app, i)) addExternalVelocityCornerNNN(i.x0, i.y0, i.z0, l, t);
+        if (intersect(Box3D(x0, x0, y0, y0, z1, z1), app, i)) addExternalVelocityCornerNNP(i.x0, i.y0, i.z0, l, t);
+        if (intersect(Box3D(x0, x0, y1, y1, z0, z0), app, i)) addExternalVelocityCornerNPN(i.x0, i.y0, i.z0, l, t);
+        if (intersect(Box3D(x0, x0, y1, y1, z1, z1), app, i)) addExternalVelocityCornerNPP(i.x0, i.y0, i.z0, l, t);
+        if (intersect(Box3D(x1, x1, y0, y0, z0, z0), app, i)) addExternalVelocityCornerPNN(i.x0, i.y0, i.z0, l, t);
+        if (intersect(Box3D(x1, x1, y0, y0, z1, z1), app, i)) addExternalVelocityCornerPNP(i.x0, i.y0, i.z0, l, t);
+        if (intersect(Box3D(x1, x1, y1, y1, z0, z0), app, i)) addExternalVelocityCornerPPN(i.x0, i.y0, i.z0, l, t);
+        if (intersect(Box3D(x1, x1, y1, y1, z1, z1), app, i)) addExternalVelocityCornerPPP(i.x0, i.y0, i.z0, l, t);
     }
-    void setVelocityConditionOnBlockBoundaries(L& l) { setVelocityConditionOnBlockBoundaries(l, l.getBoundingBox()); }
+    void setVelocityConditionOnBlockBoundaries(L& l, Box3D app, B t = D) {
+        setVelocityConditionOnBlockBoundaries(l, l.getBoundingBox(), app, t);
+    }
+    void setVelocityConditionOnBlockBoundaries(L& l, B t = D) {
+        setVelocityConditionOnBlockBoundaries(l, l.getBoundingBox(), l.getBoundingBox(), t);
+    }
+    // Pressure conditions on the six face interiors (boundaryCondition3D.hh:420-446;
+    // the reference installs nothing on edges and corners).
+    void setPressureConditionOnBlockBoundaries(L& l, Box3D block, Box3D app, B t = D) {
+        const plint x0 = block.x0, x1 = block.x1, y0 = block.y0, y1 = block.y1, z0 = block.z0, z1 = block.z1;
+        Box3D i;
+        if (intersect(Box3D(x0, x0, y0 + 1, y1 - 1, z0 + 1, z1 - 1), app, i)) addPressureBoundary0N(i, l, t);
+        if (intersect(Box3D(x1, x1, y0 + 1, y1 - 1, z0 + 1, z1 - 1), app, i)) addPressureBoundary0P(i, l, t);
+        if (intersect(Box3D(x0 + 1, x1 - 1, y0, y0, z0 + 1, z1 - 1), app, i)) addPressureBoundary1N(i, l, t);
+        if (intersect(Box3D(x0 + 1, x1 - 1, y1, y1, z0 + 1, z1 - 1), app, i)) addPressureBoundary1P(i, l, t);
+        if (intersect(Box3D(x0 + 1, x1 - 1, y0 + 1, y1 - 1, z0, z0), app, i)) addPressureBoundary2N(i, l, t);
+        if (intersect(Box3D(x0 + 1, x1 - 1, y0 + 1, y1 - 1, z1, z1), app, i)) addPressureBoundary2P(i, l, t);
+    }
+    void setPressureConditionOnBlockBoundaries(L& l, Box3D app, B t = D) {
+        setPressureConditionOnBlockBoundaries(l, l.getBoundingBox(), app, t);
+    }
+    void setPressureConditionOnBlockBoundaries(L& l, B t = D) {
+        setPressureConditionOnBlockBoundaries(l, l.getBoundingBox(), l.getBoundingBox(), t);
+    }
 private:
     int family;
     bool wrapped;

@@ -80,3 +80,41 @@ def test_coprocessor_plugin_roundtrip(gpu, tmp_path, precision):
     got = np.fromfile(fout, dtype=f0.dtype).reshape(f0.shape)
     want, _ = oracle.run(case, f0, 7)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("size,precision", [("20x20x20", "double"), ("20x31x42", "double"), ("20x31x42", "float")])
+def test_cpp_doctest_cavity_matches_palabos(gpu, tmp_path, size, precision):
+    """The jlabos regression cavity written against the C++ API
+    (setVelocityConditionOnBlockBoundaries of createLocalBoundaryCondition3D:
+    faces, edges and corners; lattice.initialize() runs the processors once):
+    populations bit-identical to compiled Palabos after 30 steps, and its
+    computeAverageEnergy / computeAverageDensity."""
+    name = f"doctest_cavity_{size}_{'f64' if precision == 'double' else 'f32'}"
+    case, _, data = load(name, GOLDEN_DIR)
+    out = tmp_path / "f.bin"
+    r = _run(["cylinder3d", case.nx, case.ny, case.nz, 30, out, precision])
+    got = np.fromfile(out, dtype=np.float64 if precision == "double" else np.float32)
+    assert hashlib.sha256(got.tobytes()).hexdigest() == str(data["sha256_s30"])
+    tol = 1e-12 if precision == "double" else 1e-6
+    assert r["avg_energy"] == pytest.approx(data["stats_s30"][3], rel=tol)
+    assert r["avg_density"] == pytest.approx(data["stats_s30"][2], rel=tol)
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_palabos_coprocessor_path_runs_on_the_engine(gpu, precision):
+    """The reference's own co-processor path with the engine behind it
+    (oracle/coprocessor_driver.cpp, compiled against the reference's sources): an
+    adapter deriving from the real plb::CoProcessor3D<T>, reached through
+    global::defaultCoProcessor3D<T>(), two uneven periodic blocks offloaded by
+    initiateCoProcessors and stepped through transferToCoProcessors /
+    MultiBlockLattice3D::collideAndStream / transferFromCoProcessors: every
+    population equals Palabos's CPU run after 6 steps, bit for bit."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "coprocessor_driver")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/coprocessor_driver is built by oracle/Makefile.ref where the reference sources are")
+    res = subprocess.run([exe, precision, "14", "9", "10", "6", "6", "1.55"], capture_output=True, text=True,
+                         timeout=300)
+    assert res.returncode == 0, res.stderr
+    r = json.loads(res.stdout.strip().splitlines()[-1])
+    assert r["handles"] == 2
+    assert r["differ"] == 0, r
