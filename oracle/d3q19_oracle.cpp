@@ -261,13 +261,36 @@ inline void addGuoForce(T* f, const T u[3], const T force[3], T omega) {
 
 // Collisions ---------------------------------------------------------------
 
-// dynamicsTemplates3D.h:2289-2368 (bgk_ma2_collision_base, D3Q19)
+// The constants of the BGK update f_i <- one_m_omega f_i + t_omega (C1 + C2 + C3),
+// and the relaxation rate they stand for.
 template<typename T>
-inline T bgk_collision(T* f, T rhoBar, const T j[3], T omega, T invRho) {
-    T one_m_omega = (T)1 - omega;
-    T t0_omega = W<T>::t(0) * omega;
-    T t1_omega = W<T>::t(1) * omega;
-    T t4_omega = W<T>::t(4) * omega;
+struct Rates { T one_m_omega, t0_omega, t1_omega, t4_omega, omega; };
+
+// Palabos's (dynamicsTemplates3D.h:2291-2294).
+template<typename T>
+inline Rates<T> palabosRates(T omega) {
+    return {(T)1 - omega, W<T>::t(0) * omega, W<T>::t(1) * omega, W<T>::t(4) * omega, omega};
+}
+
+// The 16-bit storage modes (no Palabos counterpart): omega' = 36 w with
+// w = rint(omega 2^25 / 36) 2^-25 (the product in FP64); 12w, 2w, w and
+// 1 - 36w are then exact, so the update returns rhoBar and j with the factor
+// one_m_omega + 36 w = 1 exactly, where Palabos's rounded t_k omega leave a
+// constant bias of order 1e-7 per step.
+inline Rates<float> exactRates(float omega) {
+    const float q = (float)std::nearbyint((double)omega * (33554432.0 / 36.0));
+    const float w = q * 0x1p-25f;
+    const float om = 36.0f * w;
+    return {1.0f - om, 12.0f * w, 2.0f * w, w, om};
+}
+
+// dynamicsTemplates3D.h:2289-2368 (bgk_ma2_collision_base, D3Q19), with the rates `r`
+template<typename T>
+inline T bgk_collision(T* f, T rhoBar, const T j[3], const Rates<T>& r, T invRho) {
+    T one_m_omega = r.one_m_omega;
+    T t0_omega = r.t0_omega;
+    T t1_omega = r.t1_omega;
+    T t4_omega = r.t4_omega;
     T jSqr = j[0]*j[0] + j[1]*j[1] + j[2]*j[2];
     T kx = (T)3 * j[0];
     T ky = (T)3 * j[1];
@@ -436,6 +459,15 @@ struct Engine {
         return true;
     }
 
+    // The BGK rates: Palabos's, or exactRates in the 16-bit storage modes.
+    Rates<T> bgkRates() const {
+        if (d.storage >= 2) {
+            const Rates<float> r = exactRates((float)d.omega);
+            return {(T)r.one_m_omega, (T)r.t0_omega, (T)r.t1_omega, (T)r.t4_omega, (T)r.omega};
+        }
+        return palabosRates((T)d.omega);
+    }
+
     // Base dynamics collision, with statistics (isoThermalDynamics.hh:318-330 and :1587-1600).
     void baseCollide(T* f, Stats& st, long k) const {
         T omega = (T)d.omega;
@@ -447,7 +479,7 @@ struct Engine {
             uSqr = forcedCollide(f, k, rhoBar, st);
         } else if (d.dynamics == 0) {
             get_rhoBar_j(f, rhoBar, j);
-            uSqr = bgk_collision(f, rhoBar, j, omega, invRhoOf(rhoBar));
+            uSqr = bgk_collision(f, rhoBar, j, bgkRates(), invRhoOf(rhoBar));
         } else {
             T pi[6];
             get_rhoBar_j_PiNeq(f, rhoBar, j, pi);
@@ -471,7 +503,9 @@ struct Engine {
     //   Brinkman:   the equilibrium velocity is damped, u <- (1 - alpha) u,
     //               before j_eq = rho u is formed (alpha in [0,1], 1 = solid).
     T forcedCollide(T* f, long k, T& rhoBar, Stats& st) const {
-        T omega = (T)d.omega;
+        const Rates<T> rates = bgkRates();
+        // BGK in the 16-bit modes: omega' in the drag and the Guo term as well
+        T omega = d.dynamics == 0 ? rates.omega : (T)d.omega;
         T j[3];
         T pi[6];
         T rhoBarP;
@@ -499,7 +533,7 @@ struct Engine {
             st.porous[c] += (double)(d.dynamics == 0 ? omega * (rho * u0 - jeq[c]) : (rho * u0 - jeq[c]));
         }
         T uSqr;
-        if (d.dynamics == 0) uSqr = bgk_collision(f, rhoBar, jeq, omega, invRhoOf(rhoBar));
+        if (d.dynamics == 0) uSqr = bgk_collision(f, rhoBar, jeq, rates, invRhoOf(rhoBar));
         else uSqr = rlb_collision(f, rhoBar, invRhoOf(rhoBar), jeq, pi, omega);
         if (F[0] != 0 || F[1] != 0 || F[2] != 0) addGuoForce(f, u, F, omega);
         return uSqr;
@@ -1247,6 +1281,11 @@ int oracle_objective_dual(const OracleDesc* d, const double* f0, int nsteps, int
     return 0;
 }
 
+// The 16-bit modes' BGK constants for omega: [one_m_omega, t0_omega, t1_omega, t4_omega, omega'].
+void oracle_exact_rates(float omega, float* out) {
+    const Rates<float> r = exactRates(omega);
+    out[0] = r.one_m_omega; out[1] = r.t0_omega; out[2] = r.t1_omega; out[3] = r.t4_omega; out[4] = r.omega;
+}
 void oracle_f32_to_f16(long n, const float* in, uint16_t* out) { for (long k = 0; k < n; ++k) out[k] = f32_to_f16(in[k]); }
 void oracle_f16_to_f32(long n, const uint16_t* in, float* out) { for (long k = 0; k < n; ++k) out[k] = f16_to_f32(in[k]); }
 void oracle_f32_to_bf16(long n, const float* in, uint16_t* out) { for (long k = 0; k < n; ++k) out[k] = f32_to_bf16(in[k]); }

@@ -79,6 +79,8 @@ def lib():
         _lib.oracle_objective_dual.restype = ctypes.c_int
         _lib.oracle_fields.argtypes = [ctypes.POINTER(_Desc), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib.oracle_fields.restype = ctypes.c_int
+        _lib.oracle_exact_rates.argtypes = [ctypes.c_float, ctypes.c_void_p]
+        _lib.oracle_exact_rates.restype = None
         for name in ("oracle_f32_to_f16", "oracle_f16_to_f32", "oracle_f32_to_bf16",
                      "oracle_bf16_to_f32"):
             getattr(_lib, name).argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
@@ -215,6 +217,14 @@ def equilibrium(rho, u):
     u = np.ascontiguousarray(u, dtype=np.float64)
     out = np.empty(rho.shape + (19,), dtype=np.float64)
     lib().oracle_equilibrium(rho.size, rho.ctypes.data, u.ctypes.data, out.ctypes.data)
+    return out
+
+
+def exact_rates(omega):
+    """The 16-bit storage modes' BGK constants for `omega` (FP32):
+    [1 - omega', t0 omega', t1 omega', t4 omega', omega']."""
+    out = np.empty(5, dtype=np.float32)
+    lib().oracle_exact_rates(ctypes.c_float(omega), out.ctypes.data)
     return out
 
 

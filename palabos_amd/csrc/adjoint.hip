@@ -117,7 +117,9 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     a.alpha = L.dAlpha;
     a.gAlpha = gA;
     a.tags = L.dTags;
-    a.omega = L.desc.omega;
+    // the forward's relaxation rate: 16-bit BGK relaxes with omega' (exact_rates)
+    a.omega = (st != PLBAMD_F64 && L.desc.dynamics == PLBAMD_BGK) ? (double)exact_rates((float)L.desc.omega).omega
+                                                                  : L.desc.omega;
     for (int c = 0; c < 3; ++c) a.force[c] = L.desc.force[c];
     a.invT = 1.0 / (T - obj.first_step);
     a.wdp = obj.weight_pressure_drop;

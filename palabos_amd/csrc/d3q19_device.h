@@ -99,14 +99,50 @@ __device__ __forceinline__ T feq(int i, T rhoBar, T invRho, const T j[3], T jSqr
     return weight<T>(i) * (rhoBar + (T)3. * c_j + invRho * ((T)4.5 * c_j * c_j - (T)1.5 * jSqr));
 }
 
-// BGK relaxation in i / i+9 pairs (dynamicsTemplates3D.h:2289-2368
-// bgk_ma2_collision_base). Returns |u|^2.
+// Constants of the BGK update f_i <- om1 f_i + a_k X_i (weight class k of i);
+// `omega` is the relaxation rate they stand for.
 template<typename T>
-__device__ __forceinline__ T bgk(T* f, T rhoBar, const T j[3], T omega, T invRho) {
-    const T om1 = (T)1 - omega;
-    const T a0 = w0<T>() * omega;
-    const T a1 = w1<T>() * omega;
-    const T a2 = w2<T>() * omega;
+struct BgkRates {
+    T om1, a0, a1, a2, omega;
+};
+
+// Palabos's constants (dynamicsTemplates3D.h:2291-2294): 1 - omega and t_k omega.
+template<typename T>
+__host__ __device__ inline BgkRates<T> palabos_rates(T omega) {
+    return {(T)1 - omega, w0<T>() * omega, w1<T>() * omega, w2<T>() * omega, omega};
+}
+
+// The 16-bit storage modes' constants. With Palabos's rounded t_k omega and
+// 1 - omega in FP32, the update multiplies rho-bar and j by
+// om1 + sum_k a_k (moment weights) = 1 + O(1e-7) every step: a constant bias
+// that grows linearly in time (8e-8 per step in the velocity of a smooth flow,
+// the largest round-off term of an FP32 step). Here omega is moved to
+// omega' = 36 w with w = rint(omega 2^25 / 36) 2^-25 (|omega' - omega| <= 5.4e-7
+// for omega in (0, 2]): w, 2w, 12w, 36w and 1 - 36w are then exact in FP32
+// (9 rint(...) < 2^24), a_0 = 12w, a_1 = 2w, a_2 = w are t_k omega' exactly, and
+// the factor is om1 + 36w = 1 with no rounding. q is formed in FP64, so omega'
+// is the grid point nearest to omega.
+__host__ __device__ inline BgkRates<float> exact_rates(float omega) {
+    const float q = (float)rint((double)omega * (33554432.0 / 36.0));
+    const float w = q * 0x1p-25f;
+    const float om = 36.0f * w;
+    return {1.0f - om, 12.0f * w, 2.0f * w, w, om};
+}
+
+template<typename T, bool EXACT>
+__device__ __forceinline__ BgkRates<T> bgk_rates(T omega) {
+    if constexpr (EXACT) return exact_rates(omega);
+    else return palabos_rates<T>(omega);
+}
+
+// BGK relaxation in i / i+9 pairs (dynamicsTemplates3D.h:2289-2368
+// bgk_ma2_collision_base), with the rates `r`. Returns |u|^2.
+template<typename T>
+__device__ __forceinline__ T bgk(T* f, T rhoBar, const T j[3], const BgkRates<T>& r, T invRho) {
+    const T om1 = r.om1;
+    const T a0 = r.a0;
+    const T a1 = r.a1;
+    const T a2 = r.a2;
     const T jSqr = j[0] * j[0] + j[1] * j[1] + j[2] * j[2];
     const T kx = (T)3 * j[0];
     const T ky = (T)3 * j[1];
@@ -216,6 +252,8 @@ __device__ __forceinline__ void add_guo_force(T* f, const T u[3], const T F[3], 
 
 // Bulk collision selector. DYN: 0 BGK, 1 RBGK. FORCED: Guo body force and/or
 // Brinkman porosity active (a = alpha of the cell). Returns |u|^2 and rhoBar.
+// EXACT (the 16-bit storage modes, T = float): BGK relaxes with exact_rates,
+// and omega' replaces omega in the Guo term and the Brinkman drag as well.
 //   BGK + Guo follows GuoExternalForceBGKdynamics::collide
 //   (externalForceDynamics.hh:215-232; velocity :59-74). RBGK + Guo and the
 //   Brinkman damping u <- (1 - alpha) u of the equilibrium velocity have no
@@ -224,14 +262,16 @@ __device__ __forceinline__ void add_guo_force(T* f, const T u[3], const T F[3], 
 // collision: omega * rho * (u - u_damped) for BGK, rho * (u - u_damped) for
 // regularized BGK (which sets the first moment to j_eq), u the undamped
 // half-force velocity.
-template<typename T, int DYN, bool FORCED>
-__device__ __forceinline__ T collide_bulk(T* f, T omega, const T F[3], T a, T& rhoBarOut, T drag[3]) {
+template<typename T, int DYN, bool FORCED, bool EXACT = false>
+__device__ __forceinline__ T collide_bulk(T* f, T omegaIn, const T F[3], T a, T& rhoBarOut, T drag[3]) {
+    const BgkRates<T> rates = bgk_rates<T, EXACT && DYN == 0>(omegaIn);
+    const T omega = rates.omega;
     if constexpr (!FORCED) {
         T rhoBar, j[3];
         T uSqr;
         if constexpr (DYN == 0) {
             rhobar_j(f, rhoBar, j);
-            uSqr = bgk(f, rhoBar, j, omega, inv_rho(rhoBar));
+            uSqr = bgk(f, rhoBar, j, rates, inv_rho(rhoBar));
         } else {
             T pi[6];
             rhobar_j_pineq(f, rhoBar, j, pi);
@@ -265,7 +305,7 @@ __device__ __forceinline__ T collide_bulk(T* f, T omega, const T F[3], T a, T& r
             drag[c] = (DYN == 0) ? omega * (rho * u0 - jeq[c]) : (rho * u0 - jeq[c]);
         }
         T uSqr;
-        if constexpr (DYN == 0) uSqr = bgk(f, rhoBar, jeq, omega, inv_rho(rhoBar));
+        if constexpr (DYN == 0) uSqr = bgk(f, rhoBar, jeq, rates, inv_rho(rhoBar));
         else uSqr = rlb(f, rhoBar, inv_rho(rhoBar), jeq, pi, omega);
         if (F[0] != (T)0 || F[1] != (T)0 || F[2] != (T)0) add_guo_force(f, u, F, omega);
         rhoBarOut = rhoBar;

@@ -240,7 +240,7 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, const BlockGeom&
         T rhoBar;
         T drag[3] = {0, 0, 0};
         const T alpha = (FORCED && a.alpha) ? (T)a.alpha[g.cellOffset + cell] : (T)0;
-        const T uSqr = collide_bulk<T, DYN, FORCED>(fc, omega, F, alpha, rhoBar, drag);
+        const T uSqr = collide_bulk<T, DYN, FORCED, sizeof(S) == 2>(fc, omega, F, alpha, rhoBar, drag);
         if constexpr (STATS) {
             if (inBulk) {
                 st[0] += (double)rhoBar;
@@ -1127,7 +1127,7 @@ __global__ void __launch_bounds__(256) special_step(SpecialArgs a) {
             if (kind == SK_PLANE) complete_plane(fc, sc.info, a.values + sc.value);
             T rhoBar, drag[3] = {0, 0, 0};
             const T alpha = (FORCED && a.alpha) ? (T)a.alpha[g.cellOffset + cell] : (T)0;
-            const T uSqr = collide_bulk<T, DYN, FORCED>(fc, omega, F, alpha, rhoBar, drag);
+            const T uSqr = collide_bulk<T, DYN, FORCED, sizeof(S) == 2>(fc, omega, F, alpha, rhoBar, drag);
             if (bulk) {
                 st[0] += (double)rhoBar;
                 st[1] += (double)uSqr;

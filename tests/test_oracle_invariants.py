@@ -414,3 +414,51 @@ def test_brinkman_intermediate_alpha_without_force(dynamics):
     np.testing.assert_allclose(st["porous"], removed, rtol=1e-12, atol=1e-15)
     want_j, want_pi, _ = _closed_form(dynamics, omega, np.zeros(3), a[:, 0], rho_bar, j, pi)
     assert np.abs(pi1 - want_pi).max() < 2e-15
+
+
+# -- the 16-bit modes' BGK rates -----------------------------------------------
+
+def test_exact_rates_are_exact_and_nearest():
+    """The 16-bit storage modes relax BGK with omega' and constants that carry no
+    rounding: a_k = t_k omega' exactly, the mass and momentum factors
+    om1 + a0 + 6 a1 + 12 a2 and om1 + 3 (2 a1 + 8 a2) are 1 exactly, and omega' is
+    the multiple of 36 * 2^-25 nearest to omega. A floor instead of rint fails the
+    nearest check; a finer grid (2^-26) makes 36 w inexact above omega = 1 and fails
+    the factors; a swapped weight class fails a_k = t_k omega'."""
+    rng = np.random.RandomState(3)
+    omegas = np.concatenate([rng.uniform(1e-3, 2.0, 20000), [2.0, 1.9999999, 1.0, 0.5, 1.8, 1.3, 0.6, 1e-3]])
+    grid = 36.0 * 2.0 ** -25
+    for om in omegas.astype(np.float32):
+        om1, a0, a1, a2, omp = (float(v) for v in oracle.exact_rates(float(om)))
+        assert (a0 * 3, a1 * 18, a2 * 36) == (omp, omp, omp)
+        assert om1 + a0 + 6 * a1 + 12 * a2 == 1.0
+        assert om1 + 3 * (2 * a1 + 8 * a2) == 1.0
+        assert om1 == 1.0 - omp
+        q = omp / grid
+        assert q == round(q)
+        assert abs(omp - float(om)) <= grid / 2 * (1 + 1e-12)
+
+
+def test_16bit_bgk_collision_closed_form():
+    """A spatially uniform periodic lattice streams onto itself, so one step is one
+    collision per cell: in FP16 storage it equals the FP64 BGK update
+    f (1 - omega') + omega' f_eq(rhoBar, j) to one FP16 rounding of the FP32
+    evaluation (at most one ulp of the stored value). Palabos's omega is 5e-7 away
+    and inside that tolerance: the rates themselves are pinned above."""
+    omega = 1.7
+    case = Case(2, 2, 2, storage="f16", omega=omega, periodic=(True, True, True))
+    rng = np.random.RandomState(5)
+    for trial in range(20):
+        rho = 1.0 + 0.02 * (rng.rand() - 0.5)
+        u = 0.08 * (rng.rand(3) - 0.5)
+        f = oracle.equilibrium(np.array([rho]), u[None, :])[0] + 2e-3 * (rng.rand(19) - 0.5)
+        f16 = oracle.to_storage(np.broadcast_to(f, (2, 2, 2, 19)).copy(), "f16")
+        fin = oracle.from_storage(f16, "f16")[0, 0, 0]
+        out = oracle.from_storage(oracle.run(case, f16, 1)[0], "f16")
+        assert np.array_equal(out, np.broadcast_to(out[0, 0, 0], out.shape))
+        rb = fin.sum()
+        j = C.T @ fin
+        omp = float(oracle.exact_rates(omega)[4])
+        want = fin * (1 - omp) + omp * oracle.equilibrium(np.array([1 + rb]), (j / (1 + rb))[None, :])[0]
+        ulp = np.spacing(np.abs(want).astype(np.float16)).astype(np.float64)
+        assert np.all(np.abs(out[0, 0, 0] - want) <= ulp), trial
