@@ -35,7 +35,8 @@ STORAGES = ["double", "float", "f16s", "bf16s"]
 
 def units():
     """(object name, source, extra defines) of every translation unit."""
-    out = [("engine", "engine.hip", []), ("adjoint", "adjoint.hip", []), ("noalloc", "noalloc.hip", [])]
+    out = [("engine", "engine.hip", []), ("adjoint", "adjoint.hip", []), ("noalloc", "noalloc.hip", []),
+           ("bandwidth", "bandwidth.hip", [])]
     for s in STORAGES:
         out.append((f"ops_{s}", "storage_ops.hip", [f"-DPLBAMD_S={s}"]))
         for dyn in (0, 1):
@@ -98,6 +99,41 @@ def build_cpp_examples(verbose=False):
             print("built", out)
 
 
+def resource_usage(storage, dyn, match=None):
+    """The compiler's figures for the step kernels of one translation unit
+    (gfx950 device pass with -Rpass-analysis=kernel-resource-usage): per
+    kernel, VGPRs, SGPRs, spills, static LDS and the occupancy in waves per SIMD
+    that the register count allows. Returns [(demangled name, {field: value})]."""
+    import re
+    include = ["-I", os.path.join(ROOT, "include")]
+    cmd = ([HIPCC] + FLAGS + include + [f"-DPLBAMD_S={storage}", f"-DPLBAMD_DYN={dyn}", "--cuda-device-only",
+            "-Rpass-analysis=kernel-resource-usage", "-c", os.path.join(CSRC, "step.hip"), "-o", os.devnull])
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed: " + " ".join(cmd) + "\n" + res.stderr)
+    fields = {"VGPRs": "vgpr", "TotalSGPRs": "sgpr", "VGPRs Spill": "vgpr_spill",
+              "LDS Size [bytes/block]": "static_lds", "Occupancy [waves/SIMD]": "waves_per_simd"}
+    out = []
+    for block in res.stderr.split("remark: Function Name: ")[1:]:
+        name = block.split()[0]
+        demangled = subprocess.run(["c++filt", name], capture_output=True, text=True).stdout.strip() or name
+        if match and not re.search(match, demangled):
+            continue
+        vals = {}
+        for key, short in fields.items():
+            m = re.search(re.escape(key) + r": (\S+)", block)
+            vals[short] = m.group(1) if m else "?"
+        out.append((demangled, vals))
+    return out
+
+
 if __name__ == "__main__":
-    build_library(verbose=True)
-    build_cpp_examples(verbose=True)
+    import sys
+    if len(sys.argv) >= 4 and sys.argv[1] == "--resource-usage":
+        # python -m palabos_amd._build --resource-usage <storage> <dyn> [regex]
+        for name, v in resource_usage(sys.argv[2], sys.argv[3], sys.argv[4] if len(sys.argv) > 4 else None):
+            print(f"{name:110s} vgpr {v['vgpr']:>4s} sgpr {v['sgpr']:>4s} spill {v['vgpr_spill']:>3s} "
+                  f"static_lds {v['static_lds']:>5s} waves/SIMD {v['waves_per_simd']}")
+    else:
+        build_library(verbose=True)
+        build_cpp_examples(verbose=True)

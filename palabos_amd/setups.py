@@ -98,11 +98,21 @@ def spheres_mask(shape, centres, radii, scale=1.0, periodic=(True, True, True)):
     return mask
 
 
+def tile_blocks(shape, cuts):
+    """Tiles a domain with boxes: cuts[a] lists the interior cut positions along
+    axis a (a block ends at cut - 1). Returns inclusive (x0,x1,y0,y1,z0,z1) boxes."""
+    edges = []
+    for a in range(3):
+        e = [0] + sorted(cuts[a]) + [shape[a]]
+        edges.append([(e[k], e[k + 1] - 1) for k in range(len(e) - 1)])
+    return [(x0, x1, y0, y1, z0, z1) for (x0, x1) in edges[0] for (y0, y1) in edges[1] for (z0, z1) in edges[2]]
+
+
 def uneven_blocks(shape, counts, lo, hi, seed):
     """Tiling of the domain by counts[0] x counts[1] x counts[2] blocks whose
     edges are drawn in [lo, hi] (seeded rejection sampling, summing to the extent)."""
     rng = np.random.RandomState(seed)
-    edges = []
+    cuts = []
     for a in range(3):
         n, k = shape[a], counts[a]
         if not (k * lo <= n <= k * hi):
@@ -113,6 +123,5 @@ def uneven_blocks(shape, counts, lo, hi, seed):
             if lo <= last <= hi:
                 lengths.append(last)
                 break
-        b = np.concatenate([[0], np.cumsum(lengths)])
-        edges.append([(int(b[i]), int(b[i + 1]) - 1) for i in range(k)])
-    return [(x0, x1, y0, y1, z0, z1) for (x0, x1) in edges[0] for (y0, y1) in edges[1] for (z0, z1) in edges[2]]
+        cuts.append([int(c) for c in np.cumsum(lengths)[:-1]])
+    return tile_blocks(shape, cuts)

@@ -10,6 +10,10 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
+from palabos_amd.setups import poiseuille_profile  # noqa: F401  (re-exported for the tests)
+from palabos_amd.setups import tile_blocks as uneven_blocks  # noqa: F401
+from palabos_amd.setups import uneven_blocks as seeded_uneven_blocks  # noqa: F401
+
 TAG_BULK, TAG_BB, TAG_NODYN, TAG_MEBB, TAG_WALL = 0, 1, 2, 3, 4
 
 
@@ -102,17 +106,6 @@ def sphere_tags(shape, center, radius, tag):
     tags = np.zeros(shape, dtype=np.uint8)
     tags[r2 <= radius * radius] = tag
     return tags
-
-
-def poiseuille_profile(ny, nz, u_max):
-    """u_x on the inlet interior (1..ny-2, 1..nz-2): 16 u_max y'(H-y') z'(H-z') / (Hy^2 Hz^2),
-    y', z' distances to the half-way walls at -0.5 and n-0.5."""
-    y = np.arange(1, ny - 1, dtype=np.float64) + 0.5
-    z = np.arange(1, nz - 1, dtype=np.float64) + 0.5
-    hy, hz = float(ny), float(nz)
-    py = y * (hy - y) / (hy * hy)
-    pz = z * (hz - z) / (hz * hz)
-    return 16.0 * u_max * py[:, None] * pz[None, :]
 
 
 def channel_case(nx, ny, nz, omega, u_max, storage="f64", family="zouhe", sphere=None,
@@ -282,32 +275,3 @@ def gpu_run(case, f_storage, nsteps, stats=True, initialize=False, wrapped=False
         return out, st
     finally:
         lat.close()
-
-
-def uneven_blocks(shape, cuts):
-    """Tiles a domain with boxes: cuts[a] lists the interior cut positions along
-    axis a (a block ends at cut - 1). Returns inclusive (x0,x1,y0,y1,z0,z1) boxes."""
-    edges = []
-    for a in range(3):
-        e = [0] + sorted(cuts[a]) + [shape[a]]
-        edges.append([(e[k], e[k + 1] - 1) for k in range(len(e) - 1)])
-    return [(x0, x1, y0, y1, z0, z1) for (x0, x1) in edges[0] for (y0, y1) in edges[1] for (z0, z1) in edges[2]]
-
-
-def seeded_uneven_blocks(shape, counts, lo, hi, seed):
-    """`counts[a]` pieces per axis with lengths in [lo, hi] summing to the extent,
-    by rejection sampling from a seeded generator; returns the tiling boxes."""
-    rng = np.random.RandomState(seed)
-    cuts = []
-    for a in range(3):
-        n, k = shape[a], counts[a]
-        if not (k * lo <= n <= k * hi):
-            raise ValueError(f"axis {a}: {k} pieces in [{lo}, {hi}] cannot sum to {n}")
-        while True:
-            lengths = list(rng.randint(lo, hi + 1, size=k - 1))
-            last = n - sum(lengths)
-            if lo <= last <= hi:
-                lengths.append(last)
-                break
-        cuts.append([int(c) for c in np.cumsum(lengths)[:-1]])
-    return uneven_blocks(shape, cuts)
