@@ -18,9 +18,13 @@ roofline     the collide-and-stream launches: algorithmic bytes per launch
              launch per step (even and odd steps alternate two kernels).
              "traffic" is the HBM bytes per launch from rocprofv3 counters
              (profiles/bulk_step_tg512_counters.txt).
-cpu_baseline the compiled Palabos reference (oracle/_ref/ref_driver, serial,
-             1 core) on a bounded sample (periodic BGK 128^3), or the oracle
-             restatement when the reference binary is absent.
+cpu_baseline the compiled Palabos reference (oracle/_ref/ref_driver, serial) on a
+             bounded sample (periodic BGK FP64 128^3): "value" is the aggregate
+             of one independent serial process per host core of this job's
+             share (at most 16), each on its own lattice; "single_core" is one
+             process alone; the host's CPU count and model are stated. The
+             oracle restatement (1 core) stands in when the reference binary
+             is absent.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
                        [--precision f64] [--no-variants] [--no-cpu] [--no-adjoint]
@@ -44,7 +48,8 @@ BYTES_PER_CELL = {"f64": 19 * 2 * 8, "f32": 19 * 2 * 4, "f16": 19 * 2 * 2, "bf16
 # the roofline figure is their average launch duration).
 KERNELS = {"f64": "plbamd::bulk_step (even) / plbamd::bulk_odd_staged (odd)",
            "f32": "plbamd::bulk_step (even) / plbamd::bulk_odd_staged (odd)",
-           "f16": "plbamd::bulk_pairs (even and odd)", "bf16": "plbamd::bulk_pairs (even and odd)"}
+           "f16": "plbamd::bulk_even_linear (even) / plbamd::bulk_pairs (odd)",
+           "bf16": "plbamd::bulk_even_linear (even) / plbamd::bulk_pairs (odd)"}
 
 
 def taylor_green_init(lat, n, u0):
@@ -126,16 +131,44 @@ def adjoint_ratio(repeats=3):
             "objective": J}
 
 
+def host_cpu():
+    """CPU count of this job (affinity) and of the machine, and the CPU model."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": usable, "machine_cpus": os.cpu_count(), "cpu_model": model}
+
+
 def cpu_baseline():
     ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
-    n, steps = 128, 200   # about 10-15 s of serial CPU work
+    n, steps = 128, 200   # about 10-15 s of serial CPU work per process
+    host = host_cpu()
     if os.path.exists(ref):
-        out = subprocess.run([ref, "--bench", "double", str(n), str(steps)], capture_output=True,
-                             text=True, timeout=600, check=True).stdout
-        r = json.loads(out.strip().splitlines()[-1])
-        return {"value": r["mlups"], "unit": "MLUPS", "cores": 1, "kind": "reference",
-                "sample": f"compiled Palabos (serial MultiBlockLattice3D, BGK, FP64) periodic "
-                          f"{n}^3, {steps} steps after 1 warm-up step, {r['seconds']:.1f} s"}
+        cmd = [ref, "--bench", "double", str(n), str(steps)]
+        single = json.loads(subprocess.run(cmd, capture_output=True, text=True, timeout=600,
+                                           check=True).stdout.strip().splitlines()[-1])
+        # all cores of this job's share: independent serial processes, each its own lattice
+        procs_n = max(1, min(16, host["nproc"]))
+        procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+                 for _ in range(procs_n)]
+        each = [json.loads(p.communicate(timeout=900)[0].strip().splitlines()[-1]) for p in procs]
+        total = sum(r["mlups"] for r in each)
+        return {"value": total, "unit": "MLUPS", "cores": procs_n, "kind": "reference",
+                "sample": f"compiled Palabos (serial MultiBlockLattice3D, BGK, FP64) periodic {n}^3, {steps} steps "
+                          f"after 1 warm-up step: {procs_n} independent processes at once, one per core "
+                          f"(slowest {max(r['seconds'] for r in each):.1f} s)",
+                "per_process_mlups": [r["mlups"] for r in each],
+                "single_core": {"value": single["mlups"], "cores": 1, "seconds": single["seconds"]},
+                "host": host}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from cases import Case
     from oracle import oracle
@@ -146,7 +179,7 @@ def cpu_baseline():
     oracle.run(case, f, steps)
     sec = time.perf_counter() - t0
     return {"value": n ** 3 * steps / sec / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
-            "sample": f"oracle restatement, periodic BGK FP64 {n}^3, {steps} steps, {sec:.1f} s"}
+            "sample": f"oracle restatement, periodic BGK FP64 {n}^3, {steps} steps, {sec:.1f} s", "host": host}
 
 
 def traffic_from_profile(precision, n):
@@ -163,16 +196,41 @@ def traffic_from_profile(precision, n):
 
 
 def copy_calibration(n_doubles=1 << 29, repeats=3):
-    """A plain device-to-device copy of known size (torch's copy kernel): the
-    reference that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    """A plain device-to-device copy of known size (torch's copy kernel, 4 GiB
+    read + 4 GiB written): the reference that scales rocprofv3's FETCH_SIZE /
+    WRITE_SIZE counters to bytes, and the practical streaming ceiling (its
+    read + write bytes over its duration, from HIP events on torch's stream,
+    where the copy runs)."""
     import torch
     src = torch.ones(n_doubles, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
-    for _ in range(repeats):
-        dst.copy_(src)
+    dst.copy_(src)   # untimed first touch
     torch.cuda.synchronize()
+    ms = []
+    for _ in range(repeats):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    best = min(ms)
+    del src, dst
+    torch.cuda.empty_cache()
+    # the library's probes: a 16-byte nontemporal copy of the same size, and the
+    # even step's data movement over the 512^3 FP64 lattice's bytes (19 slots in
+    # place) in 8-, 4- and 16-byte elements per thread
+    from palabos_amd import native
+    probes = {}
+    for pattern, nbytes in (("copy", 2 * n_doubles * 8), ("slots", 304 * 512 ** 3), ("slots4", 304 * 512 ** 3),
+                            ("slots16", 304 * 512 ** 3)):
+        p_ms, moved, gbs = native.bandwidth_probe(pattern, nbytes, repeats)
+        probes[pattern] = {"bytes_per_launch": moved, "ms_best": p_ms, "gbs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS}
     print(json.dumps({"copy_calibration_bytes_read": n_doubles * 8, "copy_calibration_bytes_written": n_doubles * 8,
-                      "launches": repeats}))
+                      "launches": repeats + 1, "event_ms": ms,
+                      "copy_gbs_best": 2 * n_doubles * 8 / (best * 1e-3) / 1e9,
+                      "copy_frac_of_peak": 2 * n_doubles * 8 / (best * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "probes": probes}))
 
 
 def main():

@@ -335,6 +335,23 @@ int plbamd_adjoint_gradient(int handle, int nsteps, int checkpoint_every, const 
 /* Number of completed steps. */
 int plbamd_get_time(int handle, int64_t* t);
 
+/* Streaming-bandwidth probe: the rate this GPU sustains for the memory access
+ * patterns of the collide-and-stream kernels, with their nontemporal loads and
+ * stores and no arithmetic; the practical ceiling their HBM fractions are read
+ * against. PLBAMD_PROBE_COPY copies bytes/2 into a second buffer in 16-byte
+ * elements; PLBAMD_PROBE_SLOTS reads and writes back in place 19 slot arrays
+ * of bytes/38 each (an even step's data movement) in 8-byte elements, one per
+ * thread (one FP64 cell); _SLOTS_4B and _SLOTS_16B in 4- and 16-byte ones
+ * (one FP32 cell, two FP64 cells per thread). Allocates and
+ * frees its buffers; one untimed launch, then `repeats` timed ones (HIP events
+ * on its own stream). Out: the shortest launch in ms and the bytes each launch
+ * moves (read + written). No Palabos counterpart: a measurement helper. */
+#define PLBAMD_PROBE_COPY 0
+#define PLBAMD_PROBE_SLOTS 1
+#define PLBAMD_PROBE_SLOTS_4B 2
+#define PLBAMD_PROBE_SLOTS_16B 3
+int plbamd_bandwidth_probe(int pattern, uint64_t bytes, int repeats, double* ms_best, double* bytes_moved);
+
 #ifdef __cplusplus
 }
 #endif

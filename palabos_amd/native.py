@@ -36,7 +36,7 @@ EXPORTS = (
     "plbamd_adjoint_gradient", "plbamd_get_time", "plbamd_save_binary_block", "plbamd_load_binary_block",
     "plbamd_compute_fields", "plbamd_set_solid_density", "plbamd_reduce_moments_noalloc",
     "plbamd_add_velocity_edge", "plbamd_add_velocity_corner", "plbamd_set_boundary_type",
-    "plbamd_execute_processors",
+    "plbamd_execute_processors", "plbamd_bandwidth_probe",
 )
 
 
@@ -124,6 +124,8 @@ def lib():
         "plbamd_adjoint_gradient": [i32, i32, i32, ctypes.POINTER(Objective), vp, vp, vp, vp],
         "plbamd_save_binary_block": [i32, ctypes.c_char_p, i32, i32, i32],
         "plbamd_load_binary_block": [i32, ctypes.c_char_p, i32, i32, i32],
+        "plbamd_bandwidth_probe": [i32, ctypes.c_uint64, i32, ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_double)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -143,6 +145,21 @@ def check(rc):
 
 
 _initialised = False
+
+
+PROBE_PATTERNS = {"copy": 0, "slots": 1, "slots4": 2, "slots16": 3}
+
+
+def bandwidth_probe(pattern, nbytes, repeats=3):
+    """Streaming rate of the device for `pattern` ('copy': 16-byte copy into a
+    second buffer; 'slots', 'slots4', 'slots16': 19 slots read and written back
+    in place in 8-, 4- or 16-byte elements per thread) over `nbytes` moved per
+    launch: (shortest launch ms, bytes moved, GB/s)."""
+    init()
+    ms, moved = ctypes.c_double(), ctypes.c_double()
+    check(lib().plbamd_bandwidth_probe(PROBE_PATTERNS[pattern], int(nbytes), int(repeats), ctypes.byref(ms),
+                                       ctypes.byref(moved)))
+    return ms.value, moved.value, moved.value / (ms.value * 1e-3) / 1e9
 
 
 def init(device=0):

@@ -56,3 +56,16 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "one GPU only" in r.stderr
     assert r.stdout == ""
+
+
+def test_bandwidth_probe_rejects_bad_arguments_before_the_device():
+    # Argument checks come before any device call, so this runs without a GPU.
+    import ctypes
+    from palabos_amd import native
+    lib = native.lib()
+    ms, moved = ctypes.c_double(), ctypes.c_double()
+    assert lib.plbamd_bandwidth_probe(9, 1 << 20, 1, ctypes.byref(ms), ctypes.byref(moved)) == native.EINVAL
+    assert b"unknown pattern" in lib.plbamd_last_error()
+    assert lib.plbamd_bandwidth_probe(0, 1 << 20, 0, ctypes.byref(ms), ctypes.byref(moved)) == native.EINVAL
+    assert lib.plbamd_bandwidth_probe(1, 100, 1, ctypes.byref(ms), ctypes.byref(moved)) == native.EINVAL
+    assert b"size out of range" in lib.plbamd_last_error()
