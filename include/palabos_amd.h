@@ -122,14 +122,31 @@ int plbamd_lattice_create(const plbamd_lattice_desc* desc, int* handle);
  * without overlap). Replaces the MultiBlockLattice3D(MultiBlockManagement3D,
  * ...) constructor over a SparseBlockStructure3D built with addBlock
  * (multiBlock/multiBlockLattice3D.h:75-84, sparseBlockStructure3D.h:55-57).
- * Every block is swept by the same single launch per step; blocks exchange a
- * two-cell envelope (Palabos: one cell, every step) after every second step,
- * which keeps the bulk bit-identical to a single-block lattice. All other entry
- * points take global coordinates; populations can be uploaded (and
- * initialised) only after an even number of steps (PLBAMD_ESTATE otherwise).
- * At most 65535 blocks. */
+ * On one device the tiling needs no envelopes: the blocks are merged into the
+ * fewest blocks below 2^32 cells each -- one block (a single-block lattice,
+ * bit-identical results) whenever the domain has fewer cells; above that,
+ * face-adjacent blocks with equal extents merge, z-neighbours first. All entry
+ * points take global coordinates. At most 65535 blocks. */
 int plbamd_multiblock_create(const plbamd_lattice_desc* desc, const plbamd_box* blocks, int nblocks,
                              int* handle);
+
+/* plbamd_multiblock_create with flags. PLBAMD_KEEP_BLOCKS keeps the given
+ * blocks as they are: every block is swept by the same single launch per step;
+ * blocks exchange a two-cell envelope (Palabos: one cell, every step) after
+ * every second step, which keeps the bulk bit-identical to a single-block
+ * lattice; populations can be uploaded (and initialised) only after an even
+ * number of steps (PLBAMD_ESTATE otherwise). This is the path lattices above
+ * 2^32 cells per block take; the flag selects it for any lattice. No Palabos
+ * counterpart (MultiBlockLattice3D keeps its blocks: one per MPI rank). */
+#define PLBAMD_KEEP_BLOCKS 1
+int plbamd_multiblock_create_ex(const plbamd_lattice_desc* desc, const plbamd_box* blocks, int nblocks, int flags,
+                                int* handle);
+
+/* Number of blocks the device runs the lattice as (1 for a single-block
+ * lattice and for a coalesced tiling). Replaces
+ * SparseBlockStructure3D::getNumBlocks (multiBlock/sparseBlockStructure3D.h:71)
+ * for the blocks that exist on the device. */
+int plbamd_get_block_count(int handle, int* count);
 
 /* Releases a lattice. Replaces BlockLattice3D's destructor (blockLattice3D.hh:120-130). */
 int plbamd_lattice_destroy(int handle);

@@ -779,8 +779,52 @@ int plbamd_lattice_create(const plbamd_lattice_desc* desc, int* handle) {
     return registerLattice(std::move(L), handle);
 }
 
+// Blocks of a tiling that run as one: on one device a dense tiling needs no
+// envelopes, so blocks are merged while every merged block stays below 2^32
+// cells (the per-block index limit) -- one block whenever the domain fits. Above
+// that, z-adjacent blocks with the same x and y extents merge first (their
+// envelopes are the costly z-face ones), then y, then x.
+// PLBAMD_MAX_BLOCK_CELLS lowers the limit (at least 64 cells), so that the
+// merging of tilings above it can be exercised on small lattices.
+static std::vector<plbamd_box> coalesceBlocks(const plbamd_lattice_desc& d, std::vector<plbamd_box> b) {
+    uint64_t limit = 1ull << 32;
+    if (const char* e = getenv("PLBAMD_MAX_BLOCK_CELLS"))
+        limit = std::max<uint64_t>(64, std::min<uint64_t>(limit, strtoull(e, nullptr, 10)));
+    if ((uint64_t)d.nx * d.ny * d.nz < limit) return {plbamd_box{0, d.nx - 1, 0, d.ny - 1, 0, d.nz - 1}};
+    if (b.size() > 4096) return b;   // pairwise search: large block lists stay as given
+    auto lo = [](const plbamd_box& x, int a) { return a == 0 ? x.x0 : (a == 1 ? x.y0 : x.z0); };
+    auto hi = [](const plbamd_box& x, int a) { return a == 0 ? x.x1 : (a == 1 ? x.y1 : x.z1); };
+    auto cells = [&](const plbamd_box& x) {
+        return (uint64_t)(x.x1 - x.x0 + 1) * (uint64_t)(x.y1 - x.y0 + 1) * (uint64_t)(x.z1 - x.z0 + 1);
+    };
+    for (bool merged = true; merged;) {
+        merged = false;
+        for (int axis = 2; axis >= 0 && !merged; --axis)
+            for (size_t i = 0; i < b.size() && !merged; ++i)
+                for (size_t j = 0; j < b.size() && !merged; ++j) {
+                    if (i == j || hi(b[i], axis) + 1 != lo(b[j], axis)) continue;
+                    bool same = true;
+                    for (int a = 0; a < 3; ++a)
+                        if (a != axis && (lo(b[i], a) != lo(b[j], a) || hi(b[i], a) != hi(b[j], a))) same = false;
+                    if (!same || cells(b[i]) + cells(b[j]) >= limit) continue;
+                    if (axis == 0) b[i].x1 = b[j].x1;
+                    else if (axis == 1) b[i].y1 = b[j].y1;
+                    else b[i].z1 = b[j].z1;
+                    b.erase(b.begin() + (std::ptrdiff_t)j);
+                    merged = true;
+                }
+    }
+    return b;
+}
+
 int plbamd_multiblock_create(const plbamd_lattice_desc* desc, const plbamd_box* blocks, int nblocks, int* handle) {
+    return plbamd_multiblock_create_ex(desc, blocks, nblocks, 0, handle);
+}
+
+int plbamd_multiblock_create_ex(const plbamd_lattice_desc* desc, const plbamd_box* blocks, int nblocks, int flags,
+                                int* handle) {
     if (!desc || !handle || !blocks || nblocks <= 0) return fail(PLBAMD_EINVAL, "multiblock_create: bad argument");
+    if (flags & ~PLBAMD_KEEP_BLOCKS) return fail(PLBAMD_EINVAL, "multiblock_create: unknown flags");
     PLBAMD_TRY(validateDesc(desc));
     const uint64_t cells = (uint64_t)desc->nx * desc->ny * desc->nz;
     // the blocks must tile the domain: inside, no overlap, every cell covered
@@ -800,13 +844,23 @@ int plbamd_multiblock_create(const plbamd_lattice_desc* desc, const plbamd_box* 
     }
     for (uint64_t k = 0; k < cells; ++k)
         if (!cover[k]) return fail(PLBAMD_EINVAL, "multiblock_create: the blocks do not cover the domain");
+    std::vector<plbamd_box> run(blocks, blocks + nblocks);
+    if (!(flags & PLBAMD_KEEP_BLOCKS)) run = coalesceBlocks(*desc, std::move(run));
     auto L = std::make_unique<Lattice>();
     L->desc = *desc;
-    L->multi = true;
+    L->multi = run.size() > 1;
     L->cells = cells;
-    L->bulk.assign(blocks, blocks + nblocks);
+    L->bulk = std::move(run);
     PLBAMD_TRY(setup(*L));
     return registerLattice(std::move(L), handle);
+}
+
+int plbamd_get_block_count(int handle, int* count) {
+    if (!count) return fail(PLBAMD_EINVAL, "get_block_count: null output");
+    Lattice* L = find(handle);
+    if (!L) return fail(PLBAMD_EINVAL, "get_block_count: unknown handle");
+    *count = (int)L->bulk.size();
+    return PLBAMD_OK;
 }
 
 int plbamd_lattice_destroy(int handle) {

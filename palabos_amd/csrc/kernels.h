@@ -324,8 +324,12 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
             uint32_t wall = 0;
             if constexpr (TAGS) {
                 tag = a.tags[g.cellOffset + cell];
-                if (tag == BT_SKIP) continue;
-                if (tag == BT_WALL_NEAR) wall = a.walls[g.cellOffset + cell];
+                // the link mask is zero except on wall-near cells: loaded without
+                // testing the tag. Odd steps leave skipped cells here (their lanes
+                // take no part in the ballot below); even steps test the tag after
+                // their population loads are issued.
+                if (a.walls) wall = a.walls[g.cellOffset + cell];
+                if (ODD && tag == BT_SKIP) continue;
             }
             int zm = 0, zp = 0;
             const bool vzm = shift(z, -1, g.nz, g.pz, zm), vzp = shift(z, 1, g.nz, g.pz, zp);
@@ -341,6 +345,9 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
             if constexpr (!ODD) {
 #pragma unroll
                 for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(at(rows[1][1] + (uint64_t)i * g.slot, zb[1]));
+                if constexpr (TAGS) {
+                    if (tag == BT_SKIP) continue;
+                }
             } else if (fast) {
 #pragma unroll
                 for (int i = 0; i < 19; ++i)
@@ -445,17 +452,15 @@ __global__ void __launch_bounds__(512) bulk_even_linear(StepArgs a) {
             if (comp[h]) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, inBulk, fc[h], st, wall, wf);
         }
         if constexpr (CPT == 2) {
-            if (comp[0] && comp[1]) {
+            // whole pair words only: a cell not collided here gets back the value of
+            // slot opp(i) it was loaded with (its own thread is the only writer of the
+            // pair in an even step; the special-cell kernel runs after this one)
+            if (comp[0] || comp[1]) {
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
-                    const uint32_t v = (uint32_t)Store<S>::to_raw(fc[0][i]) | ((uint32_t)Store<S>::to_raw(fc[1][i]) << 16);
-                    __builtin_nontemporal_store(v, reinterpret_cast<uint32_t*>(f + (uint64_t)opp(i) * g.slot + c0));
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < 19; ++i) {
-                    if (comp[0]) Store<S>::store(f + (uint64_t)opp(i) * g.slot + c0, fc[0][i]);
-                    if (comp[1]) Store<S>::store(f + (uint64_t)opp(i) * g.slot + c0 + 1, fc[1][i]);
+                    const uint32_t lo = Store<S>::to_raw(comp[0] ? fc[0][i] : fc[0][opp(i)]);
+                    const uint32_t hi = Store<S>::to_raw(comp[1] ? fc[1][i] : fc[1][opp(i)]);
+                    __builtin_nontemporal_store(lo | (hi << 16), reinterpret_cast<uint32_t*>(f + (uint64_t)opp(i) * g.slot + c0));
                 }
             }
         } else if (comp[0]) {
@@ -572,8 +577,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     uint8_t tag = BT_BULK;
     uint32_t wall = 0;
     if constexpr (TAGS) {
+        // the link mask is zero except on wall-near cells: loaded without testing
+        // the tag, so that no load waits for another before the population loads
         if (active) tag = a.tags[g.cellOffset + cell];
-        if (active && tag == BT_WALL_NEAR) wall = a.walls[g.cellOffset + cell];
+        if (active && a.walls) wall = a.walls[g.cellOffset + cell];
     }
     const bool compute = active && tag != BT_SKIP;
     T fc[19];
@@ -604,12 +611,16 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
             const S* src = rows[1 - cx(i)][1 - cy(i)] + (uint64_t)opp(i) * g.slot;
-            if (cz(i) == 0) {
-                // a link wall at x - c_i: f_i comes back from the cell's own slot i
-                const S* p = ((wall >> opp(i)) & 1u) ? f + (uint64_t)i * g.slot + cell : src + zc;
-                fc[i] = Store<S>::load(p);
-            } else {
-                raw[zmove_index(i)] = Store<S>::load_raw(src + zc);
+            if (cz(i) == 0) fc[i] = Store<S>::load(src + zc);
+            else raw[zmove_index(i)] = Store<S>::load_raw(src + zc);
+        }
+        if constexpr (TAGS) {
+            // a link wall at x - c_i: f_i comes back from the cell's own slot i
+            // (after every unconditional load is issued)
+            if (wall) {
+#pragma unroll
+                for (int i = 0; i < 19; ++i)
+                    if (cz(i) == 0 && ((wall >> opp(i)) & 1u)) fc[i] = Store<S>::load(f + (uint64_t)i * g.slot + cell);
             }
         }
         // Halo elements inside a periodic or segmented row: the source elements
@@ -794,10 +805,18 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
     uint32_t wallA = 0, wallB = 0;
     if constexpr (TAGS) {
         if (active) {
-            tagA = a.tags[g.cellOffset + cellA];
-            tagB = a.tags[g.cellOffset + cellB];
-            if (tagA == BT_WALL_NEAR) wallA = a.walls[g.cellOffset + cellA];
-            if (tagB == BT_WALL_NEAR) wallB = a.walls[g.cellOffset + cellB];
+            // one 16-bit tag load and one 64-bit link-mask load for the pair (cellA is
+            // even in every block, and so is every block's cell offset). The mask is
+            // zero except on wall-near cells, so it is loaded without testing the tag:
+            // no load waits for another before the population loads are issued.
+            const uint16_t tg = *reinterpret_cast<const uint16_t*>(a.tags + g.cellOffset + cellA);
+            tagA = (uint8_t)(tg & 0xffu);
+            tagB = (uint8_t)(tg >> 8);
+            if (a.walls) {
+                const uint64_t w = *reinterpret_cast<const uint64_t*>(a.walls + g.cellOffset + cellA);
+                wallA = (uint32_t)w;
+                wallB = (uint32_t)(w >> 32);
+            }
         }
     }
     const bool compA = active && tagA != BT_SKIP, compB = active && tagB != BT_SKIP;
@@ -816,16 +835,13 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
         }
         if (compA) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
         if (compB) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
-        if (compA && compB) {
+        // whole pair words: a cell not collided here gets back the value of slot
+        // opp(i) it was loaded with (only this thread writes the pair in an even step)
+        if (compA || compB) {
 #pragma unroll
             for (int i = 0; i < 19; ++i)
-                st_pair(own + (uint64_t)opp(i) * g.slot, pack16(Store<S>::to_raw(fa[i]), Store<S>::to_raw(fb[i])));
-        } else if (compA || compB) {
-#pragma unroll
-            for (int i = 0; i < 19; ++i) {
-                if (compA) Store<S>::store(own + (uint64_t)opp(i) * g.slot, fa[i]);
-                if (compB) Store<S>::store(own + (uint64_t)opp(i) * g.slot + 1, fb[i]);
-            }
+                st_pair(own + (uint64_t)opp(i) * g.slot, pack16(Store<S>::to_raw(compA ? fa[i] : fa[opp(i)]),
+                                                                Store<S>::to_raw(compB ? fb[i] : fb[opp(i)])));
         }
     } else {
         int xm = 0, xp = 0, ym = 0, yp = 0;
@@ -872,11 +888,20 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                         const uint32_t v = ld_pair(src + zc);
                         fa[i] = Store<S>::from_raw(lo16(v));
                         fb[i] = Store<S>::from_raw(hi16(v));
-                        // link walls at x - c_i: f_i comes back from the cell's own slot i
-                        if ((wallA >> opp(i)) & 1u) fa[i] = Store<S>::load(f + (uint64_t)i * g.slot + cellA);
-                        if ((wallB >> opp(i)) & 1u) fb[i] = Store<S>::load(f + (uint64_t)i * g.slot + cellB);
                     } else {
                         raw[zmove_index(i)] = ld_pair(src + zc);
+                    }
+                }
+            }
+            if constexpr (TAGS) {
+                // link walls at x - c_i: f_i comes back from the cell's own slot i
+                // (after every unconditional load is issued)
+                if (sAct && (wallA | wallB)) {
+#pragma unroll
+                    for (int i = 0; i < 19; ++i) {
+                        if (cz(i) != 0) continue;
+                        if ((wallA >> opp(i)) & 1u) fa[i] = Store<S>::load(f + (uint64_t)i * g.slot + cellA);
+                        if ((wallB >> opp(i)) & 1u) fb[i] = Store<S>::load(f + (uint64_t)i * g.slot + cellB);
                     }
                 }
             }

@@ -80,10 +80,13 @@ def _box(b, shape):
 class Block3D:
     """A lattice: one block (a Palabos BlockLattice3D / single-block
     MultiBlockLattice3D) or, with `blocks`, a multi-block lattice tiled by uneven
-    inclusive boxes (MultiBlockLattice3D over a SparseBlockStructure3D)."""
+    inclusive boxes (MultiBlockLattice3D over a SparseBlockStructure3D). The
+    device runs a tiling as one block while it has fewer than 2^32 cells
+    (bit-identical results); keep_blocks=True keeps the given blocks, with
+    their envelope exchange."""
 
     def __init__(self, nx, ny, nz, descriptor=D3Q19, dynamics=None, storage="f64",
-                 periodic=False, force=None, device=0, blocks=None):
+                 periodic=False, force=None, device=0, blocks=None, keep_blocks=False):
         if descriptor != D3Q19:
             raise ValueError("only the D3Q19 descriptor is supported")
         if dynamics is None:
@@ -106,7 +109,8 @@ class Block3D:
             check(lib().plbamd_lattice_create(ctypes.byref(d), ctypes.byref(h)))
         else:
             arr = (Box * len(blocks))(*[Box.of(b) for b in blocks])
-            check(lib().plbamd_multiblock_create(ctypes.byref(d), arr, len(blocks), ctypes.byref(h)))
+            flags = native.KEEP_BLOCKS if keep_blocks else 0
+            check(lib().plbamd_multiblock_create_ex(ctypes.byref(d), arr, len(blocks), flags, ctypes.byref(h)))
         self.blocks = None if blocks is None else [tuple(b) for b in blocks]
         self.handle = h.value
         self.shape = (int(nx), int(ny), int(nz))
@@ -116,6 +120,12 @@ class Block3D:
         self._tags = None
         self._planes = []
         self._plane_values = {}   # velocity boundaries: the values last set from here, by id
+
+    def blockCount(self):
+        """Blocks the device runs this lattice as (1 for a coalesced tiling)."""
+        n = ctypes.c_int(0)
+        check(lib().plbamd_get_block_count(self.handle, ctypes.byref(n)))
+        return n.value
 
     # -- lifetime -----------------------------------------------------------
     def close(self):

@@ -18,6 +18,7 @@ BGK, RBGK = 0, 1
 TAG_BULK, TAG_BOUNCE_BACK, TAG_NO_DYNAMICS, TAG_MOMENTUM_EXCHANGE, TAG_WALL_LINK = 0, 1, 2, 3, 4
 ZOUHE, REGULARIZED = 0, 1
 WRAPPED = 0x100
+KEEP_BLOCKS = 1           # plbamd_multiblock_create_ex: keep the given blocks
 BC_TYPES = {"dirichlet": 0, "neumann": 1, "freeslip": 2, "density": 3, "outflow": 4, "normalOutflow": 5}
 VELOCITY, PRESSURE = 0, 1
 NUM_STATS = 10
@@ -36,7 +37,8 @@ EXPORTS = (
     "plbamd_adjoint_gradient", "plbamd_get_time", "plbamd_save_binary_block", "plbamd_load_binary_block",
     "plbamd_compute_fields", "plbamd_set_solid_density", "plbamd_reduce_moments_noalloc",
     "plbamd_add_velocity_edge", "plbamd_add_velocity_corner", "plbamd_set_boundary_type",
-    "plbamd_execute_processors", "plbamd_bandwidth_probe",
+    "plbamd_execute_processors", "plbamd_bandwidth_probe", "plbamd_multiblock_create_ex",
+    "plbamd_get_block_count",
 )
 
 
@@ -99,7 +101,9 @@ def lib():
         "plbamd_init": [i32],
         "plbamd_lattice_create": [ctypes.POINTER(LatticeDesc), ctypes.POINTER(ctypes.c_int)],
         "plbamd_multiblock_create": [ctypes.POINTER(LatticeDesc), pbox, i32, ctypes.POINTER(ctypes.c_int)],
+        "plbamd_multiblock_create_ex": [ctypes.POINTER(LatticeDesc), pbox, i32, i32, ctypes.POINTER(ctypes.c_int)],
         "plbamd_lattice_destroy": [i32],
+        "plbamd_get_block_count": [i32, ctypes.POINTER(ctypes.c_int)],
         "plbamd_set_tags": [i32, vp],
         "plbamd_add_plane": [i32, i32, i32, i32, i32, pbox, ctypes.POINTER(ctypes.c_int)],
         "plbamd_set_plane_values": [i32, i32, vp, sz],

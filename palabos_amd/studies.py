@@ -184,18 +184,19 @@ def porous_geometry(shape=(384, 256, 256), count=400, r_range=(6.0, 14.0), seed=
 
 
 def porous_permeability(shape=(384, 256, 256), count=400, steps=20000, storage="f16", omega=1.7, g=1e-6,
-                        scale=1.0, blocks=None, block_seed=11, check_every=500):
+                        scale=1.0, blocks=None, block_seed=11, check_every=500, keep_blocks=False):
     """Regularized BGK driven by a Guo body force g along x through spheres made
     of link-wise walls (half-way bounce-back links), periodic in x, walls on the
     y and z faces, 60 uneven blocks (5 x 4 x 3 pieces of 32-128 cells). Darcy
     permeability k = nu <u_x> / g with <u_x> the superficial velocity (mean over
-    all cells, solid cells counting zero)."""
+    all cells, solid cells counting zero). The device runs the tiling as one
+    block unless keep_blocks (then the 60 blocks exchange their envelopes)."""
     nu = (1.0 / omega - 0.5) / 3.0
     solid = porous_geometry(shape, count, scale=scale)
     if blocks is None:
         blocks = setups.uneven_blocks(shape, (5, 4, 3), 32, 128, block_seed)
     lat = _lat.Block3D(*shape, _lat.D3Q19, _lat.RegularizedBGK(omega), storage=storage,
-                       periodic=(True, False, False), force=(g, 0.0, 0.0), blocks=blocks)
+                       periodic=(True, False, False), force=(g, 0.0, 0.0), blocks=blocks, keep_blocks=keep_blocks)
     try:
         tags = np.zeros(shape, np.uint8)
         tags[solid] = _lat.native.TAG_WALL_LINK
@@ -211,7 +212,8 @@ def porous_permeability(shape=(384, 256, 256), count=400, steps=20000, storage="
             mom = lat.reduceMoments()
             history.append(float(mom["sum_u"][0]) / mom["cells"])
         ux = history[-1]
-        return {"lattice": list(shape), "storage": storage, "blocks": len(blocks), "steps": steps,
+        return {"lattice": list(shape), "storage": storage, "blocks": len(blocks), "device_blocks": lat.blockCount(),
+                "steps": steps,
                 "porosity": float(1.0 - solid.mean()), "omega": omega, "g": g, "radius_scale": scale,
                 "superficial_ux": ux, "permeability": nu * ux / g,
                 "ux_history": history, "mlups": float(np.prod(shape)) * steps / (ms * 1e-3) / 1e6}
@@ -255,7 +257,7 @@ def brinkman_spheres(shape, centres, radii, scale=1.0, width=1.0, periodic=(True
 
 def porous_adjoint(shape=(384, 256, 256), count=400, r_range=(6.0, 14.0), seed=5, steps=2000, checkpoint_every=50,
                    window=1000, storage="f16", omega=1.7, g=1e-5, width=1.0, blocks=None, block_seed=11, fd_eps=1e-2,
-                   check_fd=True):
+                   check_fd=True, keep_blocks=False):
     """Darcy permeability of the porous medium with the spheres as a smooth
     Brinkman porosity, and its gradient with respect to a global sphere-radius
     scale s by the adjoint: K = nu <j_x> / g averaged over the last `window` of
@@ -271,7 +273,8 @@ def porous_adjoint(shape=(384, 256, 256), count=400, r_range=(6.0, 14.0), seed=5
     def run(scale, want_gradient):
         alpha, dads = brinkman_spheres(shape, centres, radii, scale=scale, width=width)
         lat = _lat.Block3D(*shape, _lat.D3Q19, _lat.RegularizedBGK(omega), storage=storage,
-                           periodic=(True, False, False), force=(g, 0.0, 0.0), blocks=blocks)
+                           periodic=(True, False, False), force=(g, 0.0, 0.0), blocks=blocks,
+                           keep_blocks=keep_blocks)
         try:
             lat.setPorosity(alpha)
             J, ga, _, (ms_f, ms_b) = lat.adjointGradient(steps, checkpoint_every, 0, 0, 0.0, 0.0,
@@ -362,6 +365,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--storage", default=None)
     ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--keep-blocks", action="store_true", help="porous studies: keep the 60 blocks on the device")
     args = ap.parse_args()
     from . import native
     native.init(0)
@@ -370,13 +374,14 @@ def main():
     elif args.study == "sphere_drag":
         r = sphere_drag(steps=args.steps or 5000, storage=args.storage or "f64")
     elif args.study == "porous":
-        r = porous_permeability(steps=args.steps or 20000, storage=args.storage or "f16", scale=args.scale)
+        r = porous_permeability(steps=args.steps or 20000, storage=args.storage or "f16", scale=args.scale,
+                                keep_blocks=args.keep_blocks)
     elif args.study == "fields":
         r = fields_throughput(n=args.n if args.n != 512 else 256, storage=args.storage or "f32")
     elif args.study == "checkpoint":
         r = checkpoint_throughput(n=args.n if args.n != 512 else 256, storage=args.storage or "f32")
     else:
-        r = porous_adjoint(steps=args.steps or 2000, storage=args.storage or "f16")
+        r = porous_adjoint(steps=args.steps or 2000, storage=args.storage or "f16", keep_blocks=args.keep_blocks)
     print(json.dumps(r))
 
 

@@ -160,7 +160,8 @@ def _forced_porous(dynamics, storage="f64", shape=(16, 10, 12)):
 def _lattice(case, blocks=None):
     dyn = palabos_amd.BGK(case.omega) if case.dynamics == "bgk" else palabos_amd.RegularizedBGK(case.omega)
     lat = palabos_amd.Block3D(case.nx, case.ny, case.nz, palabos_amd.D3Q19, dyn, storage=case.storage,
-                              periodic=case.periodic, force=case.force, blocks=blocks)
+                              periodic=case.periodic, force=case.force, blocks=blocks,
+                              keep_blocks=blocks is not None)
     lat.setPorosity(case.alpha)
     return lat
 

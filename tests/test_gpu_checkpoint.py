@@ -28,7 +28,7 @@ def load(path):
 
 def lattice(shape, storage, forced, blocks=None, periodic=False):
     return plb.Block3D(*shape, plb.D3Q19, plb.BGK(1.7), storage=storage, periodic=periodic,
-                       force=FORCE if forced else None, blocks=blocks)
+                       force=FORCE if forced else None, blocks=blocks, keep_blocks=blocks is not None)
 
 
 def tiling(shape):
@@ -95,7 +95,7 @@ def test_restart_continues_the_run(gpu, tmp_path):
     rng = np.random.RandomState(8)
     f0 = rng.standard_normal(shape + (19,)) * 0.01
     a = plb.Block3D(*shape, plb.D3Q19, plb.RegularizedBGK(1.6), storage="f64", periodic=(True, True, True),
-                    blocks=uneven_blocks(shape, [[5], [4], [6]]))
+                    blocks=uneven_blocks(shape, [[5], [4], [6]]), keep_blocks=True)
     b = plb.Block3D(*shape, plb.D3Q19, plb.RegularizedBGK(1.6), storage="f64", periodic=(True, True, True))
     try:
         a.setPopulations(f0)

@@ -164,7 +164,7 @@ def test_refusals(gpu):
     and the adjoint of a lattice with edges are refused with a message."""
     shape = (8, 9, 10)
     blocks = [(0, 3, 0, 8, 0, 9), (4, 7, 0, 8, 0, 9)]
-    lat = plb.Block3D(*shape, plb.D3Q19, plb.BGK(1.6), blocks=blocks)
+    lat = plb.Block3D(*shape, plb.D3Q19, plb.BGK(1.6), blocks=blocks, keep_blocks=True)
     try:
         lat.addExternalVelocityEdge(0, -1, -1, (1, 6, 0, 0, 0, 0))
         with pytest.raises(plb.PlbError, match="single-block"):

@@ -17,11 +17,11 @@ pytestmark = pytest.mark.gpu
 GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def _lattice_with_blocks(case, blocks):
+def _lattice_with_blocks(case, blocks, keep=True):
     dyn = palabos_amd.BGK(case.omega) if case.dynamics == "bgk" else palabos_amd.RegularizedBGK(case.omega)
     force = case.force if any(case.force) else None
     lat = palabos_amd.Block3D(case.nx, case.ny, case.nz, palabos_amd.D3Q19, dyn, storage=case.storage,
-                              periodic=case.periodic, force=force, blocks=blocks)
+                              periodic=case.periodic, force=force, blocks=blocks, keep_blocks=keep)
     if case.tags is not None:
         lat.setTags(case.tags)
     for p in case.planes:
@@ -40,10 +40,11 @@ def _dtype(storage):
     return {"f64": np.float64, "f32": np.float32}.get(storage, np.uint16)
 
 
-def _check(case, blocks, steps_list, seed=3):
+def _check(case, blocks, steps_list, seed=3, keep=True, expect_blocks=None):
     f0 = oracle.to_storage(random_state(case, seed), case.storage)
-    lat = _lattice_with_blocks(case, blocks)
+    lat = _lattice_with_blocks(case, blocks, keep)
     try:
+        assert lat.blockCount() == (len(blocks) if expect_blocks is None else expect_blocks)
         lat.setPopulations(f0)
         done = 0
         want = f0
@@ -139,6 +140,31 @@ def test_sixty_seeded_blocks(gpu):
                 force=(1e-5, 0, 0))
     case.tags = sphere_tags(shape, (20.5, 18.5, 14.5), 6.2, TAG_WALL)
     _check(case, blocks, [3, 10])
+
+
+@pytest.mark.parametrize("storage", ["f64", "f16"])
+def test_coalesced_tiling_runs_as_one_block(gpu, storage):
+    """Without keep_blocks a tiling below 2^32 cells runs as one block: the
+    60-block porous lattice (link walls, RBGK + Guo) reports one block and
+    equals the oracle bit for bit."""
+    shape = (48, 40, 30)
+    blocks = seeded_uneven_blocks(shape, (5, 4, 3), 6, 16, seed=11)
+    case = Case(*shape, storage=storage, dynamics="rbgk", omega=1.7, periodic=(True, True, True),
+                force=(1e-5, 0, 0))
+    case.tags = sphere_tags(shape, (20.5, 18.5, 14.5), 6.2, TAG_WALL)
+    _check(case, blocks, [3, 10], keep=False, expect_blocks=1)
+
+
+def test_partial_merge_under_a_cell_limit(gpu, monkeypatch):
+    """Above the per-block cell limit (lowered here to 1500 cells) only
+    z-neighbours with equal x and y extents merge: twelve 8 x 10 x 9 blocks run
+    as six of 8 x 10 x 18, bit-identical to the oracle."""
+    monkeypatch.setenv("PLBAMD_MAX_BLOCK_CELLS", "1500")
+    case = Case(24, 20, 18, storage="f64", omega=1.4, periodic=(True, False, True))
+    case.tags = sphere_tags(case.shape, (11.5, 9.5, 8.5), 4.2, TAG_WALL)
+    blocks = uneven_blocks(case.shape, [[8, 16], [10], [9]])
+    assert len(blocks) == 12
+    _check(case, blocks, [2, 7], keep=False, expect_blocks=6)
 
 
 def test_upload_after_odd_step_is_refused(gpu):

@@ -54,7 +54,7 @@ def test_noalloc_equals_reduce_moments_two_level_multiblock(gpu):
     shape = (128, 96, 96)
     blocks = seeded_uneven_blocks(shape, (3, 2, 2), 32, 64, seed=5)
     lat = plb.Block3D(*shape, plb.D3Q19, plb.BGK(1.6), storage="f16", periodic=(True, True, True),
-                           blocks=blocks)
+                           blocks=blocks, keep_blocks=True)
     try:
         rho = 1.0 + 0.01 * np.sin(np.arange(np.prod(shape)).reshape(shape) * 0.01)
         u = np.zeros(shape + (3,))
@@ -93,7 +93,7 @@ def test_noalloc_refusals(gpu):
     shape = (768, 256, 256)
     blocks = [(0, 255, 0, 255, 0, 255), (256, 511, 0, 255, 0, 255), (512, 767, 0, 255, 0, 255)]
     big = plb.Block3D(*shape, plb.D3Q19, plb.BGK(1.6), storage="f16", periodic=(True, True, True),
-                           blocks=blocks)
+                           blocks=blocks, keep_blocks=True)
     try:
         big.collideAndStream(2)
         rc, out = _sums(big, (0, 767, 0, 255, 0, 255), True)
