@@ -115,11 +115,24 @@ __device__ __forceinline__ AdjointCell adjoint_cell_of(const AdjointArgs& a) {
     c.x = (int)(rest / gy);
     return c;
 }
+// The shuffle tree and wavefront order of block_reduce_store, for the one value.
 __device__ __forceinline__ void accumulate_J(double J, double* acc) {
-    __shared__ double total[8];
-    double r[8] = {J, 0, 0, 0, 0, 0, 0, 0};
-    block_reduce_store(r, total);
-    if (threadIdx.x == 0 && threadIdx.y == 0) acc[blockIdx.x] += total[0];
+    __shared__ double red[16];
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    const int n = blockDim.x * blockDim.y;
+    const int lane = t & 63, wave = t >> 6, nWaves = (n + 63) >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(J, off, 64);
+        if (lane + off < 64 && t + off < n) J += o;
+    }
+    if (lane == 0) red[wave] = J;
+    __syncthreads();
+    if (t == 0) {
+        double r = red[0];
+        for (int w = 1; w < nWaves; ++w) r += red[w];
+        acc[blockIdx.x] += r;
+    }
 }
 
 // dj/df_k(y) for the objective (plane-mean pressure difference, drag, mean flux).
@@ -128,18 +141,43 @@ __device__ __forceinline__ void objective_gradient(const AdjointArgs& a, int gx,
     for (int k = 0; k < 19; ++k) djdf[k] = a.wdp * p + (a.wdrag * a.omega * alphaY + a.wflux) * (double)cx(k);
 }
 
-// mu(y) gathered from lambda^{t+1} (transpose of streaming with edge reflection).
+// mu(y) gathered from lambda^{t+1} (transpose of streaming with edge
+// reflection: mu_i(y) = lambda_i(y + c_i), or lambda_opp(i)(y) where y + c_i
+// leaves the block), by the lanes of a wavefront that hold consecutive cells
+// z0 .. z0 + 63 of one row (lane = z - z0; the row, and so every x/y edge test,
+// is the wavefront's). Every direction is loaded at the lane's own z in its
+// source row, aligned; the z-moving ones are then passed one lane along by a
+// cross-lane read (ds_bpermute) instead of a load straddling one more cache line
+// per wavefront. A lane whose source lies beyond the wavefront's cells, wraps
+// around a periodic row or leaves the block loads its own value. Every lane of
+// the row with z < nz takes part (their loads are what the others read).
 template<typename LT, typename C>
-__device__ __forceinline__ void gather_mu(const BlockGeom& g, const LT* lamNext, int x, int y, int z, uint64_t cell,
-                                          C mu[19]) {
+__device__ __forceinline__ void gather_mu_wave(const BlockGeom& g, const LT* lamNext, int x, int y, int z, int z0,
+                                               uint64_t cell, C mu[19]) {
+    const int lane = z - z0;
     mu[0] = (C)lamNext[cell];
 #pragma unroll
     for (int i = 1; i < 19; ++i) {
-        int dx, dy, dz;
-        const bool ok = shift(x, cx(i), g.nx, g.px, dx) && shift(y, cy(i), g.ny, g.py, dy) &&
-                        shift(z, cz(i), g.nz, g.pz, dz);
-        mu[i] = (C)(ok ? lamNext[(uint64_t)i * g.slot + lin(g, dx, dy, dz)]
-                       : lamNext[(uint64_t)opp(i) * g.slot + cell]);
+        int dx, dy;
+        const LT* refl = lamNext + (uint64_t)opp(i) * g.slot + cell;
+        if (!(shift(x, cx(i), g.nx, g.px, dx) && shift(y, cy(i), g.ny, g.py, dy))) {
+            mu[i] = (C)*refl;
+            continue;
+        }
+        const LT* row = lamNext + (uint64_t)i * g.slot + ((uint64_t)dx * g.ny + dy) * g.nz;
+        const LT v = row[z];
+        if (cz(i) == 0) {
+            mu[i] = (C)v;
+            continue;
+        }
+        const LT nb = __shfl(v, lane + cz(i), 64);
+        const int zs = z + cz(i);
+        if (zs >= z0 && zs < z0 + 64 && zs < g.nz) {
+            mu[i] = (C)nb;
+        } else {
+            int dz;
+            mu[i] = shift(z, cz(i), g.nz, g.pz, dz) ? (C)row[dz] : (C)*refl;
+        }
     }
 }
 
@@ -156,16 +194,18 @@ __device__ __forceinline__ int cx_of(uint32_t i) {
 // for a 16-bit one (whose forward arithmetic is FP32); the objective and the
 // porosity gradient accumulate in FP64.
 template<typename LT, int DYN, bool FORCED>
-__device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGeom& g, int x, int y, int z,
+__device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGeom& g, int x, int y, int z, int z0,
                                              double& J) {
     using C = LT;
     const uint64_t cell = ((uint64_t)x * g.ny + y) * g.nz + z;
     const uint64_t idx = g.cellOffset + cell;
     const LT* lamNext = reinterpret_cast<const LT*>(a.lamNext) + g.offset;
     LT* lam = reinterpret_cast<LT*>(a.lam) + g.offset;
+    // the gather first, by every cell of the row (plane cells included: their
+    // loads feed their neighbours' lanes), then the skip test
+    C mu[19];
+    gather_mu_wave<LT, C>(g, lamNext, x, y, z, z0, cell, mu);
     if (!(a.tags && a.tags[idx] == BT_SKIP)) {
-        C mu[19];
-        gather_mu<LT, C>(g, lamNext, x, y, z, cell, mu);
         const double al64 = a.alpha ? a.alpha[idx] : 0.0;
         const C al = (C)al64;
         const C dm = C(1) - al;
@@ -174,53 +214,71 @@ __device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGe
         const C j[3] = {(C)mom[a.momStride + idx], (C)mom[2 * a.momStride + idx], (C)mom[3 * a.momStride + idx]};
         const C rho = C(1) + rhoBar;
         const C F[3] = {FORCED ? (C)a.force[0] : C(0), FORCED ? (C)a.force[1] : C(0), FORCED ? (C)a.force[2] : C(0)};
-        const C Jv[3] = {dm * (j[0] + rho * F[0] / C(2)), dm * (j[1] + rho * F[1] / C(2)), dm * (j[2] + rho * F[2] / C(2))};
+        const C Jv[3] = {dm * (j[0] + rho * F[0] * C(0.5)), dm * (j[1] + rho * F[1] * C(0.5)),
+                         dm * (j[2] + rho * F[2] * C(0.5))};
         const C J2 = Jv[0] * Jv[0] + Jv[1] * Jv[1] + Jv[2] * Jv[2];
         const C w = (C)a.omega;
-        C A = C(0), B[3] = {C(0), C(0), C(0)}, gG[3] = {C(0), C(0), C(0)};
-        C M[3][3] = {{C(0), C(0), C(0)}, {C(0), C(0), C(0)}, {C(0), C(0), C(0)}};
+        // Every term below is linear in mu through its weighted moments
+        //   m0 = sum_i t_i mu_i, m1_c = sum_i t_i c_ic mu_i, m2_ce = sum_i t_i c_ic c_ie mu_i,
+        // accumulated with the velocity components as compile-time signs (a
+        // product by a zero component is not dropped under IEEE rules, so the
+        // zero terms are left out explicitly).
+        C m0 = C(0), m1[3] = {C(0), C(0), C(0)};
+        C m2[3][3] = {{C(0), C(0), C(0)}, {C(0), C(0), C(0)}, {C(0), C(0), C(0)}};
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
-            const C t = weight<C>(i);
-            const C cJ = (C)cx(i) * Jv[0] + (C)cy(i) * Jv[1] + (C)cz(i) * Jv[2];
-            const C mt = mu[i] * t;
-            A += mt * (C(1) - (C(4.5) * cJ * cJ - C(1.5) * J2) / (rho * rho));
+            const C mt = mu[i] * weight<C>(i);
+            m0 += mt;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (ci(i, c) == 0) continue;
+                m1[c] += ci(i, c) > 0 ? mt : -mt;
+#pragma unroll
+                for (int e = c; e < 3; ++e)
+                    if (ci(i, e) != 0) m2[c][e] += ci(i, c) * ci(i, e) > 0 ? mt : -mt;
+            }
+        }
+        m2[1][0] = m2[0][1];
+        m2[2][0] = m2[0][2];
+        m2[2][1] = m2[1][2];
+        const C ir = C(1) / rho, ir2 = ir * ir;
+        // sum_i t_i mu_i (c_i.Jv) c_ic = (m2 Jv)_c, sum_i t_i mu_i (c_i.Jv)^2 = Jv.m2.Jv
+        C Jm[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Jm[c] = m2[c][0] * Jv[0] + m2[c][1] * Jv[1] + m2[c][2] * Jv[2];
+        const C JmJ = Jv[0] * Jm[0] + Jv[1] * Jm[1] + Jv[2] * Jm[2];
+        const C A = m0 - (C(4.5) * JmJ - C(1.5) * J2 * m0) * ir2;
+        C B[3], gG[3] = {C(0), C(0), C(0)};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) B[c] = C(3) * m1[c] + (C(9) * Jm[c] - C(3) * Jv[c] * m0) * ir;
+        if constexpr (FORCED) {
+            const C h = C(1) - w * C(0.5);
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                B[c] += mt * (C(3) * (C)ci(i, c) + (C(9) * cJ * (C)ci(i, c) - C(3) * Jv[c]) / rho);
-            if constexpr (DYN == 1) {
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-#pragma unroll
-                    for (int e = 0; e < 3; ++e)
-                        M[c][e] += C(4.5) * mt * ((C)(ci(i, c) * ci(i, e)) - (c == e ? C(1) / C(3) : C(0)));
-            }
-            if constexpr (FORCED) {
-                const C cF = (C)cx(i) * F[0] + (C)cy(i) * F[1] + (C)cz(i) * F[2];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) gG[c] += mt * (C(-3) * F[c] + C(9) * cF * (C)ci(i, c));
-            }
+                gG[c] = (C(-3) * F[c] * m0 + C(9) * (m2[c][0] * F[0] + m2[c][1] * F[1] + m2[c][2] * F[2])) * h;
         }
-        if constexpr (FORCED) {
+        // M_ce = 4.5 sum_i t_i mu_i (c_ic c_ie - delta_ce / 3)
+        C M[3][3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) gG[c] *= (C(1) - w / C(2));
-        }
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int e = 0; e < 3; ++e) M[c][e] = C(4.5) * (c == e ? m2[c][e] - m0 * (C(1) / C(3)) : m2[c][e]);
         const C BF = B[0] * F[0] + B[1] * F[1] + B[2] * F[2];
         const C gj = gG[0] * j[0] + gG[1] * j[1] + gG[2] * j[2];
         C s, v[3];
         if constexpr (DYN == 0) {
-            s = w * (A + dm * BF / C(2)) - dm * gj / (rho * rho);
+            s = w * (A + dm * BF * C(0.5)) - dm * gj * ir2;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) v[c] = w * dm * B[c] + dm * gG[c] / rho;
+            for (int c = 0; c < 3; ++c) v[c] = w * dm * B[c] + dm * gG[c] * ir;
         } else {
             C Mj[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) Mj[c] = M[c][0] * j[0] + M[c][1] * j[1] + M[c][2] * j[2];
             const C jMj = j[0] * Mj[0] + j[1] * Mj[1] + j[2] * Mj[2];
             const C trM = M[0][0] + M[1][1] + M[2][2];
-            s = A + dm * BF / C(2) + (C(1) - w) * (jMj / (rho * rho) - trM / C(3)) - dm * gj / (rho * rho);
+            s = A + dm * BF * C(0.5) + (C(1) - w) * (jMj * ir2 - trM * (C(1) / C(3))) - dm * gj * ir2;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) v[c] = dm * B[c] - C(2) * (C(1) - w) * Mj[c] / rho + dm * gG[c] / rho;
+            for (int c = 0; c < 3; ++c) v[c] = dm * B[c] - C(2) * (C(1) - w) * Mj[c] * ir + dm * gG[c] * ir;
         }
         const int gx = g.gx0 + x;
         // dj/df of the objective (objective_gradient), in C
@@ -229,26 +287,31 @@ __device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGe
         const C invT = (C)a.invT;
 #pragma unroll
         for (int k = 0; k < 19; ++k) {
-            C val = s + v[0] * (C)cx(k) + v[1] * (C)cy(k) + v[2] * (C)cz(k);
+            C val = s;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (ci(k, c) != 0) val += ci(k, c) > 0 ? v[c] : -v[c];
             if constexpr (DYN == 0) {
                 val += (C(1) - w) * mu[k];
             } else {
+                // c_k.M.c_k
                 C cMc = C(0);
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
 #pragma unroll
-                    for (int e = 0; e < 3; ++e) cMc += (C)ci(k, c) * M[c][e] * (C)ci(k, e);
+                    for (int e = 0; e < 3; ++e)
+                        if (ci(k, c) != 0 && ci(k, e) != 0) cMc += ci(k, c) * ci(k, e) > 0 ? M[c][e] : -M[c][e];
                 val += (C(1) - w) * cMc;
             }
-            if (a.explicitTerm) val += invT * (djBase + djX * (C)cx(k));
+            if (a.explicitTerm) val += invT * (cx(k) == 0 ? djBase : (cx(k) > 0 ? djBase + djX : djBase - djX));
             lam[(uint64_t)k * g.slot + cell] = (LT)val;
         }
         if (in_bulk(g, x, y, z)) {
             const C wB = DYN == 0 ? w : C(1);
-            const C ga = -(wB * (B[0] * (j[0] + rho * F[0] / C(2)) + B[1] * (j[1] + rho * F[1] / C(2)) +
-                                 B[2] * (j[2] + rho * F[2] / C(2))) +
-                           gG[0] * (j[0] / rho + F[0] / C(2)) + gG[1] * (j[1] / rho + F[1] / C(2)) +
-                           gG[2] * (j[2] / rho + F[2] / C(2)));
+            const C ga = -(wB * (B[0] * (j[0] + rho * F[0] * C(0.5)) + B[1] * (j[1] + rho * F[1] * C(0.5)) +
+                                 B[2] * (j[2] + rho * F[2] * C(0.5))) +
+                           gG[0] * (j[0] * ir + F[0] * C(0.5)) + gG[1] * (j[1] * ir + F[1] * C(0.5)) +
+                           gG[2] * (j[2] * ir + F[2] * C(0.5)));
             double ga64 = (double)ga;
             if (a.explicitTerm) {
                 ga64 += a.invT * a.wdrag * a.omega * (double)j[0];
@@ -266,8 +329,8 @@ template<typename LT, int DYN, bool FORCED>
 __global__ void __launch_bounds__(256) adjoint_bulk(AdjointArgs a) {
     const AdjointCell c = adjoint_cell_of(a);
     double J = 0.0;
-    if (c.z < c.g.nz && c.y < c.g.ny) adjoint_cell<LT, DYN, FORCED>(a, c.g, c.x, c.y, c.z, J);
-    accumulate_J(J, a.acc);
+    if (c.z < c.g.nz && c.y < c.g.ny) adjoint_cell<LT, DYN, FORCED>(a, c.g, c.x, c.y, c.z, c.z - (int)threadIdx.x, J);
+    if (a.explicitTerm) accumulate_J(J, a.acc);   // J is zero before the objective's window
 }
 
 struct AdjointSpecialArgs {
