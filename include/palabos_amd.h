@@ -352,6 +352,36 @@ int plbamd_adjoint_gradient(int handle, int nsteps, int checkpoint_every, const 
 /* Number of completed steps. */
 int plbamd_get_time(int handle, int64_t* t);
 
+/* External moments: the lattice's rhoBar / j field (the ScalarField3D rhoBar and
+ * TensorField3D<T,3> j that Palabos's processors share), kept on the device in
+ * the storage's arithmetic type (FP64 for FP64 storage, FP32 otherwise),
+ * allocated on first use. Supported on single-block lattices of bulk BGK /
+ * RBGK cells with BounceBack and NoDynamics cells; other lattices are refused
+ * with PLBAMD_EINVAL. Host arrays: C order over the box, j with 3 values per
+ * cell.
+ * plbamd_compute_rhobar_j replaces applying BoxRhoBarJfunctional3D on `box`
+ * (dataProcessors/dataAnalysisFunctional3D.hh:528-561): every cell's
+ * computeRhoBarJ (bulk: get_rhoBar_j's partial sums; BounceBack: its density
+ * - 1 and 0; NoDynamics: 0 and 0). plbamd_get_rhobar_j / plbamd_set_rhobar_j
+ * read and write the field (ScalarField3D::get / TensorField3D::get); values
+ * set are rounded to the arithmetic type. */
+int plbamd_compute_rhobar_j(int handle, const plbamd_box* box);
+int plbamd_get_rhobar_j(int handle, const plbamd_box* box, double* rho_bar, double* j);
+int plbamd_set_rhobar_j(int handle, const plbamd_box* box, const double* rho_bar, const double* j);
+
+/* nsteps steps of ExternalRhoJcollideAndStream3D over the whole lattice
+ * (basicDynamics/dynamicsProcessor3D.hh:41-209) with the field as it is, on
+ * lattices periodic along every axis (at a non-periodic edge the reference
+ * streams through the block's envelope layer, dynamicsProcessor3D.hh:118-190,
+ * so populations enter from outside the domain; such lattices are refused with
+ * PLBAMD_EINVAL; BounceBack cells make the walls): every
+ * cell collides with the field's moments (collideExternal: BGK or regularized
+ * BGK with the given rhoBar and j, isoThermalDynamics.hh:332-338 and
+ * :1604-1613; BounceBack swaps; NoDynamics nothing), then streams. No statistics are taken
+ * (as collideExternal); the step counter advances. PLBAMD_ESTATE when the field
+ * was never computed or set. */
+int plbamd_external_collide_and_stream(int handle, int nsteps);
+
 /* Streaming-bandwidth probe: the rate this GPU sustains for the memory access
  * patterns of the collide-and-stream kernels, with their nontemporal loads and
  * stores and no arithmetic; the practical ceiling their HBM fractions are read

@@ -259,6 +259,26 @@ inline void addGuoForce(T* f, const T u[3], const T force[3], T omega) {
     f[18] += -oneOver12*mu*(force[0]*u[0] +force[1]*(-(T)1-2*u[1]+3*u[2]) +force[2]*((T)1-2*u[2]+3*u[1]));
 }
 
+// momentTemplates3D.h:482-496 (compute_PiNeq about a given rhoBar and j, with
+// invRho(rhoBar)): the stress the regularized collideExternal uses.
+template<typename T>
+inline void compute_PiNeq_given(const T* f, T rhoBar, const T j[3], T pi[6]) {
+    T sXM1 = f[1] + f[4] + f[5] + f[6] + f[7];
+    T sXP1 = f[10] + f[13] + f[14] + f[15] + f[16];
+    T sYM1 = f[2] + f[4] + f[8] + f[9] + f[14];
+    T sYP1 = f[5] + f[11] + f[13] + f[17] + f[18];
+    T sZM1 = f[3] + f[6] + f[8] + f[16] + f[18];
+    T sZP1 = f[7] + f[9] + f[12] + f[15] + f[17];
+    T invRho = invRhoOf(rhoBar);
+    T cs2 = W<T>::cs2();
+    pi[0] = sXP1 + sXM1 - cs2*rhoBar - invRho*j[0]*j[0];
+    pi[3] = sYP1 + sYM1 - cs2*rhoBar - invRho*j[1]*j[1];
+    pi[5] = sZP1 + sZM1 - cs2*rhoBar - invRho*j[2]*j[2];
+    pi[1] = f[4] - f[5] + f[13] - f[14] - invRho*j[0]*j[1];
+    pi[2] = f[6] - f[7] + f[15] - f[16] - invRho*j[0]*j[2];
+    pi[4] = f[8] - f[9] + f[17] - f[18] - invRho*j[1]*j[2];
+}
+
 // Collisions ---------------------------------------------------------------
 
 // The constants of the BGK update f_i <- one_m_omega f_i + t_omega (C1 + C2 + C3),
@@ -652,6 +672,47 @@ struct Engine {
         return tag[src] == TAG_WALL;
     }
 
+    // computeRhoBarJ of cell k (BoxRhoBarJfunctional3D, dataAnalysisFunctional3D.hh:528-561):
+    // the bulk dynamics' get_rhoBar_j (momentTemplates3D.h:411-423), BounceBack
+    // rhoBar(rho) and 0, NoDynamics 1 - SkordosFactor = 0 and 0 (core/dynamics.hh:880-885,
+    // :1061-1066). Returns false for dynamics the external processors do not cover here.
+    bool rhoBarJ(const T* f, long k, T bbRho, T& rhoBar, T j[3]) const {
+        j[0] = j[1] = j[2] = T();
+        switch (tag[k]) {
+        case TAG_BULK: get_rhoBar_j(f, rhoBar, j); return true;
+        case TAG_BB: rhoBar = bbRho - (T)1; return true;
+        case TAG_NODYN: rhoBar = T(); return true;
+        default: return false;
+        }
+    }
+
+    // collideExternal of cell k (ExternalRhoJcollideAndStream3D,
+    // dynamicsProcessor3D.hh:41-56): BGK with the given moments
+    // (isoThermalDynamics.hh:332-338), regularized BGK with PiNeq about them
+    // (:1604-1613), BounceBack its collide (core/dynamics.hh:793-798), NoDynamics
+    // nothing (:974-977). No statistics.
+    bool collideExternal(T* f, long k, T rhoBar, const T j[3]) const {
+        switch (tag[k]) {
+        case TAG_BULK:
+            if (d.alpha || alphaT || d.force[0] != 0 || d.force[1] != 0 || d.force[2] != 0) return false;
+            if (d.dynamics == 0) {
+                bgk_collision(f, rhoBar, j, bgkRates(), invRhoOf(rhoBar));
+            } else {
+                T pi[6];
+                compute_PiNeq_given(f, rhoBar, j, pi);
+                rlb_collision(f, rhoBar, invRhoOf(rhoBar), j, pi, (T)d.omega);
+            }
+            return true;
+        case TAG_BB:
+            for (int i = 1; i <= 9; ++i) std::swap(f[i], f[i+9]);
+            return true;
+        case TAG_NODYN:
+            return true;
+        default:
+            return false;
+        }
+    }
+
     // One collision of cell k, in place on f (the cell's 19 populations).
     void collideCell(T* f, long k, Stats& st) const {
         switch (tag[k]) {
@@ -962,6 +1023,34 @@ template<> struct Store<BHalf, float> {
     static BHalf store(float v) { BHalf h; h.bits = f32_to_bf16(v); return h; }
 };
 
+// Streaming of the post-collision populations `post` into `state`, with
+// same-step reflection at non-periodic edges (blockLattice3D.hh:403-427,
+// latticeTemplates3D.h:50-63, periodicity :554-624). Link-wise walls reflect the
+// same way; the force on them is the momentum 2 c_i f*_i of every reflected
+// population.
+template<typename S, typename T>
+void streamAll(const Engine<T>& eng, const std::vector<S>& post, S* state, Stats& st) {
+    const OracleDesc& d = eng.d;
+    for (int x = 0; x < d.nx; ++x) for (int y = 0; y < d.ny; ++y) for (int z = 0; z < d.nz; ++z) {
+        long k = eng.lin(x, y, z);
+        if (eng.tag[k] == TAG_WALL) continue;
+        state[k*19] = post[k*19];
+        for (int i = 1; i < 19; ++i) {
+            long src = -1;
+            if (!eng.sourceReflects(x, y, z, i, src)) {
+                state[k*19 + i] = post[src*19 + i];
+            } else {
+                state[k*19 + i] = post[k*19 + opposite(i)];
+                if (src >= 0) {  // reflected by a wall cell: momentum 2 c_o f*_o, o = opposite(i)
+                    int o = opposite(i);
+                    T fo = Store<S, T>::load(post[k*19 + o]);
+                    for (int c = 0; c < 3; ++c) st.force[c] += (double)((T)2 * C[o][c] * fo);
+                }
+            }
+        }
+    }
+}
+
 template<typename S, typename T>
 int runSteps(const OracleDesc& d, S* state, int nsteps, double* statsOut, bool initialize, bool snapshot = false) {
     Engine<T> eng(d);
@@ -980,28 +1069,7 @@ int runSteps(const OracleDesc& d, S* state, int nsteps, double* statsOut, bool i
             eng.collideCell(f, k, st);
             for (int i = 0; i < 19; ++i) post[k*19 + i] = Store<S, T>::store(f[i]);
         }
-        // Streaming with same-step reflection at non-periodic edges
-        // (blockLattice3D.hh:403-427, latticeTemplates3D.h:50-63, periodicity :554-624).
-        // Link-wise walls reflect the same way; the force on them is the
-        // momentum 2 c_i f*_i of every reflected population.
-        for (int x = 0; x < d.nx; ++x) for (int y = 0; y < d.ny; ++y) for (int z = 0; z < d.nz; ++z) {
-            long k = eng.lin(x, y, z);
-            if (eng.tag[k] == TAG_WALL) continue;
-            state[k*19] = post[k*19];
-            for (int i = 1; i < 19; ++i) {
-                long src = -1;
-                if (!eng.sourceReflects(x, y, z, i, src)) {
-                    state[k*19 + i] = post[src*19 + i];
-                } else {
-                    state[k*19 + i] = post[k*19 + opposite(i)];
-                    if (src >= 0) {  // reflected by a wall cell: momentum 2 c_o f*_o, o = opposite(i)
-                        int o = opposite(i);
-                        T fo = Store<S, T>::load(post[k*19 + o]);
-                        for (int c = 0; c < 3; ++c) st.force[c] += (double)((T)2 * C[o][c] * fo);
-                    }
-                }
-            }
-        }
+        streamAll<S, T>(eng, post, state, st);
         if (processors && !eng.runProcessors(state)) return -3;
     }
     if (statsOut) {
@@ -1027,6 +1095,44 @@ int runSteps(const OracleDesc& d, S* state, int nsteps, double* statsOut, bool i
             for (int i = 0; i < 19; ++i) state[k*19 + i] = Store<S, T>::store(f[i]);
         }
     }
+    return 0;
+}
+
+// BoxRhoBarJfunctional3D over the whole lattice: rhoBar and j of every cell, in T,
+// widened to double. -4 for cells the external processors do not cover here.
+template<typename S, typename T>
+int rhoBarJAll(const OracleDesc& d, const S* state, double bbRho, double* rhoBarOut, double* jOut) {
+    Engine<T> eng(d);
+    for (long k = 0; k < eng.N; ++k) {
+        T f[19], rb, j[3];
+        for (int i = 0; i < 19; ++i) f[i] = Store<S, T>::load(state[k*19 + i]);
+        if (!eng.rhoBarJ(f, k, (T)bbRho, rb, j)) return -4;
+        rhoBarOut[k] = (double)rb;
+        for (int c = 0; c < 3; ++c) jOut[3*k + c] = (double)j[c];
+    }
+    return 0;
+}
+
+// One ExternalRhoJcollideAndStream3D step with the field (rhoBar, j as double,
+// rounded to T as the field's type is T). Periodic lattices only: at a
+// non-periodic edge the processor streams through the block's envelope layer
+// (its extended domain, dynamicsProcessor3D.hh:118-190), whose cells Palabos
+// also collides, so populations enter from outside the domain; that is not
+// restated here (-4).
+template<typename S, typename T>
+int externalStep(const OracleDesc& d, S* state, const double* rhoBar, const double* j) {
+    Engine<T> eng(d);
+    if (eng.hasProcessors() || !d.periodic[0] || !d.periodic[1] || !d.periodic[2]) return -4;
+    std::vector<S> post((size_t)eng.N * 19);
+    for (long k = 0; k < eng.N; ++k) {
+        T f[19];
+        for (int i = 0; i < 19; ++i) f[i] = Store<S, T>::load(state[k*19 + i]);
+        const T jj[3] = {(T)j[3*k], (T)j[3*k + 1], (T)j[3*k + 2]};
+        if (!eng.collideExternal(f, k, (T)rhoBar[k], jj)) return -4;
+        for (int i = 0; i < 19; ++i) post[k*19 + i] = Store<S, T>::store(f[i]);
+    }
+    Stats st;
+    streamAll<S, T>(eng, post, state, st);
     return 0;
 }
 
@@ -1279,6 +1385,32 @@ int oracle_objective_dual(const OracleDesc* d, const double* f0, int nsteps, int
     out[0] = J.v;
     for (int k = 0; k <= kSeeds; ++k) out[1 + k] = J.d[k];
     return 0;
+}
+
+// BoxRhoBarJfunctional3D of the natural-layout state: rhoBar [N], j [N][3];
+// bb_rho: the density parameter of BounceBack cells.
+int oracle_rhobar_j(const OracleDesc* d, const void* state, double bb_rho, double* rho_bar, double* j) {
+    if (!d || !state || !rho_bar || !j) return -1;
+    switch (d->storage) {
+    case 0: return rhoBarJAll<double, double>(*d, (const double*)state, bb_rho, rho_bar, j);
+    case 1: return rhoBarJAll<float, float>(*d, (const float*)state, bb_rho, rho_bar, j);
+    case 2: return rhoBarJAll<Half, float>(*d, (const Half*)state, bb_rho, rho_bar, j);
+    case 3: return rhoBarJAll<BHalf, float>(*d, (const BHalf*)state, bb_rho, rho_bar, j);
+    }
+    return -1;
+}
+
+// One ExternalRhoJcollideAndStream3D step of the natural-layout state with the
+// field rho_bar [N], j [N][3].
+int oracle_external_step(const OracleDesc* d, void* state, const double* rho_bar, const double* j) {
+    if (!d || !state || !rho_bar || !j) return -1;
+    switch (d->storage) {
+    case 0: return externalStep<double, double>(*d, (double*)state, rho_bar, j);
+    case 1: return externalStep<float, float>(*d, (float*)state, rho_bar, j);
+    case 2: return externalStep<Half, float>(*d, (Half*)state, rho_bar, j);
+    case 3: return externalStep<BHalf, float>(*d, (BHalf*)state, rho_bar, j);
+    }
+    return -1;
 }
 
 // The 16-bit modes' BGK constants for omega: [one_m_omega, t0_omega, t1_omega, t4_omega, omega'].

@@ -79,6 +79,12 @@ def lib():
         _lib.oracle_objective_dual.restype = ctypes.c_int
         _lib.oracle_fields.argtypes = [ctypes.POINTER(_Desc), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib.oracle_fields.restype = ctypes.c_int
+        _lib.oracle_rhobar_j.argtypes = [ctypes.POINTER(_Desc), ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                                         ctypes.c_void_p]
+        _lib.oracle_rhobar_j.restype = ctypes.c_int
+        _lib.oracle_external_step.argtypes = [ctypes.POINTER(_Desc), ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p]
+        _lib.oracle_external_step.restype = ctypes.c_int
         _lib.oracle_exact_rates.argtypes = [ctypes.c_float, ctypes.c_void_p]
         _lib.oracle_exact_rates.restype = None
         for name in ("oracle_f32_to_f16", "oracle_f16_to_f32", "oracle_f32_to_bf16",
@@ -218,6 +224,39 @@ def equilibrium(rho, u):
     out = np.empty(rho.shape + (19,), dtype=np.float64)
     lib().oracle_equilibrium(rho.size, rho.ctypes.data, u.ctypes.data, out.ctypes.data)
     return out
+
+
+def rhobar_j(case, f, bb_rho=1.0):
+    """BoxRhoBarJfunctional3D of the whole lattice on the storage-typed state `f`:
+    (rhoBar [nx, ny, nz], j [nx, ny, nz, 3]) in the arithmetic type, widened to
+    float64; bb_rho is the BounceBack cells' density parameter."""
+    dtype = STORAGE_DTYPES[case.storage]
+    state = np.ascontiguousarray(f, dtype=dtype)
+    keep = _Keep()
+    d = _desc(case, keep)
+    rho_bar = np.empty(case.shape, dtype=np.float64)
+    j = np.empty(case.shape + (3,), dtype=np.float64)
+    rc = lib().oracle_rhobar_j(ctypes.byref(d), state.ctypes.data, float(bb_rho), rho_bar.ctypes.data, j.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle_rhobar_j rejected the case ({rc})")
+    return rho_bar, j
+
+
+def external_step(case, f, rho_bar, j):
+    """One ExternalRhoJcollideAndStream3D step of the storage-typed state `f`
+    with the field (rho_bar, j; rounded to the arithmetic type); returns the new
+    state."""
+    dtype = STORAGE_DTYPES[case.storage]
+    state = np.array(f, dtype=dtype, copy=True, order="C")
+    rb = np.ascontiguousarray(rho_bar, dtype=np.float64)
+    jj = np.ascontiguousarray(j, dtype=np.float64)
+    assert rb.shape == case.shape and jj.shape == case.shape + (3,)
+    keep = _Keep()
+    d = _desc(case, keep)
+    rc = lib().oracle_external_step(ctypes.byref(d), state.ctypes.data, rb.ctypes.data, jj.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle_external_step rejected the case ({rc})")
+    return state
 
 
 def exact_rates(omega):

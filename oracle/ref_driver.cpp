@@ -31,6 +31,13 @@
 //   dump s1 s2 ...                    steps after which populations are written (0 = after initialize)
 //   force fx fy fz                    optional; ForcedD3Q19 + GuoExternalForceBGKdynamics (bgk, no planes)
 //   fields 1                          optional; also write the analysis fields at every dump
+//   external a                        optional; every step is Palabos's external-moment cycle instead
+//                                     of collideAndStream: BoxRhoBarJfunctional3D fills rhoBar and j
+//                                     (dataAnalysisFunctional3D.hh:528-561), j_x += a * (1 + rhoBar) on
+//                                     every cell (in T), ExternalRhoJcollideAndStream3D collides with them
+//                                     and streams (dynamicsProcessor3D.hh:41-209); the field after the
+//                                     last coupling is written as prefix_rhobarj_s<step>.bin
+//                                     (T, [nx][ny][nz][4]: rhoBar jx jy jz)
 //   out prefix
 // Outputs: prefix_s<step>.bin (T, [nx][ny][nz][19]) and prefix_stats.txt lines;
 // with "fields", prefix_fields_s<step>.bin (T, [nx][ny][nz][7]: computeDensity,
@@ -125,6 +132,8 @@ struct CaseSpec {
     bool forced = false;
     double force[3] = {0, 0, 0};
     bool fields = false;
+    bool external = false;
+    double externalA = 0;
 };
 
 std::vector<char> readFile(std::string const& name) {
@@ -153,6 +162,7 @@ CaseSpec parseCase(std::string const& name) {
         else if (key == "tags") s >> c.tags;
         else if (key == "out") s >> c.out;
         else if (key == "force") { s >> c.force[0] >> c.force[1] >> c.force[2]; c.forced = true; }
+        else if (key == "external") { s >> c.externalA; c.external = true; }
         else if (key == "bc") {
             BcSpec b;
             s >> b.kind >> b.dir >> b.ori >> b.box.x0 >> b.box.x1 >> b.box.y0 >> b.box.y1
@@ -389,8 +399,27 @@ int runCase(CaseSpec const& c) {
     plint maxStep = 0;
     for (pluint i = 0; i < c.dumps.size(); ++i) if (c.dumps[i] > maxStep) maxStep = c.dumps[i];
     std::vector<T> buf((size_t)(c.nx*c.ny*c.nz*Q));
+    // the external-moment fields: the lattice's own block management, its periodicity
+    MultiScalarField3D<T> rhoBarField((MultiBlock3D&)lat);
+    MultiTensorField3D<T,3> jField((MultiBlock3D&)lat);
+    for (int a = 0; a < 3; ++a) {
+        rhoBarField.periodicity().toggle(a, c.periodic[a] != 0);
+        jField.periodicity().toggle(a, c.periodic[a] != 0);
+    }
+    std::vector<MultiBlock3D*> rhoBarJargs;
+    rhoBarJargs.push_back(&lat);
+    rhoBarJargs.push_back(&rhoBarField);
+    rhoBarJargs.push_back(&jField);
     for (plint step = 0; step <= maxStep; ++step) {
-        if (step > 0) lat.collideAndStream();
+        if (step > 0 && c.external) {
+            applyProcessingFunctional(new BoxRhoBarJfunctional3D<T,D>(), lat.getBoundingBox(), rhoBarJargs);
+            for (plint x = 0; x < c.nx; ++x) for (plint y = 0; y < c.ny; ++y) for (plint z = 0; z < c.nz; ++z)
+                jField.get(x, y, z)[0] += (T)c.externalA * ((T)1 + rhoBarField.get(x, y, z));
+            jField.duplicateOverlaps(modif::staticVariables);
+            applyProcessingFunctional(new ExternalRhoJcollideAndStream3D<T,D>(), lat.getBoundingBox(), rhoBarJargs);
+        } else if (step > 0) {
+            lat.collideAndStream();
+        }
         bool dumpNow = false;
         for (pluint i = 0; i < c.dumps.size(); ++i) if (c.dumps[i] == step) dumpNow = true;
         if (!dumpNow) continue;
@@ -403,6 +432,18 @@ int runCase(CaseSpec const& c) {
         FILE* o = std::fopen(name, "wb");
         std::fwrite(&buf[0], sizeof(T), buf.size(), o);
         std::fclose(o);
+        if (c.external && step > 0) {
+            std::vector<T> fb((size_t)(c.nx*c.ny*c.nz*4));
+            size_t k = 0;
+            for (plint x = 0; x < c.nx; ++x) for (plint y = 0; y < c.ny; ++y) for (plint z = 0; z < c.nz; ++z) {
+                fb[k++] = rhoBarField.get(x, y, z);
+                for (int a = 0; a < 3; ++a) fb[k++] = jField.get(x, y, z)[a];
+            }
+            std::snprintf(name, sizeof(name), "%s_rhobarj_s%ld.bin", c.out.c_str(), (long)step);
+            FILE* fo = std::fopen(name, "wb");
+            std::fwrite(&fb[0], sizeof(T), fb.size(), fo);
+            std::fclose(fo);
+        }
         if (c.fields) {
             std::auto_ptr<MultiScalarField3D<T> > rho = computeDensity(lat);
             std::auto_ptr<MultiScalarField3D<T> > rhoBar = computeRhoBar(lat);

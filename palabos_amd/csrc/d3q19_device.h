@@ -91,6 +91,22 @@ __device__ __forceinline__ void rhobar_j_pineq(const T* f, T& rhoBar, T j[3], T 
     pi[4] = f[8] - f[9] + f[17] - f[18] - ir * j[1] * j[2];
 }
 
+// The off-equilibrium stress of the populations for a given rhoBar and j (the
+// moments of an external field), order xx xy xz yy yz zz
+// (momentTemplates3D.h:482-496 compute_PiNeq with invRho(rhoBar)).
+template<typename T>
+__device__ __forceinline__ void pineq_given(const T* f, T rhoBar, const T j[3], T pi[6]) {
+    Surfaces<T> s(f);
+    const T ir = inv_rho(rhoBar);
+    const T c = cs2<T>();
+    pi[0] = s.xp + s.xm - c * rhoBar - ir * j[0] * j[0];
+    pi[3] = s.yp + s.ym - c * rhoBar - ir * j[1] * j[1];
+    pi[5] = s.zp + s.zm - c * rhoBar - ir * j[2] * j[2];
+    pi[1] = f[4] - f[5] + f[13] - f[14] - ir * j[0] * j[1];
+    pi[2] = f[6] - f[7] + f[15] - f[16] - ir * j[0] * j[2];
+    pi[4] = f[8] - f[9] + f[17] - f[18] - ir * j[1] * j[2];
+}
+
 // Second-order equilibrium of one population
 // (dynamicsTemplates3D.h:1994-1997 bgk_ma2_equilibrium).
 template<typename T>
@@ -310,6 +326,23 @@ __device__ __forceinline__ T collide_bulk(T* f, T omegaIn, const T F[3], T a, T&
         if (F[0] != (T)0 || F[1] != (T)0 || F[2] != (T)0) add_guo_force(f, u, F, omega);
         rhoBarOut = rhoBar;
         return uSqr;
+    }
+}
+
+// Collision with the moments of an external field instead of the populations'
+// own: BGKdynamics::collideExternal (isoThermalDynamics.hh:332-338,
+// bgk_ma2_collision with the given rhoBar and j) and
+// RegularizedBGKdynamics::collideExternal (isoThermalDynamics.hh:1604-1613,
+// PiNeq of the populations about the given moments, then rlb_collision). No
+// statistics, as there. EXACT: the 16-bit storage modes' BGK rates.
+template<typename T, int DYN, bool EXACT>
+__device__ __forceinline__ void collide_external(T* f, T omega, T rhoBar, const T j[3]) {
+    if constexpr (DYN == 0) {
+        (void)bgk(f, rhoBar, j, bgk_rates<T, EXACT>(omega), inv_rho(rhoBar));
+    } else {
+        T pi[6];
+        pineq_given(f, rhoBar, j, pi);
+        (void)rlb(f, rhoBar, inv_rho(rhoBar), j, pi, omega);
     }
 }
 

@@ -129,6 +129,7 @@ struct Lattice {
     // data processors after the stream (fd_boundary.h): records of the four
     // phases [procPhase[p], procPhase[p + 1])
     ProcCell* dProc = nullptr;
+    void* dExtField = nullptr;        // external rhoBar / j field, [4][allocCells] of the arithmetic type
     uint32_t procPhase[5] = {0, 0, 0, 0, 0};
     bool anyCopy = false;             // some boundary copies its value (the device values change)
 
@@ -138,7 +139,7 @@ struct Lattice {
     ~Lattice() {
         hipFree(arena); hipFree(dGeom); hipFree(dTags); hipFree(dSpecial); hipFree(dValues); hipFree(dAlpha);
         hipFree(dBulkPartials); hipFree(dBulkLevel1); hipFree(dSpecialPartials); hipFree(dStats); hipFree(dSegs); hipFree(dSegs1); hipFree(dTasks1);
-        hipFree(dWgBlock); hipFree(dTasks); hipFree(dWalls); hipFree(dEvenBlock); hipFree(dProc);
+        hipFree(dWgBlock); hipFree(dTasks); hipFree(dWalls); hipFree(dEvenBlock); hipFree(dProc); hipFree(dExtField);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
         if (stream) hipStreamDestroy(stream);
@@ -213,6 +214,11 @@ int moments(Lattice& L, const plbamd_box& b, double* rho, double* u, double* sum
 // (launches only).
 template<typename S>
 int launchProcessors(Lattice& L, int parity);
+// The external-moment field and step (external.h): op 0 compute on b, 1 get, 2 set.
+template<typename S>
+int externalField(Lattice& L, const plbamd_box& b, double* rhoBar, double* j, int op);
+template<typename S>
+int externalSteps(Lattice& L, int nsteps);
 
 // Finds the device address of the fixed scratch array of
 // plbamd_reduce_moments_noalloc on `device`, which must be current (noalloc.hip).

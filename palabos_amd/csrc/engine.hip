@@ -855,6 +855,63 @@ int plbamd_multiblock_create_ex(const plbamd_lattice_desc* desc, const plbamd_bo
     return registerLattice(std::move(L), handle);
 }
 
+// What the external-moment processors support here: a single-block lattice of
+// bulk BGK / RBGK cells with BounceBack and NoDynamics cells.
+static int externalSupported(Lattice& L, const char* who) {
+    PLBAMD_TRY(rebuild(L));
+    if (L.multi || L.nSpecial || L.dWalls || L.dAlpha || L.dProc || L.desc.force[0] != 0 || L.desc.force[1] != 0 ||
+        L.desc.force[2] != 0)
+        return fail(PLBAMD_EINVAL, std::string(who) + ": external rhoBar / j need a single-block lattice of bulk, "
+                                                      "BounceBack and NoDynamics cells (no boundary objects, "
+                                                      "momentum-exchange nodes, link walls, body force or porosity)");
+    return PLBAMD_OK;
+}
+
+static int externalFieldOp(int handle, const plbamd_box* box, double* rhoBar, double* j, int op, const char* who) {
+    Lattice* L = find(handle);
+    if (!L) return fail(PLBAMD_EINVAL, std::string(who) + ": unknown handle");
+    if (!box || !boxValid(*L, *box)) return fail(PLBAMD_EINVAL, std::string(who) + ": box outside the lattice");
+    if (op != 0 && (!rhoBar || !j)) return fail(PLBAMD_EINVAL, std::string(who) + ": null array");
+    PLBAMD_TRY(externalSupported(*L, who));
+    switch (L->desc.storage) {
+    case PLBAMD_F64: return externalField<double>(*L, *box, rhoBar, j, op);
+    case PLBAMD_F32: return externalField<float>(*L, *box, rhoBar, j, op);
+    case PLBAMD_F16: return externalField<f16s>(*L, *box, rhoBar, j, op);
+    default: return externalField<bf16s>(*L, *box, rhoBar, j, op);
+    }
+}
+
+int plbamd_compute_rhobar_j(int handle, const plbamd_box* box) {
+    return externalFieldOp(handle, box, nullptr, nullptr, 0, "compute_rhobar_j");
+}
+
+int plbamd_get_rhobar_j(int handle, const plbamd_box* box, double* rhoBar, double* j) {
+    return externalFieldOp(handle, box, rhoBar, j, 1, "get_rhobar_j");
+}
+
+int plbamd_set_rhobar_j(int handle, const plbamd_box* box, const double* rhoBar, const double* j) {
+    return externalFieldOp(handle, box, const_cast<double*>(rhoBar), const_cast<double*>(j), 2, "set_rhobar_j");
+}
+
+int plbamd_external_collide_and_stream(int handle, int nsteps) {
+    Lattice* L = find(handle);
+    if (!L) return fail(PLBAMD_EINVAL, "external_collide_and_stream: unknown handle");
+    if (nsteps < 0) return fail(PLBAMD_EINVAL, "external_collide_and_stream: negative step count");
+    PLBAMD_TRY(externalSupported(*L, "external_collide_and_stream"));
+    // at a non-periodic edge Palabos's processor streams through the block's
+    // envelope layer (populations enter from cells outside the domain)
+    if (!L->desc.periodic[0] || !L->desc.periodic[1] || !L->desc.periodic[2])
+        return fail(PLBAMD_EINVAL, "external_collide_and_stream: periodic lattices only (at a non-periodic edge "
+                                   "ExternalRhoJcollideAndStream3D streams through the block envelope; use "
+                                   "BounceBack cells as walls)");
+    switch (L->desc.storage) {
+    case PLBAMD_F64: return externalSteps<double>(*L, nsteps);
+    case PLBAMD_F32: return externalSteps<float>(*L, nsteps);
+    case PLBAMD_F16: return externalSteps<f16s>(*L, nsteps);
+    default: return externalSteps<bf16s>(*L, nsteps);
+    }
+}
+
 int plbamd_get_block_count(int handle, int* count) {
     if (!count) return fail(PLBAMD_EINVAL, "get_block_count: null output");
     Lattice* L = find(handle);

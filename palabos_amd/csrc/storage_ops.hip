@@ -2,6 +2,7 @@
 // step: envelope exchange, host transfers, checkpoint files, equilibrium
 // initialisation and moments.
 #include "engine.h"
+#include "external.h"
 
 #if !defined(PLBAMD_S)
 #error "compile with -DPLBAMD_S=<storage type>"
@@ -322,6 +323,81 @@ int launchProcessors(Lattice& L, int parity) {
     return PLBAMD_OK;
 }
 
+// The external-moment field of a single-block lattice: op 0 fills it on box b
+// from the populations (BoxRhoBarJfunctional3D), 1 copies its values on b to the
+// host, 2 sets them from the host. Allocated on first use, [4][allocCells].
+template<typename S>
+int externalField(Lattice& L, const plbamd_box& b, double* rhoBar, double* j, int op) {
+    using T = typename Store<S>::T;
+    PLBAMD_TRY(rebuild(L));
+    if (!L.dExtField) {
+        PLBAMD_HIP(hipMalloc(&L.dExtField, 4 * L.allocCells * sizeof(T)));
+        PLBAMD_HIP(hipMemsetAsync(L.dExtField, 0, 4 * L.allocCells * sizeof(T), L.stream));
+    }
+    T* field = static_cast<T*>(L.dExtField);
+    BoxArgs a;
+    if (!boxArgs(L, 0, b, a)) return fail(PLBAMD_EINVAL, "rhobar_j: box outside the lattice");
+    const uint64_t n = boxCells(b);
+    const dim3 grid = gridFor((uint64_t)a.ilx * a.ily * a.ilz);
+    if (op == 0) {
+        if (a.parity)
+            hipLaunchKernelGGL((rhobar_j_box<S, 1>), grid, dim3(256), 0, L.stream, L.arena, a, L.dTags,
+                               L.solidRho[0], field, L.allocCells);
+        else
+            hipLaunchKernelGGL((rhobar_j_box<S, 0>), grid, dim3(256), 0, L.stream, L.arena, a, L.dTags,
+                               L.solidRho[0], field, L.allocCells);
+        PLBAMD_HIP(hipGetLastError());
+        PLBAMD_HIP(hipStreamSynchronize(L.stream));
+        return PLBAMD_OK;
+    }
+    DeviceBuffers buf;
+    double* d = static_cast<double*>(buf.get(4 * n * sizeof(double)));
+    if (!d) return fail(PLBAMD_ENOMEM, "rhobar_j: staging allocation");
+    if (op == 1) {
+        hipLaunchKernelGGL((field_get<T>), grid, dim3(256), 0, L.stream, a, (const T*)field, L.allocCells, d, d + n);
+        PLBAMD_HIP(hipGetLastError());
+        PLBAMD_HIP(hipMemcpyAsync(rhoBar, d, n * sizeof(double), hipMemcpyDeviceToHost, L.stream));
+        PLBAMD_HIP(hipMemcpyAsync(j, d + n, 3 * n * sizeof(double), hipMemcpyDeviceToHost, L.stream));
+    } else {
+        PLBAMD_HIP(hipMemcpyAsync(d, rhoBar, n * sizeof(double), hipMemcpyHostToDevice, L.stream));
+        PLBAMD_HIP(hipMemcpyAsync(d + n, j, 3 * n * sizeof(double), hipMemcpyHostToDevice, L.stream));
+        hipLaunchKernelGGL((field_set<T>), grid, dim3(256), 0, L.stream, a, field, L.allocCells,
+                           (const double*)d, (const double*)(d + n));
+        PLBAMD_HIP(hipGetLastError());
+    }
+    PLBAMD_HIP(hipStreamSynchronize(L.stream));
+    return PLBAMD_OK;
+}
+
+// nsteps ExternalRhoJcollideAndStream3D steps with the field as it is.
+template<typename S>
+int externalSteps(Lattice& L, int nsteps) {
+    using T = typename Store<S>::T;
+    PLBAMD_TRY(rebuild(L));
+    if (!L.dExtField) return fail(PLBAMD_ESTATE, "external_collide_and_stream: no rhoBar / j field (compute or set it first)");
+    const BlockGeom& g = L.geom[0];
+    const dim3 grid = itemsGrid((uint64_t)g.nx * g.ny * g.nz);
+    const T* field = static_cast<const T*>(L.dExtField);
+    const bool rbgk = L.desc.dynamics == PLBAMD_RBGK;
+    for (int s = 0; s < nsteps; ++s) {
+        const bool odd = (L.time & 1) != 0;
+        if (rbgk) {
+            if (odd) hipLaunchKernelGGL((external_step<S, 1, true>), grid, dim3(256), 0, L.stream, L.arena, g, L.dTags, field, L.desc.omega);
+            else hipLaunchKernelGGL((external_step<S, 1, false>), grid, dim3(256), 0, L.stream, L.arena, g, L.dTags, field, L.desc.omega);
+        } else {
+            if (odd) hipLaunchKernelGGL((external_step<S, 0, true>), grid, dim3(256), 0, L.stream, L.arena, g, L.dTags, field, L.desc.omega);
+            else hipLaunchKernelGGL((external_step<S, 0, false>), grid, dim3(256), 0, L.stream, L.arena, g, L.dTags, field, L.desc.omega);
+        }
+        PLBAMD_HIP(hipGetLastError());
+        ++L.time;
+    }
+    L.statsValid = false;
+    PLBAMD_HIP(hipStreamSynchronize(L.stream));
+    return PLBAMD_OK;
+}
+
+template int externalField<PLBAMD_S>(Lattice&, const plbamd_box&, double*, double*, int);
+template int externalSteps<PLBAMD_S>(Lattice&, int);
 template int launchProcessors<PLBAMD_S>(Lattice&, int);
 template int exchange<PLBAMD_S>(Lattice&);
 template hipError_t transferDevice<PLBAMD_S>(Lattice&, const plbamd_box&, int, void*, bool);

@@ -477,6 +477,33 @@ class Block3D:
     def velocity(self, box=None):
         return self.macroscopic(box)[1]
 
+    # -- external moments (dynamicsProcessor3D.hh:41-209, dataAnalysisFunctional3D.hh:528-561) --
+    def computeRhoBarJ(self, box=None):
+        """BoxRhoBarJfunctional3D on the box: the lattice's rhoBar / j field
+        (on the device) takes every cell's computeRhoBarJ."""
+        b = _box(box, self.shape)
+        check(lib().plbamd_compute_rhobar_j(self.handle, ctypes.byref(b)))
+
+    def getRhoBarJ(self, box=None):
+        """The rhoBar / j field on the box: ([nx, ny, nz], [nx, ny, nz, 3]) float64."""
+        b = _box(box, self.shape)
+        rho_bar = np.empty(b.shape(), dtype=np.float64)
+        j = np.empty(b.shape() + (3,), dtype=np.float64)
+        check(lib().plbamd_get_rhobar_j(self.handle, ctypes.byref(b), rho_bar.ctypes.data, j.ctypes.data))
+        return rho_bar, j
+
+    def setRhoBarJ(self, rho_bar, j, box=None):
+        """Sets the rhoBar / j field on the box (rounded to the arithmetic type)."""
+        b = _box(box, self.shape)
+        rho_bar = np.ascontiguousarray(rho_bar, dtype=np.float64).reshape(b.shape())
+        j = np.ascontiguousarray(j, dtype=np.float64).reshape(b.shape() + (3,))
+        check(lib().plbamd_set_rhobar_j(self.handle, ctypes.byref(b), rho_bar.ctypes.data, j.ctypes.data))
+
+    def externalCollideAndStream(self, nsteps=1):
+        """ExternalRhoJcollideAndStream3D: nsteps steps colliding with the
+        rhoBar / j field as it is (no statistics)."""
+        check(lib().plbamd_external_collide_and_stream(self.handle, int(nsteps)))
+
     def reduceMoments(self, box=None):
         b = _box(box, self.shape)
         out = np.zeros(5, dtype=np.float64)

@@ -38,7 +38,8 @@ EXPORTS = (
     "plbamd_compute_fields", "plbamd_set_solid_density", "plbamd_reduce_moments_noalloc",
     "plbamd_add_velocity_edge", "plbamd_add_velocity_corner", "plbamd_set_boundary_type",
     "plbamd_execute_processors", "plbamd_bandwidth_probe", "plbamd_multiblock_create_ex",
-    "plbamd_get_block_count",
+    "plbamd_get_block_count", "plbamd_compute_rhobar_j", "plbamd_get_rhobar_j", "plbamd_set_rhobar_j",
+    "plbamd_external_collide_and_stream",
 )
 
 
@@ -104,6 +105,10 @@ def lib():
         "plbamd_multiblock_create_ex": [ctypes.POINTER(LatticeDesc), pbox, i32, i32, ctypes.POINTER(ctypes.c_int)],
         "plbamd_lattice_destroy": [i32],
         "plbamd_get_block_count": [i32, ctypes.POINTER(ctypes.c_int)],
+        "plbamd_compute_rhobar_j": [i32, pbox],
+        "plbamd_get_rhobar_j": [i32, pbox, vp, vp],
+        "plbamd_set_rhobar_j": [i32, pbox, vp, vp],
+        "plbamd_external_collide_and_stream": [i32, i32],
         "plbamd_set_tags": [i32, vp],
         "plbamd_add_plane": [i32, i32, i32, i32, i32, pbox, ctypes.POINTER(ctypes.c_int)],
         "plbamd_set_plane_values": [i32, i32, vp, sz],

@@ -37,8 +37,9 @@ def main(only=None):
         if only and name not in only:
             continue
         case, init, meta = build(name)
-        pops, stats = reference.run(case, init, spec["dumps"], bcstyle=spec.get("bcstyle", "dynamics"),
-                                    blocks=spec.get("blocks"))
+        res = reference.run(case, init, spec["dumps"], bcstyle=spec.get("bcstyle", "dynamics"),
+                            blocks=spec.get("blocks"), external=spec.get("external"))
+        pops, stats = res[0], res[1]
         out = {"case_json": np.array(json.dumps(meta)), "init_sha256": np.array(sha(init))}
         for s in spec["dumps"]:
             st = stats[s]
@@ -60,6 +61,8 @@ def main(only=None):
                 out[f"midplane_u_s{s}"] = np.stack([(sXP - sXM) / rho, (sYP - sYM) / rho, (sZP - sZM) / rho], -1)
             else:
                 out[f"s{s}"] = pops[s]
+            if spec.get("external") is not None:
+                out[f"rhobarj_s{s}"] = res[2][s]
         path = os.path.join(HERE, name + ".npz")
         np.savez_compressed(path, **out)
         print(f"{name}: {os.path.getsize(path) / 1024:.0f} KiB")

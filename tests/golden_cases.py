@@ -52,6 +52,11 @@ GOLDEN = {
     "doctest_cavity_58x58x58_f64": {"dumps": [30], "summary": True, "bcstyle": "wrapped"},
     "doctest_cavity_20x31x42_f64": {"dumps": [30], "summary": True, "bcstyle": "wrapped"},
     "doctest_cavity_20x31x42_f32": {"dumps": [30], "summary": True, "bcstyle": "wrapped"},
+    # Palabos's external-moment cycle in place of collideAndStream: BoxRhoBarJfunctional3D,
+    # j_x += a (1 + rhoBar), ExternalRhoJcollideAndStream3D (periodic, BounceBack and NoDynamics cells)
+    "external_bgk_f64": {"dumps": [1, 6], "external": 2e-4},
+    "external_rbgk_f32": {"dumps": [1, 5], "external": 2e-4},
+    "external_rbgk_f64": {"dumps": [4], "external": -3e-4},
 }
 
 
@@ -131,6 +136,14 @@ def _make(name):
         case = Case(*shape, storage=storage, omega=1.7,
                     planes=block_boundary_objects(shape, "zouhe", bc_type="neumann", kind="pressure"))
         init_seed = 3
+    elif base.startswith("external_"):
+        case = Case(8, 9, 10, storage=storage, dynamics="rbgk" if "rbgk" in base else "bgk", omega=1.6,
+                    periodic=(True, True, True))
+        tags = np.zeros(case.shape, np.uint8)
+        tags[3:5, 3:6, 4:6] = TAG_BB
+        tags[6, 1:3, 1:3] = TAG_NODYN
+        case.tags = tags
+        init_seed = 19
     elif base.startswith("doctest_cavity_"):
         nx, ny, nz = (int(v) for v in base.split("_")[2].split("x"))
         case = doctest_cavity_case(nx, ny, nz, storage=storage)

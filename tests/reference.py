@@ -18,14 +18,18 @@ def available():
 
 
 def run(case, init_f64, dumps, bcstyle="dynamics", blocks=None, timeout=1800, fields=False, block_bcs=None,
-        family=None):
+        family=None, external=None):
     """Returns ({step: populations in Palabos's T}, {step: stats row}); with
     fields, also {step: [nx,ny,nz,7] analysis fields} (computeDensity,
     computeRhoBar, computeVelocity x y z, computeKineticEnergy,
     computeVelocityNorm) as a third item. block_bcs: [(kind 'vel' | 'press',
     bc type name, per-cell values of the whole lattice)] installed with
     set{Velocity,Pressure}ConditionOnBlockBoundaries; family: the boundary
-    factory when the case has no planes of its own."""
+    factory when the case has no planes of its own. external: the coupling
+    amplitude a of the external-moment cycle (BoxRhoBarJfunctional3D, j_x +=
+    a (1 + rhoBar), ExternalRhoJcollideAndStream3D) that replaces
+    collideAndStream; then {step: [nx,ny,nz,4] rhoBar jx jy jz field} is
+    returned as the last item."""
     T = {"f64": "double", "f32": "float"}[case.storage]
     with tempfile.TemporaryDirectory() as tmp:
         lines = [f"T {T}", f"size {case.nx} {case.ny} {case.nz}", f"dyn {case.dynamics}",
@@ -71,6 +75,8 @@ def run(case, init_f64, dumps, bcstyle="dynamics", blocks=None, timeout=1800, fi
             lines.append("block " + " ".join(map(str, b)))
         if fields:
             lines.append("fields 1")
+        if external is not None:
+            lines.append(f"external {float(external)!r}")
         lines.append("dump " + " ".join(map(str, dumps)))
         out = os.path.join(tmp, "out")
         lines.append(f"out {out}")
@@ -86,10 +92,14 @@ def run(case, init_f64, dumps, bcstyle="dynamics", blocks=None, timeout=1800, fi
             stats[int(v[0])] = {"stored_avg_rho": float(v[1]), "stored_avg_energy": float(v[2]),
                                 "avg_rho": float(v[3]), "avg_energy": float(v[4]),
                                 "force": np.array([float(v[5]), float(v[6]), float(v[7])])}
+        extra = []
         if fields:
-            flds = {s: np.fromfile(f"{out}_fields_s{s}.bin", dtype=dt).reshape(case.shape + (7,)) for s in dumps}
-            return pops, stats, flds
-        return pops, stats
+            extra.append({s: np.fromfile(f"{out}_fields_s{s}.bin", dtype=dt).reshape(case.shape + (7,))
+                          for s in dumps})
+        if external is not None:
+            extra.append({s: np.fromfile(f"{out}_rhobarj_s{s}.bin", dtype=dt).reshape(case.shape + (4,))
+                          for s in dumps if s > 0})
+        return (pops, stats, *extra)
 
 
 def checkpoint(mode, T, forced, shape, src, dst, enforce_uint=False, timeout=300):

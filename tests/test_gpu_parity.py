@@ -224,7 +224,7 @@ GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 @pytest.mark.parametrize("name", sorted(n for n, s in GOLDEN.items()
-                                        if not s.get("summary") and "wrapped" not in n and
+                                        if not s.get("summary") and "wrapped" not in n and "external" not in s and
                                         not n.startswith(("blockbc_", "pressure_neumann"))))
 def test_gpu_matches_palabos_golden(gpu, name):
     """The HIP engine reproduces compiled Palabos's populations bit for bit."""

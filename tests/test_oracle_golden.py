@@ -15,7 +15,29 @@ from golden_cases import GOLDEN, load
 from oracle import oracle
 
 GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-SMALL = sorted(n for n, s in GOLDEN.items() if not s.get("summary"))
+SMALL = sorted(n for n, s in GOLDEN.items() if not s.get("summary") and "external" not in s)
+EXTERNAL = sorted(n for n, s in GOLDEN.items() if "external" in s)
+
+
+def external_cycle(case, f, a, lattice=None):
+    """One external-moment cycle as the reference driver runs it: rhoBar, j of
+    every cell (BoxRhoBarJfunctional3D), j_x += a (1 + rhoBar) in the arithmetic
+    type, one ExternalRhoJcollideAndStream3D step. With `lattice` (a Block3D) the
+    device does the processors; the oracle otherwise. Returns (state or None,
+    the field used)."""
+    T = np.float64 if case.storage == "f64" else np.float32
+    if lattice is None:
+        rb, j = oracle.rhobar_j(case, f)
+    else:
+        lattice.computeRhoBarJ()
+        rb, j = lattice.getRhoBarJ()
+    rb, j = rb.astype(T), j.astype(T)
+    j[..., 0] = j[..., 0] + T(a) * (T(1) + rb)
+    if lattice is None:
+        return oracle.external_step(case, f, rb, j), (rb, j)
+    lattice.setRhoBarJ(rb, j)
+    lattice.externalCollideAndStream(1)
+    return None, (rb, j)
 
 
 def _stats_check(name, ost, ref_row, case):
@@ -44,6 +66,48 @@ def test_oracle_matches_palabos(name):
             f"{name} step {s}: {np.count_nonzero(got != want)} populations differ, "
             f"max {np.abs(got.astype(float) - want.astype(float)).max():.3e}")
         _stats_check(name, ost, data[f"stats_s{s}"], case)
+
+
+@pytest.mark.parametrize("name", EXTERNAL)
+def test_oracle_external_cycle_matches_palabos(name):
+    """BoxRhoBarJfunctional3D, a coupling j_x += a (1 + rhoBar), and
+    ExternalRhoJcollideAndStream3D (dynamicsProcessor3D.hh:41-209) with BGK or
+    regularized BGK, BounceBack and NoDynamics cells on a periodic lattice: the
+    populations and the field used match compiled Palabos bit for bit."""
+    case, init, data = load(name, GOLDEN_DIR)
+    a = GOLDEN[name]["external"]
+    f = oracle.to_storage(init, case.storage)
+    done = 0
+    for s in GOLDEN[name]["dumps"]:
+        while done < s:
+            f, (rb, j) = external_cycle(case, f, a)
+            done += 1
+        assert np.array_equal(f, data[f"s{s}"]), f"step {s}: {np.count_nonzero(f != data[f's{s}'])} differ"
+        field = data[f"rhobarj_s{s}"]
+        assert np.array_equal(rb, field[..., 0]) and np.array_equal(j, field[..., 1:])
+
+
+def test_external_cycle_without_coupling_is_the_plain_step():
+    """With the field left as BoxRhoBarJfunctional3D computes it, one external
+    step equals one collideAndStream (same moments, same collision): bit-exact,
+    BGK and regularized BGK."""
+    for dyn in ("bgk", "rbgk"):
+        case, init, _ = load("external_bgk_f64", GOLDEN_DIR)
+        case.dynamics = dyn
+        f0 = oracle.to_storage(init, "f64")
+        ext, _ = external_cycle(case, f0, 0.0)
+        plain, _ = oracle.run(case, f0, 1)
+        assert np.array_equal(ext, plain), dyn
+
+
+def test_external_step_refuses_non_periodic_lattices():
+    """At a non-periodic edge Palabos's processor streams through the block
+    envelope (populations enter from outside the domain): not restated, refused."""
+    case, init, _ = load("external_bgk_f64", GOLDEN_DIR)
+    case.periodic = (True, False, True)
+    rb, j = oracle.rhobar_j(case, init)
+    with pytest.raises(ValueError):
+        oracle.external_step(case, init, rb, j)
 
 
 def test_multiblock_equals_single_block():
