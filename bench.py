@@ -8,7 +8,9 @@ Palabos's own timing runs toggle them off). The primary line is FP64; the FP32
 and FP16-storage runs of the same workload are reported under "variants";
 "adjoint" gives the reverse-sweep / forward time ratio of the hand-written
 adjoint on the Brinkman porosity channel (128 x 64 x 64, 500 steps, checkpoint
-every 50, FP64).
+every 50, FP64), with the default recording budget (segments recorded during
+the forward sweep when device memory holds them) and with every segment
+recomputed.
 
 value        MLUPS = cells * steps / wall seconds / 1e6 over the timed steps,
              populations resident in HBM before the timed region starts.
@@ -98,8 +100,9 @@ def run_precision(precision, n, steps, warmup, omega=1.8, u0=0.04):
 def adjoint_ratio(repeats=3):
     """Brinkman porosity channel 128 x 64 x 64 (Zou-He Poiseuille inlet u_max = 0.04,
     pressure outlet, omega = 1.6, porosity from 32 seeded Gaussian blobs), 500
-    steps, checkpoint every 50: forward (with checkpoints) and reverse-sweep
-    (recompute + adjoint) times from HIP events, best of `repeats`."""
+    steps, checkpoint every 50: forward (with checkpoints, or records) and
+    reverse-sweep (recompute + adjoint) times from HIP events, best of `repeats`,
+    with the default recording budget and with none (every segment recomputed)."""
     import palabos_amd as plb
     from palabos_amd import setups
     nx, ny, nz, u_max = 128, 64, 64, 0.04
@@ -109,26 +112,34 @@ def adjoint_ratio(repeats=3):
     prof = setups.poiseuille_profile(ny, nz, u_max)
     vel = np.zeros(((ny - 2) * (nz - 2), 3))
     vel[:, 0] = prof.reshape(-1)
-    best = None
-    for _ in range(repeats):
-        lat = plb.Block3D(nx, ny, nz, plb.D3Q19, plb.BGK(1.6), storage="f64")
-        try:
-            p_in = lat.addVelocityBoundary(0, -1, (0, 0, 1, ny - 2, 1, nz - 2))
-            lat.setBoundaryVelocity(p_in, vel)
-            p_out = lat.addPressureBoundary(0, 1, (nx - 1, nx - 1, 1, ny - 2, 1, nz - 2))
-            lat.setBoundaryDensity(p_out, np.ones((ny - 2) * (nz - 2)))
-            lat.setPorosity(alpha)
-            lat.setPopulations(np.zeros((nx, ny, nz, 19)))
-            J, _, _, (ms_f, ms_b) = lat.adjointGradient(500, 50, 1, nx - 2, 1.0, 1.0)
-        finally:
-            lat.close()
-        if best is None or ms_f + ms_b < best[0] + best[1]:
-            best = (ms_f, ms_b, J)
-    ms_f, ms_b, J = best
+    def best_of(budget):
+        best = None
+        for _ in range(repeats):
+            lat = plb.Block3D(nx, ny, nz, plb.D3Q19, plb.BGK(1.6), storage="f64")
+            try:
+                p_in = lat.addVelocityBoundary(0, -1, (0, 0, 1, ny - 2, 1, nz - 2))
+                lat.setBoundaryVelocity(p_in, vel)
+                p_out = lat.addPressureBoundary(0, 1, (nx - 1, nx - 1, 1, ny - 2, 1, nz - 2))
+                lat.setBoundaryDensity(p_out, np.ones((ny - 2) * (nz - 2)))
+                lat.setPorosity(alpha)
+                lat.setPopulations(np.zeros((nx, ny, nz, 19)))
+                lat.setAdjointRecordBytes(budget)
+                J, _, _, (ms_f, ms_b) = lat.adjointGradient(500, 50, 1, nx - 2, 1.0, 1.0)
+                kept = lat.adjointRecordInfo()[0]
+            finally:
+                lat.close()
+            if best is None or ms_f + ms_b < best[0] + best[1]:
+                best = (ms_f, ms_b, J, kept)
+        return best
+
+    ms_f, ms_b, J, kept = best_of(-1)
+    ms_f0, ms_b0, J0, _ = best_of(0)
     cells_steps = nx * ny * nz * 500
     return {"workload": "brinkman_channel_128x64x64_500steps_checkpoint50_f64", "forward_ms": ms_f,
             "reverse_ms": ms_b, "ratio": ms_b / ms_f, "forward_mlups": cells_steps / ms_f / 1e3,
-            "objective": J}
+            "objective": J, "segments": 10, "segments_recorded_forward": kept,
+            "forward_ms_full_recompute": ms_f0, "reverse_ms_full_recompute": ms_b0,
+            "ratio_full_recompute": ms_b0 / ms_f0, "same_objective_full_recompute": J0 == J}
 
 
 def host_cpu():

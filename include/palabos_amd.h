@@ -337,8 +337,9 @@ typedef struct {
  * order; 0 for pressure planes; may be NULL). Hand-written discrete adjoint of
  * the collide-and-stream (BGK or regularized BGK, Guo body force, Brinkman
  * porosity) with checkpoints every `checkpoint_every` steps (even) and
- * recomputation; timing_ms (may be NULL) receives the forward and the backward
- * device time. FP64 or 16-bit storage (the 16-bit forward is differentiated
+ * recomputation of the segments not recorded in the forward sweep
+ * (plbamd_set_adjoint_record_bytes); timing_ms (may be NULL) receives the
+ * forward and the backward device time. FP64 or 16-bit storage (the 16-bit forward is differentiated
  * through its FP32 arithmetic, the storage rounding taken as the identity;
  * every block even along z); no per-cell tags (the walls are the domain
  * edges); planes only on single-block BGK lattices without a body force, with
@@ -348,6 +349,19 @@ typedef struct {
  * No Palabos counterpart. */
 int plbamd_adjoint_gradient(int handle, int nsteps, int checkpoint_every, const plbamd_objective* objective,
                             double* J, double* grad_alpha, double* grad_plane_scale, double* timing_ms);
+
+/* Device memory plbamd_adjoint_gradient may use for records made during its
+ * forward sweep: the moments (and plane populations) of every step of the
+ * last segments, which then need no recompute in the reverse sweep (the same
+ * kernels on the same states, so the same records and the same gradient).
+ * bytes >= 0: at most that many; -1 (the default): the free device memory at
+ * the call beyond a reserve of max(4 GiB, 1/32 of the device); 0: every
+ * segment is recomputed. No Palabos counterpart. */
+int plbamd_set_adjoint_record_bytes(int handle, int64_t bytes);
+
+/* Segments the last plbamd_adjoint_gradient call recorded in its forward
+ * sweep, and the device bytes of those records. No Palabos counterpart. */
+int plbamd_get_adjoint_record_info(int handle, int* kept_segments, int64_t* bytes);
 
 /* Number of completed steps. */
 int plbamd_get_time(int handle, int64_t* t);

@@ -76,8 +76,8 @@ struct AdjointArgs {
     const double* planeF;     // f^t of the special (plane) records, 19 per record
     const void* lamNext;      // lambda^{t+1} of this block (slot stride g.slot), FP64 or FP32
     void* lam;                // lambda^t of this block
-    const double* alpha;      // porosity by cellOffset + cell, or null
-    double* gAlpha;           // gradient accumulator by cellOffset + cell
+    const void* alpha;        // porosity by cellOffset + cell, type LT (the forward's arithmetic type), or null
+    void* gAlpha;             // gradient accumulator by cellOffset + cell, type LT, or null
     const uint8_t* tags;      // BulkTag by cellOffset + cell (BT_SKIP: plane cells), or null
     BlockGeom g;
     double omega;
@@ -191,8 +191,10 @@ __device__ __forceinline__ int cx_of(uint32_t i) {
 
 // Reverse step of one bulk (non-plane) cell; plane cells are left to adjoint_planes.
 // The arithmetic type C is the adjoint state's: FP64 for an FP64 forward, FP32
-// for a 16-bit one (whose forward arithmetic is FP32); the objective and the
-// porosity gradient accumulate in FP64.
+// for a 16-bit one (whose forward arithmetic is FP32), which is also the type
+// of the porosity and of the porosity-gradient accumulator (one segment's sum
+// for a 16-bit forward, added into an FP64 total); the objective accumulates
+// in FP64.
 template<typename LT, int DYN, bool FORCED>
 __device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGeom& g, int x, int y, int z, int z0,
                                              double& J) {
@@ -206,8 +208,8 @@ __device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGe
     C mu[19];
     gather_mu_wave<LT, C>(g, lamNext, x, y, z, z0, cell, mu);
     if (!(a.tags && a.tags[idx] == BT_SKIP)) {
-        const double al64 = a.alpha ? a.alpha[idx] : 0.0;
-        const C al = (C)al64;
+        const C al = a.alpha ? reinterpret_cast<const LT*>(a.alpha)[idx] : C(0);
+        const double al64 = (double)al;
         const C dm = C(1) - al;
         const LT* mom = reinterpret_cast<const LT*>(a.mom);
         const C rhoBar = (C)mom[idx];
@@ -319,7 +321,7 @@ __device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGe
                 J += a.invT * (a.wdp * p64 * (double)rho + a.wdrag * a.omega * al64 * (double)j[0] +
                                a.wflux * (double)j[0]);
             }
-            if (a.gAlpha) a.gAlpha[idx] += ga64;
+            if (a.gAlpha) reinterpret_cast<LT*>(a.gAlpha)[idx] += (LT)ga64;
         }
     }
 }
@@ -419,7 +421,7 @@ __global__ void __launch_bounds__(256) adjoint_final(AdjointArgs a) {
         const uint64_t cell = ((uint64_t)x * g.ny + y) * g.nz + z;
         const uint64_t idx = g.cellOffset + cell;
         const bool plane = a.tags && a.tags[idx] == BT_SKIP;
-        const double al = (a.alpha && !plane) ? a.alpha[idx] : 0.0;
+        const double al = (a.alpha && !plane) ? (double)reinterpret_cast<const LT*>(a.alpha)[idx] : 0.0;
         const LT* mom = reinterpret_cast<const LT*>(a.mom);
         const double rhoBar = (double)mom[idx];
         const double jx = (double)mom[a.momStride + idx];
@@ -429,12 +431,22 @@ __global__ void __launch_bounds__(256) adjoint_final(AdjointArgs a) {
 #pragma unroll
         for (int k = 0; k < 19; ++k) lam[(uint64_t)k * g.slot + cell] = (LT)(a.invT * djdf[k]);
         if (in_bulk(g, x, y, z)) {
-            if (a.gAlpha && !plane) a.gAlpha[idx] += a.invT * a.wdrag * a.omega * jx;
+            if (a.gAlpha && !plane) reinterpret_cast<LT*>(a.gAlpha)[idx] += (LT)(a.invT * a.wdrag * a.omega * jx);
             const double p = (gx == a.xin ? a.inW : 0.0) - (gx == a.xout ? a.outW : 0.0);
             J += a.invT * (a.wdp * p * (1.0 + rhoBar) + a.wdrag * a.omega * al * jx + a.wflux * jx);
         }
     }
     accumulate_J(J, a.acc);
+}
+
+// The FP32 gradient accumulator of a 16-bit adjoint (one segment's sum) added
+// into the FP64 gradient, and cleared for the next segment.
+static __global__ void __launch_bounds__(256) flush_gradient(float* seg, double* total, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        total[i] += (double)seg[i];
+        seg[i] = 0.0f;
+    }
 }
 
 // Sum of the per-workgroup accumulators of the bulk and plane kernels (one

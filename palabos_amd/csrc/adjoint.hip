@@ -2,7 +2,9 @@
 // checkpoint every K steps, then segments backwards: recompute the segment from
 // its checkpoint with the forward kernels, recording per step the pre-collision
 // moments (rhoBar, j) of every cell and the populations of the plane cells, then
-// K reverse steps.
+// K reverse steps. The last segments, as many as the recording budget holds,
+// are recorded during the forward sweep instead and need no recompute (the
+// same kernels on the same states: the same records).
 #include "engine.h"
 
 namespace plbamd {
@@ -63,12 +65,18 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     const bool lam64 = st == PLBAMD_F64;
     const size_t lamSize = lam64 ? sizeof(double) : sizeof(float);
     // the recorded moments have the adjoint state's type (MomentType of the storage)
+    // The porosity gradient: FP64 total; a 16-bit adjoint accumulates each segment
+    // in FP32 (gA32, 8 bytes per cell and step less) and adds it to the total.
     char *ck = nullptr, *work = nullptr, *lamA = nullptr, *lamB = nullptr, *mom = nullptr;
     double *pf = nullptr, *gA = nullptr, *gS = nullptr, *acc = nullptr, *sum = nullptr;
+    float* gA32 = nullptr;
+    char* keepMom = nullptr;    // records of the segments recorded in the forward sweep
+    double* keepPf = nullptr;
     uint16_t* dAdjBlock = nullptr;
     uint32_t* dAdjBegin = nullptr;
     auto release = [&]() {
-        hipFree(ck); hipFree(work); hipFree(mom); hipFree(pf); hipFree(lamA); hipFree(lamB); hipFree(gA);
+        hipFree(ck); hipFree(work); hipFree(mom); hipFree(pf); hipFree(lamA); hipFree(lamB); hipFree(gA); hipFree(gA32);
+        hipFree(keepMom); hipFree(keepPf);
         hipFree(gS); hipFree(acc); hipFree(sum); hipFree(dAdjBlock); hipFree(dAdjBegin);
     };
     const uint64_t nAcc = (uint64_t)adjWGs + planeWGs;   // J accumulators: bulk workgroups, then plane ones
@@ -79,6 +87,7 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     if (err == hipSuccess) err = hipMalloc(&lamA, arenaElems * lamSize);
     if (err == hipSuccess) err = hipMalloc(&lamB, arenaElems * lamSize);
     if (err == hipSuccess) err = hipMalloc(&gA, cells * sizeof(double));
+    if (err == hipSuccess && !lam64) err = hipMalloc(&gA32, cells * sizeof(float));
     if (err == hipSuccess) err = hipMalloc(&gS, std::max<uint32_t>(1, L.nSpecial) * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&acc, nAcc * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&sum, 8 * sizeof(double));
@@ -93,7 +102,52 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
         release();
         return fail(PLBAMD_ENOMEM, "adjoint: cannot allocate the checkpoint and adjoint buffers");
     }
+    // Segments recorded in the forward sweep: the last nKeep, whose records
+    // (len per segment, one more for f^T) fit the budget. Record r of kept
+    // segment c is keep[keepFirst[c] + r].
+    const uint64_t recBytes = momElems * lamSize + pfElems * sizeof(double);
+    uint64_t budget = 0;
+    if (L.adjointRecordBytes >= 0) {
+        budget = (uint64_t)L.adjointRecordBytes;
+    } else {
+        size_t freeB = 0, totalB = 0;
+        if (hipMemGetInfo(&freeB, &totalB) == hipSuccess) {
+            const uint64_t reserve = std::max<uint64_t>(4ull << 30, totalB / 32);
+            budget = freeB > reserve ? freeB - reserve : 0;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    int nKeep = 0;
+    uint64_t keepRecs = 1;
+    while (nKeep < nCk) {
+        const int c = nCk - 1 - nKeep;
+        const uint64_t more = keepRecs + (uint64_t)std::min(K, T - c * K);
+        if (more * recBytes > budget) break;
+        keepRecs = more;
+        ++nKeep;
+    }
+    if (nKeep > 0) {
+        if (hipMalloc(&keepMom, keepRecs * momElems * lamSize) != hipSuccess ||
+            hipMalloc(&keepPf, keepRecs * pfElems * sizeof(double)) != hipSuccess) {
+            // the budget was not there after all: recompute everything
+            (void)hipGetLastError();
+            hipFree(keepMom);
+            hipFree(keepPf);
+            keepMom = nullptr;
+            keepPf = nullptr;
+            nKeep = 0;
+        }
+    }
+    const int keepFirst = nCk - nKeep;   // first kept segment
+    auto keepMomAt = [&](int c, int r) {
+        return keepMom + ((uint64_t)(c - keepFirst) * K + (uint64_t)r) * momElems * lamSize;
+    };
+    auto keepPfAt = [&](int c, int r) { return keepPf + ((uint64_t)(c - keepFirst) * K + (uint64_t)r) * pfElems; };
+    L.adjointKept = nKeep;
+    L.adjointKeptBytes = nKeep > 0 ? (int64_t)(keepRecs * recBytes) : 0;
     if (err == hipSuccess) err = hipMemsetAsync(gA, 0, cells * sizeof(double), L.stream);
+    if (err == hipSuccess && gA32) err = hipMemsetAsync(gA32, 0, cells * sizeof(float), L.stream);
     if (err == hipSuccess) err = hipMemsetAsync(gS, 0, std::max<uint32_t>(1, L.nSpecial) * sizeof(double), L.stream);
     if (err == hipSuccess) err = hipMemsetAsync(acc, 0, nAcc * sizeof(double), L.stream);
     // cells the odd steps skip (outermost envelope rows) keep zero moments
@@ -101,12 +155,38 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     const bool saveStats = L.statsOn;
     L.statsOn = false;
     float msF = 0, msB = 0;
-    // forward, with checkpoints (even time: the AA array is in natural layout)
+    if (err == hipSuccess && nKeep > 0) err = hipMemsetAsync(keepMom, 0, keepRecs * momElems * lamSize, L.stream);
+    // forward, with checkpoints (even time: the AA array is in natural layout);
+    // the kept segments record their steps instead
     if (err == hipSuccess) err = hipEventRecord(L.ev0, L.stream);
     for (int c = 0; c < nCk && err == hipSuccess; ++c) {
-        err = hipMemcpyAsync(ck + (uint64_t)c * stateBytes, L.arena, stateBytes, hipMemcpyDeviceToDevice, L.stream);
-        if (err != hipSuccess) break;
-        if (step(L, std::min(K, T - c * K)) != PLBAMD_OK) err = hipErrorUnknown;
+        const int len = std::min(K, T - c * K);
+        if (c < keepFirst) {
+            err = hipMemcpyAsync(ck + (uint64_t)c * stateBytes, L.arena, stateBytes, hipMemcpyDeviceToDevice, L.stream);
+            if (err != hipSuccess) break;
+            if (step(L, len) != PLBAMD_OK) err = hipErrorUnknown;
+            continue;
+        }
+        for (int m = 0; m < len && err == hipSuccess; ++m) {
+            Recording r;
+            r.moments = keepMomAt(c, m);
+            r.momStride = cells;
+            r.specialF = L.nSpecial ? keepPfAt(c, m) : nullptr;
+            if (recordStep(L, (L.time & 1) != 0, r) != PLBAMD_OK) err = hipErrorUnknown;
+            L.time += 1;
+        }
+    }
+    if (err == hipSuccess && nKeep > 0) {
+        // the record of f^T, on a copy (the lattice stays at step T)
+        err = hipMemcpyAsync(work, L.arena, stateBytes, hipMemcpyDeviceToDevice, L.stream);
+        Recording r;
+        r.moments = keepMomAt(nCk - 1, T - (nCk - 1) * K);
+        r.momStride = cells;
+        r.specialF = L.nSpecial ? keepPfAt(nCk - 1, T - (nCk - 1) * K) : nullptr;
+        void* const real = L.arena;
+        L.arena = work;
+        if (err == hipSuccess && recordStep(L, (L.time & 1) != 0, r) != PLBAMD_OK) err = hipErrorUnknown;
+        L.arena = real;
     }
     if (err == hipSuccess) err = hipEventRecord(L.ev1, L.stream);
     if (err == hipSuccess) err = hipEventSynchronize(L.ev1);
@@ -115,7 +195,7 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     AdjointArgs a{};
     a.momStride = cells;
     a.alpha = L.dAlpha;
-    a.gAlpha = gA;
+    a.gAlpha = lam64 ? (void*)gA : (void*)gA32;
     a.tags = L.dTags;
     // the forward's relaxation rate: 16-bit BGK relaxes with omega' (exact_rates)
     a.omega = (st != PLBAMD_F64 && L.desc.dynamics == PLBAMD_BGK) ? (double)exact_rates((float)L.desc.omega).omega
@@ -178,18 +258,26 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     if (err == hipSuccess) err = hipEventRecord(L.ev0, L.stream);
     for (int c = nCk - 1; c >= 0 && err == hipSuccess; --c) {
         const int t0 = c * K, len = std::min(K, T - t0);
-        err = hipMemcpyAsync(work, ck + (uint64_t)c * stateBytes, stateBytes, hipMemcpyDeviceToDevice, L.stream);
-        // steps t0 .. t0 + len - 1 (and, in the last segment, one more for f^T)
-        const int nRec = (c == nCk - 1) ? len + 1 : len;
-        for (int m = 0; m < nRec && err == hipSuccess; ++m) {
-            Recording r;
-            r.moments = mom + (uint64_t)m * momElems * lamSize;
-            r.momStride = cells;
-            r.specialF = L.nSpecial ? pf + (uint64_t)m * pfElems : nullptr;
-            if (recordStep(L, (m & 1) != 0, r) != PLBAMD_OK) err = hipErrorUnknown;
+        // the segment's records: kept from the forward sweep, or recomputed
+        char* segMom = mom;
+        double* segPf = pf;
+        if (c >= keepFirst) {
+            segMom = keepMomAt(c, 0);
+            segPf = keepPfAt(c, 0);
+        } else {
+            err = hipMemcpyAsync(work, ck + (uint64_t)c * stateBytes, stateBytes, hipMemcpyDeviceToDevice, L.stream);
+            // steps t0 .. t0 + len - 1 (and, in the last segment, one more for f^T)
+            const int nRec = (c == nCk - 1) ? len + 1 : len;
+            for (int m = 0; m < nRec && err == hipSuccess; ++m) {
+                Recording r;
+                r.moments = mom + (uint64_t)m * momElems * lamSize;
+                r.momStride = cells;
+                r.specialF = L.nSpecial ? pf + (uint64_t)m * pfElems : nullptr;
+                if (recordStep(L, (m & 1) != 0, r) != PLBAMD_OK) err = hipErrorUnknown;
+            }
         }
         if (err == hipSuccess && c == nCk - 1) {
-            a.mom = mom + (uint64_t)len * momElems * lamSize;
+            a.mom = segMom + (uint64_t)len * momElems * lamSize;
             a.explicitTerm = 1;
             a.lam = lamNext;
             if (lam64) hipLaunchKernelGGL(adjoint_final<double>, agrid, ablock, 0, L.stream, a);
@@ -199,8 +287,8 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
         }
         for (int m = len - 1; m >= 0 && err == hipSuccess; --m) {
             const int t = t0 + m;
-            a.mom = mom + (uint64_t)m * momElems * lamSize;
-            a.planeF = pf + (uint64_t)m * pfElems;
+            a.mom = segMom + (uint64_t)m * momElems * lamSize;
+            a.planeF = segPf + (uint64_t)m * pfElems;
             a.explicitTerm = t > obj.first_step ? 1 : 0;
             a.lamNext = lamNext;
             a.lam = lamCur;
@@ -231,6 +319,11 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
             if (err == hipSuccess && L.nSpecial) err = hipStreamWaitEvent(L.stream, evOut, 0);
             exchangeLam(lamCur);
             std::swap(lamNext, lamCur);
+        }
+        if (err == hipSuccess && gA32) {
+            hipLaunchKernelGGL(flush_gradient, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, L.stream, gA32, gA,
+                               cells);
+            err = hipGetLastError();
         }
     }
     L.arena = arena;

@@ -90,6 +90,11 @@ struct Lattice {
     void* arena = nullptr;
     int64_t time = 0;
     bool statsOn = true;
+    // adjoint: device bytes for moments recorded during the forward sweep (the
+    // last segments then need no recompute); -1: the free memory beyond a reserve
+    int64_t adjointRecordBytes = -1;
+    int adjointKept = 0;               // segments the last adjoint call recorded in its forward sweep
+    int64_t adjointKeptBytes = 0;      // and their device bytes
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -104,7 +109,7 @@ struct Lattice {
     SpecialCell* dSpecial = nullptr;
     uint32_t nSpecial = 0;
     double* dValues = nullptr;
-    double* dAlpha = nullptr;
+    void* dAlpha = nullptr;        // porosity per cell in the arithmetic type (double for FP64 storage, else float)
     double* dBulkPartials = nullptr;
     double* dBulkLevel1 = nullptr;  // kStatsLevel1 records: first level of the statistics reduction
     uint64_t nBulkPartials = 0;       // = workgroups of the bulk launch

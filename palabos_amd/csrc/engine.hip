@@ -172,7 +172,11 @@ int setup(Lattice& L) {
         if (const char* e = getenv("PLBAMD_PACKED")) packed = packedW > 0 && atoi(e) != 0;
         if (packed) tx = packedW;
     }
+    // Staged odd steps take one row segment per workgroup, also for short rows:
+    // two or four 128-cell rows per workgroup measured slower on the 256x128x128
+    // channel (FP32 odd step 138.7, 141.2, 147.4 us; FP64 262.4, 264.4, 271.4 us).
     const int ty = (staged || pairs) ? 1 : std::max(1, txMax / tx);
+
     L.bulkBlock = dim3(tx, ty);
     L.zChunk = tx * (pairs ? 2 : cpt);
     L.staged = staged;
@@ -651,8 +655,16 @@ int rebuild(Lattice& L) {
         PLBAMD_HIP(hipMemcpy(L.dValues, values.data(), values.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     if (!alpha.empty()) {
-        PLBAMD_HIP(hipMalloc(&L.dAlpha, alpha.size() * sizeof(double)));
-        PLBAMD_HIP(hipMemcpy(L.dAlpha, alpha.data(), alpha.size() * sizeof(double), hipMemcpyHostToDevice));
+        // in the arithmetic type of the storage (FP32 below FP64 storage): the
+        // kernels use the porosity converted to it, so the values are the same
+        if (L.desc.storage == PLBAMD_F64) {
+            PLBAMD_HIP(hipMalloc(&L.dAlpha, alpha.size() * sizeof(double)));
+            PLBAMD_HIP(hipMemcpy(L.dAlpha, alpha.data(), alpha.size() * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> a32(alpha.begin(), alpha.end());
+            PLBAMD_HIP(hipMalloc(&L.dAlpha, a32.size() * sizeof(float)));
+            PLBAMD_HIP(hipMemcpy(L.dAlpha, a32.data(), a32.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
     }
     L.dirty = false;
     return PLBAMD_OK;
@@ -1213,6 +1225,21 @@ int plbamd_adjoint_gradient(int handle, int nsteps, int checkpoint_every, const 
     Lattice* L = find(handle);
     if (!L || !objective) return fail(PLBAMD_EINVAL, "adjoint_gradient: bad argument");
     return adjointGradient(*L, nsteps, checkpoint_every, *objective, J, grad_alpha, grad_plane_scale, timing_ms);
+}
+
+int plbamd_set_adjoint_record_bytes(int handle, int64_t bytes) {
+    Lattice* L = find(handle);
+    if (!L || bytes < -1) return fail(PLBAMD_EINVAL, "set_adjoint_record_bytes: bad argument");
+    L->adjointRecordBytes = bytes;
+    return PLBAMD_OK;
+}
+
+int plbamd_get_adjoint_record_info(int handle, int* kept_segments, int64_t* bytes) {
+    Lattice* L = find(handle);
+    if (!L || !kept_segments || !bytes) return fail(PLBAMD_EINVAL, "get_adjoint_record_info: bad argument");
+    *kept_segments = L->adjointKept;
+    *bytes = L->adjointKeptBytes;
+    return PLBAMD_OK;
 }
 
 int plbamd_get_time(int handle, int64_t* t) {

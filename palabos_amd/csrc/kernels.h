@@ -172,7 +172,7 @@ struct StepArgs {
     double omega;
     double force[3];
     const uint8_t* tags;     // BulkTag per cell (all blocks) or null
-    const double* alpha;     // Brinkman porosity per cell (all blocks) or null
+    const void* alpha;       // Brinkman porosity per cell (all blocks), in the arithmetic type, or null
     double* partials;        // 8 doubles per workgroup when statistics are on
     void* moments;           // MOMENTS: rhoBar, jx, jy, jz of the pre-collision state (MomentType<S>),
     uint64_t momStride;      //          [4][momStride] by cell (all blocks)
@@ -239,7 +239,7 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, const BlockGeom&
         const T F[3] = {(T)a.force[0], (T)a.force[1], (T)a.force[2]};
         T rhoBar;
         T drag[3] = {0, 0, 0};
-        const T alpha = (FORCED && a.alpha) ? (T)a.alpha[g.cellOffset + cell] : (T)0;
+        const T alpha = (FORCED && a.alpha) ? reinterpret_cast<const T*>(a.alpha)[g.cellOffset + cell] : (T)0;
         const T uSqr = collide_bulk<T, DYN, FORCED, sizeof(S) == 2>(fc, omega, F, alpha, rhoBar, drag);
         if constexpr (STATS) {
             if (inBulk) {
@@ -526,6 +526,22 @@ __host__ __device__ constexpr int zmove_index(int i) {
 }
 constexpr int kZMoving = 10;
 
+// The m-th (0-4) direction with c_z == SZ and its z-moving index, for a
+// per-lane m (a chain of selects: no indexed array).
+template<int SZ>
+__device__ __forceinline__ void zdir(int m, int& dir, int& k) {
+    dir = 0;
+    k = 0;
+    int n = 0;
+#pragma unroll
+    for (int i = 1; i < 19; ++i) {
+        if (cz(i) == SZ) {
+            if (m == n) { dir = i; k = zmove_index(i); }
+            ++n;
+        }
+    }
+}
+
 // FP32 statistics variants without forcing: asking for 6 waves per SIMD keeps
 // them at 80 VGPRs or fewer with no spills (99 otherwise, 4 waves), which the
 // short-row lattices need (one 128-lane workgroup per row). Every other
@@ -623,11 +639,33 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
                     if (cz(i) == 0 && ((wall >> opp(i)) & 1u)) fc[i] = Store<S>::load(f + (uint64_t)i * g.slot + cell);
             }
         }
+        if constexpr (TAGS) {
+            // z-moving directions with a link wall at x - c_i: the cell's own slot
+            // i, issued before the barrier so that its latency overlaps it
+            if (compute && wall) {
+#pragma unroll
+                for (int i = 0; i < 19; ++i)
+                    if (cz(i) != 0 && ((wall >> opp(i)) & 1u)) fc[i] = Store<S>::load(f + (uint64_t)i * g.slot + cell);
+            }
+        }
+        // Non-periodic row ends: the edge cell's five inputs from beyond the end
+        // are its own slots i (reflection). Lanes 0-4 load the low end's, lanes
+        // 5-9 the high end's: one load instruction for the ten, into the halo
+        // positions that the edge cells read after the barrier. (One lane loading
+        // all five per end measured slower than the edge cell reading its own
+        // slots: 14.2 against 14.6 GLUPS on the FP64 sphere channel.)
+        const bool loEdge = !vlo && t < 5, hiEdge = !vhi && t >= 5 && t < 10;
+        int eDir = 0, eK = 0;
+        R eVal = R();
+        if (loEdge || hiEdge) {
+            int dHi = 0, kHi = 0;
+            zdir<1>(t, eDir, eK);
+            zdir<-1>(t - 5, dHi, kHi);
+            if (hiEdge) { eDir = dHi; eK = kHi; }
+            eVal = Store<S>::load_raw(f + (uint64_t)eDir * g.slot + rowBase + (loEdge ? z0 : z0 + len - 1));
+        }
         // Halo elements inside a periodic or segmented row: the source elements
-        // beyond the segment ends. (At a non-periodic row end the edge cell reads
-        // its reflection from its own slot after the barrier: measured faster
-        // here than staging it through the halo as the pair kernel does, 14.6
-        // against 14.2 GLUPS on the FP64 sphere channel.)
+        // beyond the segment ends.
         R halo[kZMoving];
         const bool loLane = t == 0 && vlo, hiLane = t == W - 1 && vhi;
         if (loLane || hiLane) {
@@ -646,6 +684,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
             for (int k = 0; k < kZMoving; ++k) buf[k * P + hp] = halo[k];
         }
+        if (loEdge || hiEdge) buf[eK * P + (loEdge ? 0 : len + 1)] = eVal;
         if constexpr (TAGS) {
             if (active) blocked[t] = compute ? wall : 0xffffffffu;
         }
@@ -653,13 +692,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
         if (compute) {
 #pragma unroll
             for (int i = 0; i < 19; ++i) {
-                if (cz(i) != 0) {
-                    const int k = zmove_index(i);
-                    int zs = 0;
-                    const bool ok = shift(z, -cz(i), g.nz, g.pz, zs) && !((wall >> opp(i)) & 1u);
-                    fc[i] = ok ? Store<S>::from_raw((R)buf[k * P + t + 1 - cz(i)])
-                               : Store<S>::load(f + (uint64_t)i * g.slot + cell);   // edge or wall reflection
-                }
+                if (cz(i) != 0 && !((wall >> opp(i)) & 1u))
+                    fc[i] = Store<S>::from_raw((R)buf[zmove_index(i) * P + t + 1 - cz(i)]);
             }
         }
         __syncthreads();   // every LDS input is read before the outputs reuse the buffer
@@ -677,8 +711,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
                 } else {
                     const int k = zmove_index(i);
                     int zd = 0;
-                    if (shift(z, cz(i), g.nz, g.pz, zd) && !reflect) buf[k * P + t + 1 + cz(i)] = Store<S>::to_raw(fc[i]);
-                    else Store<S>::store(f + (uint64_t)opp(i) * g.slot + cell, fc[i]);   // edge or wall reflection
+                    if (!shift(z, cz(i), g.nz, g.pz, zd))
+                        buf[k * P + (cz(i) < 0 ? len + 1 : 0)] = Store<S>::to_raw(fc[i]);   // row end: for the edge lanes
+                    else if (!reflect)
+                        buf[k * P + t + 1 + cz(i)] = Store<S>::to_raw(fc[i]);
+                    else
+                        Store<S>::store(f + (uint64_t)opp(i) * g.slot + cell, fc[i]);   // link-wall reflection
                 }
             }
         }
@@ -708,6 +746,22 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
                 if constexpr (TAGS) h = h && !((blocked[cz(i) < 0 ? 0 : len - 1] >> i) & 1u);
                 if (h) Store<S>::store_raw(dst + (cz(i) < 0 ? zlo : zhi), halo[k]);
             }
+        }
+        // Row-end reflections, one store instruction: lanes 0-4 write the low edge
+        // cell's outputs along c_z = -1 (staged at position len + 1), lanes 5-9 the
+        // high edge cell's along c_z = +1 (position 0), each into the cell's own
+        // slot opp(i), when the edge cell was computed here.
+        if (loEdge || hiEdge) {
+            int dOut = 0, kOut = 0, dHi = 0, kHi = 0;
+            zdir<-1>(t, dOut, kOut);
+            zdir<1>(t - 5, dHi, kHi);
+            if (hiEdge) { dOut = dHi; kOut = kHi; }
+            const int ce = loEdge ? 0 : len - 1;
+            bool done = true;
+            if constexpr (TAGS) done = blocked[ce] != 0xffffffffu;
+            if (done)
+                Store<S>::store_raw(f + (uint64_t)opp(dOut) * g.slot + rowBase + z0 + ce,
+                                    (R)buf[kOut * P + (loEdge ? len + 1 : 0)]);
         }
     }
     store_bulk_stats<STATS>(st, wf, a.partials, wg);
@@ -1091,7 +1145,7 @@ struct SpecialArgs {
     const BlockGeom* blocks;
     double omega;
     double force[3];
-    const double* alpha;  // Brinkman porosity per cell (all blocks) or null
+    const void* alpha;    // Brinkman porosity per cell (all blocks), in the arithmetic type, or null
     const SpecialCell* cells;
     uint32_t count;
     const double* values;
@@ -1151,7 +1205,7 @@ __global__ void __launch_bounds__(256) special_step(SpecialArgs a) {
         } else {
             if (kind == SK_PLANE) complete_plane(fc, sc.info, a.values + sc.value);
             T rhoBar, drag[3] = {0, 0, 0};
-            const T alpha = (FORCED && a.alpha) ? (T)a.alpha[g.cellOffset + cell] : (T)0;
+            const T alpha = (FORCED && a.alpha) ? reinterpret_cast<const T*>(a.alpha)[g.cellOffset + cell] : (T)0;
             const T uSqr = collide_bulk<T, DYN, FORCED, sizeof(S) == 2>(fc, omega, F, alpha, rhoBar, drag);
             if (bulk) {
                 st[0] += (double)rhoBar;

@@ -403,6 +403,19 @@ class Block3D:
                                             ctypes.byref(J), ga.ctypes.data, gs.ctypes.data, tm.ctypes.data))
         return J.value, ga, gs[:len(self._planes)], (tm[0], tm[1])
 
+    def setAdjointRecordBytes(self, nbytes):
+        """Device memory adjointGradient may use for the records of the last
+        segments made during its forward sweep (those segments are then not
+        recomputed; the gradient is the same): -1 (default) the free memory
+        beyond a reserve, 0 none, otherwise at most nbytes."""
+        check(lib().plbamd_set_adjoint_record_bytes(self.handle, int(nbytes)))
+
+    def adjointRecordInfo(self):
+        """(segments recorded in the last adjointGradient's forward sweep, their bytes)."""
+        n, b = ctypes.c_int(0), ctypes.c_int64(0)
+        check(lib().plbamd_get_adjoint_record_info(self.handle, ctypes.byref(n), ctypes.byref(b)))
+        return n.value, b.value
+
     def toggleInternalStatistics(self, on):
         check(lib().plbamd_toggle_statistics(self.handle, int(bool(on))))
 

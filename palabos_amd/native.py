@@ -34,7 +34,7 @@ EXPORTS = (
     "plbamd_upload", "plbamd_download", "plbamd_initialize_at_equilibrium",
     "plbamd_collide_and_stream", "plbamd_collide_and_stream_timed", "plbamd_toggle_statistics",
     "plbamd_get_statistics", "plbamd_compute_macroscopic", "plbamd_reduce_moments",
-    "plbamd_adjoint_gradient", "plbamd_get_time", "plbamd_save_binary_block", "plbamd_load_binary_block",
+    "plbamd_adjoint_gradient", "plbamd_set_adjoint_record_bytes", "plbamd_get_adjoint_record_info", "plbamd_get_time", "plbamd_save_binary_block", "plbamd_load_binary_block",
     "plbamd_compute_fields", "plbamd_set_solid_density", "plbamd_reduce_moments_noalloc",
     "plbamd_add_velocity_edge", "plbamd_add_velocity_corner", "plbamd_set_boundary_type",
     "plbamd_execute_processors", "plbamd_bandwidth_probe", "plbamd_multiblock_create_ex",
@@ -131,6 +131,8 @@ def lib():
         "plbamd_set_solid_density": [i32, i32, ctypes.c_double],
         "plbamd_get_time": [i32, ctypes.POINTER(ctypes.c_int64)],
         "plbamd_adjoint_gradient": [i32, i32, i32, ctypes.POINTER(Objective), vp, vp, vp, vp],
+        "plbamd_set_adjoint_record_bytes": [i32, ctypes.c_int64],
+        "plbamd_get_adjoint_record_info": [i32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64)],
         "plbamd_save_binary_block": [i32, ctypes.c_char_p, i32, i32, i32],
         "plbamd_load_binary_block": [i32, ctypes.c_char_p, i32, i32, i32],
         "plbamd_bandwidth_probe": [i32, ctypes.c_uint64, i32, ctypes.POINTER(ctypes.c_double),

@@ -1,7 +1,7 @@
 """The benchmark configurations as runnable studies on the engine.
 
     python -m palabos_amd.studies taylor_green [--n 512 --steps 2000]
-    python -m palabos_amd.studies sphere_drag  [--steps 5000 --storage f64]
+    python -m palabos_amd.studies sphere_drag  [--steps 5000 --storage f64 --ramp 0 --average-last 2000]
     python -m palabos_amd.studies porous       [--steps 20000 --storage f16]
     python -m palabos_amd.studies porous_adjoint [--steps 2000 --storage f16]
 
@@ -122,14 +122,20 @@ def taylor_green_precision(n=512, steps=2000, omega=1.8, u0=0.04, storages=("f32
 # Channel flow past a sphere: drag coefficient
 
 def sphere_drag(nx=256, ny=128, nz=128, radius=16.0, center=(64.0, 64.0, 64.0), u_max=0.05, re=100.0,
-                steps=5000, storage="f64", average_last=2000, sample_every=10, ramp=1500):
+                steps=5000, storage="f64", average_last=2000, sample_every=10, ramp=0):
     """Zou-He Poiseuille inlet (x = 0, u_max), Zou-He pressure outlet (rho = 1),
     half-way bounce-back side walls (the domain edges), a momentum-exchange
-    sphere. Re = u_max 2 r / nu. The inlet velocity rises as sin^2 over the
-    first `ramp` steps (an impulsive start fills the channel with pressure
-    waves that the velocity and pressure planes reflect, and that dominate the
-    force for thousands of steps). C_d = 2 F_x / (rho u^2 pi r^2) with u = u_max
-    and with the mean inlet velocity; F_x averaged over the last steps."""
+    sphere. Re = u_max 2 r / nu. ramp = 0 is the impulsive start (the full
+    profile from the first step); otherwise the inlet velocity rises as sin^2
+    over the first `ramp` steps. An impulsive start fills the channel with
+    pressure waves that the velocity and pressure planes reflect (a quarter-wave
+    resonator of period 4 nx / c_s steps, damped only by the viscosity), so
+    the force oscillates about its mean for far longer than 5000 steps.
+    C_d = 2 F_x / (rho u^2 pi r^2) with u = u_max and with the mean inlet
+    velocity; F_x sampled every `sample_every` steps over the last
+    `average_last` steps. Reported with the band of the samples (min, max,
+    standard deviation) and the mean over the whole oscillation periods the
+    window holds (upward crossings of the mean)."""
     nu = u_max * 2.0 * radius / re
     omega = 1.0 / (3.0 * nu + 0.5)
     lat = _lat.Block3D(nx, ny, nz, _lat.D3Q19, _lat.BGK(omega), storage=storage)
@@ -166,9 +172,18 @@ def sphere_drag(nx=256, ny=128, nz=128, radius=16.0, center=(64.0, 64.0, 64.0), 
         fx = np.array(fx)
         u_mean = float(prof.sum()) / ((ny - 2) * (nz - 2))
         area = math.pi * radius * radius
+        cd = 2.0 / (u_max ** 2 * area)
+        up = np.nonzero((fx[:-1] < fx.mean()) & (fx[1:] >= fx.mean()))[0] + 1
+        periods = len(up) - 1
+        whole = fx[up[0]:up[-1]] if periods > 0 else fx
         return {"lattice": [nx, ny, nz], "storage": storage, "steps": steps, "re": re, "omega": omega, "ramp": ramp,
                 "sphere_cells": int((tags != 0).sum()), "fx_mean": float(fx.mean()), "fx_std": float(fx.std()),
-                "cd_u_max": 2.0 * fx.mean() / (u_max ** 2 * area),
+                "cd_u_max": cd * fx.mean(), "cd_u_max_min": cd * float(fx.min()), "cd_u_max_max": cd * float(fx.max()),
+                "cd_u_max_std": cd * float(fx.std()),
+                "swing_pct": 50.0 * float(fx.max() - fx.min()) / abs(float(fx.mean())),
+                "oscillation_periods": periods,
+                "period_steps": sample_every * float(up[-1] - up[0]) / periods if periods > 0 else None,
+                "cd_u_max_whole_periods": cd * float(whole.mean()),
                 "cd_u_mean": 2.0 * fx.mean() / (u_mean ** 2 * area),
                 "mlups": nx * ny * nz * main / (ms * 1e-3) / 1e6, "timed_steps": main}
     finally:
@@ -257,38 +272,53 @@ def brinkman_spheres(shape, centres, radii, scale=1.0, width=1.0, periodic=(True
 
 def porous_adjoint(shape=(384, 256, 256), count=400, r_range=(6.0, 14.0), seed=5, steps=2000, checkpoint_every=50,
                    window=1000, storage="f16", omega=1.7, g=1e-5, width=1.0, blocks=None, block_seed=11, fd_eps=1e-2,
-                   check_fd=True, keep_blocks=False):
+                   check_fd=True, keep_blocks=False, record_bytes=-1, compare_full_recompute=True):
     """Darcy permeability of the porous medium with the spheres as a smooth
     Brinkman porosity, and its gradient with respect to a global sphere-radius
     scale s by the adjoint: K = nu <j_x> / g averaged over the last `window` of
     `steps` steps from rest, dK/ds = nu / g sum_x dJ/dalpha(x) dalpha/ds(x);
     checked against central finite differences in s (K(s +- eps) from two more
     runs). Regularized BGK with a Guo force, periodic in x, walls on the y and z
-    faces, 60 uneven blocks, checkpoints every `checkpoint_every` steps."""
+    faces, 60 uneven blocks, checkpoints every `checkpoint_every` steps.
+    record_bytes: the device memory for segments recorded during the forward
+    sweep (-1: the free memory beyond a reserve; 0: every segment recomputed);
+    with compare_full_recompute the gradient run is repeated with every segment
+    recomputed, for its cost and to confirm the same gradient."""
     nu = (1.0 / omega - 0.5) / 3.0
     centres, radii = setups.random_spheres(shape, count, r_range, seed, periodic=(True, False, False))
     if blocks is None:
         blocks = setups.uneven_blocks(shape, (5, 4, 3), 32, 128, block_seed)
 
-    def run(scale, want_gradient):
+    def run(scale, want_gradient, budget=record_bytes):
         alpha, dads = brinkman_spheres(shape, centres, radii, scale=scale, width=width)
         lat = _lat.Block3D(*shape, _lat.D3Q19, _lat.RegularizedBGK(omega), storage=storage,
                            periodic=(True, False, False), force=(g, 0.0, 0.0), blocks=blocks,
                            keep_blocks=keep_blocks)
         try:
             lat.setPorosity(alpha)
+            lat.setAdjointRecordBytes(budget)
             J, ga, _, (ms_f, ms_b) = lat.adjointGradient(steps, checkpoint_every, 0, 0, 0.0, 0.0,
                                                          weight_flux=1.0, first_step=steps - window)
+            kept = lat.adjointRecordInfo()
         finally:
             lat.close()
         dK = nu / g * float((ga * dads).sum()) if want_gradient else None
-        return nu * J / g, dK, ms_f, ms_b, float(1.0 - alpha.mean())
+        print(f"[porous_adjoint] scale {scale} record budget {budget}: forward {ms_f:.1f} ms, reverse {ms_b:.1f} ms, "
+              f"{kept[0]} segments recorded forward", file=sys.stderr, flush=True)
+        return nu * J / g, dK, ms_f, ms_b, float(1.0 - alpha.mean()), kept
 
-    K, dK, ms_f, ms_b, porosity = run(1.0, True)
+    K, dK, ms_f, ms_b, porosity, kept = run(1.0, True)
     out = {"lattice": list(shape), "storage": storage, "blocks": len(blocks), "steps": steps,
            "checkpoint_every": checkpoint_every, "window": window, "omega": omega, "g": g, "width": width,
            "porosity": porosity, "permeability": K, "dK_dscale_adjoint": dK,
-           "forward_ms": ms_f, "reverse_ms": ms_b, "reverse_over_forward": ms_b / ms_f}
+           "forward_ms": ms_f, "reverse_ms": ms_b, "reverse_over_forward": ms_b / ms_f,
+           "segments_recorded_forward": kept[0], "segments": -(-steps // checkpoint_every),
+           "recorded_bytes": kept[1]}
+    if compare_full_recompute and kept[0] > 0:
+        K0, dK0, f0, b0, _, _ = run(1.0, True, budget=0)
+        out.update({"forward_ms_full_recompute": f0, "reverse_ms_full_recompute": b0,
+                    "reverse_over_forward_full_recompute": b0 / f0,
+                    "same_result_full_recompute": bool(K0 == K and dK0 == dK)})
     if check_fd:
         Kp = run(1.0 + fd_eps, False)[0]
         Km = run(1.0 - fd_eps, False)[0]
@@ -366,13 +396,18 @@ def main():
     ap.add_argument("--storage", default=None)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--keep-blocks", action="store_true", help="porous studies: keep the 60 blocks on the device")
+    ap.add_argument("--record-bytes", type=int, default=-1,
+                    help="porous_adjoint: device bytes for forward-recorded segments (-1 auto, 0 none)")
+    ap.add_argument("--ramp", type=int, default=0, help="sphere_drag: inlet ramp in steps (0: impulsive start)")
+    ap.add_argument("--average-last", type=int, default=2000, help="sphere_drag: force-averaging window in steps")
     args = ap.parse_args()
     from . import native
     native.init(0)
     if args.study == "taylor_green":
         r = taylor_green_precision(n=args.n, steps=args.steps or 2000)
     elif args.study == "sphere_drag":
-        r = sphere_drag(steps=args.steps or 5000, storage=args.storage or "f64")
+        r = sphere_drag(steps=args.steps or 5000, storage=args.storage or "f64", ramp=args.ramp,
+                        average_last=args.average_last)
     elif args.study == "porous":
         r = porous_permeability(steps=args.steps or 20000, storage=args.storage or "f16", scale=args.scale,
                                 keep_blocks=args.keep_blocks)
@@ -381,7 +416,8 @@ def main():
     elif args.study == "checkpoint":
         r = checkpoint_throughput(n=args.n if args.n != 512 else 256, storage=args.storage or "f32")
     else:
-        r = porous_adjoint(steps=args.steps or 2000, storage=args.storage or "f16", keep_blocks=args.keep_blocks)
+        r = porous_adjoint(steps=args.steps or 2000, storage=args.storage or "f16", keep_blocks=args.keep_blocks,
+                           record_bytes=args.record_bytes)
     print(json.dumps(r))
 
 

@@ -209,3 +209,48 @@ def test_sixteen_bit_adjoint_close_to_fp64(gpu, storage):
     tol = 2e-3 if storage == "f16" else 2e-2
     assert abs(J16 - J64) <= tol * abs(J64)
     assert np.linalg.norm(g16 - g64) <= tol * np.linalg.norm(g64)
+
+
+
+@pytest.mark.parametrize("case_name", ["planes_f64", "forced_f64_blocks", "forced_f16_blocks"])
+def test_forward_recorded_segments_change_nothing(gpu, case_name):
+    """Segments recorded during the forward sweep (no recompute) give the same J
+    and gradients, bit for bit, as recomputing every segment: budgets of every
+    segment (the default), none, the last segment and the last two. T = 18,
+    K = 4: segments of 4, 4, 4, 4 and 2 steps, 19 records in all (f^T once)."""
+    if case_name == "planes_f64":
+        case = brinkman_channel(20, 10, 12, 1.6, 0.04, n_blobs=4)
+        blocks = None
+        args, kw = (1, 18, 1.0, 1.0), {}
+        f0 = rest_state(case)
+    else:
+        case = _forced_porous("rbgk", storage="f64" if "f64" in case_name else "f16")
+        blocks = uneven_blocks(case.shape, [[8], [], [6]])
+        args, kw = (1, case.nx - 2, 0.5, 1.0), {"weight_flux": 2.0, "first_step": 6}
+        f0 = oracle.to_storage(random_state(case, 3) * 0.3, case.storage)
+    T, K = 18, 4
+    records = {5: 19, 0: 0, 1: 3, 2: 7}    # records held by the last n segments
+    per = None
+    results = []
+    for kept in (5, 0, 1, 2):
+        lat = gpu_lattice(case) if blocks is None else _lattice(case, blocks)
+        try:
+            lat.setPopulations(f0)
+            if kept == 5:
+                lat.setAdjointRecordBytes(-1)
+            else:
+                lat.setAdjointRecordBytes(records[kept] * per + per // 2)
+            J, ga, gs, _ = lat.adjointGradient(T, K, *args, **kw)
+            n, nbytes = lat.adjointRecordInfo()
+            print(f"[record] {case_name} kept {n} ({nbytes} B) J {J!r}", flush=True)
+            assert n == kept
+            if kept == 5:
+                assert nbytes % 19 == 0
+                per = nbytes // 19
+            results.append((J, ga.copy(), np.array(gs)))
+        finally:
+            lat.close()
+    for J, ga, gs in results[1:]:
+        assert J == results[0][0]
+        assert np.array_equal(ga, results[0][1])
+        assert np.array_equal(gs, results[0][2])

@@ -34,6 +34,8 @@ def test_sphere_drag_small(gpu):
     assert r["fx_mean"] > 0
     assert r["fx_std"] < 0.1 * r["fx_mean"]   # the ramped start keeps the pressure waves small
     assert 1.0 < r["cd_u_mean"] < 100.0
+    assert r["cd_u_max_min"] <= r["cd_u_max"] <= r["cd_u_max_max"]
+    assert r["cd_u_max_std"] > 0 and r["swing_pct"] > 0
 
 
 def test_porous_permeability_small(gpu):
