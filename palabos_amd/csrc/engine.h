@@ -120,6 +120,7 @@ struct Lattice {
     bool packed = false;           // bulk_pairs with whole rows per workgroup
     bool zWrap = false;            // every block periodic along z
     int ldsPitch = 0;              // packed: LDS pair words per staged direction
+    bool cachedZ = false;          // staged odd steps with ordinary stores for the z-moving outputs (CACHEDZ)
     size_t pairsLds = 0;           // its dynamic LDS bytes (odd steps)
     size_t stagedLds = 0;          // its dynamic LDS bytes
     double* dSpecialPartials = nullptr;

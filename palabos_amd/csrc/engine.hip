@@ -7,6 +7,14 @@
 
 namespace plbamd {
 
+// Population bytes up to which the staged odd step writes its z-moving outputs
+// with ordinary stores (bulk_odd_staged CACHEDZ). Measured on periodic n^3
+// lattices (step time, nontemporal -> ordinary): FP64 128^3 -9 %, 224^3 -6 %,
+// 384^3 (8.0 GiB) -1.4 %, 512^3 (19 GiB) +1.5 %; FP32 192^3 -11 %, 384^3
+// -0.6 %, 512^3 (9.5 GiB) +2 %; the sphere channel 256x128x128 -10 % (FP32)
+// and -11 % (FP64).
+constexpr size_t kCachedZBytes = size_t(9) << 30;
+
 thread_local std::string g_lastError;
 
 int fail(int code, const std::string& msg) {
@@ -77,6 +85,7 @@ int setup(Lattice& L) {
     int txMax = storageSize(d.storage) >= 8 ? 256 : 512, cpt = 1;
     L.xcdRemap = 1;
     if (const char* e = getenv("PLBAMD_XCD_REMAP")) L.xcdRemap = atoi(e) != 0;
+
     // at most 512 threads: the step kernels are compiled for 512-thread workgroups
     // (a 1024-thread bound caps them at 128 registers and the heavier variants spill)
     if (const char* e = getenv("PLBAMD_BULK_TX")) txMax = std::max(64, std::min(512, atoi(e) / 64 * 64));
@@ -234,6 +243,10 @@ int setup(Lattice& L) {
     }
     L.elemSize = storageSize(d.storage);
     const size_t bytes = offset * L.elemSize;
+    // ordinary stores for the staged odd step's z-moving outputs on lattices of
+    // at most kCachedZBytes of populations (PLBAMD_CACHED_Z=0/1 overrides)
+    L.cachedZ = bytes <= kCachedZBytes;
+    if (const char* e = getenv("PLBAMD_CACHED_Z")) L.cachedZ = atoi(e) != 0;
     if (hipMalloc(&L.arena, bytes) != hipSuccess) {
         (void)hipGetLastError();
         return fail(PLBAMD_ENOMEM, "lattice_create: cannot allocate " + std::to_string(bytes) + " bytes");

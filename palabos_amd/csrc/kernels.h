@@ -546,7 +546,13 @@ __device__ __forceinline__ void zdir(int m, int& dir, int& k) {
 // them at 80 VGPRs or fewer with no spills (99 otherwise, 4 waves), which the
 // short-row lattices need (one 128-lane workgroup per row). Every other
 // instantiation is unconstrained (1 = the default).
-template<typename S, int DYN, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false>
+//
+// CACHEDZ: the z-moving outputs (staged rows, halo elements, row-end
+// reflections) are written with ordinary stores instead of nontemporal ones.
+// Chosen for lattices small enough that those lines can stay in the device's
+// last-level cache until the next step reads them (the lines that two
+// workgroups write in parts also merge there).
+template<typename S, int DYN, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false, bool CACHEDZ = false>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     (sizeof(S) == 4 && STATS && !FORCED) ? 6 : 1))) bulk_odd_staged(StepArgs a) {
     using T = typename Store<S>::T;
@@ -724,6 +730,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 
         // 3. aligned stores of the z-moving destination rows; position q takes the
         // output of cell q - c_z when that cell is in this segment and was computed
+        auto storeZ = [](S* p, R v) {
+            if constexpr (CACHEDZ) Store<S>::store_raw_cached(p, v);
+            else Store<S>::store_raw(p, v);
+        };
 #pragma unroll
         for (int k = 0; k < kZMoving; ++k) raw[k] = (R)buf[k * P + t + 1];
         if (loLane || hiLane) {
@@ -739,12 +749,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
                 const int p = t - cz(i);
                 bool ok = active && p >= 0 && p < len;
                 if constexpr (TAGS) ok = ok && !((blocked[p < 0 ? 0 : (p >= len ? 0 : p)] >> i) & 1u);
-                if (ok) Store<S>::store_raw(dst + z, raw[k]);
+                if (ok) storeZ(dst + z, raw[k]);
                 // halo position: the low one takes cell 0's output for c_z < 0,
                 // the high one cell len-1's output for c_z > 0
                 bool h = cz(i) < 0 ? loLane : hiLane;
                 if constexpr (TAGS) h = h && !((blocked[cz(i) < 0 ? 0 : len - 1] >> i) & 1u);
-                if (h) Store<S>::store_raw(dst + (cz(i) < 0 ? zlo : zhi), halo[k]);
+                if (h) storeZ(dst + (cz(i) < 0 ? zlo : zhi), halo[k]);
             }
         }
         // Row-end reflections, one store instruction: lanes 0-4 write the low edge
@@ -759,9 +769,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
             const int ce = loEdge ? 0 : len - 1;
             bool done = true;
             if constexpr (TAGS) done = blocked[ce] != 0xffffffffu;
-            if (done)
-                Store<S>::store_raw(f + (uint64_t)opp(dOut) * g.slot + rowBase + z0 + ce,
-                                    (R)buf[kOut * P + (loEdge ? len + 1 : 0)]);
+            if (done) storeZ(f + (uint64_t)opp(dOut) * g.slot + rowBase + z0 + ce, (R)buf[kOut * P + (loEdge ? len + 1 : 0)]);
         }
     }
     store_bulk_stats<STATS>(st, wf, a.partials, wg);

@@ -59,7 +59,10 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record) {
         if constexpr (std::is_same<S, double>::value) {
 #define PLBAMD_REC(O, TG, FO)                                                                                   \
     do {                                                                                                        \
-        if (O && L.staged)                                                                                      \
+        if (O && L.staged && L.cachedZ)                                                                         \
+            hipLaunchKernelGGL((bulk_odd_staged<double, DYN, TG, false, FO, true, true>), grid, L.bulkBlock,     \
+                               L.stagedLds, L.stream, a);                                                       \
+        else if (O && L.staged)                                                                                 \
             hipLaunchKernelGGL((bulk_odd_staged<double, DYN, TG, false, FO, true>), grid, L.bulkBlock,           \
                                L.stagedLds, L.stream, a);                                                       \
         else                                                                                                    \
@@ -119,7 +122,10 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record) {
     } else {
 #define PLBAMD_BULK(O, TG, ST, FO)                                                                              \
     do {                                                                                                        \
-        if (O && L.staged)                                                                                      \
+        if (O && L.staged && L.cachedZ)                                                                         \
+            hipLaunchKernelGGL((bulk_odd_staged<S, DYN, TG, ST, FO, false, true>), grid, L.bulkBlock,            \
+                               L.stagedLds, L.stream, a);                                                       \
+        else if (O && L.staged)                                                                                 \
             hipLaunchKernelGGL((bulk_odd_staged<S, DYN, TG, ST, FO>), grid, L.bulkBlock, L.stagedLds, L.stream,  \
                                a);                                                                              \
         else                                                                                                    \

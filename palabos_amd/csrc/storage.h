@@ -1,6 +1,8 @@
 // Population storage types of the engine: FP64, FP32, and 16-bit shifted
 // populations (IEEE half or bfloat16) with FP32 arithmetic. Loads and stores
 // carry the nontemporal hint: every population is touched once per step.
+// store_raw_cached is an ordinary store (the staged odd step's z-moving
+// outputs on lattices small enough for the last-level cache to keep them).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,6 +19,7 @@ template<> struct Store<double> {
     using R = double;   // raw stored value
     static __device__ __forceinline__ R load_raw(const double* p) { return __builtin_nontemporal_load(p); }
     static __device__ __forceinline__ void store_raw(double* p, R v) { __builtin_nontemporal_store(v, p); }
+    static __device__ __forceinline__ void store_raw_cached(double* p, R v) { *p = v; }
     static __device__ __forceinline__ T from_raw(R v) { return v; }
     static __device__ __forceinline__ R to_raw(T v) { return v; }
     static __device__ __forceinline__ double load(const double* p) { return __builtin_nontemporal_load(p); }
@@ -29,6 +32,7 @@ template<> struct Store<float> {
     using R = float;
     static __device__ __forceinline__ R load_raw(const float* p) { return __builtin_nontemporal_load(p); }
     static __device__ __forceinline__ void store_raw(float* p, R v) { __builtin_nontemporal_store(v, p); }
+    static __device__ __forceinline__ void store_raw_cached(float* p, R v) { *p = v; }
     static __device__ __forceinline__ T from_raw(R v) { return v; }
     static __device__ __forceinline__ R to_raw(T v) { return v; }
     static __device__ __forceinline__ float load(const float* p) { return __builtin_nontemporal_load(p); }
@@ -45,6 +49,7 @@ template<> struct Store<f16s> {
     static __device__ __forceinline__ void store_raw(f16s* p, R v) {
         __builtin_nontemporal_store(v, reinterpret_cast<uint16_t*>(p));
     }
+    static __device__ __forceinline__ void store_raw_cached(f16s* p, R v) { *reinterpret_cast<uint16_t*>(p) = v; }
     static __device__ __forceinline__ T from_raw(R b) {
         _Float16 h;
         __builtin_memcpy(&h, &b, 2);
@@ -86,6 +91,7 @@ template<> struct Store<bf16s> {
     static __device__ __forceinline__ void store_raw(bf16s* p, R v) {
         __builtin_nontemporal_store(v, reinterpret_cast<uint16_t*>(p));
     }
+    static __device__ __forceinline__ void store_raw_cached(bf16s* p, R v) { *reinterpret_cast<uint16_t*>(p) = v; }
     static __device__ __forceinline__ T from_raw(R b) { return __uint_as_float((uint32_t)b << 16); }
     static __device__ __forceinline__ R to_raw(T v) { return bf16_round(v); }
     static __device__ __forceinline__ float load(const bf16s* p) {
