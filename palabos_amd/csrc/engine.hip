@@ -7,13 +7,15 @@
 
 namespace plbamd {
 
-// Population bytes up to which the staged odd step writes its z-moving outputs
-// with ordinary stores (bulk_odd_staged CACHEDZ). Measured on periodic n^3
+// Cells up to which the odd steps write their z-moving outputs with ordinary
+// stores (bulk_odd_staged and bulk_pairs CACHEDZ). Measured on periodic n^3
 // lattices (step time, nontemporal -> ordinary): FP64 128^3 -9 %, 224^3 -6 %,
-// 384^3 (8.0 GiB) -1.4 %, 512^3 (19 GiB) +1.5 %; FP32 192^3 -11 %, 384^3
-// -0.6 %, 512^3 (9.5 GiB) +2 %; the sphere channel 256x128x128 -10 % (FP32)
-// and -11 % (FP64).
-constexpr size_t kCachedZBytes = size_t(9) << 30;
+// 384^3 -1.4 %, 512^3 +1.5 %; FP32 192^3 -11 %, 384^3 -0.6 %, 512^3 +2 %;
+// FP16 192^3 -7 %, 256^3 -7 %, 384^3 -1.4 %, 512^3 +1.8 %: the turn lies
+// between 384^3 and 512^3 cells whatever the storage. The sphere channel
+// 256x128x128 -10 % (FP32) and -11 % (FP64); the 384x256x256 porous medium
+// (FP16) -3 %.
+constexpr uint64_t kCachedZCells = uint64_t(1) << 26;
 
 thread_local std::string g_lastError;
 
@@ -243,9 +245,9 @@ int setup(Lattice& L) {
     }
     L.elemSize = storageSize(d.storage);
     const size_t bytes = offset * L.elemSize;
-    // ordinary stores for the staged odd step's z-moving outputs on lattices of
-    // at most kCachedZBytes of populations (PLBAMD_CACHED_Z=0/1 overrides)
-    L.cachedZ = bytes <= kCachedZBytes;
+    // ordinary stores for the odd steps' z-moving outputs on lattices of at most
+    // kCachedZCells cells (PLBAMD_CACHED_Z=0/1 overrides)
+    L.cachedZ = cellOffset <= kCachedZCells;
     if (const char* e = getenv("PLBAMD_CACHED_Z")) L.cachedZ = atoi(e) != 0;
     if (hipMalloc(&L.arena, bytes) != hipSuccess) {
         (void)hipGetLastError();

@@ -803,8 +803,10 @@ __device__ __forceinline__ uint32_t pack16(uint16_t lo, uint16_t hi) { return (u
 // 2W cells of one row and the row is wavefront-uniform. ZWRAP: every block is
 // periodic along z (no z edge: the edge handling is compiled out, which keeps
 // the kernel within 80 registers).
+// CACHEDZ (odd steps): ordinary stores for the z-moving rows, the single halves
+// and the reflections, as bulk_odd_staged's CACHEDZ.
 template<typename S, int DYN, bool ODD, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false, bool PACKED = false,
-         bool ZWRAP = false>
+         bool ZWRAP = false, bool CACHEDZ = false>
 __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
     static_assert(sizeof(typename Store<S>::R) == 2, "pairs of 16-bit values");
     using T = typename Store<S>::T;
@@ -955,7 +957,10 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                     }
                 }
             }
-            if constexpr (TAGS) {
+            // TAGS: wavefronts with a link wall take the wall loads (most take none)
+            bool waveWalls = false;
+            if constexpr (TAGS) waveWalls = __ballot(sAct && (wallA | wallB) != 0u) != 0;
+            if (waveWalls) {
                 // link walls at x - c_i: f_i comes back from the cell's own slot i
                 // (after every unconditional load is issued)
                 if (sAct && (wallA | wallB)) {
@@ -970,11 +975,12 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             // TAGS: the own slot-i pair where a link wall reflects a z-moving
             // direction, loaded with the other loads (not after the barrier)
             uint32_t wref[kZMoving];
-            if constexpr (TAGS) {
+#pragma unroll
+            for (int k = 0; k < kZMoving; ++k) wref[k] = 0;
+            if (waveWalls) {
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
                     if (cz(i) == 0) continue;
-                    wref[zmove_index(i)] = 0;
                     if (sAct && (((wallA | wallB) >> opp(i)) & 1u))
                         wref[zmove_index(i)] = ld_pair(f + (uint64_t)i * g.slot + cellA);
                 }
@@ -1044,18 +1050,49 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                     fb[i] = Store<S>::from_raw(rb);
                 }
             }
+            // A plain wavefront: every lane's two cells collided here with no link
+            // wall, and so the cells next to its pair that its stores take outputs
+            // from (segment ends excepted: those are not this segment's). Its stores
+            // below are those of a lattice without tags; the others test every
+            // direction. (With tags, the tag and wall tests cost a third of the odd
+            // step even where no cell is near a wall.)
+            bool plainWave = true;
+            if constexpr (TAGS) {
+                bool lanePlain = true;
+                if (sAct)
+                    lanePlain = compA && compB && (wallA | wallB) == 0u && (p == 0 || blocked[cb + 2 * p - 1] == 0u) &&
+                                (2 * p + 2 >= len || blocked[cb + 2 * p + 2] == 0u);
+                plainWave = __ballot(lanePlain) == __ballot(1);
+            }
             __syncthreads();   // every LDS input is read before the outputs reuse the buffer
             if (sCompA) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
             if (sCompB) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
-            if (sAct) {
+            // outputs: aligned directions to HBM, z-moving ones to LDS. PL: a plain
+            // wavefront (or no tags): every active cell collided here, no walls.
+            // reflections and z-moving rows: nontemporal, or ordinary (CACHEDZ)
+            auto storeZ = [](S* p, T v) {
+                if constexpr (CACHEDZ) Store<S>::store_raw_cached(p, Store<S>::to_raw(v));
+                else Store<S>::store(p, v);
+            };
+            auto storeRawZ = [](S* p, uint16_t v) {
+                if constexpr (CACHEDZ) Store<S>::store_raw_cached(p, v);
+                else Store<S>::store_raw(p, v);
+            };
+            auto storePairZ = [](S* p, uint32_t v) {
+                if constexpr (CACHEDZ) *reinterpret_cast<uint32_t*>(p) = v;
+                else st_pair(p, v);
+            };
+            auto emitOutputs = [&](auto plainTag) {
+                constexpr bool PL = decltype(plainTag)::value;
+                if (!sAct) return;
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
                     // link walls at x + c_i: the output goes back to the cell's own slot opp(i)
-                    const bool rA = compA && ((wallA >> i) & 1u), rB = compB && ((wallB >> i) & 1u);
-                    if (rA) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellA, fa[i]);
-                    if (rB) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellB, fb[i]);
+                    const bool rA = !PL && compA && ((wallA >> i) & 1u), rB = !PL && compB && ((wallB >> i) & 1u);
+                    const bool sA = PL || (compA && !rA), sB = PL || (compB && !rB);
+                    if (rA) storeZ(f + (uint64_t)opp(i) * g.slot + cellA, fa[i]);
+                    if (rB) storeZ(f + (uint64_t)opp(i) * g.slot + cellB, fb[i]);
                     if (cz(i) == 0) {
-                        const bool sA = compA && !rA, sB = compB && !rB;
                         S* dst = rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot + zA;
                         if (sA && sB) st_pair(dst, pack16(Store<S>::to_raw(fa[i]), Store<S>::to_raw(fb[i])));
                         else {
@@ -1065,57 +1102,70 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                     } else {
                         buf[zmove_index(i) * P + q + 1] = pack16(Store<S>::to_raw(fa[i]), Store<S>::to_raw(fb[i]));
                         // edge reflection where the destination leaves a non-periodic z edge
-                        if (cz(i) > 0 && !vzB && compB && !rB) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellB, fb[i]);
-                        if (cz(i) < 0 && !vzA && compA && !rA) Store<S>::store(f + (uint64_t)opp(i) * g.slot + cellA, fa[i]);
+                        if (cz(i) > 0 && !vzB && sB) storeZ(f + (uint64_t)opp(i) * g.slot + cellB, fb[i]);
+                        if (cz(i) < 0 && !vzA && sA) storeZ(f + (uint64_t)opp(i) * g.slot + cellA, fa[i]);
                     }
                 }
-            }
+            };
+            if (plainWave) emitOutputs(std::true_type{});
+            else emitOutputs(std::integral_constant<bool, !TAGS>{});
             __syncthreads();
             // destination pair words: position (zA, zB) takes, for c_z = +1, the
             // outputs of cells zA - 1 and zA; for c_z = -1, of cells zB and zB + 1
-            // segment-local cell c was collided here and sent its direction-i output along c_i
-            auto computed = [&](int c, int i) {
-                if (c < 0 || c >= len) return false;
-                if constexpr (TAGS) return ((blocked[cb + c] >> i) & 1u) == 0;
-                return true;
-            };
+            auto emitRows = [&](auto plainTag) {
+                constexpr bool PL = decltype(plainTag)::value;
+                // segment-local cell c was collided here and sent its direction-i output
+                // along c_i; `near`: c is this lane's or next to its pair, which a plain
+                // wavefront has checked
+                auto computed = [&](int c, int i, bool near) {
+                    if (c < 0 || c >= len) return false;
+                    if constexpr (TAGS) {
+                        if (!(PL && near)) return ((blocked[cb + c] >> i) & 1u) == 0;
+                    }
+                    return true;
+                };
 #pragma unroll
-            for (int i = 0; i < 19; ++i) {
-                if (cz(i) == 0) continue;
-                const uint32_t* row = buf + zmove_index(i) * P;
-                S* dst = rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot;
-                if (sAct) {
-                    uint16_t lo, hi;
-                    bool okLo, okHi;
+                for (int i = 0; i < 19; ++i) {
+                    if (cz(i) == 0) continue;
+                    const uint32_t* row = buf + zmove_index(i) * P;
+                    S* dst = rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot;
+                    if (sAct) {
+                        uint16_t lo, hi;
+                        bool okLo, okHi;
+                        if (cz(i) > 0) {
+                            lo = hi16(row[q]);
+                            okLo = computed(2 * p - 1, i, true);
+                            hi = lo16(row[q + 1]);
+                            okHi = computed(2 * p, i, true);
+                        } else {
+                            lo = hi16(row[q + 1]);
+                            okLo = computed(2 * p + 1, i, true);
+                            hi = lo16(row[q + 2]);
+                            okHi = computed(2 * p + 2, i, true);
+                        }
+                        if (okLo && okHi) storePairZ(dst + zA, pack16(lo, hi));
+                        else {
+                            if (okLo) storeRawZ(dst + zA, lo);
+                            if (okHi) storeRawZ(dst + zB, hi);
+                        }
+                    }
+                    // single halves beyond the segment: c_z = +1 sends the last cell's
+                    // output to z0 + len; c_z = -1 sends the first cell's to z0 - 1 (the
+                    // end cell may be another wavefront's: always tested)
                     if (cz(i) > 0) {
-                        lo = hi16(row[q]);
-                        okLo = computed(2 * p - 1, i);
-                        hi = lo16(row[q + 1]);
-                        okHi = computed(2 * p, i);
+                        int zd = 0;
+                        if (segOk && p == lanes - 1 && shift(z0 + len - 1, 1, g.nz, g.pz, zd) &&
+                            computed(len - 1, i, false))
+                            storeRawZ(dst + zd, hi16(row[sb + nPairs]));
                     } else {
-                        lo = hi16(row[q + 1]);
-                        okLo = computed(2 * p + 1, i);
-                        hi = lo16(row[q + 2]);
-                        okHi = computed(2 * p + 2, i);
-                    }
-                    if (okLo && okHi) st_pair(dst + zA, pack16(lo, hi));
-                    else {
-                        if (okLo) Store<S>::store_raw(dst + zA, lo);
-                        if (okHi) Store<S>::store_raw(dst + zB, hi);
+                        int zd = 0;
+                        if (segOk && p == 0 && shift(z0, -1, g.nz, g.pz, zd) && computed(0, i, false))
+                            storeRawZ(dst + zd, lo16(row[sb + 1]));
                     }
                 }
-                // single halves beyond the segment: c_z = +1 sends the last cell's
-                // output to z0 + len; c_z = -1 sends the first cell's to z0 - 1
-                if (cz(i) > 0) {
-                    int zd = 0;
-                    if (segOk && p == lanes - 1 && shift(z0 + len - 1, 1, g.nz, g.pz, zd) && computed(len - 1, i))
-                        Store<S>::store_raw(dst + zd, hi16(row[sb + nPairs]));
-                } else {
-                    int zd = 0;
-                    if (segOk && p == 0 && shift(z0, -1, g.nz, g.pz, zd) && computed(0, i))
-                        Store<S>::store_raw(dst + zd, lo16(row[sb + 1]));
-                }
-            }
+            };
+            if (plainWave) emitRows(std::true_type{});
+            else emitRows(std::integral_constant<bool, !TAGS>{});
         }
     }
     store_bulk_stats<STATS>(st, wf, a.partials, wg);

@@ -81,6 +81,9 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record) {
         if (L.packed)                                                                                           \
             hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, false, FO, true, true>), grid, L.bulkBlock,           \
                                O ? L.pairsLds : 0, L.stream, a);                                                \
+        else if (O && L.cachedZ)                                                                                \
+            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, false, FO, true, false, false, O>), grid,           \
+                               L.bulkBlock, L.pairsLds, L.stream, a);                                           \
         else                                                                                                    \
             hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, false, FO, true>), grid, L.bulkBlock,                 \
                                O ? L.pairsLds : 0, L.stream, a);                                                \
@@ -98,12 +101,21 @@ int launchStepTyped(Lattice& L, bool odd, bool stats, const Recording* record) {
         if constexpr (sizeof(typename Store<S>::R) == 2) {
 #define PLBAMD_PAIRS(O, TG, ST, FO)                                                                             \
     do {                                                                                                        \
-        if (L.packed)                                                                                           \
+        if (L.packed && O && L.cachedZ)                                                                         \
+            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO, false, true, false, O>), grid, L.bulkBlock, \
+                               L.pairsLds, L.stream, a);                                                        \
+        else if (L.packed)                                                                                      \
             hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO, false, true>), grid, L.bulkBlock,             \
                                O ? L.pairsLds : 0, L.stream, a);                                                \
+        else if (O && L.zWrap && L.cachedZ)                                                                     \
+            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO, false, false, O, O>), grid, L.bulkBlock, \
+                               L.pairsLds, L.stream, a);                                                        \
         else if (O && L.zWrap)                                                                                  \
             hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO, false, false, true>), grid, L.bulkBlock,      \
                                L.pairsLds, L.stream, a);                                                        \
+        else if (O && L.cachedZ)                                                                                \
+            hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO, false, false, false, O>), grid,             \
+                               L.bulkBlock, L.pairsLds, L.stream, a);                                           \
         else                                                                                                    \
             hipLaunchKernelGGL((bulk_pairs<S, DYN, O, TG, ST, FO>), grid, L.bulkBlock, O ? L.pairsLds : 0,      \
                                L.stream, a);                                                                    \
