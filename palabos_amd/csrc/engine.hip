@@ -81,10 +81,11 @@ int setup(Lattice& L) {
     const int per[3] = {d.periodic[0] != 0, d.periodic[1] != 0, d.periodic[2] != 0};
     // Launch shape: TX threads along z (a multiple of the 64-lane wavefront),
     // CPT cells each per row segment; TY rows per workgroup when a row is short.
-    // Measured on MI355X (512^3 periodic): 256 threads for FP64, 512 for the
-    // 4- and 2-byte storages (fewer, larger workgroups for the cheaper cells),
-    // workgroups remapped so that neighbouring row segments share an XCD.
-    int txMax = storageSize(d.storage) >= 8 ? 256 : 512, cpt = 1;
+    // Up to 512 threads, so that a 512-cell row is one segment of the staged odd
+    // step (a periodic row then wraps through LDS with no halo loads: FP64 at
+    // 512^3 18.3k against 17.9k MLUPS with 256-thread segments), workgroups
+    // remapped so that neighbouring row segments share an XCD.
+    int txMax = 512, cpt = 1;
     L.xcdRemap = 1;
     if (const char* e = getenv("PLBAMD_XCD_REMAP")) L.xcdRemap = atoi(e) != 0;
 

@@ -623,6 +623,11 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
         int zlo = 0, zhi = 0;
         const bool vlo = shift(z0, -1, g.nz, g.pz, zlo);
         const bool vhi = shift(z0 + len - 1, 1, g.nz, g.pz, zhi);
+        // A whole periodic row in this segment: the elements beyond its ends are
+        // its other end's, already in LDS. LDS position 1 + j holds cell j, so the
+        // positions 0 and len + 1 wrap to len and 1 (no halo loads or stores).
+        const bool wrapRow = g.pz && len == g.nz;
+        auto wrapPos = [&](int s) { return !wrapRow ? s : (s == 0 ? len : (s == len + 1 ? 1 : s)); };
 
         // 1. loads, all issued before any is consumed: the 9 aligned directions
         // to registers, the 10 z-moving source-row elements to registers and then
@@ -673,7 +678,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
         // Halo elements inside a periodic or segmented row: the source elements
         // beyond the segment ends.
         R halo[kZMoving];
-        const bool loLane = t == 0 && vlo, hiLane = t == W - 1 && vhi;
+        const bool loLane = t == 0 && vlo && !wrapRow, hiLane = t == W - 1 && vhi && !wrapRow;
         if (loLane || hiLane) {
             const int zh = loLane ? zlo : zhi;
 #pragma unroll
@@ -699,7 +704,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
             for (int i = 0; i < 19; ++i) {
                 if (cz(i) != 0 && !((wall >> opp(i)) & 1u))
-                    fc[i] = Store<S>::from_raw((R)buf[zmove_index(i) * P + t + 1 - cz(i)]);
+                    fc[i] = Store<S>::from_raw((R)buf[zmove_index(i) * P + wrapPos(t + 1 - cz(i))]);
             }
         }
         __syncthreads();   // every LDS input is read before the outputs reuse the buffer
@@ -720,7 +725,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
                     if (!shift(z, cz(i), g.nz, g.pz, zd))
                         buf[k * P + (cz(i) < 0 ? len + 1 : 0)] = Store<S>::to_raw(fc[i]);   // row end: for the edge lanes
                     else if (!reflect)
-                        buf[k * P + t + 1 + cz(i)] = Store<S>::to_raw(fc[i]);
+                        buf[k * P + wrapPos(t + 1 + cz(i))] = Store<S>::to_raw(fc[i]);
                     else
                         Store<S>::store(f + (uint64_t)opp(i) * g.slot + cell, fc[i]);   // link-wall reflection
                 }
@@ -746,7 +751,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
             if (cz(i) != 0) {
                 const int k = zmove_index(i);
                 S* dst = rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot;
-                const int p = t - cz(i);
+                int p = t - cz(i);
+                if (wrapRow) p = p < 0 ? p + len : (p >= len ? p - len : p);
                 bool ok = active && p >= 0 && p < len;
                 if constexpr (TAGS) ok = ok && !((blocked[p < 0 ? 0 : (p >= len ? 0 : p)] >> i) & 1u);
                 if (ok) storeZ(dst + z, raw[k]);
@@ -942,6 +948,12 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             int zlo = 0, zhi = 0;
             const bool vlo = shift(z0, -2, g.nz, g.pz, zlo);
             const bool vhi = shift(z0 + len - 1, 1, g.nz, g.pz, zhi);
+            // A whole periodic row in this segment: the pair words beyond its ends
+            // are its other end's, already in LDS (pair j at sb + 1 + j): read and
+            // stored from there, with no halo loads and no single-half stores.
+            const bool wrapRow = g.pz && len == g.nz;
+            const int qLo = (wrapRow && p == 0) ? sb + nPairs : sb + p;          // LDS word of pair p - 1
+            const int qHi = (wrapRow && p == nPairs - 1) ? sb + 1 : sb + p + 2;  // LDS word of pair p + 1
             const int zc = active ? zA : z0;
             uint32_t raw[kZMoving];
             if (sAct) {
@@ -992,7 +1004,7 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             // edge reflection is read from LDS like any other input.
             uint32_t halo[kZMoving];
             const bool segOk = rowOk && allValid;
-            const bool loLane = segOk && p == 0 && vlo, hiLane = segOk && p == lanes - 1 && vhi;
+            const bool loLane = segOk && p == 0 && vlo && !wrapRow, hiLane = segOk && p == lanes - 1 && vhi && !wrapRow;
             const bool loEdge = !ZWRAP && segOk && p == 0 && !vlo, hiEdge = !ZWRAP && segOk && p == lanes - 1 && !vhi;
             const bool edge = loEdge || hiEdge;
             if (loLane || hiLane || edge) {
@@ -1035,11 +1047,11 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                     const bool wA = (wallA >> opp(i)) & 1u, wB = (wallB >> opp(i)) & 1u;
                     uint16_t ra, rb;
                     if (cz(i) > 0) {
-                        ra = hi16(row[q]);
+                        ra = hi16(row[qLo]);
                         rb = lo16(row[q + 1]);
                     } else {
                         ra = hi16(row[q + 1]);
-                        rb = lo16(row[q + 2]);
+                        rb = lo16(row[qHi]);
                     }
                     if constexpr (TAGS) {
                         // link wall at x - c_i: f_i comes back from the cell's own slot i
@@ -1133,15 +1145,15 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                         uint16_t lo, hi;
                         bool okLo, okHi;
                         if (cz(i) > 0) {
-                            lo = hi16(row[q]);
-                            okLo = computed(2 * p - 1, i, true);
+                            lo = hi16(row[qLo]);
+                            okLo = (wrapRow && p == 0) ? computed(len - 1, i, false) : computed(2 * p - 1, i, true);
                             hi = lo16(row[q + 1]);
                             okHi = computed(2 * p, i, true);
                         } else {
                             lo = hi16(row[q + 1]);
                             okLo = computed(2 * p + 1, i, true);
-                            hi = lo16(row[q + 2]);
-                            okHi = computed(2 * p + 2, i, true);
+                            hi = lo16(row[qHi]);
+                            okHi = (wrapRow && p == nPairs - 1) ? computed(0, i, false) : computed(2 * p + 2, i, true);
                         }
                         if (okLo && okHi) storePairZ(dst + zA, pack16(lo, hi));
                         else {
@@ -1152,6 +1164,7 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                     // single halves beyond the segment: c_z = +1 sends the last cell's
                     // output to z0 + len; c_z = -1 sends the first cell's to z0 - 1 (the
                     // end cell may be another wavefront's: always tested)
+                    if (wrapRow) continue;
                     if (cz(i) > 0) {
                         int zd = 0;
                         if (segOk && p == lanes - 1 && shift(z0 + len - 1, 1, g.nz, g.pz, zd) &&
