@@ -117,7 +117,10 @@ __device__ __forceinline__ int find_block(const BlockGeom* blocks, int nBlocks, 
 
 // Block reduction of an 8-double record (fixed order: deterministic): a
 // shuffle tree inside each wavefront, then the wavefront sums in order. Slot 2
-// is a maximum, the others are sums. Up to 1024 threads per workgroup.
+// is a maximum, the others are sums. Up to 1024 threads per workgroup. Only
+// the first N entries are reduced; the others are known to be zero and are
+// written as zero.
+template<int N = 8>
 __device__ __forceinline__ void block_reduce_store(double v[8], double* out) {
     __shared__ double red[16][8];
     const int t = threadIdx.x + threadIdx.y * blockDim.x;
@@ -126,26 +129,26 @@ __device__ __forceinline__ void block_reduce_store(double v[8], double* out) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < N; ++k) {
             const double o = __shfl_down(v[k], off, 64);
             if (lane + off < 64 && t + off < n) v[k] = (k == 2) ? fmax(v[k], o) : v[k] + o;
         }
     }
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) red[wave][k] = v[k];
+        for (int k = 0; k < N; ++k) red[wave][k] = v[k];
     }
     __syncthreads();
     if (t == 0) {
         double r[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) r[k] = red[0][k];
+        for (int k = 0; k < N; ++k) r[k] = red[0][k];
         for (int w = 1; w < nWaves; ++w) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) r[k] = (k == 2) ? fmax(r[k], red[w][k]) : r[k] + red[w][k];
+            for (int k = 0; k < N; ++k) r[k] = (k == 2) ? fmax(r[k], red[w][k]) : r[k] + red[w][k];
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) out[k] = r[k];
+        for (int k = 0; k < 8; ++k) out[k] = k < N ? r[k] : 0.0;
     }
     __syncthreads();
 }
@@ -273,12 +276,21 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, const BlockGeom&
 
 // Statistics of a bulk workgroup: two records, [rhoBar uSqr maxUSqr count Px Py Pz -]
 // and the link-wall force [Fx Fy - Fz - - - -].
-template<bool STATS>
-__device__ __forceinline__ void store_bulk_stats(double st[8], const double wf[3], double* partials, uint32_t wg) {
+// Without link walls (walls: the lattice's wall masks, null when it has none)
+// the force record is zero and written as such, without a reduction; without
+// a force (FORCED false) so are the porous-drag entries 4-6.
+template<bool STATS, bool FORCED>
+__device__ __forceinline__ void store_bulk_stats(double st[8], const double wf[3], double* partials, uint32_t wg,
+                                                 const void* walls) {
     if constexpr (STATS) {
-        block_reduce_store(st, partials + 16 * (uint64_t)wg);
-        double w[8] = {wf[0], wf[1], 0, wf[2], 0, 0, 0, 0};
-        block_reduce_store(w, partials + 16 * (uint64_t)wg + 8);
+        block_reduce_store<FORCED ? 7 : 4>(st, partials + 16 * (uint64_t)wg);
+        if (walls) {
+            double w[8] = {wf[0], wf[1], 0, wf[2], 0, 0, 0, 0};
+            block_reduce_store(w, partials + 16 * (uint64_t)wg + 8);
+        } else if (threadIdx.x + threadIdx.y == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) partials[16 * (uint64_t)wg + 8 + k] = 0.0;
+        }
     }
 }
 
@@ -393,7 +405,7 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
             }
         }
     }
-    store_bulk_stats<STATS>(st, wf, a.partials, wg);
+    store_bulk_stats<STATS, FORCED>(st, wf, a.partials, wg, a.walls);
 }
 
 // ---------------------------------------------------------------------------
@@ -468,7 +480,7 @@ __global__ void __launch_bounds__(512) bulk_even_linear(StepArgs a) {
             for (int i = 0; i < 19; ++i) Store<S>::store(f + (uint64_t)opp(i) * g.slot + c0, fc[0][i]);
         }
     }
-    store_bulk_stats<STATS>(st, wf, a.partials, wg);
+    store_bulk_stats<STATS, FORCED>(st, wf, a.partials, wg, a.walls);
 }
 
 // ---------------------------------------------------------------------------
@@ -542,10 +554,12 @@ __device__ __forceinline__ void zdir(int m, int& dir, int& k) {
     }
 }
 
-// FP32 statistics variants without forcing: asking for 6 waves per SIMD keeps
-// them at 80 VGPRs or fewer with no spills (99 otherwise, 4 waves), which the
-// short-row lattices need (one 128-lane workgroup per row). Every other
-// instantiation is unconstrained (1 = the default).
+// FP32 statistics variants without forcing: asking for 7 waves per SIMD keeps
+// them at 72 VGPRs (2-6 spilled; 99 unconstrained, 4 waves), which the
+// short-row lattices need (one 128-lane workgroup per row): the sphere
+// channel's FP32 odd step with statistics took 131 us at 6 waves (79 VGPRs, no
+// spills) and takes 126 us at 7. Every other instantiation is unconstrained
+// (1 = the default).
 //
 // CACHEDZ: the z-moving outputs (staged rows, halo elements, row-end
 // reflections) are written with ordinary stores instead of nontemporal ones.
@@ -554,7 +568,7 @@ __device__ __forceinline__ void zdir(int m, int& dir, int& k) {
 // workgroups write in parts also merge there).
 template<typename S, int DYN, bool TAGS, bool STATS, bool FORCED, bool MOMENTS = false, bool CACHEDZ = false>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
-    (sizeof(S) == 4 && STATS && !FORCED) ? 6 : 1))) bulk_odd_staged(StepArgs a) {
+    (sizeof(S) == 4 && STATS && !FORCED) ? 7 : 1))) bulk_odd_staged(StepArgs a) {
     using T = typename Store<S>::T;
     using R = typename Store<S>::R;
     // LDS words of at least 32 bits: 16-bit values would put two lanes on one bank word
@@ -778,7 +792,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
             if (done) storeZ(f + (uint64_t)opp(dOut) * g.slot + rowBase + z0 + ce, (R)buf[kOut * P + (loEdge ? len + 1 : 0)]);
         }
     }
-    store_bulk_stats<STATS>(st, wf, a.partials, wg);
+    store_bulk_stats<STATS, FORCED>(st, wf, a.partials, wg, a.walls);
 }
 
 // ---------------------------------------------------------------------------
@@ -1181,7 +1195,7 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             else emitRows(std::integral_constant<bool, !TAGS>{});
         }
     }
-    store_bulk_stats<STATS>(st, wf, a.partials, wg);
+    store_bulk_stats<STATS, FORCED>(st, wf, a.partials, wg, a.walls);
 }
 
 // ---------------------------------------------------------------------------
