@@ -1248,6 +1248,7 @@ T objectiveTerm(const OracleDesc& d, const std::vector<T>& st, const Engine<T>& 
     const T omega = (T)d.omega;
     for (int x = 0; x < d.nx; ++x) for (int y = 0; y < d.ny; ++y) for (int z = 0; z < d.nz; ++z) {
         const long k = eng.lin(x, y, z);
+        if (eng.tag[k] == TAG_WALL) continue;   // solid of link-wise walls: not fluid, counts zero
         const T* f = &st[k * 19];
         T rho = (T)1;
         for (int i = 0; i < 19; ++i) rho = rho + f[i];

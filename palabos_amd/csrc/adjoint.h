@@ -78,7 +78,9 @@ struct AdjointArgs {
     void* lam;                // lambda^t of this block
     const void* alpha;        // porosity by cellOffset + cell, type LT (the forward's arithmetic type), or null
     void* gAlpha;             // gradient accumulator by cellOffset + cell, type LT, or null
-    const uint8_t* tags;      // BulkTag by cellOffset + cell (BT_SKIP: plane cells), or null
+    const uint8_t* tags;      // BulkTag by cellOffset + cell (BT_SKIP: plane cells, or link-wall solid), or null
+    const uint32_t* walls;    // link-wall mask by cellOffset + cell (bit i: wall at x + c_i), or null
+    int solidSkip;            // 1: the BT_SKIP cells are link-wall solid (no planes): no objective there
     BlockGeom g;
     double omega;
     double force[3];
@@ -207,6 +209,16 @@ __device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGe
     // loads feed their neighbours' lanes), then the skip test
     C mu[19];
     gather_mu_wave<LT, C>(g, lamNext, x, y, z, z0, cell, mu);
+    if (a.walls) {
+        // a link wall at x + c_i sends f_i back into the cell's own slot opp(i)
+        // (the forward's reflection): mu_i = lambda_opp(i) of the cell itself
+        const uint32_t wall = a.walls[idx];
+        if (wall) {
+#pragma unroll
+            for (int i = 1; i < 19; ++i)
+                if ((wall >> i) & 1u) mu[i] = (C)lamNext[(uint64_t)opp(i) * g.slot + cell];
+        }
+    }
     if (!(a.tags && a.tags[idx] == BT_SKIP)) {
         const C al = a.alpha ? reinterpret_cast<const LT*>(a.alpha)[idx] : C(0);
         const double al64 = (double)al;
@@ -430,7 +442,7 @@ __global__ void __launch_bounds__(256) adjoint_final(AdjointArgs a) {
         objective_gradient(a, gx, al, djdf);
 #pragma unroll
         for (int k = 0; k < 19; ++k) lam[(uint64_t)k * g.slot + cell] = (LT)(a.invT * djdf[k]);
-        if (in_bulk(g, x, y, z)) {
+        if (in_bulk(g, x, y, z) && !(plane && a.solidSkip)) {
             if (a.gAlpha && !plane) reinterpret_cast<LT*>(a.gAlpha)[idx] += (LT)(a.invT * a.wdrag * a.omega * jx);
             const double p = (gx == a.xin ? a.inW : 0.0) - (gx == a.xout ? a.outW : 0.0);
             J += a.invT * (a.wdp * p * (1.0 + rhoBar) + a.wdrag * a.omega * al * jx + a.wflux * jx);

@@ -15,7 +15,15 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     if (st != PLBAMD_F64 && st != PLBAMD_F16 && st != PLBAMD_BF16)
         return fail(PLBAMD_EINVAL, "adjoint: FP64 or 16-bit storage");
     const bool forced = L.desc.force[0] != 0 || L.desc.force[1] != 0 || L.desc.force[2] != 0;
-    if (!L.hostTags.empty()) return fail(PLBAMD_EINVAL, "adjoint: no per-cell tags (walls are the domain edges)");
+    if (!L.hostTags.empty()) {
+        // link-wise walls only: their reflection is the transposed streaming's,
+        // and their solid cells take no part
+        for (uint8_t t : L.hostTags)
+            if (t != PLBAMD_TAG_BULK && t != PLBAMD_TAG_WALL_LINK)
+                return fail(PLBAMD_EINVAL, "adjoint: of the per-cell tags only link-wise walls are differentiated "
+                                           "(not BounceBack, NoDynamics or momentum exchange)");
+        if (!L.planes.empty()) return fail(PLBAMD_EINVAL, "adjoint: link-wise walls with boundary planes are not differentiated");
+    }
     if (T <= 0 || K <= 0 || (K & 1)) return fail(PLBAMD_EINVAL, "adjoint: steps > 0 and an even checkpoint interval");
     if (obj.first_step < 0 || obj.first_step >= T) return fail(PLBAMD_EINVAL, "adjoint: first_step in [0, nsteps)");
     if (L.time & 1) return fail(PLBAMD_ESTATE, "adjoint: start after an even number of steps");
@@ -197,6 +205,8 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     a.alpha = L.dAlpha;
     a.gAlpha = lam64 ? (void*)gA : (void*)gA32;
     a.tags = L.dTags;
+    a.walls = L.dWalls;
+    a.solidSkip = L.hostTags.empty() ? 0 : 1;
     // the forward's relaxation rate: 16-bit BGK relaxes with omega' (exact_rates)
     a.omega = (st != PLBAMD_F64 && L.desc.dynamics == PLBAMD_BGK) ? (double)exact_rates((float)L.desc.omega).omega
                                                                   : L.desc.omega;

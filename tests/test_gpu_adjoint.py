@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 import palabos_amd
-from cases import Case, channel_case, gpu_lattice, random_state, rest_state, uneven_blocks
+from cases import (TAG_BB, TAG_WALL, Case, channel_case, gpu_lattice, random_state, rest_state, sphere_tags,
+                   uneven_blocks)
 from oracle import oracle
 from palabos_amd import setups
 
@@ -254,3 +255,46 @@ def test_forward_recorded_segments_change_nothing(gpu, case_name):
         assert J == results[0][0]
         assert np.array_equal(ga, results[0][1])
         assert np.array_equal(gs, results[0][2])
+
+
+@pytest.mark.parametrize("dynamics", ["bgk", "rbgk"])
+@pytest.mark.parametrize("tiling", ["single", "blocks"])
+def test_link_wall_adjoint_matches_oracle(gpu, dynamics, tiling):
+    """Guo-forced porous flow around a sphere of link-wise walls (the forward's
+    half-way bounce-back medium): the reverse step reflects at the wall links as
+    the forward's streaming does, the solid cells take no part and count zero in
+    the objective. Against the oracle's dual numbers through the same medium."""
+    case = _forced_porous(dynamics)
+    tags = sphere_tags(case.shape, (8.0, 5.0, 6.0), 2.6, TAG_WALL)
+    case.tags = tags
+    f0 = random_state(case, 5) * 0.3
+    fluid = (tags.ravel() == 0) & (case.alpha.ravel() > 0.1)
+    seeds = np.sort(np.random.RandomState(1).choice(np.flatnonzero(fluid), 16, replace=False))
+    T, K, t0 = 14, 4, 8
+    blocks = uneven_blocks(case.shape, [[7], [4], [5]]) if tiling == "blocks" else None
+    lat = _lattice(case, blocks)
+    try:
+        lat.setTags(tags)
+        lat.setPopulations(f0)
+        J, ga, _, _ = lat.adjointGradient(T, K, 1, case.nx - 2, 0.5, 1.0, weight_flux=2.0, first_step=t0)
+    finally:
+        lat.close()
+    Jd, dJa, _ = oracle.objective_dual(case, f0, T, 1, case.nx - 2, 0.5, 1.0, seeds, w_flux=2.0, first_step=t0)
+    print(f"[link-wall adjoint] {dynamics} {tiling} J {J!r} {Jd!r}", flush=True)
+    assert J == pytest.approx(Jd, rel=1e-11)
+    np.testing.assert_allclose(ga.ravel()[seeds], dJa, rtol=1e-8, atol=1e-15)
+    assert np.all(ga[tags == TAG_WALL] == 0.0)
+
+
+def test_adjoint_refuses_other_solid_tags(gpu):
+    """BounceBack, NoDynamics and momentum-exchange cells are not differentiated."""
+    case = _forced_porous("bgk")
+    lat = _lattice(case)
+    try:
+        tags = np.zeros(case.shape, np.uint8)
+        tags[4, 4, 4] = TAG_BB
+        lat.setTags(tags)
+        with pytest.raises(palabos_amd.PlbError, match="only link-wise walls"):
+            lat.adjointGradient(4, 2, 1, case.nx - 2, 0.5, 1.0)
+    finally:
+        lat.close()
