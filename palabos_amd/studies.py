@@ -4,6 +4,7 @@
     python -m palabos_amd.studies sphere_drag  [--steps 5000 --storage f64 --ramp 0 --average-last 2000]
     python -m palabos_amd.studies porous       [--steps 20000 --storage f16]
     python -m palabos_amd.studies porous_adjoint [--steps 2000 --storage f16]
+    python -m palabos_amd.studies porous_walls_adjoint [--steps 2000 --storage f16]
 
 Each prints one JSON object. Everything runs on the device through the
 native library; the host only builds the geometry, reads back reductions and
@@ -327,6 +328,56 @@ def porous_adjoint(shape=(384, 256, 256), count=400, r_range=(6.0, 14.0), seed=5
     return out
 
 
+def porous_walls_adjoint(shape=(384, 256, 256), count=400, steps=2000, checkpoint_every=50, window=1000,
+                         storage="f16", omega=1.7, g=1e-5, s0=0.1, blocks=None, block_seed=11, fd_eps=1e-2,
+                         check_fd=True, record_bytes=-1):
+    """The permeability of the link-wall sphere medium (the forward's medium of
+    `porous_permeability`: half-way bounce-back links, regularized BGK, Guo
+    force, periodic in x, 60 uneven blocks) with a smooth Brinkman porosity
+    alpha = s alpha0 laid over its fluid (alpha0: 32 seeded Gaussian blobs
+    through a sigmoid, zero in the solid), and dK/ds at s = s0 by the adjoint
+    through the walls: dK/ds = nu / g sum_x dJ/dalpha(x) alpha0(x), checked
+    against central finite differences in s."""
+    nu = (1.0 / omega - 0.5) / 3.0
+    solid = porous_geometry(shape, count)
+    alpha0 = setups.brinkman_blobs(shape, n_blobs=32, seed=4)
+    alpha0[solid] = 0.0
+    if blocks is None:
+        blocks = setups.uneven_blocks(shape, (5, 4, 3), 32, 128, block_seed)
+    tags = np.zeros(shape, np.uint8)
+    tags[solid] = _lat.native.TAG_WALL_LINK
+
+    def run(s, want_gradient):
+        lat = _lat.Block3D(*shape, _lat.D3Q19, _lat.RegularizedBGK(omega), storage=storage,
+                           periodic=(True, False, False), force=(g, 0.0, 0.0), blocks=blocks)
+        try:
+            lat.setTags(tags)
+            lat.setPorosity(s * alpha0)
+            lat.setAdjointRecordBytes(record_bytes)
+            J, ga, _, (ms_f, ms_b) = lat.adjointGradient(steps, checkpoint_every, 0, 0, 0.0, 0.0,
+                                                         weight_flux=1.0, first_step=steps - window)
+            kept = lat.adjointRecordInfo()[0]
+        finally:
+            lat.close()
+        dK = nu / g * float((ga * alpha0).sum()) if want_gradient else None
+        print(f"[porous_walls_adjoint] s {s}: forward {ms_f:.1f} ms, reverse {ms_b:.1f} ms, {kept} segments recorded "
+              f"forward", file=sys.stderr, flush=True)
+        return nu * J / g, dK, ms_f, ms_b, kept
+
+    K, dK, ms_f, ms_b, kept = run(s0, True)
+    out = {"lattice": list(shape), "storage": storage, "blocks": len(blocks), "steps": steps,
+           "checkpoint_every": checkpoint_every, "window": window, "omega": omega, "g": g, "s0": s0,
+           "porosity": float(1.0 - solid.mean()), "permeability": K, "dK_ds_adjoint": dK,
+           "forward_ms": ms_f, "reverse_ms": ms_b, "reverse_over_forward": ms_b / ms_f,
+           "segments_recorded_forward": kept, "segments": -(-steps // checkpoint_every)}
+    if check_fd:
+        Kp = run(s0 + fd_eps, False)[0]
+        Km = run(s0 - fd_eps, False)[0]
+        fd = (Kp - Km) / (2 * fd_eps)
+        out.update({"dK_ds_fd": fd, "fd_eps": fd_eps, "relative_difference": abs(dK - fd) / abs(fd)})
+    return out
+
+
 def checkpoint_throughput(n=256, storage="f32", T="f32", directory=None):
     """Saves an n^3 lattice as a Palabos checkpoint file and loads it back into
     a second lattice (wall-clock times of both calls, file size, and that the
@@ -389,13 +440,14 @@ def fields_throughput(n=256, storage="f32", repeats=3):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["taylor_green", "sphere_drag", "porous", "porous_adjoint", "checkpoint",
-                                      "fields"])
+    ap.add_argument("study", choices=["taylor_green", "sphere_drag", "porous", "porous_adjoint",
+                                      "porous_walls_adjoint", "checkpoint", "fields"])
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--storage", default=None)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--keep-blocks", action="store_true", help="porous studies: keep the 60 blocks on the device")
+    ap.add_argument("--fd-eps", type=float, default=1e-2, help="adjoint studies: finite-difference step in the scale")
     ap.add_argument("--record-bytes", type=int, default=-1,
                     help="porous_adjoint: device bytes for forward-recorded segments (-1 auto, 0 none)")
     ap.add_argument("--ramp", type=int, default=0, help="sphere_drag: inlet ramp in steps (0: impulsive start)")
@@ -411,13 +463,16 @@ def main():
     elif args.study == "porous":
         r = porous_permeability(steps=args.steps or 20000, storage=args.storage or "f16", scale=args.scale,
                                 keep_blocks=args.keep_blocks)
+    elif args.study == "porous_walls_adjoint":
+        r = porous_walls_adjoint(steps=args.steps or 2000, storage=args.storage or "f16", record_bytes=args.record_bytes,
+                                 fd_eps=args.fd_eps)
     elif args.study == "fields":
         r = fields_throughput(n=args.n if args.n != 512 else 256, storage=args.storage or "f32")
     elif args.study == "checkpoint":
         r = checkpoint_throughput(n=args.n if args.n != 512 else 256, storage=args.storage or "f32")
     else:
         r = porous_adjoint(steps=args.steps or 2000, storage=args.storage or "f16", keep_blocks=args.keep_blocks,
-                           record_bytes=args.record_bytes)
+                           record_bytes=args.record_bytes, fd_eps=args.fd_eps)
     print(json.dumps(r))
 
 
