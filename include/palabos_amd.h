@@ -341,8 +341,9 @@ typedef struct {
  * (plbamd_set_adjoint_record_bytes); timing_ms (may be NULL) receives the
  * forward and the backward device time. FP64 or 16-bit storage (the 16-bit forward is differentiated
  * through its FP32 arithmetic, the storage rounding taken as the identity;
- * every block even along z); no per-cell tags (the walls are the domain
- * edges); planes only on single-block BGK lattices without a body force, with
+ * every block even along z); of the per-cell tags only link-wise walls
+ * (reflected as the forward reflects them; solid cells count zero in J, and
+ * not together with planes); planes only on single-block BGK lattices without a body force, with
  * zero porosity on them; started after an even number of steps. Multi-block
  * lattices refresh the first envelope layer of the adjoint state after every
  * reverse step.
