@@ -153,9 +153,11 @@ __device__ __forceinline__ void objective_gradient(const AdjointArgs& a, int gx,
 // per wavefront. A lane whose source lies beyond the wavefront's cells, wraps
 // around a periodic row or leaves the block loads its own value. Every lane of
 // the row with z < nz takes part (their loads are what the others read).
+// wall: the cell's link-wall mask (bit i: wall at x + c_i): those directions
+// take lambda_opp(i) of the cell itself, loaded beside the row loads.
 template<typename LT, typename C>
 __device__ __forceinline__ void gather_mu_wave(const BlockGeom& g, const LT* lamNext, int x, int y, int z, int z0,
-                                               uint64_t cell, C mu[19]) {
+                                               uint64_t cell, C mu[19], uint32_t wall = 0u) {
     const int lane = z - z0;
     mu[0] = (C)lamNext[cell];
 #pragma unroll
@@ -166,15 +168,19 @@ __device__ __forceinline__ void gather_mu_wave(const BlockGeom& g, const LT* lam
             mu[i] = (C)*refl;
             continue;
         }
+        const bool wr = (wall >> i) & 1u;
         const LT* row = lamNext + (uint64_t)i * g.slot + ((uint64_t)dx * g.ny + dy) * g.nz;
         const LT v = row[z];
+        const LT w = wr ? *refl : LT(0);
         if (cz(i) == 0) {
-            mu[i] = (C)v;
+            mu[i] = wr ? (C)w : (C)v;
             continue;
         }
         const LT nb = __shfl(v, lane + cz(i), 64);
         const int zs = z + cz(i);
-        if (zs >= z0 && zs < z0 + 64 && zs < g.nz) {
+        if (wr) {
+            mu[i] = (C)w;
+        } else if (zs >= z0 && zs < z0 + 64 && zs < g.nz) {
             mu[i] = (C)nb;
         } else {
             int dz;
@@ -207,18 +213,12 @@ __device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGe
     LT* lam = reinterpret_cast<LT*>(a.lam) + g.offset;
     // the gather first, by every cell of the row (plane cells included: their
     // loads feed their neighbours' lanes), then the skip test
+    // a link wall at x + c_i sends f_i back into the cell's own slot opp(i) (the
+    // forward's reflection): mu_i = lambda_opp(i) of the cell itself. The mask
+    // is loaded first, so that those loads go out with the gather's.
+    const uint32_t wall = a.walls ? a.walls[idx] : 0u;
     C mu[19];
-    gather_mu_wave<LT, C>(g, lamNext, x, y, z, z0, cell, mu);
-    if (a.walls) {
-        // a link wall at x + c_i sends f_i back into the cell's own slot opp(i)
-        // (the forward's reflection): mu_i = lambda_opp(i) of the cell itself
-        const uint32_t wall = a.walls[idx];
-        if (wall) {
-#pragma unroll
-            for (int i = 1; i < 19; ++i)
-                if ((wall >> i) & 1u) mu[i] = (C)lamNext[(uint64_t)opp(i) * g.slot + cell];
-        }
-    }
+    gather_mu_wave<LT, C>(g, lamNext, x, y, z, z0, cell, mu, wall);
     if (!(a.tags && a.tags[idx] == BT_SKIP)) {
         const C al = a.alpha ? reinterpret_cast<const LT*>(a.alpha)[idx] : C(0);
         const double al64 = (double)al;
