@@ -1137,7 +1137,12 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             else emitOutputs(std::integral_constant<bool, !TAGS>{});
             __syncthreads();
             // destination pair words: position (zA, zB) takes, for c_z = +1, the
-            // outputs of cells zA - 1 and zA; for c_z = -1, of cells zB and zB + 1
+            // outputs of cells zA - 1 and zA; for c_z = -1, of cells zB and zB + 1. Half
+            // words are stored where a cell sent no output there: beyond a non-periodic
+            // z edge (that half is another row's edge reflection) and at the ends of a
+            // row split into segments (the halves beyond a segment end are the
+            // neighbouring segment's).
+            const bool segmented = len < g.nz;
             auto emitRows = [&](auto plainTag) {
                 constexpr bool PL = decltype(plainTag)::value;
                 // segment-local cell c was collided here and sent its direction-i output
@@ -1150,6 +1155,18 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                     }
                     return true;
                 };
+                // the same as words, once per lane: bit i set where the cell sent its
+                // direction-i output along c_i
+                auto sent = [&](int c, bool near) -> uint32_t {
+                    if (c < 0 || c >= len) return 0u;
+                    if constexpr (TAGS) {
+                        if (!(PL && near)) return ~blocked[cb + c];
+                    }
+                    return ~0u;
+                };
+                const uint32_t sentA = sent(2 * p, true), sentB = sent(2 * p + 1, true);
+                const uint32_t sentLo = (wrapRow && p == 0) ? sent(len - 1, false) : sent(2 * p - 1, true);
+                const uint32_t sentHi = (wrapRow && p == nPairs - 1) ? sent(0, false) : sent(2 * p + 2, true);
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
                     if (cz(i) == 0) continue;
@@ -1160,34 +1177,36 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                         bool okLo, okHi;
                         if (cz(i) > 0) {
                             lo = hi16(row[qLo]);
-                            okLo = (wrapRow && p == 0) ? computed(len - 1, i, false) : computed(2 * p - 1, i, true);
+                            okLo = (sentLo >> i) & 1u;
                             hi = lo16(row[q + 1]);
-                            okHi = computed(2 * p, i, true);
+                            okHi = (sentA >> i) & 1u;
                         } else {
                             lo = hi16(row[q + 1]);
-                            okLo = computed(2 * p + 1, i, true);
+                            okLo = (sentB >> i) & 1u;
                             hi = lo16(row[qHi]);
-                            okHi = (wrapRow && p == nPairs - 1) ? computed(0, i, false) : computed(2 * p + 2, i, true);
+                            okHi = (sentHi >> i) & 1u;
                         }
                         if (okLo && okHi) storePairZ(dst + zA, pack16(lo, hi));
-                        else {
-                            if (okLo) storeRawZ(dst + zA, lo);
-                            if (okHi) storeRawZ(dst + zB, hi);
-                        }
+                        else if (okLo || okHi) storeRawZ(dst + (okLo ? zA : zB), okLo ? lo : hi);
                     }
-                    // single halves beyond the segment: c_z = +1 sends the last cell's
-                    // output to z0 + len; c_z = -1 sends the first cell's to z0 - 1 (the
-                    // end cell may be another wavefront's: always tested)
-                    if (wrapRow) continue;
+                }
+                // single halves beyond the segment: c_z = +1 sends the last cell's
+                // output to z0 + len; c_z = -1 sends the first cell's to z0 - 1 (the
+                // end cell may be another wavefront's: always tested)
+                if (!segmented) return;
+                int zdHi = 0, zdLo = 0;
+                const bool beyondHi = segOk && p == lanes - 1 && shift(z0 + len - 1, 1, g.nz, g.pz, zdHi);
+                const bool beyondLo = segOk && p == 0 && shift(z0, -1, g.nz, g.pz, zdLo);
+                if (!beyondHi && !beyondLo) return;
+#pragma unroll
+                for (int i = 0; i < 19; ++i) {
+                    if (cz(i) == 0) continue;
+                    const uint32_t* row = buf + zmove_index(i) * P;
+                    S* dst = rows[1 + cx(i)][1 + cy(i)] + (uint64_t)i * g.slot;
                     if (cz(i) > 0) {
-                        int zd = 0;
-                        if (segOk && p == lanes - 1 && shift(z0 + len - 1, 1, g.nz, g.pz, zd) &&
-                            computed(len - 1, i, false))
-                            storeRawZ(dst + zd, hi16(row[sb + nPairs]));
+                        if (beyondHi && computed(len - 1, i, false)) storeRawZ(dst + zdHi, hi16(row[sb + nPairs]));
                     } else {
-                        int zd = 0;
-                        if (segOk && p == 0 && shift(z0, -1, g.nz, g.pz, zd) && computed(0, i, false))
-                            storeRawZ(dst + zd, lo16(row[sb + 1]));
+                        if (beyondLo && computed(0, i, false)) storeRawZ(dst + zdLo, lo16(row[sb + 1]));
                     }
                 }
             };
