@@ -939,7 +939,7 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
         // rows on a block edge: per-lane edge selection, no LDS (the outermost
         // envelope layer is not collided: see bulk_odd_staged)
         if (rowOk && !allValid && !outer_envelope_row(g, x, y)) {
-            if (compA)
+                if (compA)
                 odd_cell_general<S, DYN, STATS, FORCED, MOMENTS>(a, g, f, zA, cellA, tagA, inBulkA, xs, ys, vx, vy, st,
                                                                  wallA, wf);
             if (compB)
@@ -989,12 +989,16 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
             if (waveWalls) {
                 // link walls at x - c_i: f_i comes back from the cell's own slot i
                 // (after every unconditional load is issued)
+                // (one pair load for both cells)
                 if (sAct && (wallA | wallB)) {
 #pragma unroll
                     for (int i = 0; i < 19; ++i) {
                         if (cz(i) != 0) continue;
-                        if ((wallA >> opp(i)) & 1u) fa[i] = Store<S>::load(f + (uint64_t)i * g.slot + cellA);
-                        if ((wallB >> opp(i)) & 1u) fb[i] = Store<S>::load(f + (uint64_t)i * g.slot + cellB);
+                        if (((wallA | wallB) >> opp(i)) & 1u) {
+                            const uint32_t v = ld_pair(f + (uint64_t)i * g.slot + cellA);
+                            if ((wallA >> opp(i)) & 1u) fa[i] = Store<S>::from_raw(lo16(v));
+                            if ((wallB >> opp(i)) & 1u) fb[i] = Store<S>::from_raw(hi16(v));
+                        }
                     }
                 }
             }
