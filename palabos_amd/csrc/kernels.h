@@ -210,6 +210,23 @@ template<typename S>
 using MomentType = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
 
 template<typename T, typename M>
+__device__ __forceinline__ void moments_of(const T* fc, M out[4]) {
+    double rb = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const double v = (double)fc[i];
+        rb += v;
+        jx += cx(i) * v;
+        jy += cy(i) * v;
+        jz += cz(i) * v;
+    }
+    out[0] = (M)rb;
+    out[1] = (M)jx;
+    out[2] = (M)jy;
+    out[3] = (M)jz;
+}
+
+template<typename T, typename M>
 __device__ __forceinline__ void store_moments(const T* fc, M* m, uint64_t stride, uint64_t idx) {
     double rb = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
@@ -224,6 +241,20 @@ __device__ __forceinline__ void store_moments(const T* fc, M* m, uint64_t stride
     m[stride + idx] = (M)jx;
     m[2 * stride + idx] = (M)jy;
     m[3 * stride + idx] = (M)jz;
+}
+
+// The records of a pair of cells (idx even, stride even: 16-bit lattices),
+// computed one cell at a time: one 8-byte store per moment where both cells
+// record, so that a wavefront's stores cover whole lines.
+__device__ __forceinline__ void store_moments_pair(const float ra[4], const float rb[4], bool recA, bool recB,
+                                                   float* m, uint64_t stride, uint64_t idx) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float* p = m + k * stride + idx;
+        if (recA && recB) *reinterpret_cast<float2*>(p) = make_float2(ra[k], rb[k]);
+        else if (recA) p[0] = ra[k];
+        else if (recB) p[1] = rb[k];
+    }
 }
 
 // Collision of one cell by its bulk tag: background dynamics (with moments
@@ -919,8 +950,16 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                 fb[i] = Store<S>::from_raw(hi16(v));
             }
         }
-        if (compA) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
-        if (compB) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
+        const bool recA = MOMENTS && compA && (tagA == BT_BULK || tagA == BT_WALL_NEAR);
+        const bool recB = MOMENTS && compB && (tagB == BT_BULK || tagB == BT_WALL_NEAR);
+        float ra[4] = {0, 0, 0, 0}, rb[4] = {0, 0, 0, 0};
+        if (recA) moments_of(fa, ra);
+        if (compA) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
+        if (recB) moments_of(fb, rb);
+        if (compB) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
+        if constexpr (MOMENTS)
+            store_moments_pair(ra, rb, recA, recB, reinterpret_cast<float*>(a.moments), a.momStride,
+                               g.cellOffset + cellA);
         // whole pair words, also for pairs of two cells not collided here (no holes in
         // the lines written): such a cell gets back the value of slot opp(i) it was
         // loaded with (only this thread writes the pair in an even step)
@@ -1098,8 +1137,17 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                 plainWave = __ballot(lanePlain) == __ballot(1);
             }
             __syncthreads();   // every LDS input is read before the outputs reuse the buffer
-            if (sCompA) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
-            if (sCompB) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
+            // MOMENTS: the records of the pair stored together (collide_cell records one cell)
+            const bool recA = MOMENTS && sCompA && (tagA == BT_BULK || tagA == BT_WALL_NEAR);
+            const bool recB = MOMENTS && sCompB && (tagB == BT_BULK || tagB == BT_WALL_NEAR);
+            float ra[4] = {0, 0, 0, 0}, rb[4] = {0, 0, 0, 0};
+            if (recA) moments_of(fa, ra);
+            if (sCompA) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
+            if (recB) moments_of(fb, rb);
+            if (sCompB) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
+            if constexpr (MOMENTS)
+                store_moments_pair(ra, rb, recA, recB, reinterpret_cast<float*>(a.moments), a.momStride,
+                                   g.cellOffset + cellA);
             // outputs: aligned directions to HBM, z-moving ones to LDS. PL: a plain
             // wavefront (or no tags): every active cell collided here, no walls.
             // reflections and z-moving rows: nontemporal, or ordinary (CACHEDZ)
