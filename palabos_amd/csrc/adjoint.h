@@ -153,11 +153,13 @@ __device__ __forceinline__ void objective_gradient(const AdjointArgs& a, int gx,
 // per wavefront. A lane whose source lies beyond the wavefront's cells, wraps
 // around a periodic row or leaves the block loads its own value. Every lane of
 // the row with z < nz takes part (their loads are what the others read).
-// wall: the cell's link-wall mask (bit i: wall at x + c_i): those directions
-// take lambda_opp(i) of the cell itself, loaded beside the row loads.
-template<typename LT, typename C>
+// WALLS, wall: the cell's link-wall mask (bit i: wall at x + c_i): those
+// directions take lambda_opp(i) of the cell itself, loaded beside the row loads.
+// A lattice without link walls takes the kernel without them: no conditional
+// load between the row loads, which then go out back to back.
+template<typename LT, typename C, bool WALLS>
 __device__ __forceinline__ void gather_mu_wave(const BlockGeom& g, const LT* lamNext, int x, int y, int z, int z0,
-                                               uint64_t cell, C mu[19], uint32_t wall = 0u) {
+                                               uint64_t cell, C mu[19], uint32_t wall) {
     const int lane = z - z0;
     mu[0] = (C)lamNext[cell];
 #pragma unroll
@@ -168,7 +170,7 @@ __device__ __forceinline__ void gather_mu_wave(const BlockGeom& g, const LT* lam
             mu[i] = (C)*refl;
             continue;
         }
-        const bool wr = (wall >> i) & 1u;
+        const bool wr = WALLS && ((wall >> i) & 1u);
         const LT* row = lamNext + (uint64_t)i * g.slot + ((uint64_t)dx * g.ny + dy) * g.nz;
         const LT v = row[z];
         const LT w = wr ? *refl : LT(0);
@@ -203,7 +205,7 @@ __device__ __forceinline__ int cx_of(uint32_t i) {
 // of the porosity and of the porosity-gradient accumulator (one segment's sum
 // for a 16-bit forward, added into an FP64 total); the objective accumulates
 // in FP64.
-template<typename LT, int DYN, bool FORCED>
+template<typename LT, int DYN, bool FORCED, bool WALLS>
 __device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGeom& g, int x, int y, int z, int z0,
                                              double& J) {
     using C = LT;
@@ -216,9 +218,9 @@ __device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGe
     // a link wall at x + c_i sends f_i back into the cell's own slot opp(i) (the
     // forward's reflection): mu_i = lambda_opp(i) of the cell itself. The mask
     // is loaded first, so that those loads go out with the gather's.
-    const uint32_t wall = a.walls ? a.walls[idx] : 0u;
+    const uint32_t wall = WALLS ? a.walls[idx] : 0u;
     C mu[19];
-    gather_mu_wave<LT, C>(g, lamNext, x, y, z, z0, cell, mu, wall);
+    gather_mu_wave<LT, C, WALLS>(g, lamNext, x, y, z, z0, cell, mu, wall);
     if (!(a.tags && a.tags[idx] == BT_SKIP)) {
         const C al = a.alpha ? reinterpret_cast<const LT*>(a.alpha)[idx] : C(0);
         const double al64 = (double)al;
@@ -339,11 +341,13 @@ __device__ __forceinline__ void adjoint_cell(const AdjointArgs& a, const BlockGe
 }
 
 // Reverse step of every bulk cell of every block (64 x 4 cells per workgroup).
-template<typename LT, int DYN, bool FORCED>
+// WALLS: the lattice has link-wall masks (a.walls not null).
+template<typename LT, int DYN, bool FORCED, bool WALLS>
 __global__ void __launch_bounds__(256) adjoint_bulk(AdjointArgs a) {
     const AdjointCell c = adjoint_cell_of(a);
     double J = 0.0;
-    if (c.z < c.g.nz && c.y < c.g.ny) adjoint_cell<LT, DYN, FORCED>(a, c.g, c.x, c.y, c.z, c.z - (int)threadIdx.x, J);
+    if (c.z < c.g.nz && c.y < c.g.ny)
+        adjoint_cell<LT, DYN, FORCED, WALLS>(a, c.g, c.x, c.y, c.z, c.z - (int)threadIdx.x, J);
     if (a.explicitTerm) accumulate_J(J, a.acc);   // J is zero before the objective's window
 }
 

@@ -224,19 +224,25 @@ int adjointGradient(Lattice& L, int T, int K, const plbamd_objective& obj, doubl
     a.wgBegin = dAdjBegin;
     a.acc = acc;
     const bool rbgk = L.desc.dynamics == PLBAMD_RBGK;
-#define PLBAMD_ADJ(LT) \
+    const bool walls = a.walls != nullptr;
+#define PLBAMD_ADJ(LT, W) \
     do { \
         if (rbgk) { \
-            if (forced) hipLaunchKernelGGL((adjoint_bulk<LT, 1, true>), gr, ablock, 0, L.stream, args); \
-            else hipLaunchKernelGGL((adjoint_bulk<LT, 1, false>), gr, ablock, 0, L.stream, args); \
+            if (forced) hipLaunchKernelGGL((adjoint_bulk<LT, 1, true, W>), gr, ablock, 0, L.stream, args); \
+            else hipLaunchKernelGGL((adjoint_bulk<LT, 1, false, W>), gr, ablock, 0, L.stream, args); \
         } else { \
-            if (forced) hipLaunchKernelGGL((adjoint_bulk<LT, 0, true>), gr, ablock, 0, L.stream, args); \
-            else hipLaunchKernelGGL((adjoint_bulk<LT, 0, false>), gr, ablock, 0, L.stream, args); \
+            if (forced) hipLaunchKernelGGL((adjoint_bulk<LT, 0, true, W>), gr, ablock, 0, L.stream, args); \
+            else hipLaunchKernelGGL((adjoint_bulk<LT, 0, false, W>), gr, ablock, 0, L.stream, args); \
         } \
     } while (0)
     auto reverseKernel = [&](const AdjointArgs& args, const dim3& gr) {
-        if (lam64) PLBAMD_ADJ(double);
-        else PLBAMD_ADJ(float);
+        if (lam64) {
+            if (walls) PLBAMD_ADJ(double, true);
+            else PLBAMD_ADJ(double, false);
+        } else {
+            if (walls) PLBAMD_ADJ(float, true);
+            else PLBAMD_ADJ(float, false);
+        }
     };
 #undef PLBAMD_ADJ
     auto exchangeLam = [&](char* lam) {
