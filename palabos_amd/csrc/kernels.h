@@ -950,14 +950,17 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                 fb[i] = Store<S>::from_raw(hi16(v));
             }
         }
-        const bool recA = MOMENTS && compA && (tagA == BT_BULK || tagA == BT_WALL_NEAR);
-        const bool recB = MOMENTS && compB && (tagB == BT_BULK || tagB == BT_WALL_NEAR);
+        // tagged lattices: the records of the pair stored together (collide_cell
+        // records one cell; measured faster only with tags)
+        constexpr bool PAIR_REC = MOMENTS && TAGS;
+        const bool recA = PAIR_REC && compA && (tagA == BT_BULK || tagA == BT_WALL_NEAR);
+        const bool recB = PAIR_REC && compB && (tagB == BT_BULK || tagB == BT_WALL_NEAR);
         float ra[4] = {0, 0, 0, 0}, rb[4] = {0, 0, 0, 0};
         if (recA) moments_of(fa, ra);
-        if (compA) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
+        if (compA) collide_cell<S, DYN, STATS, FORCED, MOMENTS && !PAIR_REC>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
         if (recB) moments_of(fb, rb);
-        if (compB) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
-        if constexpr (MOMENTS)
+        if (compB) collide_cell<S, DYN, STATS, FORCED, MOMENTS && !PAIR_REC>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
+        if constexpr (PAIR_REC)
             store_moments_pair(ra, rb, recA, recB, reinterpret_cast<float*>(a.moments), a.momStride,
                                g.cellOffset + cellA);
         // whole pair words, also for pairs of two cells not collided here (no holes in
@@ -1137,15 +1140,19 @@ __global__ void __launch_bounds__(512) bulk_pairs(StepArgs a) {
                 plainWave = __ballot(lanePlain) == __ballot(1);
             }
             __syncthreads();   // every LDS input is read before the outputs reuse the buffer
-            // MOMENTS: the records of the pair stored together (collide_cell records one cell)
-            const bool recA = MOMENTS && sCompA && (tagA == BT_BULK || tagA == BT_WALL_NEAR);
-            const bool recB = MOMENTS && sCompB && (tagB == BT_BULK || tagB == BT_WALL_NEAR);
+            // MOMENTS with tags: the records of the pair stored together (collide_cell
+            // records one cell; measured faster only with tags)
+            constexpr bool PAIR_REC = MOMENTS && TAGS;
+            const bool recA = PAIR_REC && sCompA && (tagA == BT_BULK || tagA == BT_WALL_NEAR);
+            const bool recB = PAIR_REC && sCompB && (tagB == BT_BULK || tagB == BT_WALL_NEAR);
             float ra[4] = {0, 0, 0, 0}, rb[4] = {0, 0, 0, 0};
             if (recA) moments_of(fa, ra);
-            if (sCompA) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
+            if (sCompA)
+                collide_cell<S, DYN, STATS, FORCED, MOMENTS && !PAIR_REC>(a, g, cellA, tagA, inBulkA, fa, st, wallA, wf);
             if (recB) moments_of(fb, rb);
-            if (sCompB) collide_cell<S, DYN, STATS, FORCED, false>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
-            if constexpr (MOMENTS)
+            if (sCompB)
+                collide_cell<S, DYN, STATS, FORCED, MOMENTS && !PAIR_REC>(a, g, cellB, tagB, inBulkB, fb, st, wallB, wf);
+            if constexpr (PAIR_REC)
                 store_moments_pair(ra, rb, recA, recB, reinterpret_cast<float*>(a.moments), a.momStride,
                                    g.cellOffset + cellA);
             // outputs: aligned directions to HBM, z-moving ones to LDS. PL: a plain
