@@ -388,6 +388,9 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
             if constexpr (!ODD) {
 #pragma unroll
                 for (int i = 0; i < 19; ++i) fc[i] = Store<S>::load(at(rows[1][1] + (uint64_t)i * g.slot, zb[1]));
+                if constexpr (TAGS) {
+                    if (tag == BT_SKIP) continue;
+                }
             } else if (fast) {
 #pragma unroll
                 for (int i = 0; i < 19; ++i)
@@ -411,12 +414,8 @@ __global__ void __launch_bounds__(512) bulk_step(StepArgs a) {
                                                        fc, st, wall, wf);
 
             if constexpr (!ODD) {
-                // a skipped cell (collide_cell leaves it as loaded) is written back
-                // unchanged: no holes in the lines the stores write
-                const bool skip = TAGS && tag == BT_SKIP;
 #pragma unroll
-                for (int i = 0; i < 19; ++i)
-                    Store<S>::store(at(rows[1][1] + (uint64_t)opp(i) * g.slot, zb[1]), skip ? fc[opp(i)] : fc[i]);
+                for (int i = 0; i < 19; ++i) Store<S>::store(at(rows[1][1] + (uint64_t)opp(i) * g.slot, zb[1]), fc[i]);
             } else if (fast) {
 #pragma unroll
                 for (int i = 0; i < 19; ++i)
@@ -495,11 +494,14 @@ __global__ void __launch_bounds__(512) bulk_even_linear(StepArgs a) {
             }
             if (comp[h]) collide_cell<S, DYN, STATS, FORCED, MOMENTS>(a, g, cell, tag, inBulk, fc[h], st, wall, wf);
         }
-        // Every cell is written back, a cell not collided here with the value of slot
-        // opp(i) it was loaded with (its own thread is the only writer of the cell in
-        // an even step; the special-cell kernel runs after this one): the stores
-        // leave no holes in the lines they write, which cost a read-modify-write at
-        // the memory (porous medium with 7 % solid cells: even step 377 -> 334 us).
+        // 16-bit pairs: every pair is written back, a cell not collided here with the
+        // value of slot opp(i) it was loaded with (its own thread is the only writer
+        // of the pair in an even step; the special-cell kernel runs after this one):
+        // the stores leave no holes in the lines they write, which cost a
+        // read-modify-write at the memory (porous medium with 7 % solid cells: even
+        // step 377 -> 334 us). One cell per thread keeps its conditional stores (the
+        // write-back would keep the loaded values live through the collision: 75 ->
+        // 103 registers).
         if constexpr (CPT == 2) {
 #pragma unroll
             for (int i = 0; i < 19; ++i) {
@@ -507,10 +509,9 @@ __global__ void __launch_bounds__(512) bulk_even_linear(StepArgs a) {
                 const uint32_t hi = Store<S>::to_raw(comp[1] ? fc[1][i] : fc[1][opp(i)]);
                 __builtin_nontemporal_store(lo | (hi << 16), reinterpret_cast<uint32_t*>(f + (uint64_t)opp(i) * g.slot + c0));
             }
-        } else {
+        } else if (comp[0]) {
 #pragma unroll
-            for (int i = 0; i < 19; ++i)
-                Store<S>::store(f + (uint64_t)opp(i) * g.slot + c0, comp[0] ? fc[0][i] : fc[0][opp(i)]);
+            for (int i = 0; i < 19; ++i) Store<S>::store(f + (uint64_t)opp(i) * g.slot + c0, fc[0][i]);
         }
     }
     store_bulk_stats<STATS, FORCED>(st, wf, a.partials, wg, a.walls);
